@@ -1,0 +1,246 @@
+/*
+ * voxelrt.h -- C ABI of libvoxelrt_hip, the MI355X (gfx950) implementation of
+ * VoxelRT's base frame pipeline: packed-u32 voxel grid + one word per 8^3
+ * brick, two-level Amanatides-Woo DDA, 1-spp path tracer with sun-cone NEE,
+ * temporal reprojection and a 3-pass a-trous filter.
+ *
+ * Every entry point returns int (VRT_OK = 0, < 0 = error) and never throws;
+ * vrt_last_error() gives the message of the most recent failure on a context.
+ * Host arrays are owned by the caller and are read or written before the call
+ * returns. Device memory is owned by the context and its scene handles.
+ * A context is externally synchronised: one caller thread at a time (the
+ * serving loop's internal worker thread is the only other user).
+ *
+ * Reference interfaces replaced (paths into astrilux1/VoxelRT):
+ *   - the WebGPU frame: src/main.js:680-803 (tick: pathtrace -> temporal ->
+ *     atrous x3), uniforms src/shaders/common.wgsl:13-35, per-pass bindings
+ *     pathtrace.wgsl:19-30, temporal.wgsl:7-12, atrous.wgsl:8-18;
+ *   - the scene buffers: src/main.js:187-203 (voxels, bricks, brickMasks
+ *     written once from generateScene, src/scene.js:232-476);
+ *   - the traversal: src/shaders/voxel.wgsl:73-157 (trace / traceShadow);
+ *   - the HDR capture: src/main.js:628-659;
+ *   - VoxelBench's Technique::frame (VoxelBench/pipeline/technique.h:8-15),
+ *     through the adapter in voxelrt_technique.h.
+ */
+#ifndef VOXELRT_H
+#define VOXELRT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes --------------------------------------------------------- */
+#define VRT_OK              0
+#define VRT_ERR_INVALID    -1  /* bad argument, shape or handle */
+#define VRT_ERR_DEVICE     -2  /* HIP runtime or kernel launch failure */
+#define VRT_ERR_NO_MEMORY  -3  /* device allocation failed */
+#define VRT_ERR_BUDGET     -4  /* request exceeds the context's HBM budget */
+#define VRT_ERR_NOT_READY  -5  /* serving: ticket not finished yet */
+#define VRT_ERR_CANCELLED  -6  /* serving: request cancelled / rejected */
+#define VRT_ERR_NO_DEVICE  -7  /* no HIP device visible */
+
+/* ---- constants shared with the reference shaders ------------------------- */
+#define VRT_BRICK            8          /* scene.js:6, BRICK */
+#define VRT_MASK_WORDS      16          /* 512-bit occupancy mask per brick */
+#define VRT_VOXEL_SIZE   0.0625f        /* scene.js:5, 1/16 m */
+#define VRT_EMISSIVE_SCALE 24.0f        /* common.wgsl:11 */
+
+/* Face ids follow VoxelBench (core.h:106): 0..5 = +x,-x,+y,-y,+z,-z, meaning
+ * the outward normal of the face that was hit.  VRT_FACE_NONE marks a hit
+ * whose normal is the zero vector (ray started inside a solid voxel while
+ * moving parallel to the entry axis); VRT_FACE_MISS marks a miss. */
+#define VRT_FACE_NONE  6
+#define VRT_FACE_MISS  255
+#define VRT_VOXEL_MISS 0xFFFFFFFFu
+
+/* uniforms flag bits (params0.z, common.wgsl:20-22, main.js:413-416) */
+#define VRT_FLAG_TEMPORAL  1u
+#define VRT_FLAG_DENOISE   2u
+
+/* ---- opaque handles ------------------------------------------------------- */
+typedef struct vrt_ctx vrt_ctx;
+typedef struct vrt_scene vrt_scene;
+
+typedef struct vrt_create_info {
+  int32_t  device;        /* HIP device ordinal */
+  uint32_t flags;         /* reserved, 0 */
+  uint64_t hbm_budget;    /* bytes for scenes + per-view state; 0 = 90% of free */
+} vrt_create_info;
+
+/* One traced ray's result (16 B).  Output of vrt_trace_rays; the WGSL Hit
+ * (voxel.wgsl:59-63) extended with the voxel index, face id and the DDA's two
+ * step counts. */
+typedef struct vrt_hit {
+  float    t;             /* world metres, -1 on miss (voxel.wgsl:125) */
+  uint32_t voxel;         /* x + y*nx + z*nx*ny, VRT_VOXEL_MISS on miss */
+  uint16_t brick_steps;   /* brick-loop iterations (one brick word each) */
+  uint16_t voxel_steps;   /* voxel-loop iterations (one mask word each) */
+  uint8_t  face;          /* VRT_FACE_*, see above */
+  uint8_t  _pad[3];
+} vrt_hit;
+
+/* Per-view uniforms, byte-for-byte the 304-byte Uniforms block of
+ * common.wgsl:13-35 (column-major matrices).  Fields the base path does not
+ * read (ReSTIR params2/3/5/6) are carried for layout compatibility. */
+typedef struct vrt_uniforms {
+  float    inv_view_proj[16];   /* 0   */
+  float    prev_view_proj[16];  /* 64  */
+  float    cam_pos[4];          /* 128 xyz metres, w = time */
+  float    prev_cam_pos[4];     /* 144 */
+  float    sun_dir[4];          /* 160 xyz toward the sun, w = cos(radius) */
+  float    sun_radiance[4];     /* 176 rgb irradiance, w = sky intensity */
+  uint32_t params0[4];          /* 192 x = frame (+seed), y = bounces, z = flags */
+  float    params1[4];          /* 208 x = width, y = height, z = exposure */
+  uint32_t params2[4];          /* 224 restir (unused by the base path) */
+  float    params3[4];          /* 240 */
+  float    params4[4];          /* 256 y = max temporal history */
+  float    params5[4];          /* 272 */
+  float    params6[4];          /* 288 */
+} vrt_uniforms;
+
+/* Accumulation (temporal history) storage format, main.js:148-149. */
+#define VRT_ACCUM_RGBA16F 0
+#define VRT_ACCUM_RGBA32F 1
+
+/* One view to render.  Per-view device state (G-buffer and accumulation
+ * ping-pong, denoise targets) is kept resident in the context, keyed by
+ * view_id, and reallocated when width/height/accum_format change. */
+typedef struct vrt_view {
+  const vrt_scene* scene;
+  vrt_uniforms     u;
+  float    firefly_clamp;       /* FCLAMP_VALUE (pathtrace.wgsl:35); <=0 off */
+  uint32_t view_id;             /* key for resident history */
+  uint32_t spp;                 /* samples per pixel, >=1 (1 = reference) */
+  uint32_t accum_format;        /* VRT_ACCUM_* */
+} vrt_view;
+
+/* Optional host destinations for one view's outputs (NULL = not copied).
+ * Layouts are row-major, width*height pixels, the reference's texture
+ * formats (main.js:341-349). */
+typedef struct vrt_frame {
+  uint16_t* radiance;   /* rgba16f demodulated radiance (pathtrace out)   */
+  uint8_t*  albedo;     /* rgba8unorm sqrt(albedo)                        */
+  float*    gbuf;       /* rgba32f (normal.xyz, t) of this frame          */
+  void*     accum;      /* accumulation, rgba16f or rgba32f per format    */
+  uint16_t* denoised;   /* rgba16f final a-trous output (denoise flag)    */
+  float*    hdr;        /* rgb f32 illum*(alb8/255)^2, main.js:628-659    */
+  /* filled on return */
+  uint64_t  rays;       /* rays traced for this view (primary+shadow+bounce) */
+  uint64_t  brick_steps;
+  uint64_t  voxel_steps;
+} vrt_frame;
+
+/* Aggregate counters since vrt_create or the last vrt_stats_reset. */
+typedef struct vrt_stats {
+  uint64_t rays;
+  uint64_t brick_steps;
+  uint64_t voxel_steps;
+  uint64_t hits;
+  uint64_t frames;
+  double   gpu_ms_pathtrace;   /* HIP-event time of the last render call */
+  double   gpu_ms_temporal;
+  double   gpu_ms_atrous;
+  double   gpu_ms_frame;       /* pathtrace start .. last a-trous end */
+  uint64_t hbm_resident;       /* bytes held by scenes + view state */
+  uint64_t hbm_budget;
+} vrt_stats;
+
+/* ---- context -------------------------------------------------------------- */
+int  vrt_create(const vrt_create_info* info, vrt_ctx** out);
+void vrt_destroy(vrt_ctx* ctx);
+const char* vrt_last_error(const vrt_ctx* ctx);
+int  vrt_device_count(int32_t* count);
+
+/* ---- scenes --------------------------------------------------------------- *
+ * vrt_scene_upload copies a dense packed-u32 grid (x fastest, scene.js:234;
+ * dims multiples of 8) to the device and builds the brick words and 512-bit
+ * masks there (the build scene.js:411-433 does on the CPU). */
+int vrt_scene_upload(vrt_ctx* ctx, const uint32_t* vox, uint32_t nx, uint32_t ny,
+                     uint32_t nz, uint64_t scene_key, vrt_scene** out);
+int vrt_scene_release(vrt_ctx* ctx, vrt_scene* scene);
+/* Read back the device-built brick words (nbricks) and masks (nbricks*16). */
+int vrt_scene_bricks(vrt_ctx* ctx, const vrt_scene* scene, uint32_t* bricks,
+                     uint32_t* masks);
+int vrt_scene_dims(const vrt_scene* scene, uint32_t dims[3], uint64_t* bytes);
+
+/* ---- traversal (voxel.wgsl:73-153) ---------------------------------------- *
+ * rays: n records of 6 floats (origin xyz in metres, direction xyz).
+ * Host arrays in/out. */
+int vrt_trace_rays(vrt_ctx* ctx, const vrt_scene* scene, const float* rays,
+                   uint64_t n, float tmax, vrt_hit* hits);
+/* Same on device-resident buffers (hipMalloc'd by the caller), enqueued on
+ * the context's stream; *ms receives the kernel's HIP-event time. */
+int vrt_trace_rays_device(vrt_ctx* ctx, const vrt_scene* scene,
+                          const float* d_rays, uint64_t n, float tmax,
+                          vrt_hit* d_hits, float* ms);
+
+/* ---- frames ---------------------------------------------------------------- *
+ * Render nviews views: pathtrace -> temporal -> (if VRT_FLAG_DENOISE)
+ * a-trous steps 1/2/4, per view.  nviews > 1 runs the batched form: all
+ * views' pixels share one persistent launch per pass.  outs may be NULL. */
+int vrt_render(vrt_ctx* ctx, const vrt_view* views, uint32_t nviews,
+               vrt_frame* outs);
+/* Drop the resident state of one view (history restarts). */
+int vrt_view_release(vrt_ctx* ctx, uint32_t view_id);
+
+/* Single passes on host arrays, for the WebGPU pass interface
+ * (temporal.wgsl:7-12, atrous.wgsl:8-18). */
+int vrt_temporal_pass(vrt_ctx* ctx, const vrt_uniforms* u,
+                      const uint16_t* radiance, const float* gbuf_cur,
+                      const void* accum_prev, const float* gbuf_prev,
+                      uint32_t accum_format, void* accum_out);
+int vrt_atrous_pass(vrt_ctx* ctx, const vrt_uniforms* u, const void* src,
+                    uint32_t src_format, const float* gbuf, int32_t step,
+                    uint16_t* dst);
+
+/* ---- stats ------------------------------------------------------------------ */
+int vrt_stats_get(vrt_ctx* ctx, vrt_stats* out);
+int vrt_stats_reset(vrt_ctx* ctx);
+
+/* ---- host helpers (no device work) ------------------------------------------ *
+ * Camera matrices exactly as src/camera.js:47-56 + src/math.js (double math,
+ * rounded to f32): outputs column-major viewProj and invViewProj. */
+void vrt_camera_matrices(const double pos[3], double yaw, double pitch,
+                         double fov_y, double aspect, double near_z,
+                         double far_z, float view_proj[16],
+                         float inv_view_proj[16]);
+/* Fill a vrt_uniforms the way main.js:399-426 does for the base preset. */
+/* Camera: fov 70 deg, near 0.05, far 400 (camera.js:10-12).  prev_view_proj
+ * / prev_cam_pos may be NULL (first frame: current pose, main.js:401). */
+void vrt_uniforms_init(vrt_uniforms* u, const double pos[3], double yaw,
+                       double pitch, const float prev_view_proj[16],
+                       const double prev_cam_pos[3], uint32_t width,
+                       uint32_t height, uint32_t frame, uint32_t bounces,
+                       uint32_t flags, double sun_azimuth,
+                       double sun_elevation, double sun_cos_radius,
+                       float max_history);
+
+/* ---- seeded scene generators (host, no device work) ------------------------ */
+#define VRT_SCENE_RANDOM_BRICKS 1  /* traversal microbench grid */
+#define VRT_SCENE_WORLD         2  /* terrain + Cornell room + boxes + trees + lanterns */
+#define VRT_SCENE_VOXELRT       3  /* restatement of generateScene(N, variant), scene.js:232-409 */
+
+typedef struct vrt_scene_desc {
+  uint32_t kind;               /* VRT_SCENE_* */
+  uint32_t nx, ny, nz;         /* VOXELRT: cubic, nx used */
+  uint64_t seed;
+  float    p_brick, p_voxel;   /* RANDOM_BRICKS: brick / voxel occupancy probability */
+  float    h_min, h_max;       /* WORLD: terrain height range (voxels) */
+  uint32_t room, room_h;       /* WORLD: Cornell room interior extent / height, 0 = none */
+  uint32_t boxes, trees, lanterns;  /* WORLD: object counts */
+  uint32_t variant;            /* VOXELRT: 0 = 'default', 1 = 'lamps' */
+} vrt_scene_desc;
+
+/* Fill vox (nx*ny*nz packed words, x fastest). */
+int vrt_generate_scene(const vrt_scene_desc* desc, uint32_t* vox);
+/* Spawn pose of a generated scene (scene.js:463-472 for VRT_SCENE_VOXELRT;
+ * outside the room's doorway, looking in, for VRT_SCENE_WORLD). */
+int vrt_scene_spawn(const vrt_scene_desc* desc, double pos[3], double* yaw, double* pitch);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VOXELRT_H */

@@ -1,0 +1,87 @@
+// vrt_internal.h -- host-side state of libvoxelrt_hip (not part of the ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/voxelrt.h"
+#include "vrt_trace.h"
+
+namespace vrt {
+
+// Owning device allocation (hipMalloc / hipFree).
+struct DeviceBuffer {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  DeviceBuffer() = default;
+  DeviceBuffer(const DeviceBuffer&) = delete;
+  DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+  DeviceBuffer(DeviceBuffer&& o) noexcept : ptr(o.ptr), bytes(o.bytes) {
+    o.ptr = nullptr;
+    o.bytes = 0;
+  }
+  DeviceBuffer& operator=(DeviceBuffer&& o) noexcept {
+    if (this != &o) {
+      release();
+      ptr = o.ptr;
+      bytes = o.bytes;
+      o.ptr = nullptr;
+      o.bytes = 0;
+    }
+    return *this;
+  }
+  ~DeviceBuffer() { release(); }
+  hipError_t alloc(size_t n);
+  hipError_t ensure(size_t n);  // grow-only
+  void release();
+};
+
+}  // namespace vrt
+
+struct vrt_scene {
+  uint32_t nx = 0, ny = 0, nz = 0;
+  uint64_t nbricks = 0, nocc = 0;
+  uint64_t bytes = 0;
+  uint64_t key = 0;
+  uint64_t lru = 0;
+  vrt::DeviceBuffer d_bricks;  // one word per 8^3 brick: 0 or 1 + pool slot
+  vrt::DeviceBuffer d_pool;    // nocc x 528 words (mask + voxels)
+  vrt::SceneDev dev() const;
+};
+
+// Per-view resident frame targets (main.js:341-349): radiance rgba16f,
+// albedo rgba8unorm, gbuf x2 rgba32f, accum x2 (rgba16f | rgba32f),
+// denoise x2 rgba16f.
+struct ViewState {
+  uint32_t w = 0, h = 0, fmt = 0;
+  uint64_t bytes = 0;
+  uint64_t frames = 0;  // frames rendered: parity = frames & 1 (main.js:695)
+  vrt::DeviceBuffer radiance, albedo;
+  vrt::DeviceBuffer gbuf[2], accum[2], den[2];
+};
+
+struct vrt_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[8] = {};
+  std::string err;
+  uint64_t budget = 0, resident = 0;
+  uint64_t clock = 0;
+  uint64_t frames = 0;
+  double ms_pathtrace = 0, ms_temporal = 0, ms_atrous = 0, ms_frame = 0;
+  int trace_blocks = 0;  // 0 = default grid for k_trace_rays
+  vrt::DeviceBuffer d_counters;  // u64 x4: rays, brick steps, voxel steps, hits
+  vrt::DeviceBuffer scratch_rays, scratch_hits, scratch_hdr;
+  std::vector<vrt_scene*> scenes;
+  std::unordered_map<uint32_t, ViewState> views;
+
+  ~vrt_ctx();
+  int fail(int code, const char* fmt, ...);
+  int reserve(uint64_t bytes);
+  void unreserve(uint64_t bytes);
+  int counters_to_host(uint64_t out[4]);
+  int view_state(uint32_t id, uint32_t w, uint32_t h, uint32_t fmt, ViewState** out);
+};

@@ -1,0 +1,34 @@
+// vrt_kernels.h -- launchers of the gfx950 kernels (vrt_kernels.hip), called
+// by the runtime (vrt_runtime.cpp).  All launches go on the given stream.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/voxelrt.h"
+
+namespace vrt {
+
+using Uniforms = vrt_uniforms;
+struct SceneDev;
+
+hipError_t launch_brick_masks(hipStream_t st, const uint32_t* vox, int nx, int ny, int bgx,
+                              int bgy, int bz0, int nbz, uint32_t* masks, uint32_t* flags);
+hipError_t launch_brick_words(hipStream_t st, const uint32_t* flags, const uint32_t* slots,
+                              uint64_t nb, uint32_t* bricks);
+hipError_t launch_pack_pool(hipStream_t st, const uint32_t* vox, int nx, int ny, int bgx, int bgy,
+                            int bz0, int nbz, const uint32_t* masks, const uint32_t* bricks,
+                            uint32_t* pool);
+hipError_t launch_trace_rays(hipStream_t st, const SceneDev& s, const float* rays, uint64_t n,
+                             float tmax, void* hits, unsigned long long* counters, int blocks);
+hipError_t launch_pathtrace(hipStream_t st, const SceneDev& s, const Uniforms& u, float fclamp,
+                            uint32_t spp, void* radiance, void* albedo, void* gbuf,
+                            unsigned long long* counters);
+hipError_t launch_temporal(hipStream_t st, const Uniforms& u, const void* radiance,
+                           const void* gcur, const void* aprev, const void* gprev, void* aout,
+                           int fmt);
+hipError_t launch_atrous(hipStream_t st, const Uniforms& u, const void* src, int src_fmt,
+                         const void* gbuf, int step, void* dst);
+hipError_t launch_hdr(hipStream_t st, uint32_t npx, const void* illum, int fmt, const void* albedo,
+                      float* hdr);
+
+}  // namespace vrt

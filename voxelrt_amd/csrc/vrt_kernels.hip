@@ -1,0 +1,622 @@
+// vrt_kernels.hip -- gfx950 kernels of libvoxelrt_hip.
+//
+//   k_brick_masks   scene upload: 512-bit masks + occupancy per 8^3 brick
+//                   (scene.js:411-433), one 64-lane wave per brick, one ballot
+//                   per 8x8 layer
+//   k_brick_words   occupied-brick slots -> brick words (0 / 1+slot)
+//   k_pack_pool     copy each occupied brick's mask + voxels into its slot
+//   k_trace_rays    voxel.wgsl:73-153 on a ray list (traversal microbench)
+//   k_pathtrace     pathtrace.wgsl:248-330, base branch
+//   k_temporal      temporal.wgsl:17-77
+//   k_atrous        atrous.wgsl:26-83 (LDS tile with a 2*step halo)
+//   k_hdr           main.js:628-659 HDR capture
+// No MFMA anywhere: nothing here is a dense contraction.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "vrt_device.h"
+#include "vrt_kernels.h"
+#include "vrt_trace.h"
+
+namespace vrt {
+
+// ---------------------------------------------------------------------------
+// counters: wave-reduce, one 64-bit atomic per wave
+// ---------------------------------------------------------------------------
+VRT_D uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+VRT_D void count4(unsigned long long* c, uint32_t a, uint32_t b, uint32_t d, uint32_t e) {
+  a = wave_sum(a);
+  b = wave_sum(b);
+  d = wave_sum(d);
+  e = wave_sum(e);
+  if ((threadIdx.x & 63u) == 0u && c != nullptr) {
+    atomicAdd(c + 0, (unsigned long long)a);
+    atomicAdd(c + 1, (unsigned long long)b);
+    atomicAdd(c + 2, (unsigned long long)d);
+    atomicAdd(c + 3, (unsigned long long)e);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// scene build
+// ---------------------------------------------------------------------------
+// One wave per brick of a slab of whole brick layers (bz in [bz0, bz0+nbz)).
+// vox points at voxel (0, 0, bz0*8).  masks/flags are indexed by the global
+// brick index.
+__global__ __launch_bounds__(256) void k_brick_masks(const uint32_t* __restrict__ vox, int nx,
+                                                     int ny, int bgx, int bgy, int bz0, int nbz,
+                                                     uint32_t* __restrict__ masks,
+                                                     uint32_t* __restrict__ flags) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint64_t wave = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const uint64_t nslab = (uint64_t)bgx * bgy * nbz;
+  if (wave >= nslab) return;
+  const int bx = (int)(wave % bgx);
+  const int by = (int)((wave / bgx) % bgy);
+  const int bzl = (int)(wave / ((uint64_t)bgx * bgy));
+  const int lx = (int)(lane & 7u), ly = (int)(lane >> 3);
+  const uint64_t gbi = (uint64_t)bx + (uint64_t)by * bgx + (uint64_t)(bz0 + bzl) * bgx * bgy;
+  uint32_t any = 0;
+  uint32_t mylo = 0, myhi = 0;
+#pragma unroll
+  for (int lz = 0; lz < 8; ++lz) {
+    const uint64_t vi = (uint64_t)(bx * 8 + lx) + (uint64_t)(by * 8 + ly) * nx +
+                        (uint64_t)(bzl * 8 + lz) * (uint64_t)nx * ny;
+    const uint64_t b = __ballot(vox[vi] != 0u);
+    // words 2*lz (bits 0..31 of the layer) and 2*lz+1 (bits 32..63)
+    if (lane == (uint32_t)(2 * lz)) mylo = (uint32_t)b;
+    if (lane == (uint32_t)(2 * lz + 1)) myhi = (uint32_t)(b >> 32);
+    any |= (b != 0ull) ? 1u : 0u;
+  }
+  if (lane < 16u) masks[gbi * 16 + lane] = (lane & 1u) ? myhi : mylo;
+  if (lane == 0u) flags[gbi] = any;
+}
+
+// bricks[i] = flags[i] ? 1 + slot[i] : 0, slot = exclusive prefix of flags
+__global__ __launch_bounds__(256) void k_brick_words(const uint32_t* __restrict__ flags,
+                                                     const uint32_t* __restrict__ slots,
+                                                     uint64_t nb, uint32_t* __restrict__ bricks) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb) return;
+  bricks[i] = flags[i] ? slots[i] + 1u : 0u;
+}
+
+// One wave per brick of the slab; occupied bricks copy mask + voxels.
+__global__ __launch_bounds__(256) void k_pack_pool(const uint32_t* __restrict__ vox, int nx, int ny,
+                                                   int bgx, int bgy, int bz0, int nbz,
+                                                   const uint32_t* __restrict__ masks,
+                                                   const uint32_t* __restrict__ bricks,
+                                                   uint32_t* __restrict__ pool) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint64_t wave = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const uint64_t nslab = (uint64_t)bgx * bgy * nbz;
+  if (wave >= nslab) return;
+  const int bx = (int)(wave % bgx);
+  const int by = (int)((wave / bgx) % bgy);
+  const int bzl = (int)(wave / ((uint64_t)bgx * bgy));
+  const uint64_t gbi = (uint64_t)bx + (uint64_t)by * bgx + (uint64_t)(bz0 + bzl) * bgx * bgy;
+  const uint32_t w = bricks[gbi];
+  if (w == 0u) return;
+  uint32_t* dst = pool + (uint64_t)(w - 1u) * POOL_WORDS;
+  if (lane < 16u) dst[lane] = masks[gbi * 16 + lane];
+  const int lx = (int)(lane & 7u), ly = (int)(lane >> 3);
+#pragma unroll
+  for (int lz = 0; lz < 8; ++lz) {
+    const uint64_t vi = (uint64_t)(bx * 8 + lx) + (uint64_t)(by * 8 + ly) * nx +
+                        (uint64_t)(bzl * 8 + lz) * (uint64_t)nx * ny;
+    dst[16 + lz * 64 + (int)lane] = vox[vi];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// traversal microbench: one ray per lane, grid-stride
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_trace_rays(SceneDev s, const float* __restrict__ rays,
+                                                    uint64_t n, float tmax,
+                                                    uint4* __restrict__ hits,
+                                                    unsigned long long* __restrict__ counters) {
+  uint32_t nb = 0, nv = 0, nh = 0, nr = 0;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  // every lane of a wave runs the same number of outer iterations so the
+  // counter reduction below sees all 64 lanes
+  const uint64_t base0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (uint64_t i = base0; i < n; i += stride) {
+    const float* r = rays + i * 6;
+    TraceResult h;
+    trace(s, mk3(r[0], r[1], r[2]), mk3(r[3], r[4], r[5]), tmax, h);
+    uint4 rec;
+    rec.x = __builtin_bit_cast(uint32_t, h.t);
+    rec.y = h.voxel;
+    rec.z = (h.bsteps & 0xffffu) | (h.vsteps << 16);
+    rec.w = h.face;
+    hits[i] = rec;
+    nb += h.bsteps;
+    nv += h.vsteps;
+    nh += h.t >= 0.0f ? 1u : 0u;
+    nr += 1u;
+  }
+  count4(counters, nr, nb, nv, nh);
+}
+
+// ---------------------------------------------------------------------------
+// camera / sky / sampling, common.wgsl:90-139
+// ---------------------------------------------------------------------------
+VRT_D float4 mat_vec4(const float* m, float x, float y, float z, float w) {
+  float4 o;
+  o.x = ((m[0] * x + m[4] * y) + m[8] * z) + m[12] * w;
+  o.y = ((m[1] * x + m[5] * y) + m[9] * z) + m[13] * w;
+  o.z = ((m[2] * x + m[6] * y) + m[10] * z) + m[14] * w;
+  o.w = ((m[3] * x + m[7] * y) + m[11] * z) + m[15] * w;
+  return o;
+}
+
+VRT_D f3 camera_ray(const Uniforms& u, float uvx, float uvy) {
+  const float ndcx = uvx * 2.0f - 1.0f;
+  const float ndcy = 1.0f - uvy * 2.0f;
+  const float4 far = mat_vec4(u.inv_view_proj, ndcx, ndcy, 1.0f, 1.0f);
+  const f3 p = mk3(far.x / far.w, far.y / far.w, far.z / far.w);
+  return normalize(p - mk3(u.cam_pos[0], u.cam_pos[1], u.cam_pos[2]));
+}
+
+VRT_D f3 sky_color(const Uniforms& u, f3 rd, bool disc) {
+  const float up = vclamp(rd.y, -1.0f, 1.0f);
+  const float tt = pow_r(vmax(up, 0.0f), 0.55f);
+  f3 sky = mix(mk3(0.55f, 0.68f, 0.85f), mk3(0.15f, 0.32f, 0.62f), tt);
+  if (up < 0.0f) sky = mix(sky, mk3(0.28f, 0.26f, 0.24f), vmin(1.0f, -up * 4.0f));
+  const f3 sd = mk3(u.sun_dir[0], u.sun_dir[1], u.sun_dir[2]);
+  const float sunAmt = vmax(dot(rd, sd), 0.0f);
+  sky = sky + (mk3(1.0f, 0.75f, 0.5f) * pow8(sunAmt)) * 0.2f;
+  if (disc && sunAmt > u.sun_dir[3]) {
+    const float solidAngle = TWO_PI_F * (1.0f - u.sun_dir[3]);
+    sky = sky + mk3(u.sun_radiance[0], u.sun_radiance[1], u.sun_radiance[2]) / vmax(solidAngle, 1e-4f);
+  }
+  return sky * u.sun_radiance[3];
+}
+
+VRT_D f3 sample_sun_dir(const Uniforms& u, uint32_t& rng) {
+  const float r0 = rand01(rng);
+  const float r1 = rand01(rng);
+  const float cosT = mix(1.0f, u.sun_dir[3], r0);
+  const float sinT = vsqrt(vmax(0.0f, 1.0f - cosT * cosT));
+  const float phi = TWO_PI_F * r1;
+  const f3 w = mk3(u.sun_dir[0], u.sun_dir[1], u.sun_dir[2]);
+  const f3 a = vabs(w.x) > 0.9f ? mk3(0.0f, 1.0f, 0.0f) : mk3(1.0f, 0.0f, 0.0f);
+  const f3 tang = normalize(cross(a, w));
+  const f3 bitan = cross(w, tang);
+  float s, c;
+  sincos_r(phi, &s, &c);
+  return normalize((tang * (c * sinT) + bitan * (s * sinT)) + w * cosT);
+}
+
+VRT_D f3 cosine_hemisphere(f3 n, uint32_t& rng) {
+  const float r0 = rand01(rng);
+  const float r1 = rand01(rng);
+  const float phi = TWO_PI_F * r0;
+  const float sr = vsqrt(r1);
+  const f3 a = vabs(n.x) > 0.9f ? mk3(0.0f, 1.0f, 0.0f) : mk3(1.0f, 0.0f, 0.0f);
+  const f3 tang = normalize(cross(a, n));
+  const f3 bitan = cross(n, tang);
+  float s, c;
+  sincos_r(phi, &s, &c);
+  return normalize((tang * (c * sr) + bitan * (s * sr)) + n * vsqrt(vmax(0.0f, 1.0f - r1)));
+}
+
+// ---------------------------------------------------------------------------
+// path tracer, one pixel per lane
+// ---------------------------------------------------------------------------
+struct PixelOut {
+  f3 radiance;
+  f3 albedo_sqrt;
+  float4 gbuf;
+};
+
+struct RayCount {
+  uint32_t rays, bsteps, vsteps, hits;
+  VRT_D void add(const TraceResult& h) {
+    rays += 1u;
+    bsteps += h.bsteps;
+    vsteps += h.vsteps;
+    hits += h.t >= 0.0f ? 1u : 0u;
+  }
+};
+
+VRT_D PixelOut shade_pixel(const SceneDev& s, const Uniforms& u, float fclamp, uint32_t px,
+                           uint32_t py, uint32_t frame, uint32_t w, uint32_t h, RayCount& rc) {
+  PixelOut res;
+  uint32_t rng = init_rng(px, py, frame);
+  const float jx = rand01(rng) - 0.5f;
+  const float jy = rand01(rng) - 0.5f;
+  const float uvx = (((float)px + 0.5f) + jx) / (float)w;
+  const float uvy = (((float)py + 0.5f) + jy) / (float)h;
+  const f3 ro = mk3(u.cam_pos[0], u.cam_pos[1], u.cam_pos[2]);
+  f3 rd = camera_ray(u, uvx, uvy);
+
+  TraceResult h0;
+  trace(s, ro, rd, 1e4f, h0);
+  rc.add(h0);
+  if (h0.t < 0.0f) {
+    res.radiance = sky_color(u, rd, true);
+    res.albedo_sqrt = mk3(1.0f, 1.0f, 1.0f);
+    res.gbuf = make_float4(-rd.x, -rd.y, -rd.z, -1.0f);
+    return res;
+  }
+  const Material mat1 = unpack_material(h0.mat);
+  const f3 x1 = ro + rd * h0.t;
+  const f3 n1 = h0.n;
+  res.albedo_sqrt = mk3(vsqrt(mat1.albedo.x), vsqrt(mat1.albedo.y), vsqrt(mat1.albedo.z));
+  res.gbuf = make_float4(n1.x, n1.y, n1.z, h0.t);
+
+  f3 radiance = mk3(0.0f, 0.0f, 0.0f);
+  f3 tp = mk3(1.0f, 1.0f, 1.0f);
+  f3 pos = x1, n = n1;
+  Material mat = mat1;
+  const uint32_t nb = u.params0[1];
+  const f3 sunRad = mk3(u.sun_radiance[0], u.sun_radiance[1], u.sun_radiance[2]);
+  for (uint32_t bounce = 0; bounce <= nb; ++bounce) {
+    if (mat.emissive > 0.0f) radiance = radiance + (tp * mat.albedo) * (mat.emissive * EMISSIVE_SCALE);
+    const f3 surf = pos + n * 1e-3f;
+    const f3 l = sample_sun_dir(u, rng);
+    const float ndl = dot(n, l);
+    if (ndl > 0.0f && l.y > 0.0f) {
+      TraceResult hs;
+      trace(s, surf, l, 1e4f, hs);
+      rc.add(hs);
+      if (!(hs.t >= 0.0f)) radiance = radiance + (((tp * mat.albedo) * INV_PI_F) * ndl) * sunRad;
+    }
+    if (bounce == nb) break;
+    rd = cosine_hemisphere(n, rng);
+    tp = tp * mat.albedo;
+    if (bounce >= 2u) {
+      const float p = vclamp(vmax(tp.x, vmax(tp.y, tp.z)), 0.05f, 0.95f);
+      if (rand01(rng) > p) break;
+      tp = tp / p;
+    }
+    TraceResult hb;
+    trace(s, surf, rd, 1e4f, hb);
+    rc.add(hb);
+    if (hb.t < 0.0f) {
+      radiance = radiance + tp * sky_color(u, rd, false);
+      break;
+    }
+    pos = surf + rd * hb.t;
+    n = hb.n;
+    mat = unpack_material(hb.mat);
+  }
+  if (!(fclamp <= 0.0f))
+    radiance = mk3(vmin(radiance.x, fclamp), vmin(radiance.y, fclamp), vmin(radiance.z, fclamp));
+  const f3 den = mk3(vmax(mat1.albedo.x, 1e-3f), vmax(mat1.albedo.y, 1e-3f), vmax(mat1.albedo.z, 1e-3f));
+  res.radiance = radiance / den;
+  return res;
+}
+
+// Pixel p of a launch covering TILE x TILE tiles, row-major tiles: keeps the
+// 64 lanes of a wave on a compact 16x4 patch (coherent primary rays).
+VRT_D bool tile_pixel(uint32_t w, uint32_t h, uint32_t& px, uint32_t& py) {
+  const uint32_t tiles_x = (w + 15u) >> 4;
+  const uint32_t tile = blockIdx.x;
+  px = (tile % tiles_x) * 16u + (threadIdx.x & 15u);
+  py = (tile / tiles_x) * 16u + (threadIdx.x >> 4);
+  return px < w && py < h;
+}
+
+__global__ __launch_bounds__(256) void k_pathtrace(SceneDev s, Uniforms u, float fclamp,
+                                                   uint32_t spp, ushort4* __restrict__ radiance,
+                                                   uchar4* __restrict__ albedo,
+                                                   float4* __restrict__ gbuf,
+                                                   unsigned long long* __restrict__ counters) {
+  const uint32_t w = (uint32_t)u.params1[0], h = (uint32_t)u.params1[1];
+  uint32_t px, py;
+  const bool inb = tile_pixel(w, h, px, py);
+  RayCount rc{0u, 0u, 0u, 0u};
+  if (inb) {
+    const uint64_t i = (uint64_t)py * w + px;
+    f3 acc = mk3(0.0f, 0.0f, 0.0f);
+    PixelOut first;
+    for (uint32_t smp = 0; smp < spp; ++smp) {
+      PixelOut r = shade_pixel(s, u, fclamp, px, py, u.params0[0] * spp + smp, w, h, rc);
+      if (smp == 0) first = r;
+      acc = acc + r.radiance;
+    }
+    const f3 o = acc / (float)spp;
+    radiance[i] = make_ushort4(f32_to_f16(o.x), f32_to_f16(o.y), f32_to_f16(o.z), f32_to_f16(1.0f));
+    albedo[i] = make_uchar4((uint8_t)f32_to_unorm8(first.albedo_sqrt.x),
+                            (uint8_t)f32_to_unorm8(first.albedo_sqrt.y),
+                            (uint8_t)f32_to_unorm8(first.albedo_sqrt.z), 255);
+    gbuf[i] = first.gbuf;
+  }
+  count4(counters, rc.rays, rc.bsteps, rc.vsteps, rc.hits);
+}
+
+// ---------------------------------------------------------------------------
+// temporal.wgsl:17-77
+// ---------------------------------------------------------------------------
+template <int FMT>
+VRT_D float4 load_accum(const void* p, uint64_t i) {
+  if (FMT == 1) return ((const float4*)p)[i];
+  const ushort4 v = ((const ushort4*)p)[i];
+  return make_float4(f16_to_f32(v.x), f16_to_f32(v.y), f16_to_f32(v.z), f16_to_f32(v.w));
+}
+template <int FMT>
+VRT_D void store_accum(void* p, uint64_t i, float4 v) {
+  if (FMT == 1) {
+    ((float4*)p)[i] = v;
+  } else {
+    ((ushort4*)p)[i] = make_ushort4(f32_to_f16(v.x), f32_to_f16(v.y), f32_to_f16(v.z), f32_to_f16(v.w));
+  }
+}
+
+template <int FMT>
+__global__ __launch_bounds__(256) void k_temporal(Uniforms u, const ushort4* __restrict__ radiance,
+                                                  const float4* __restrict__ gcur,
+                                                  const void* __restrict__ aprev,
+                                                  const float4* __restrict__ gprev,
+                                                  void* __restrict__ aout) {
+  const uint32_t w = (uint32_t)u.params1[0], h = (uint32_t)u.params1[1];
+  uint32_t px, py;
+  if (!tile_pixel(w, h, px, py)) return;
+  const uint64_t i = (uint64_t)py * w + px;
+  const ushort4 rv = radiance[i];
+  const f3 cur = mk3(f16_to_f32(rv.x), f16_to_f32(rv.y), f16_to_f32(rv.z));
+  const float4 g = gcur[i];
+  const bool temporalOn = (u.params0[2] & 1u) != 0u;
+  if (g.w < 0.0f || !temporalOn) {
+    store_accum<FMT>(aout, i, make_float4(cur.x, cur.y, cur.z, 1.0f));
+    return;
+  }
+  const float uvx = ((float)px + 0.5f) / (float)w;
+  const float uvy = ((float)py + 0.5f) / (float)h;
+  const f3 worldPos = mk3(u.cam_pos[0], u.cam_pos[1], u.cam_pos[2]) + camera_ray(u, uvx, uvy) * g.w;
+  const float4 prevClip = mat_vec4(u.prev_view_proj, worldPos.x, worldPos.y, worldPos.z, 1.0f);
+  f3 color = cur;
+  float history = 1.0f;
+  if (prevClip.w > 0.0f) {
+    const float ndcx = prevClip.x / prevClip.w;
+    const float ndcy = prevClip.y / prevClip.w;
+    const float pux = ndcx * 0.5f + 0.5f;
+    const float puy = (-ndcy) * 0.5f + 0.5f;
+    if (pux > 0.0f && puy > 0.0f && pux < 1.0f && puy < 1.0f) {
+      int ppx = (int)(pux * (float)w);
+      int ppy = (int)(puy * (float)h);
+      ppx = ppx > (int)w - 1 ? (int)w - 1 : ppx;
+      ppy = ppy > (int)h - 1 ? (int)h - 1 : ppy;
+      const float4 pg = gprev[(uint64_t)ppy * w + ppx];
+      const f3 dwp = worldPos - mk3(u.prev_cam_pos[0], u.prev_cam_pos[1], u.prev_cam_pos[2]);
+      const float expectedT = vsqrt(dot(dwp, dwp));
+      const bool depthOk = pg.w > 0.0f && vabs(pg.w - expectedT) < 0.08f * expectedT + 0.02f;
+      const bool normalOk = dot(mk3(pg.x, pg.y, pg.z), mk3(g.x, g.y, g.z)) > 0.85f;
+      if (depthOk && normalOk) {
+        // textureSampleLevel(linear, clamp-to-edge) restated in FP32
+        const float tcx = pux * (float)w - 0.5f;
+        const float tcy = puy * (float)h - 0.5f;
+        const float flx = vfloor(tcx), fly = vfloor(tcy);
+        const float fx = tcx - flx, fy = tcy - fly;
+        int x0 = (int)flx, y0 = (int)fly;
+        int x1 = x0 + 1, y1 = y0 + 1;
+        x0 = x0 < 0 ? 0 : (x0 > (int)w - 1 ? (int)w - 1 : x0);
+        x1 = x1 < 0 ? 0 : (x1 > (int)w - 1 ? (int)w - 1 : x1);
+        y0 = y0 < 0 ? 0 : (y0 > (int)h - 1 ? (int)h - 1 : y0);
+        y1 = y1 < 0 ? 0 : (y1 > (int)h - 1 ? (int)h - 1 : y1);
+        const float4 t00 = load_accum<FMT>(aprev, (uint64_t)y0 * w + x0);
+        const float4 t10 = load_accum<FMT>(aprev, (uint64_t)y0 * w + x1);
+        const float4 t01 = load_accum<FMT>(aprev, (uint64_t)y1 * w + x0);
+        const float4 t11 = load_accum<FMT>(aprev, (uint64_t)y1 * w + x1);
+        float4 prev;
+        prev.x = mix(mix(t00.x, t10.x, fx), mix(t01.x, t11.x, fx), fy);
+        prev.y = mix(mix(t00.y, t10.y, fx), mix(t01.y, t11.y, fx), fy);
+        prev.z = mix(mix(t00.z, t10.z, fx), mix(t01.z, t11.z, fx), fy);
+        prev.w = mix(mix(t00.w, t10.w, fx), mix(t01.w, t11.w, fx), fy);
+        history = vmin(prev.w, u.params4[1]) + 1.0f;
+        const float alpha = 1.0f / history;
+        color = mix(mk3(prev.x, prev.y, prev.z), cur, alpha);
+      }
+    }
+  }
+  store_accum<FMT>(aout, i, make_float4(color.x, color.y, color.z, history));
+}
+
+// ---------------------------------------------------------------------------
+// atrous.wgsl:26-83.  A 16x16 tile stages its (16+4*step)^2 neighbourhood of
+// colour (as f32x4) and G-buffer in LDS; every tap then reads LDS.
+// ---------------------------------------------------------------------------
+template <int FMT, int STEP>
+__global__ __launch_bounds__(256) void k_atrous(Uniforms u, const void* __restrict__ src,
+                                                const float4* __restrict__ gbuf,
+                                                ushort4* __restrict__ dst) {
+  constexpr int T = 16;
+  constexpr int HALO = 2 * STEP;
+  constexpr int S = T + 2 * HALO;
+  __shared__ float4 sc[S * S];
+  __shared__ float4 sg[S * S];
+  const int w = (int)u.params1[0], h = (int)u.params1[1];
+  const int tiles_x = (w + T - 1) / T;
+  const int ox = (int)(blockIdx.x % (uint32_t)tiles_x) * T - HALO;
+  const int oy = (int)(blockIdx.x / (uint32_t)tiles_x) * T - HALO;
+  for (int k = (int)threadIdx.x; k < S * S; k += 256) {
+    const int lx = k % S, ly = k / S;
+    const int gx = ox + lx, gy = oy + ly;
+    float4 c = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float4 g = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
+    if (gx >= 0 && gy >= 0 && gx < w && gy < h) {
+      const uint64_t j = (uint64_t)gy * w + gx;
+      c = load_accum<FMT>(src, j);
+      g = gbuf[j];
+    }
+    sc[k] = c;
+    sg[k] = g;
+  }
+  __syncthreads();
+  const int tx = (int)(threadIdx.x & 15u), ty = (int)(threadIdx.x >> 4);
+  const int x = ox + HALO + tx, y = oy + HALO + ty;
+  if (x >= w || y >= h) return;
+  const int cidx = (ty + HALO) * S + (tx + HALO);
+  const float4 center = sc[cidx];
+  const float4 g = sg[cidx];
+  const uint64_t i = (uint64_t)y * w + x;
+  if (g.w < 0.0f) {
+    dst[i] = make_ushort4(f32_to_f16(center.x), f32_to_f16(center.y), f32_to_f16(center.z),
+                          f32_to_f16(center.w));
+    return;
+  }
+  const float K[3] = {0.375f, 0.25f, 0.0625f};
+  const f3 lw = mk3(0.2126f, 0.7152f, 0.0722f);
+  const float history = vmax(center.w, 1.0f);
+  const float lumC = dot(mk3(center.x, center.y, center.z), lw);
+  const float relax = 2.0f / vsqrt(history);
+  const float sigmaL = vmax(0.08f, relax);
+  f3 sum = mk3(center.x, center.y, center.z) * (K[0] * K[0]);
+  float wSum = K[0] * K[0];
+  const f3 gn = mk3(g.x, g.y, g.z);
+  const float dscale = 0.05f * g.w + 0.01f;
+#pragma unroll
+  for (int dy = -2; dy <= 2; ++dy) {
+#pragma unroll
+    for (int dx = -2; dx <= 2; ++dx) {
+      if (dx == 0 && dy == 0) continue;
+      const int qx = x + dx * STEP, qy = y + dy * STEP;
+      if (qx < 0 || qy < 0 || qx >= w || qy >= h) continue;
+      const int l = (ty + HALO + dy * STEP) * S + (tx + HALO + dx * STEP);
+      const float4 gq = sg[l];
+      if (gq.w < 0.0f) continue;
+      const float4 cq = sc[l];
+      const float wKernel = K[dx < 0 ? -dx : dx] * K[dy < 0 ? -dy : dy];
+      const float wNormal = pow32(vmax(dot(gn, mk3(gq.x, gq.y, gq.z)), 0.0f));
+      const float wDepth = exp_r(-vabs(g.w - gq.w) / dscale);
+      const float wLum = exp_r(-vabs(lumC - dot(mk3(cq.x, cq.y, cq.z), lw)) / sigmaL);
+      const float wt = ((wKernel * wNormal) * wDepth) * wLum;
+      sum = sum + mk3(cq.x, cq.y, cq.z) * wt;
+      wSum += wt;
+    }
+  }
+  const f3 o = sum / vmax(wSum, 1e-6f);
+  dst[i] = make_ushort4(f32_to_f16(o.x), f32_to_f16(o.y), f32_to_f16(o.z), f32_to_f16(center.w));
+}
+
+// ---------------------------------------------------------------------------
+// main.js:628-659: hdr = illum * (alb8/255)^2, computed in double like JS
+// ---------------------------------------------------------------------------
+template <int FMT>
+__global__ __launch_bounds__(256) void k_hdr(uint32_t npx, const void* __restrict__ illum,
+                                             const uchar4* __restrict__ albedo,
+                                             float* __restrict__ hdr) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npx) return;
+  const float4 il = load_accum<FMT>(illum, i);
+  const uchar4 a = albedo[i];
+  const double ar = (double)a.x / 255.0, ag = (double)a.y / 255.0, ab = (double)a.z / 255.0;
+  hdr[i * 3 + 0] = (float)((double)il.x * ar * ar);
+  hdr[i * 3 + 1] = (float)((double)il.y * ag * ag);
+  hdr[i * 3 + 2] = (float)((double)il.z * ab * ab);
+}
+
+}  // namespace vrt
+
+// ---------------------------------------------------------------------------
+// host-side launchers (declared in vrt_kernels.h)
+// ---------------------------------------------------------------------------
+namespace vrt {
+
+static inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+hipError_t launch_brick_masks(hipStream_t st, const uint32_t* vox, int nx, int ny, int bgx,
+                              int bgy, int bz0, int nbz, uint32_t* masks, uint32_t* flags) {
+  const uint64_t waves = (uint64_t)bgx * bgy * nbz;
+  hipLaunchKernelGGL(k_brick_masks, dim3(div_up(waves, 4)), dim3(256), 0, st, vox, nx, ny, bgx,
+                     bgy, bz0, nbz, masks, flags);
+  return hipGetLastError();
+}
+
+hipError_t launch_brick_words(hipStream_t st, const uint32_t* flags, const uint32_t* slots,
+                              uint64_t nb, uint32_t* bricks) {
+  hipLaunchKernelGGL(k_brick_words, dim3(div_up(nb, 256)), dim3(256), 0, st, flags, slots, nb,
+                     bricks);
+  return hipGetLastError();
+}
+
+hipError_t launch_pack_pool(hipStream_t st, const uint32_t* vox, int nx, int ny, int bgx, int bgy,
+                            int bz0, int nbz, const uint32_t* masks, const uint32_t* bricks,
+                            uint32_t* pool) {
+  const uint64_t waves = (uint64_t)bgx * bgy * nbz;
+  hipLaunchKernelGGL(k_pack_pool, dim3(div_up(waves, 4)), dim3(256), 0, st, vox, nx, ny, bgx, bgy,
+                     bz0, nbz, masks, bricks, pool);
+  return hipGetLastError();
+}
+
+hipError_t launch_trace_rays(hipStream_t st, const SceneDev& s, const float* rays, uint64_t n,
+                             float tmax, void* hits, unsigned long long* counters, int blocks) {
+  if (blocks <= 0) blocks = 256 * 16;
+  const uint32_t need = div_up(n, 256);
+  const uint32_t nb = need < (uint32_t)blocks ? need : (uint32_t)blocks;
+  hipLaunchKernelGGL(k_trace_rays, dim3(nb > 0 ? nb : 1), dim3(256), 0, st, s, rays, n, tmax,
+                     (uint4*)hits, counters);
+  return hipGetLastError();
+}
+
+hipError_t launch_pathtrace(hipStream_t st, const SceneDev& s, const Uniforms& u, float fclamp,
+                            uint32_t spp, void* radiance, void* albedo, void* gbuf,
+                            unsigned long long* counters) {
+  const uint32_t w = (uint32_t)u.params1[0], h = (uint32_t)u.params1[1];
+  const uint32_t tiles = div_up(w, 16) * div_up(h, 16);
+  hipLaunchKernelGGL(k_pathtrace, dim3(tiles), dim3(256), 0, st, s, u, fclamp, spp,
+                     (ushort4*)radiance, (uchar4*)albedo, (float4*)gbuf, counters);
+  return hipGetLastError();
+}
+
+hipError_t launch_temporal(hipStream_t st, const Uniforms& u, const void* radiance,
+                           const void* gcur, const void* aprev, const void* gprev, void* aout,
+                           int fmt) {
+  const uint32_t w = (uint32_t)u.params1[0], h = (uint32_t)u.params1[1];
+  const uint32_t tiles = div_up(w, 16) * div_up(h, 16);
+  if (fmt == 1)
+    hipLaunchKernelGGL(k_temporal<1>, dim3(tiles), dim3(256), 0, st, u, (const ushort4*)radiance,
+                       (const float4*)gcur, aprev, (const float4*)gprev, aout);
+  else
+    hipLaunchKernelGGL(k_temporal<0>, dim3(tiles), dim3(256), 0, st, u, (const ushort4*)radiance,
+                       (const float4*)gcur, aprev, (const float4*)gprev, aout);
+  return hipGetLastError();
+}
+
+template <int FMT>
+static void atrous_fmt(hipStream_t st, const Uniforms& u, const void* src, const void* gbuf,
+                       int step, void* dst, uint32_t tiles) {
+  switch (step) {
+    case 1:
+      hipLaunchKernelGGL((k_atrous<FMT, 1>), dim3(tiles), dim3(256), 0, st, u, src,
+                         (const float4*)gbuf, (ushort4*)dst);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_atrous<FMT, 2>), dim3(tiles), dim3(256), 0, st, u, src,
+                         (const float4*)gbuf, (ushort4*)dst);
+      break;
+    default:
+      hipLaunchKernelGGL((k_atrous<FMT, 4>), dim3(tiles), dim3(256), 0, st, u, src,
+                         (const float4*)gbuf, (ushort4*)dst);
+      break;
+  }
+}
+
+hipError_t launch_atrous(hipStream_t st, const Uniforms& u, const void* src, int src_fmt,
+                         const void* gbuf, int step, void* dst) {
+  const uint32_t w = (uint32_t)u.params1[0], h = (uint32_t)u.params1[1];
+  const uint32_t tiles = div_up(w, 16) * div_up(h, 16);
+  if (step != 1 && step != 2 && step != 4) return hipErrorInvalidValue;
+  if (src_fmt == 1) atrous_fmt<1>(st, u, src, gbuf, step, dst, tiles);
+  else atrous_fmt<0>(st, u, src, gbuf, step, dst, tiles);
+  return hipGetLastError();
+}
+
+hipError_t launch_hdr(hipStream_t st, uint32_t npx, const void* illum, int fmt, const void* albedo,
+                      float* hdr) {
+  if (fmt == 1)
+    hipLaunchKernelGGL(k_hdr<1>, dim3(div_up(npx, 256)), dim3(256), 0, st, npx, illum,
+                       (const uchar4*)albedo, hdr);
+  else
+    hipLaunchKernelGGL(k_hdr<0>, dim3(div_up(npx, 256)), dim3(256), 0, st, npx, illum,
+                       (const uchar4*)albedo, hdr);
+  return hipGetLastError();
+}
+
+}  // namespace vrt

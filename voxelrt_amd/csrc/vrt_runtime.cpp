@@ -1,0 +1,647 @@
+// vrt_runtime.cpp -- host runtime and C ABI of libvoxelrt_hip.
+//
+// Owns the HIP device state: one stream per context (default priority), the
+// resident scenes (brick words + compacted brick pool), the per-view frame
+// targets (G-buffer / accumulation ping-pong and denoise targets, the
+// reference's textures of main.js:341-349), and the frame orchestration of
+// main.js:680-747 (pathtrace -> temporal -> a-trous x3).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/voxelrt.h"
+#include "vrt_internal.h"
+#include "vrt_kernels.h"
+#include "vrt_trace.h"
+
+namespace vrt {
+hipError_t exclusive_scan_u32(hipStream_t st, void* d_tmp, size_t* tmp_bytes, const uint32_t* in,
+                              uint32_t* out, uint64_t n);
+}
+
+using vrt::DeviceBuffer;
+
+// ---------------------------------------------------------------------------
+// error helpers
+// ---------------------------------------------------------------------------
+int vrt_ctx::fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  err = buf;
+  return code;
+}
+
+#define HIP_TRY(ctx, expr)                                                                     \
+  do {                                                                                         \
+    hipError_t e_ = (expr);                                                                    \
+    if (e_ != hipSuccess)                                                                      \
+      return (ctx)->fail(e_ == hipErrorOutOfMemory ? VRT_ERR_NO_MEMORY : VRT_ERR_DEVICE,       \
+                         "%s failed: %s", #expr, hipGetErrorString(e_));                       \
+  } while (0)
+
+#define GUARD_BEGIN try {
+#define GUARD_END(ctx)                                                                         \
+  }                                                                                            \
+  catch (const std::bad_alloc&) {                                                              \
+    return (ctx) ? (ctx)->fail(VRT_ERR_NO_MEMORY, "host allocation failed") : VRT_ERR_NO_MEMORY; \
+  }                                                                                            \
+  catch (...) {                                                                                \
+    return (ctx) ? (ctx)->fail(VRT_ERR_DEVICE, "unexpected exception") : VRT_ERR_DEVICE;       \
+  }
+
+namespace vrt {
+
+hipError_t DeviceBuffer::alloc(size_t n) {
+  release();
+  if (n == 0) return hipSuccess;
+  hipError_t e = hipMalloc(&ptr, n);
+  if (e != hipSuccess) {
+    ptr = nullptr;
+    return e;
+  }
+  bytes = n;
+  return hipSuccess;
+}
+void DeviceBuffer::release() {
+  if (ptr) (void)hipFree(ptr);
+  ptr = nullptr;
+  bytes = 0;
+}
+hipError_t DeviceBuffer::ensure(size_t n) {
+  if (n <= bytes) return hipSuccess;
+  return alloc(n);
+}
+
+}  // namespace vrt
+
+static uint64_t view_bytes(uint32_t w, uint32_t h, uint32_t fmt) {
+  const uint64_t px = (uint64_t)w * h;
+  return px * (8 + 4 + 16 * 2 + (fmt == VRT_ACCUM_RGBA32F ? 16 : 8) * 2 + 8 * 2);
+}
+
+int vrt_ctx::reserve(uint64_t bytes) {
+  if (resident + bytes > budget)
+    return fail(VRT_ERR_BUDGET, "HBM budget exceeded: %llu resident + %llu requested > %llu",
+                (unsigned long long)resident, (unsigned long long)bytes, (unsigned long long)budget);
+  resident += bytes;
+  return VRT_OK;
+}
+void vrt_ctx::unreserve(uint64_t bytes) { resident = bytes > resident ? 0 : resident - bytes; }
+
+int vrt_ctx::counters_to_host(uint64_t out[4]) {
+  HIP_TRY(this, hipMemcpyAsync(out, d_counters.ptr, 4 * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                               stream));
+  HIP_TRY(this, hipStreamSynchronize(stream));
+  return VRT_OK;
+}
+
+vrt::SceneDev vrt_scene::dev() const {
+  vrt::SceneDev s;
+  s.bricks = (const uint32_t*)d_bricks.ptr;
+  s.pool = (const uint32_t*)d_pool.ptr;
+  s.nx = (int)nx;
+  s.ny = (int)ny;
+  s.nz = (int)nz;
+  s.bgx = (int)(nx / 8);
+  s.bgy = (int)(ny / 8);
+  s.bgz = (int)(nz / 8);
+  s.voxel_size = VRT_VOXEL_SIZE;
+  s.inv_voxel = 1.0f / VRT_VOXEL_SIZE;
+  return s;
+}
+
+// ---------------------------------------------------------------------------
+// view state
+// ---------------------------------------------------------------------------
+int vrt_ctx::view_state(uint32_t id, uint32_t w, uint32_t h, uint32_t fmt, ViewState** out) {
+  auto it = views.find(id);
+  if (it != views.end()) {
+    ViewState& v = it->second;
+    if (v.w == w && v.h == h && v.fmt == fmt) {
+      *out = &v;
+      return VRT_OK;
+    }
+    unreserve(v.bytes);
+    views.erase(it);
+  }
+  const uint64_t bytes = view_bytes(w, h, fmt);
+  int rc = reserve(bytes);
+  if (rc) return rc;
+  ViewState& v = views[id];
+  v.w = w;
+  v.h = h;
+  v.fmt = fmt;
+  v.bytes = bytes;
+  v.frames = 0;
+  const uint64_t px = (uint64_t)w * h;
+  const uint64_t abytes = px * (fmt == VRT_ACCUM_RGBA32F ? 16 : 8);
+  hipError_t e = hipSuccess;
+  if (e == hipSuccess) e = v.radiance.alloc(px * 8);
+  if (e == hipSuccess) e = v.albedo.alloc(px * 4);
+  for (int k = 0; k < 2 && e == hipSuccess; ++k) e = v.gbuf[k].alloc(px * 16);
+  for (int k = 0; k < 2 && e == hipSuccess; ++k) e = v.accum[k].alloc(abytes);
+  for (int k = 0; k < 2 && e == hipSuccess; ++k) e = v.den[k].alloc(px * 8);
+  // WebGPU zero-initialises textures: history reads before the first write
+  // see zeros (and gbuf.w = 0 fails the depth test).
+  for (int k = 0; k < 2 && e == hipSuccess; ++k) e = hipMemsetAsync(v.gbuf[k].ptr, 0, px * 16, stream);
+  for (int k = 0; k < 2 && e == hipSuccess; ++k) e = hipMemsetAsync(v.accum[k].ptr, 0, abytes, stream);
+  if (e != hipSuccess) {
+    unreserve(bytes);
+    views.erase(id);
+    return fail(e == hipErrorOutOfMemory ? VRT_ERR_NO_MEMORY : VRT_ERR_DEVICE,
+                "view state allocation failed: %s", hipGetErrorString(e));
+  }
+  *out = &v;
+  return VRT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int vrt_device_count(int32_t* count) {
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (count) *count = (e == hipSuccess) ? n : 0;
+  return e == hipSuccess ? VRT_OK : VRT_ERR_NO_DEVICE;
+}
+
+int vrt_create(const vrt_create_info* info, vrt_ctx** out) {
+  if (!out) return VRT_ERR_INVALID;
+  *out = nullptr;
+  vrt_ctx* ctx = nullptr;
+  try {
+    ctx = new vrt_ctx();
+  } catch (...) {
+    return VRT_ERR_NO_MEMORY;
+  }
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+    delete ctx;
+    return VRT_ERR_NO_DEVICE;
+  }
+  ctx->device = info ? info->device : 0;
+  if (ctx->device < 0 || ctx->device >= n) {
+    delete ctx;
+    return VRT_ERR_INVALID;
+  }
+  if (hipSetDevice(ctx->device) != hipSuccess ||
+      hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete ctx;
+    return VRT_ERR_DEVICE;
+  }
+  for (auto& e : ctx->ev)
+    if (hipEventCreate(&e) != hipSuccess) {
+      delete ctx;
+      return VRT_ERR_DEVICE;
+    }
+  size_t freeb = 0, total = 0;
+  (void)hipMemGetInfo(&freeb, &total);
+  ctx->budget = (info && info->hbm_budget) ? info->hbm_budget : (uint64_t)(0.9 * (double)freeb);
+  if (ctx->d_counters.alloc(4 * sizeof(uint64_t)) != hipSuccess ||
+      hipMemsetAsync(ctx->d_counters.ptr, 0, 4 * sizeof(uint64_t), ctx->stream) != hipSuccess) {
+    delete ctx;
+    return VRT_ERR_DEVICE;
+  }
+  *out = ctx;
+  return VRT_OK;
+}
+
+void vrt_destroy(vrt_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  delete ctx;
+}
+
+const char* vrt_last_error(const vrt_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+// ---------------------------------------------------------------------------
+// scenes
+// ---------------------------------------------------------------------------
+int vrt_scene_upload(vrt_ctx* ctx, const uint32_t* vox, uint32_t nx, uint32_t ny, uint32_t nz,
+                     uint64_t scene_key, vrt_scene** out) {
+  if (!ctx || !vox || !out) return ctx ? ctx->fail(VRT_ERR_INVALID, "null argument") : VRT_ERR_INVALID;
+  GUARD_BEGIN
+  *out = nullptr;
+  if (nx == 0 || ny == 0 || nz == 0 || nx % 8 || ny % 8 || nz % 8)
+    return ctx->fail(VRT_ERR_INVALID, "grid dims %ux%ux%u must be positive multiples of 8", nx, ny, nz);
+  if ((uint64_t)nx * ny * nz > 0xFFFFFFFEull)
+    return ctx->fail(VRT_ERR_INVALID, "grid has more than 2^32-2 voxels");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  const int bgx = (int)nx / 8, bgy = (int)ny / 8, bgz = (int)nz / 8;
+  const uint64_t nb = (uint64_t)bgx * bgy * bgz;
+  const uint64_t layer_bytes = (uint64_t)nx * ny * 8 * 4;  // one brick layer of voxels
+  const uint64_t stage_cap = 512ull << 20;
+  const int slab = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)bgz, stage_cap / layer_bytes));
+  const bool single = slab >= bgz;
+
+  std::unique_ptr<vrt_scene> sc(new vrt_scene());
+  sc->nx = nx; sc->ny = ny; sc->nz = nz;
+  sc->key = scene_key;
+  sc->nbricks = nb;
+  vrt::DeviceBuffer stage, masks, flags, slots, tmp;
+  HIP_TRY(ctx, stage.alloc(layer_bytes * slab));
+  HIP_TRY(ctx, masks.alloc(nb * 64));
+  HIP_TRY(ctx, flags.alloc(nb * 4));
+  HIP_TRY(ctx, slots.alloc(nb * 4));
+  // pass 1: masks + occupancy flags
+  for (int bz0 = 0; bz0 < bgz; bz0 += slab) {
+    const int nbz = std::min(slab, bgz - bz0);
+    HIP_TRY(ctx, hipMemcpyAsync(stage.ptr, vox + (uint64_t)bz0 * 8 * nx * ny, layer_bytes * nbz,
+                                hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, vrt::launch_brick_masks(ctx->stream, (const uint32_t*)stage.ptr, (int)nx, (int)ny,
+                                         bgx, bgy, bz0, nbz, (uint32_t*)masks.ptr,
+                                         (uint32_t*)flags.ptr));
+  }
+  // slots = exclusive prefix of flags
+  size_t tmp_bytes = 0;
+  HIP_TRY(ctx, vrt::exclusive_scan_u32(ctx->stream, nullptr, &tmp_bytes, (const uint32_t*)flags.ptr,
+                                       (uint32_t*)slots.ptr, nb));
+  HIP_TRY(ctx, tmp.alloc(std::max<size_t>(tmp_bytes, 16)));
+  HIP_TRY(ctx, vrt::exclusive_scan_u32(ctx->stream, tmp.ptr, &tmp_bytes, (const uint32_t*)flags.ptr,
+                                       (uint32_t*)slots.ptr, nb));
+  uint32_t last[2] = {0, 0};
+  HIP_TRY(ctx, hipMemcpyAsync(&last[0], (uint32_t*)slots.ptr + (nb - 1), 4, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipMemcpyAsync(&last[1], (uint32_t*)flags.ptr + (nb - 1), 4, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  const uint64_t nocc = (uint64_t)last[0] + last[1];
+  sc->nocc = nocc;
+  const uint64_t bytes = nb * 4 + std::max<uint64_t>(nocc, 1) * vrt::POOL_WORDS * 4;
+  int rc = ctx->reserve(bytes);
+  if (rc) return rc;
+  sc->bytes = bytes;
+  hipError_t e = sc->d_bricks.alloc(nb * 4);
+  if (e == hipSuccess) e = sc->d_pool.alloc(std::max<uint64_t>(nocc, 1) * vrt::POOL_WORDS * 4);
+  if (e != hipSuccess) {
+    ctx->unreserve(bytes);
+    return ctx->fail(e == hipErrorOutOfMemory ? VRT_ERR_NO_MEMORY : VRT_ERR_DEVICE,
+                     "scene allocation failed: %s", hipGetErrorString(e));
+  }
+  auto undo = [&](int code, const char* what, hipError_t err) {
+    ctx->unreserve(bytes);
+    return ctx->fail(code, "%s: %s", what, hipGetErrorString(err));
+  };
+  e = vrt::launch_brick_words(ctx->stream, (const uint32_t*)flags.ptr, (const uint32_t*)slots.ptr, nb,
+                              (uint32_t*)sc->d_bricks.ptr);
+  if (e != hipSuccess) return undo(VRT_ERR_DEVICE, "brick words", e);
+  // pass 2: pack occupied bricks into the pool
+  for (int bz0 = 0; bz0 < bgz; bz0 += slab) {
+    const int nbz = std::min(slab, bgz - bz0);
+    if (!single) {
+      e = hipMemcpyAsync(stage.ptr, vox + (uint64_t)bz0 * 8 * nx * ny, layer_bytes * nbz,
+                         hipMemcpyHostToDevice, ctx->stream);
+      if (e != hipSuccess) return undo(VRT_ERR_DEVICE, "upload", e);
+    }
+    e = vrt::launch_pack_pool(ctx->stream, (const uint32_t*)stage.ptr, (int)nx, (int)ny, bgx, bgy, bz0,
+                              nbz, (const uint32_t*)masks.ptr, (const uint32_t*)sc->d_bricks.ptr,
+                              (uint32_t*)sc->d_pool.ptr);
+    if (e != hipSuccess) return undo(VRT_ERR_DEVICE, "pack", e);
+  }
+  e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) return undo(VRT_ERR_DEVICE, "scene build", e);
+  sc->lru = ++ctx->clock;
+  vrt_scene* raw = sc.release();
+  ctx->scenes.push_back(raw);
+  *out = raw;
+  return VRT_OK;
+  GUARD_END(ctx)
+}
+
+int vrt_scene_release(vrt_ctx* ctx, vrt_scene* scene) {
+  if (!ctx || !scene) return VRT_ERR_INVALID;
+  GUARD_BEGIN
+  auto it = std::find(ctx->scenes.begin(), ctx->scenes.end(), scene);
+  if (it == ctx->scenes.end()) return ctx->fail(VRT_ERR_INVALID, "unknown scene handle");
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  ctx->unreserve(scene->bytes);
+  ctx->scenes.erase(it);
+  delete scene;
+  return VRT_OK;
+  GUARD_END(ctx)
+}
+
+int vrt_scene_dims(const vrt_scene* scene, uint32_t dims[3], uint64_t* bytes) {
+  if (!scene) return VRT_ERR_INVALID;
+  if (dims) { dims[0] = scene->nx; dims[1] = scene->ny; dims[2] = scene->nz; }
+  if (bytes) *bytes = scene->bytes;
+  return VRT_OK;
+}
+
+int vrt_scene_bricks(vrt_ctx* ctx, const vrt_scene* scene, uint32_t* bricks, uint32_t* masks) {
+  if (!ctx || !scene) return VRT_ERR_INVALID;
+  GUARD_BEGIN
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  const uint64_t nb = scene->nbricks;
+  std::vector<uint32_t> words(nb);
+  HIP_TRY(ctx, hipMemcpyAsync(words.data(), scene->d_bricks.ptr, nb * 4, hipMemcpyDeviceToHost, ctx->stream));
+  std::vector<uint32_t> pool;
+  if (masks) {
+    pool.resize(std::max<uint64_t>(scene->nocc, 1) * vrt::POOL_WORDS);
+    HIP_TRY(ctx, hipMemcpyAsync(pool.data(), scene->d_pool.ptr, pool.size() * 4, hipMemcpyDeviceToHost,
+                                ctx->stream));
+  }
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  for (uint64_t i = 0; i < nb; ++i) {
+    if (bricks) bricks[i] = words[i];
+    if (masks) {
+      if (words[i] == 0) std::memset(masks + i * 16, 0, 64);
+      else std::memcpy(masks + i * 16, pool.data() + (uint64_t)(words[i] - 1) * vrt::POOL_WORDS, 64);
+    }
+  }
+  return VRT_OK;
+  GUARD_END(ctx)
+}
+
+// ---------------------------------------------------------------------------
+// traversal
+// ---------------------------------------------------------------------------
+int vrt_trace_rays_device(vrt_ctx* ctx, const vrt_scene* scene, const float* d_rays, uint64_t n,
+                          float tmax, vrt_hit* d_hits, float* ms) {
+  if (!ctx || !scene || (n && (!d_rays || !d_hits)))
+    return ctx ? ctx->fail(VRT_ERR_INVALID, "null argument") : VRT_ERR_INVALID;
+  GUARD_BEGIN
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  if (n == 0) { if (ms) *ms = 0.0f; return VRT_OK; }
+  const vrt::SceneDev s = scene->dev();
+  HIP_TRY(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+  HIP_TRY(ctx, vrt::launch_trace_rays(ctx->stream, s, d_rays, n, tmax, d_hits,
+                                      (unsigned long long*)ctx->d_counters.ptr, ctx->trace_blocks));
+  HIP_TRY(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
+  if (ms) {
+    HIP_TRY(ctx, hipEventSynchronize(ctx->ev[1]));
+    HIP_TRY(ctx, hipEventElapsedTime(ms, ctx->ev[0], ctx->ev[1]));
+  }
+  return VRT_OK;
+  GUARD_END(ctx)
+}
+
+int vrt_trace_rays(vrt_ctx* ctx, const vrt_scene* scene, const float* rays, uint64_t n, float tmax,
+                   vrt_hit* hits) {
+  if (!ctx || !scene || (n && (!rays || !hits)))
+    return ctx ? ctx->fail(VRT_ERR_INVALID, "null argument") : VRT_ERR_INVALID;
+  GUARD_BEGIN
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  if (n == 0) return VRT_OK;
+  HIP_TRY(ctx, ctx->scratch_rays.ensure(n * 24));
+  HIP_TRY(ctx, ctx->scratch_hits.ensure(n * 16));
+  HIP_TRY(ctx, hipMemcpyAsync(ctx->scratch_rays.ptr, rays, n * 24, hipMemcpyHostToDevice, ctx->stream));
+  const vrt::SceneDev s = scene->dev();
+  HIP_TRY(ctx, vrt::launch_trace_rays(ctx->stream, s, (const float*)ctx->scratch_rays.ptr, n, tmax,
+                                      ctx->scratch_hits.ptr, (unsigned long long*)ctx->d_counters.ptr,
+                                      ctx->trace_blocks));
+  HIP_TRY(ctx, hipMemcpyAsync(hits, ctx->scratch_hits.ptr, n * 16, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return VRT_OK;
+  GUARD_END(ctx)
+}
+
+// ---------------------------------------------------------------------------
+// frames
+// ---------------------------------------------------------------------------
+static int validate_view(vrt_ctx* ctx, const vrt_view& v) {
+  if (!v.scene) return ctx->fail(VRT_ERR_INVALID, "view without scene");
+  if (std::find(ctx->scenes.begin(), ctx->scenes.end(), v.scene) == ctx->scenes.end())
+    return ctx->fail(VRT_ERR_INVALID, "view references an unknown scene");
+  const float w = v.u.params1[0], h = v.u.params1[1];
+  if (!(w >= 1.0f && h >= 1.0f && w <= 16384.0f && h <= 16384.0f) || w != std::floor(w) ||
+      h != std::floor(h))
+    return ctx->fail(VRT_ERR_INVALID, "bad view size %gx%g", (double)w, (double)h);
+  if (v.spp < 1 || v.spp > 4096) return ctx->fail(VRT_ERR_INVALID, "spp must be in [1, 4096]");
+  if (v.accum_format > 1) return ctx->fail(VRT_ERR_INVALID, "bad accum_format");
+  if (v.u.params0[1] > 64) return ctx->fail(VRT_ERR_INVALID, "bounces must be <= 64");
+  return VRT_OK;
+}
+
+static int copy_out(vrt_ctx* ctx, ViewState& vs, const vrt_view& v, vrt_frame* fo, int parity,
+                    bool denoised) {
+  const uint64_t px = (uint64_t)vs.w * vs.h;
+  const uint64_t abytes = px * (vs.fmt == VRT_ACCUM_RGBA32F ? 16 : 8);
+  if (fo->radiance)
+    HIP_TRY(ctx, hipMemcpyAsync(fo->radiance, vs.radiance.ptr, px * 8, hipMemcpyDeviceToHost, ctx->stream));
+  if (fo->albedo)
+    HIP_TRY(ctx, hipMemcpyAsync(fo->albedo, vs.albedo.ptr, px * 4, hipMemcpyDeviceToHost, ctx->stream));
+  if (fo->gbuf)
+    HIP_TRY(ctx, hipMemcpyAsync(fo->gbuf, vs.gbuf[parity].ptr, px * 16, hipMemcpyDeviceToHost, ctx->stream));
+  if (fo->accum)
+    HIP_TRY(ctx, hipMemcpyAsync(fo->accum, vs.accum[parity].ptr, abytes, hipMemcpyDeviceToHost, ctx->stream));
+  if (fo->denoised && denoised)
+    HIP_TRY(ctx, hipMemcpyAsync(fo->denoised, vs.den[0].ptr, px * 8, hipMemcpyDeviceToHost, ctx->stream));
+  if (fo->hdr) {
+    // main.js:629-632: the denoised target when denoise is on, else the
+    // accumulation (rgba32f only when that is its format)
+    const void* illum = denoised ? vs.den[0].ptr : vs.accum[parity].ptr;
+    const int fmt = denoised ? 0 : (int)vs.fmt;
+    HIP_TRY(ctx, ctx->scratch_hdr.ensure(px * 12));
+    HIP_TRY(ctx, vrt::launch_hdr(ctx->stream, (uint32_t)px, illum, fmt, vs.albedo.ptr,
+                                 (float*)ctx->scratch_hdr.ptr));
+    HIP_TRY(ctx, hipMemcpyAsync(fo->hdr, ctx->scratch_hdr.ptr, px * 12, hipMemcpyDeviceToHost, ctx->stream));
+  }
+  (void)v;
+  return VRT_OK;
+}
+
+int vrt_render(vrt_ctx* ctx, const vrt_view* views, uint32_t nviews, vrt_frame* outs) {
+  if (!ctx || (nviews && !views)) return ctx ? ctx->fail(VRT_ERR_INVALID, "null argument") : VRT_ERR_INVALID;
+  GUARD_BEGIN
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  for (uint32_t k = 0; k < nviews; ++k) {
+    int rc = validate_view(ctx, views[k]);
+    if (rc) return rc;
+  }
+  std::vector<ViewState*> states(nviews);
+  for (uint32_t k = 0; k < nviews; ++k) {
+    const vrt_view& v = views[k];
+    int rc = ctx->view_state(v.view_id, (uint32_t)v.u.params1[0], (uint32_t)v.u.params1[1],
+                             v.accum_format, &states[k]);
+    if (rc) return rc;
+  }
+  uint64_t before[4] = {0, 0, 0, 0};
+  const bool want_counts = outs != nullptr;
+  if (want_counts) {
+    int rc = ctx->counters_to_host(before);
+    if (rc) return rc;
+  }
+  HIP_TRY(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+  for (uint32_t k = 0; k < nviews; ++k) {
+    const vrt_view& v = views[k];
+    ViewState& vs = *states[k];
+    const int parity = (int)(vs.frames & 1u);
+    HIP_TRY(ctx, vrt::launch_pathtrace(ctx->stream, v.scene->dev(), v.u, v.firefly_clamp, v.spp,
+                                       vs.radiance.ptr, vs.albedo.ptr, vs.gbuf[parity].ptr,
+                                       (unsigned long long*)ctx->d_counters.ptr));
+  }
+  HIP_TRY(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
+  for (uint32_t k = 0; k < nviews; ++k) {
+    const vrt_view& v = views[k];
+    ViewState& vs = *states[k];
+    const int parity = (int)(vs.frames & 1u);
+    HIP_TRY(ctx, vrt::launch_temporal(ctx->stream, v.u, vs.radiance.ptr, vs.gbuf[parity].ptr,
+                                      vs.accum[1 - parity].ptr, vs.gbuf[1 - parity].ptr,
+                                      vs.accum[parity].ptr, (int)vs.fmt));
+  }
+  HIP_TRY(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
+  for (uint32_t k = 0; k < nviews; ++k) {
+    const vrt_view& v = views[k];
+    ViewState& vs = *states[k];
+    if (!(v.u.params0[2] & VRT_FLAG_DENOISE)) continue;
+    const int parity = (int)(vs.frames & 1u);
+    // main.js:502-507: accum -> dA (1) -> dB (2) -> dA (4)
+    HIP_TRY(ctx, vrt::launch_atrous(ctx->stream, v.u, vs.accum[parity].ptr, (int)vs.fmt, vs.gbuf[parity].ptr, 1,
+                                    vs.den[0].ptr));
+    HIP_TRY(ctx, vrt::launch_atrous(ctx->stream, v.u, vs.den[0].ptr, 0, vs.gbuf[parity].ptr, 2, vs.den[1].ptr));
+    HIP_TRY(ctx, vrt::launch_atrous(ctx->stream, v.u, vs.den[1].ptr, 0, vs.gbuf[parity].ptr, 4, vs.den[0].ptr));
+  }
+  HIP_TRY(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
+  for (uint32_t k = 0; k < nviews && outs; ++k) {
+    ViewState& vs = *states[k];
+    const int parity = (int)(vs.frames & 1u);
+    int rc = copy_out(ctx, vs, views[k], &outs[k], parity, (views[k].u.params0[2] & VRT_FLAG_DENOISE) != 0);
+    if (rc) return rc;
+  }
+  HIP_TRY(ctx, hipEventSynchronize(ctx->ev[3]));
+  float t01 = 0, t12 = 0, t23 = 0, t03 = 0;
+  HIP_TRY(ctx, hipEventElapsedTime(&t01, ctx->ev[0], ctx->ev[1]));
+  HIP_TRY(ctx, hipEventElapsedTime(&t12, ctx->ev[1], ctx->ev[2]));
+  HIP_TRY(ctx, hipEventElapsedTime(&t23, ctx->ev[2], ctx->ev[3]));
+  HIP_TRY(ctx, hipEventElapsedTime(&t03, ctx->ev[0], ctx->ev[3]));
+  ctx->ms_pathtrace = t01;
+  ctx->ms_temporal = t12;
+  ctx->ms_atrous = t23;
+  ctx->ms_frame = t03;
+  for (uint32_t k = 0; k < nviews; ++k) states[k]->frames++;
+  ctx->frames += nviews;
+  if (want_counts) {
+    uint64_t after[4];
+    int rc = ctx->counters_to_host(after);
+    if (rc) return rc;
+    // per-view split is not tracked by the shared counters: the totals of the
+    // call go to the first frame record when several views are rendered
+    for (uint32_t k = 0; k < nviews; ++k) {
+      outs[k].rays = k == 0 ? after[0] - before[0] : 0;
+      outs[k].brick_steps = k == 0 ? after[1] - before[1] : 0;
+      outs[k].voxel_steps = k == 0 ? after[2] - before[2] : 0;
+    }
+  }
+  return VRT_OK;
+  GUARD_END(ctx)
+}
+
+int vrt_view_release(vrt_ctx* ctx, uint32_t view_id) {
+  if (!ctx) return VRT_ERR_INVALID;
+  GUARD_BEGIN
+  auto it = ctx->views.find(view_id);
+  if (it == ctx->views.end()) return ctx->fail(VRT_ERR_INVALID, "unknown view id %u", view_id);
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  ctx->unreserve(it->second.bytes);
+  ctx->views.erase(it);
+  return VRT_OK;
+  GUARD_END(ctx)
+}
+
+int vrt_temporal_pass(vrt_ctx* ctx, const vrt_uniforms* u, const uint16_t* radiance,
+                      const float* gbuf_cur, const void* accum_prev, const float* gbuf_prev,
+                      uint32_t accum_format, void* accum_out) {
+  if (!ctx || !u || !radiance || !gbuf_cur || !accum_prev || !gbuf_prev || !accum_out || accum_format > 1)
+    return ctx ? ctx->fail(VRT_ERR_INVALID, "bad argument") : VRT_ERR_INVALID;
+  GUARD_BEGIN
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  const uint64_t px = (uint64_t)u->params1[0] * (uint64_t)u->params1[1];
+  const uint64_t ab = px * (accum_format ? 16 : 8);
+  vrt::DeviceBuffer r, gc, ap, gp, ao;
+  HIP_TRY(ctx, r.alloc(px * 8));
+  HIP_TRY(ctx, gc.alloc(px * 16));
+  HIP_TRY(ctx, ap.alloc(ab));
+  HIP_TRY(ctx, gp.alloc(px * 16));
+  HIP_TRY(ctx, ao.alloc(ab));
+  HIP_TRY(ctx, hipMemcpyAsync(r.ptr, radiance, px * 8, hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(ctx, hipMemcpyAsync(gc.ptr, gbuf_cur, px * 16, hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(ctx, hipMemcpyAsync(ap.ptr, accum_prev, ab, hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(ctx, hipMemcpyAsync(gp.ptr, gbuf_prev, px * 16, hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(ctx, vrt::launch_temporal(ctx->stream, *u, r.ptr, gc.ptr, ap.ptr, gp.ptr, ao.ptr, (int)accum_format));
+  HIP_TRY(ctx, hipMemcpyAsync(accum_out, ao.ptr, ab, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return VRT_OK;
+  GUARD_END(ctx)
+}
+
+int vrt_atrous_pass(vrt_ctx* ctx, const vrt_uniforms* u, const void* src, uint32_t src_format,
+                    const float* gbuf, int32_t step, uint16_t* dst) {
+  if (!ctx || !u || !src || !gbuf || !dst || src_format > 1 || (step != 1 && step != 2 && step != 4))
+    return ctx ? ctx->fail(VRT_ERR_INVALID, "bad argument (step must be 1, 2 or 4)") : VRT_ERR_INVALID;
+  GUARD_BEGIN
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  const uint64_t px = (uint64_t)u->params1[0] * (uint64_t)u->params1[1];
+  const uint64_t sb = px * (src_format ? 16 : 8);
+  vrt::DeviceBuffer s, g, d;
+  HIP_TRY(ctx, s.alloc(sb));
+  HIP_TRY(ctx, g.alloc(px * 16));
+  HIP_TRY(ctx, d.alloc(px * 8));
+  HIP_TRY(ctx, hipMemcpyAsync(s.ptr, src, sb, hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(ctx, hipMemcpyAsync(g.ptr, gbuf, px * 16, hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(ctx, vrt::launch_atrous(ctx->stream, *u, s.ptr, (int)src_format, g.ptr, step, d.ptr));
+  HIP_TRY(ctx, hipMemcpyAsync(dst, d.ptr, px * 8, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return VRT_OK;
+  GUARD_END(ctx)
+}
+
+int vrt_stats_get(vrt_ctx* ctx, vrt_stats* out) {
+  if (!ctx || !out) return VRT_ERR_INVALID;
+  GUARD_BEGIN
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  uint64_t c[4];
+  int rc = ctx->counters_to_host(c);
+  if (rc) return rc;
+  std::memset(out, 0, sizeof(*out));
+  out->rays = c[0];
+  out->brick_steps = c[1];
+  out->voxel_steps = c[2];
+  out->hits = c[3];
+  out->frames = ctx->frames;
+  out->gpu_ms_pathtrace = ctx->ms_pathtrace;
+  out->gpu_ms_temporal = ctx->ms_temporal;
+  out->gpu_ms_atrous = ctx->ms_atrous;
+  out->gpu_ms_frame = ctx->ms_frame;
+  out->hbm_resident = ctx->resident;
+  out->hbm_budget = ctx->budget;
+  return VRT_OK;
+  GUARD_END(ctx)
+}
+
+int vrt_stats_reset(vrt_ctx* ctx) {
+  if (!ctx) return VRT_ERR_INVALID;
+  GUARD_BEGIN
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  HIP_TRY(ctx, hipMemsetAsync(ctx->d_counters.ptr, 0, 4 * sizeof(uint64_t), ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->frames = 0;
+  return VRT_OK;
+  GUARD_END(ctx)
+}
+
+}  // extern "C"
+
+vrt_ctx::~vrt_ctx() {
+  for (vrt_scene* s : scenes) delete s;
+  scenes.clear();
+  views.clear();
+  for (auto& e : ev)
+    if (e) (void)hipEventDestroy(e);
+  if (stream) (void)hipStreamDestroy(stream);
+}
