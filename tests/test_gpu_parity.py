@@ -1,0 +1,260 @@
+"""GPU parity: libvoxelrt_hip (through the C ABI) against the CPU oracle.
+
+Integer/index outputs (brick words, masks, voxel index, face, step counts)
+and every stored texel (f16 radiance, unorm8 albedo, f32 G-buffer, f16/f32
+accumulation, f16 a-trous output, f32 HDR) must match bit for bit: both sides
+compute FP32 in the same operation order without fused multiply-add and with
+the same own transcendental functions.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import voxelrt_amd as v
+from voxelrt_amd import configs as cf
+from tests.chain import OracleChain, assert_bits_equal, f16, to_oracle_uniforms
+
+pytestmark = pytest.mark.gpu
+
+
+def _hits_equal(got, want, name=""):
+    for f in ("voxel", "face", "brick_steps", "voxel_steps"):
+        bad = np.flatnonzero(got[f] != want[f])
+        assert len(bad) == 0, f"{name} {f}: {len(bad)} mismatches, first {bad[:5]} got {got[f][bad[:5]]} want {want[f][bad[:5]]}"
+    tb = np.flatnonzero(got["t"].view(np.uint32) != want["t"].view(np.uint32))
+    assert len(tb) == 0, f"{name} t: {len(tb)} bit mismatches, first {tb[:5]} got {got['t'][tb[:5]]} want {want['t'][tb[:5]]}"
+
+
+# ---------------------------------------------------------------------------
+# scene build (scene.js:411-433 on the device)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("shape,p", [((64, 64, 64), 0.25), ((48, 32, 64), 0.6), ((32, 32, 32), 0.0),
+                                     ((16, 24, 8), 1.0)])
+def test_scene_bricks_match_oracle(ctx, shape, p):
+    nz, ny, nx = shape
+    vox = v.generate_scene(v.SCENE_RANDOM_BRICKS, n=nx, ny=ny, nz=nz, seed=9, p_brick=p, p_voxel=0.5)
+    if p == 1.0:
+        vox[:] = 0xFF00FF01  # every voxel solid
+    sc = ctx.upload(vox)
+    words, masks = sc.bricks()
+    ob, om = oracle.build_bricks(vox)
+    assert np.array_equal(words != 0, ob != 0)
+    assert np.array_equal(masks, om)
+    # slots are a permutation of 1..nocc in brick order
+    occ = words[words != 0]
+    assert np.array_equal(occ, np.arange(1, len(occ) + 1, dtype=np.uint32))
+    sc.release()
+
+
+def test_scene_build_multi_slab(ctx):
+    """A grid larger than one 512 MiB staging slab (re-uploaded per slab)."""
+    vox = v.generate_scene(v.SCENE_RANDOM_BRICKS, n=1024, ny=256, nz=1024, seed=4, p_brick=0.02,
+                           p_voxel=0.5)
+    sc = ctx.upload(vox)
+    words, masks = sc.bricks()
+    ob, om = oracle.build_bricks(vox)
+    assert np.array_equal(words != 0, ob != 0)
+    assert np.array_equal(masks, om)
+    # a few rays through it
+    rays = cf.traverse_rays(1 << 14, n=256, seed=3) * np.array([4, 1, 4, 1, 1, 1], np.float32)
+    got = ctx.trace_rays(sc, rays)
+    want = oracle.trace_rays(oracle.OracleGrid(vox), rays)
+    _hits_equal(got, want, "1024x256x1024")
+    sc.release()
+
+
+# ---------------------------------------------------------------------------
+# traversal (voxel.wgsl:73-153)
+# ---------------------------------------------------------------------------
+@pytest.fixture(scope="module", params=[0.05, 0.25, 0.6])
+def traverse(request, ctx):
+    vox = cf.traverse_scene(request.param)
+    sc = ctx.upload(vox)
+    yield request.param, vox, sc, oracle.OracleGrid(vox)
+    sc.release()
+
+
+def test_trace_bitexact_traverse(ctx, traverse):
+    p, vox, sc, og = traverse
+    rays = cf.traverse_rays(1 << 20, seed=int(p * 100))
+    got = ctx.trace_rays(sc, rays)
+    want = oracle.trace_rays(og, rays)
+    _hits_equal(got, want, f"p={p}")
+    assert (got["t"] >= 0).mean() > 0.5
+
+
+def test_trace_full_size_properties(ctx, traverse):
+    """2^24 rays: every hit lands on a solid voxel, misses carry the miss
+    markers, and a 2^16 random subset matches the oracle exactly."""
+    p, vox, sc, og = traverse
+    rays = cf.traverse_rays(1 << 24, seed=77)
+    got = ctx.trace_rays(sc, rays)
+    hit = got["t"] >= 0
+    flat = vox.reshape(-1)
+    assert np.all(flat[got["voxel"][hit]] != 0)
+    assert np.all(got["voxel"][~hit] == v.VOXEL_MISS) and np.all(got["face"][~hit] == v.FACE_MISS)
+    assert np.all(got["face"][hit] <= v.FACE_NONE)
+    idx = np.random.default_rng(1).choice(len(rays), 1 << 16, replace=False)
+    _hits_equal(got[idx], oracle.trace_rays(og, rays[idx]), "subset")
+
+
+def _edge_rays(n: int):
+    ext = n * cf.VOXEL_SIZE
+    rng = np.random.default_rng(3)
+    rays = []
+    axes = np.array([[1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1]], np.float32)
+    for d in axes:  # axis-aligned, from inside and outside the grid
+        for _ in range(64):
+            o = rng.uniform(-0.5 * ext, 1.5 * ext, 3).astype(np.float32)
+            rays.append(np.concatenate([o, d]))
+    for _ in range(256):  # one or two zero components, signed zeros
+        d = rng.normal(size=3).astype(np.float32)
+        d[rng.integers(0, 3)] = 0.0
+        if rng.random() < 0.5:
+            d[rng.integers(0, 3)] = -0.0
+        d /= np.linalg.norm(d) + 1e-30
+        o = rng.uniform(0, ext, 3).astype(np.float32)
+        rays.append(np.concatenate([o, d]))
+    for _ in range(256):  # origins exactly on voxel / brick boundaries and grid faces
+        o = (rng.integers(0, n + 1, 3) * cf.VOXEL_SIZE).astype(np.float32)
+        d = rng.normal(size=3).astype(np.float32)
+        d /= np.linalg.norm(d)
+        rays.append(np.concatenate([o, d]))
+    for _ in range(256):  # diagonal directions: exact tMax ties
+        o = (rng.integers(0, n, 3) * cf.VOXEL_SIZE + cf.VOXEL_SIZE / 2).astype(np.float32)
+        d = np.array([rng.choice([-1, 1]) for _ in range(3)], np.float32) / np.float32(np.sqrt(3))
+        rays.append(np.concatenate([o, d]))
+    return np.ascontiguousarray(np.stack(rays), np.float32)
+
+
+@pytest.mark.parametrize("kind", ["random", "empty", "full", "noncubic"])
+@pytest.mark.parametrize("tmax", [1e4, 1.5])
+def test_trace_edge_cases(ctx, kind, tmax):
+    n = 64
+    if kind == "random":
+        vox = v.generate_scene(v.SCENE_RANDOM_BRICKS, n=n, seed=5, p_brick=0.3, p_voxel=0.3)
+    elif kind == "empty":
+        vox = np.zeros((n, n, n), np.uint32)
+    elif kind == "full":
+        vox = np.full((n, n, n), 0x00808080, np.uint32)
+    else:
+        vox = v.generate_scene(v.SCENE_RANDOM_BRICKS, n=96, ny=32, nz=48, seed=6, p_brick=0.2, p_voxel=0.4)
+    sc = ctx.upload(vox)
+    rays = _edge_rays(n)
+    _hits_equal(ctx.trace_rays(sc, rays, tmax), oracle.trace_rays(oracle.OracleGrid(vox), rays, tmax),
+                f"{kind} tmax={tmax}")
+    sc.release()
+
+
+# ---------------------------------------------------------------------------
+# path tracer (pathtrace.wgsl:248-330)
+# ---------------------------------------------------------------------------
+@pytest.fixture(scope="module")
+def frame128(ctx):
+    vox = cf.frame_128_scene()
+    sc = ctx.upload(vox)
+    yield vox, sc, oracle.OracleGrid(vox)
+    sc.release()
+
+
+def _render_one(ctx, sc, view_id, u, fclamp, spp, w, h, accum_format=0):
+    view = v.View(sc.handle, u, fclamp, view_id, spp, accum_format)
+    outs = {"radiance": np.zeros((h, w, 4), np.uint16), "albedo": np.zeros((h, w, 4), np.uint8),
+            "gbuf": np.zeros((h, w, 4), np.float32)}
+    fr = ctx.render([view], [outs])
+    ctx.release_view(view_id)
+    return outs, fr[0]
+
+
+def test_pathtrace_frame128_bitexact(ctx, frame128):
+    vox, sc, og = frame128
+    pos, yaw, pitch = cf.frame_128_pose()
+    p = cf.FRAME_128
+    u, _ = v.make_uniforms(pos, yaw, pitch, p["width"], p["height"], frame=0, bounces=p["bounces"],
+                           sun_azimuth=p["sun_azimuth"], sun_elevation=p["sun_elevation"],
+                           sun_cos_radius=p["sun_cos_radius"])
+    outs, fr = _render_one(ctx, sc, 900, u, p["fclamp"], 1, p["width"], p["height"])
+    rad, alb, gb, cnt = oracle.pathtrace(og, to_oracle_uniforms(u), p["fclamp"], 1)
+    assert_bits_equal("gbuf", outs["gbuf"], gb)
+    assert_bits_equal("albedo", outs["albedo"], alb)
+    assert_bits_equal("radiance", outs["radiance"], rad)
+    assert (fr.rays, fr.brick_steps, fr.voxel_steps) == (cnt.rays, cnt.brick_steps, cnt.voxel_steps)
+
+
+@pytest.mark.parametrize("bounces,spp,fclamp,frame", [(4, 3, 0.0, 7), (0, 1, 48.0, 123456789), (6, 2, 10.0, 3)])
+def test_pathtrace_variants_bitexact(ctx, frame128, bounces, spp, fclamp, frame):
+    """Russian roulette (bounce >= 2), several samples, clamp off/on, seeds."""
+    vox, sc, og = frame128
+    pos, yaw, pitch = cf.frame_128_pose()
+    w, h = 160, 96
+    u, _ = v.make_uniforms(pos, yaw + 0.3, pitch, w, h, frame=frame, bounces=bounces)
+    outs, fr = _render_one(ctx, sc, 901, u, fclamp, spp, w, h)
+    rad, alb, gb, cnt = oracle.pathtrace(og, to_oracle_uniforms(u), fclamp, spp)
+    assert_bits_equal("gbuf", outs["gbuf"], gb)
+    assert_bits_equal("albedo", outs["albedo"], alb)
+    assert_bits_equal("radiance", outs["radiance"], rad)
+    assert fr.rays == cnt.rays
+
+
+def test_pathtrace_converged_relmse(ctx, frame128):
+    """4096 spp over a crop: the GPU estimate against the oracle's own
+    4096-spp estimate of the same pixels, relMSE < 1e-3."""
+    vox, sc, og = frame128
+    pos, yaw, pitch = cf.frame_128_pose()
+    w, h = 64, 36
+    u, _ = v.make_uniforms(pos, yaw, pitch, w, h, frame=0, bounces=2, sun_elevation=cf.SUN_40DEG["sun_elevation"],
+                           sun_cos_radius=cf.SUN_40DEG["sun_cos_radius"])
+    outs, _ = _render_one(ctx, sc, 902, u, 10.0, 4096, w, h)
+    rad, _, _, _ = oracle.pathtrace(og, to_oracle_uniforms(u), 10.0, 4096)
+    a, b = f16(outs["radiance"])[..., :3], f16(rad)[..., :3]
+    rel = np.mean((a - b) ** 2 / (b * b + 1e-2))
+    assert rel < 1e-3, rel
+
+
+# ---------------------------------------------------------------------------
+# temporal + a-trous (temporal.wgsl, atrous.wgsl)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("fmt", [0, 1])
+def test_filter_passes_bitexact(ctx, frame128, fmt):
+    vox, sc, og = frame128
+    pos, yaw, pitch = cf.frame_128_pose()
+    w, h = 200, 120
+    u0, vp0 = v.make_uniforms(pos, yaw, pitch, w, h, frame=0, flags=3)
+    pos1 = (pos[0] + 0.05, pos[1], pos[2] + 0.02)
+    u1, _ = v.make_uniforms(pos1, yaw + 0.01, pitch, w, h, frame=1, flags=3, prev_view_proj=vp0,
+                            prev_cam_pos=pos)
+    o0, o1 = to_oracle_uniforms(u0), to_oracle_uniforms(u1)
+    rad0, alb0, gb0, _ = oracle.pathtrace(og, o0, 48.0, 1)
+    acc0 = oracle.temporal(o0, rad0, gb0, np.zeros((h, w, 4), np.uint16 if fmt == 0 else np.float32),
+                           np.zeros((h, w, 4), np.float32), fmt)
+    rad1, alb1, gb1, _ = oracle.pathtrace(og, o1, 48.0, 1)
+    want = oracle.temporal(o1, rad1, gb1, acc0, gb0, fmt)
+    got = ctx.temporal_pass(u1, rad1, gb1, acc0, gb0, fmt)
+    assert_bits_equal("temporal", got, want)
+    assert (f16(want)[..., 3] if fmt == 0 else want[..., 3]).max() == 2.0  # history grew
+    src = want
+    for step in (1, 2, 4):
+        want_d = oracle.atrous(o1, src, gb1, step, fmt if step == 1 else 0)
+        got_d = ctx.atrous_pass(u1, src, gb1, step, fmt if step == 1 else 0)
+        assert_bits_equal(f"atrous step {step}", got_d, want_d)
+        src = want_d
+
+
+def test_frame_sequence_bitexact(ctx):
+    """Five frames of an orbit through the whole chain with temporal and
+    denoise on: every target of every frame matches the oracle chain."""
+    desc = dict(cf.FRAME_1080_SCENE)
+    vox = v.generate_scene(**desc)
+    sc = ctx.upload(vox)
+    og = oracle.OracleGrid(vox)
+    w, h = 240, 136
+    r = v.Renderer(ctx, sc, w, h, bounces=2, temporal=True, denoise=True, fclamp=48.0, view_id=950)
+    chain = OracleChain(og, w, h, 0)
+    for i in range(5):
+        r.set_pose(*cf.world_orbit(desc, i, 64))
+        got = r.frame(("radiance", "albedo", "gbuf", "accum", "denoised", "hdr"))
+        want = chain.frame(to_oracle_uniforms(r.last_view.u), 48.0)
+        for k in ("radiance", "albedo", "gbuf", "accum", "denoised", "hdr"):
+            assert_bits_equal(f"frame {i} {k}", got[k], want[k])
+    ctx.release_view(950)
+    sc.release()

@@ -119,6 +119,7 @@ vrt::SceneDev vrt_scene::dev() const {
   s.bgz = (int)(nz / 8);
   s.voxel_size = VRT_VOXEL_SIZE;
   s.inv_voxel = 1.0f / VRT_VOXEL_SIZE;
+  s.zero = 0.0f;
   return s;
 }
 

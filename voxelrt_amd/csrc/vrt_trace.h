@@ -25,6 +25,10 @@ struct SceneDev {
   int nx, ny, nz;
   int bgx, bgy, bgz;
   float voxel_size, inv_voxel;
+  // Always 0.0f.  Passed at run time so that (zero - x) stays a real
+  // subtraction: the gfx950 instruction selector folds a literal (0.0f - x)
+  // feeding a multiply into a negated operand, which turns +0 into -0.
+  float zero;
 };
 
 struct TraceResult {
@@ -71,7 +75,7 @@ VRT_D void trace(const SceneDev& s, f3 roW, f3 rd, float maxTWorld, TraceResult&
     adl[c] = vabs(inv[c]);
     sd[c] = sign_i(rdv[c]);
     ge[c] = rdv[c] >= 0.0f ? 1.0f : 0.0f;
-    float t0 = (0.0f - ro[c]) * inv[c];
+    float t0 = (s.zero - ro[c]) * inv[c];
     float t1 = ((float)G[c] - ro[c]) * inv[c];
     tmin[c] = vmin(t0, t1);
     tmax[c] = vmax(t0, t1);

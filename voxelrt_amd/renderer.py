@@ -50,6 +50,7 @@ class Renderer:
         self.prev_view_proj = None
         self.prev_cam_pos = None
         self.pose = ((0.0, 0.0, 0.0), 0.0, 0.0)
+        self.last_view = None
 
     def set_pose(self, pos, yaw, pitch):
         self.pose = (tuple(pos), float(yaw), float(pitch))
@@ -74,6 +75,7 @@ class Renderer:
     def frame(self, outputs=()):
         """Render one frame; `outputs` names the targets to read back."""
         v = self.view()
+        self.last_view = v
         outs = None
         arrays = {}
         if outputs:
