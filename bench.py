@@ -1,0 +1,206 @@
+"""Benchmark: VoxelRT's base frame pipeline at 1920x1080 on a 256^3 grid.
+
+One step = one frame of a 64-frame camera orbit: pathtrace (1 spp, 2 bounces,
+sun-cone NEE) -> temporal reprojection (rgba16f history, cap 64) -> a-trous
+steps 1/2/4, on one MI355X, scene and frame targets resident in HBM.
+
+Prints one JSON line:
+  value        Mrays/s over the timed frames (primary + shadow + bounce rays,
+               counted in the kernel; the counts equal the oracle's)
+  ms_per_step  ms per frame (wall clock over the timed frames)
+  roofline     the path-trace kernel (dominant): algorithmic bytes per launch
+               (4 B per brick step + 4 B per voxel step + 4 B per hit voxel +
+               28 B of stores per pixel) / its HIP-event duration, vs 8 TB/s
+  cpu_baseline the CPU oracle on 3 frames of the same workload (host cores)
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "Mrays/s & ms/frame @1080p 256³; batched/served p50/p99 latency vs plain"
+
+
+def _pmc_traffic(name: str):
+    """HBM bytes per launch from a committed rocprofv3 --pmc summary, if any
+    (profiles/<name>: a line 'traffic_bytes_per_launch: <int>')."""
+    path = os.path.join(ROOT, "profiles", name)
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(float(line.split(":", 1)[1]))
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(vox, frames, w, h, fclamp, desc):
+    """The CPU oracle on `frames` frames of the workload (all host cores)."""
+    import numpy as np  # noqa: F401
+
+    import oracle
+    from tests.chain import OracleChain
+    from voxelrt_amd import configs as cf
+
+    threads = min(os.cpu_count() or 1, 64)
+    g = oracle.OracleGrid(vox)
+    chain = OracleChain(g, w, h, 0, threads=threads)
+    prev_vp = prev_pos = None
+    rays = 0
+    t0 = time.perf_counter()
+    for i in range(frames):
+        pos, yaw, pitch = cf.world_orbit(desc, i)
+        u, vp = oracle.make_uniforms(pos, yaw, pitch, w, h, frame=i, bounces=2, flags=3,
+                                     prev_view_proj=prev_vp, prev_cam_pos=prev_pos)
+        out = chain.frame(u, fclamp)
+        rays += out["counters"].rays
+        prev_vp, prev_pos = vp, pos
+    dt = time.perf_counter() - t0
+    return {"value": round(rays / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "ms_per_frame": round(dt / frames * 1e3, 1),
+            "sample": f"{frames} frames of the 1920x1080 orbit (pathtrace + temporal + 3 a-trous), "
+                      f"oracle C++ FP32, OpenMP over rows"}
+
+
+def traverse(args):
+    """Traversal microbench: 256^3 random bricks, 2^24 rays resident in HBM,
+    hits (voxel, face, steps, t) written back to HBM.  One JSON line per
+    occupancy p.  Algorithmic bytes per ray: 24 in + 16 out + 4 per brick step
+    + 4 per voxel step + 4 per hit."""
+    import numpy as np
+    import torch
+
+    import voxelrt_amd as v
+    from voxelrt_amd import configs as cf
+
+    n_rays = 1 << 24
+    rays = cf.traverse_rays(n_rays, seed=77)
+    ctx = v.Context(0)
+    d_rays = torch.from_numpy(rays).to("cuda:0")
+    d_hits = torch.empty(n_rays * 4, dtype=torch.int32, device="cuda:0")
+    torch.cuda.synchronize()
+    for p in (0.05, 0.25, 0.6):
+        sc = ctx.upload(cf.traverse_scene(p))
+        for _ in range(args.warmup):
+            ctx.trace_rays_device(sc, d_rays.data_ptr(), n_rays, d_hits.data_ptr())
+        s0 = ctx.stats()
+        ms = []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ms.append(ctx.trace_rays_device(sc, d_rays.data_ptr(), n_rays, d_hits.data_ptr()))
+        s1 = ctx.stats()
+        wall = time.perf_counter() - t0
+        K = args.steps
+        bs = (s1["brick_steps"] - s0["brick_steps"]) / K
+        vs = (s1["voxel_steps"] - s0["voxel_steps"]) / K
+        hits = (s1["hits"] - s0["hits"]) / K
+        alg = 40 * n_rays + 4 * bs + 4 * vs + 4 * hits
+        kms = float(np.mean(ms))
+        gbs = alg / (kms * 1e-3) / 1e9
+        print(json.dumps({"metric": "traversal Mrays/s (256^3, 2^24 rays)", "p": p,
+                          "value": round(n_rays / (kms * 1e-3) / 1e6, 1), "unit": "Mrays/s",
+                          "kernel_ms": round(kms, 4), "wall_ms_per_launch": round(wall / K * 1e3, 4),
+                          "brick_steps_per_ray": round(bs / n_rays, 3),
+                          "voxel_steps_per_ray": round(vs / n_rays, 3), "hit_frac": round(hits / n_rays, 4),
+                          "roofline": {"bound": "hbm", "kernel": "k_trace_rays", "achieved": round(gbs, 1),
+                                       "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                       "frac": round(gbs / HBM_PEAK_GBS, 4),
+                                       "alg_bytes_per_launch": int(alg)}}), flush=True)
+        sc.release()
+    ctx.close()
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=32)
+    ap.add_argument("--warmup", type=int, default=8)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-frames", type=int, default=8)
+    ap.add_argument("--workload", default="frame_1080", choices=["frame_1080", "traverse"])
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: this build has no multi-GPU path (--gpus {args.gpus} "
+              f"requested). The frame shards per view; see the design notes.", file=sys.stderr)
+        sys.exit(2)
+
+    if args.workload == "traverse":
+        return traverse(args)
+
+    import voxelrt_amd as v
+    from voxelrt_amd import configs as cf
+
+    desc = dict(cf.FRAME_1080_SCENE)
+    p = cf.FRAME_1080
+    w, h = p["width"], p["height"]
+    vox = cf.frame_1080_scene()
+    ctx = v.Context(0)
+    sc = ctx.upload(vox)
+    r = v.Renderer(ctx, sc, w, h, bounces=p["bounces"], temporal=True, denoise=True,
+                   fclamp=p["fclamp"], max_history=p["max_history"], view_id=1)
+
+    def step(i):
+        r.set_pose(*cf.world_orbit(desc, i % cf.ORBIT_FRAMES))
+        r.frame()
+
+    for i in range(args.warmup):
+        step(i)
+    ctx.stats()  # stream sync: barrier before the timed region
+    s0 = ctx.stats()
+    pt_ms = tp_ms = at_ms = 0.0
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        step(args.warmup + k)
+        st = ctx.stats()
+        pt_ms += st["gpu_ms_pathtrace"]
+        tp_ms += st["gpu_ms_temporal"]
+        at_ms += st["gpu_ms_atrous"]
+    s1 = ctx.stats()  # stream sync: barrier after
+    wall = time.perf_counter() - t0
+
+    K = args.steps
+    rays = s1["rays"] - s0["rays"]
+    bsteps = s1["brick_steps"] - s0["brick_steps"]
+    vsteps = s1["voxel_steps"] - s0["voxel_steps"]
+    hits = s1["hits"] - s0["hits"]
+    ms_frame = wall / K * 1e3
+    mrays = rays / wall / 1e6
+    # path-trace kernel: algorithmic bytes per launch
+    alg_bytes = (4 * bsteps + 4 * vsteps + 4 * hits) / K + 28 * w * h
+    pt_avg_ms = pt_ms / K
+    achieved = alg_bytes / (pt_avg_ms * 1e-3) / 1e9
+    out = {
+        "metric": METRIC, "value": round(mrays, 2), "unit": "Mrays/s", "n_gpus": 1,
+        "steps": K, "warmup": args.warmup, "ms_per_step": round(ms_frame, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded 256^3 terrain + Cornell room + 3 lanterns + 40 trees)",
+        "config": {"workload": "frame_1080: 1920x1080, 256^3 grid, 1 spp, 2 bounces, temporal "
+                               "(rgba16f, cap 64) + a-trous 1/2/4, 64-frame orbit",
+                   "width": w, "height": h, "grid": 256, "spp": 1, "bounces": 2},
+        "roofline": {"bound": "hbm", "kernel": "k_pathtrace", "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": _pmc_traffic("pathtrace_frame1080_pmc.txt"),
+                     "alg_bytes_per_launch": int(alg_bytes), "avg_launch_ms": round(pt_avg_ms, 4)},
+        "breakdown": {"gpu_ms_pathtrace": round(pt_avg_ms, 4), "gpu_ms_temporal": round(tp_ms / K, 4),
+                      "gpu_ms_atrous3": round(at_ms / K, 4),
+                      "rays_per_frame": rays // K, "brick_steps_per_ray": round(bsteps / max(rays, 1), 2),
+                      "voxel_steps_per_ray": round(vsteps / max(rays, 1), 2)},
+    }
+    sc.release()
+    ctx.close()
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(vox, args.cpu_frames, w, h, p["fclamp"], desc)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
