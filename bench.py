@@ -88,8 +88,24 @@ def traverse(args):
     d_rays = torch.from_numpy(rays).to("cuda:0")
     d_hits = torch.empty(n_rays * 4, dtype=torch.int32, device="cuda:0")
     torch.cuda.synchronize()
+    variants = {"lane": dict(trace_kernel=0),
+                "persistent": dict(trace_kernel=1, trace_blocks=2048, trace_steps=8, trace_min_refill=16)}
     for p in (0.05, 0.25, 0.6):
         sc = ctx.upload(cf.traverse_scene(p))
+        for name in ("lane", "persistent", "autotuned"):
+            if name == "autotuned":
+                tuned = ctx.autotune(sc, 1 << 22).as_dict()
+            else:
+                ctx.set_tuning(**variants[name])
+            _traverse_one(args, ctx, sc, p, name, d_rays, d_hits, n_rays,
+                          tuned if name == "autotuned" else variants[name])
+        sc.release()
+    ctx.close()
+
+
+def _traverse_one(args, ctx, sc, p, name, d_rays, d_hits, n_rays, tuning):
+    import numpy as np
+    if True:
         for _ in range(args.warmup):
             ctx.trace_rays_device(sc, d_rays.data_ptr(), n_rays, d_hits.data_ptr())
         s0 = ctx.stats()
@@ -106,7 +122,8 @@ def traverse(args):
         alg = 40 * n_rays + 4 * bs + 4 * vs + 4 * hits
         kms = float(np.mean(ms))
         gbs = alg / (kms * 1e-3) / 1e9
-        print(json.dumps({"metric": "traversal Mrays/s (256^3, 2^24 rays)", "p": p,
+        print(json.dumps({"metric": "traversal Mrays/s (256^3, 2^24 rays)", "p": p, "variant": name,
+                          "tuning": tuning,
                           "value": round(n_rays / (kms * 1e-3) / 1e6, 1), "unit": "Mrays/s",
                           "kernel_ms": round(kms, 4), "wall_ms_per_launch": round(wall / K * 1e3, 4),
                           "brick_steps_per_ray": round(bs / n_rays, 3),
@@ -115,8 +132,6 @@ def traverse(args):
                                        "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                        "frac": round(gbs / HBM_PEAK_GBS, 4),
                                        "alg_bytes_per_launch": int(alg)}}), flush=True)
-        sc.release()
-    ctx.close()
 
 
 def main():

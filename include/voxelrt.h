@@ -59,6 +59,11 @@ extern "C" {
 /* uniforms flag bits (params0.z, common.wgsl:20-22, main.js:413-416) */
 #define VRT_FLAG_TEMPORAL  1u
 #define VRT_FLAG_DENOISE   2u
+/* Indirect light only (no reference counterpart; for VoxelBench techniques,
+ * which estimate indirect light, technique.h:2-3): the primary vertex adds
+ * no emission and no sun NEE, and a primary miss returns 0 instead of the
+ * sky.  The RNG stream is consumed exactly as without the flag. */
+#define VRT_FLAG_INDIRECT  4u
 
 /* ---- opaque handles ------------------------------------------------------- */
 typedef struct vrt_ctx vrt_ctx;
@@ -195,6 +200,30 @@ int vrt_temporal_pass(vrt_ctx* ctx, const vrt_uniforms* u,
 int vrt_atrous_pass(vrt_ctx* ctx, const vrt_uniforms* u, const void* src,
                     uint32_t src_format, const float* gbuf, int32_t step,
                     uint16_t* dst);
+
+/* ---- launch tuning ----------------------------------------------------------- *
+ * Kernel variants and launch shapes.  Every variant computes identical
+ * results; only speed differs.  vrt_autotune measures candidates on the
+ * given scene and keeps the fastest in the context.  No reference
+ * counterpart: WebGPU dispatches fixed 8x8 workgroups (main.js:728-734). */
+typedef struct vrt_tuning {
+  uint32_t trace_kernel;      /* 0 = one ray per lane, 1 = persistent ray queue */
+  uint32_t trace_blocks;      /* persistent grid (256-thread blocks), 0 = default */
+  uint32_t trace_steps;       /* DDA iterations between queue refill checks */
+  uint32_t trace_min_refill;  /* idle lanes (of 64) that trigger a refill */
+  uint32_t pt_kernel;         /* path tracer: 0 = one pixel per lane, 1 = persistent */
+  uint32_t pt_blocks;
+  uint32_t pt_steps;
+  uint32_t pt_min_refill;
+  uint32_t reserved[8];
+} vrt_tuning;
+int vrt_tuning_get(vrt_ctx* ctx, vrt_tuning* out);
+int vrt_tuning_set(vrt_ctx* ctx, const vrt_tuning* in);
+/* Time the candidate variants on `scene` (traversal on `nrays` random rays,
+ * path tracing on a width x height view at the scene's spawn-like pose
+ * given by u) and keep the fastest.  Either part may be skipped with 0. */
+int vrt_autotune(vrt_ctx* ctx, const vrt_scene* scene, uint64_t nrays,
+                 const vrt_uniforms* u, vrt_tuning* chosen);
 
 /* ---- stats ------------------------------------------------------------------ */
 int vrt_stats_get(vrt_ctx* ctx, vrt_stats* out);

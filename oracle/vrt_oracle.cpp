@@ -484,8 +484,9 @@ PixelResult shade_pixel(const orc_grid* g, const vrt_uniforms* u, float fclamp,
   TraceOut h0;
   trace_impl(g, ro, rd, 1e4f, &h0);
   count(cnt, h0);
+  const bool indirect = (u->params0[2] & VRT_FLAG_INDIRECT) != 0u;
   if (h0.t < 0.0f) {
-    res.radiance = sky_color(u, rd, true);
+    res.radiance = indirect ? mk(0.0f, 0.0f, 0.0f) : sky_color(u, rd, true);
     res.albedo_sqrt = mk(1.0f, 1.0f, 1.0f);
     res.gbuf[0] = -rd.x; res.gbuf[1] = -rd.y; res.gbuf[2] = -rd.z; res.gbuf[3] = -1.0f;
     return res;
@@ -504,12 +505,13 @@ PixelResult shade_pixel(const orc_grid* g, const vrt_uniforms* u, float fclamp,
   const uint32_t nb = u->params0[1];
   const v3 sunRad = mk(u->sun_radiance[0], u->sun_radiance[1], u->sun_radiance[2]);
   for (uint32_t bounce = 0; bounce <= nb; ++bounce) {
-    if (mat.emissive > 0.0f)
+    const bool direct = !(indirect && bounce == 0u);
+    if (mat.emissive > 0.0f && direct)
       radiance = add(radiance, muls(mul(tp, mat.albedo), mat.emissive * EMISSIVE_SCALE));
     v3 surf = add(pos, muls(n, 1e-3f));
     v3 l = sample_sun_dir(u, &rng);
     float ndl = dot3(n, l);
-    if (ndl > 0.0f && l.y > 0.0f) {
+    if (ndl > 0.0f && l.y > 0.0f && direct) {
       TraceOut hs;
       trace_impl(g, surf, l, 1e4f, &hs);
       count(cnt, hs);

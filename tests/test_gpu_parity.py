@@ -74,13 +74,28 @@ def traverse(request, ctx):
     sc.release()
 
 
-def test_trace_bitexact_traverse(ctx, traverse):
+TRACE_VARIANTS = [dict(trace_kernel=0), dict(trace_kernel=1, trace_blocks=2048, trace_steps=8, trace_min_refill=16),
+                  dict(trace_kernel=1, trace_blocks=512, trace_steps=1, trace_min_refill=1),
+                  dict(trace_kernel=1, trace_blocks=4096, trace_steps=16, trace_min_refill=64)]
+
+
+@pytest.fixture(params=range(len(TRACE_VARIANTS)), ids=["lane", "persist", "persist-s1", "persist-s16"])
+def trace_variant(request, ctx):
+    old = ctx.tuning().as_dict()
+    ctx.set_tuning(**TRACE_VARIANTS[request.param])
+    yield TRACE_VARIANTS[request.param]
+    ctx.set_tuning(**old)
+
+
+def test_trace_bitexact_traverse(ctx, traverse, trace_variant):
     p, vox, sc, og = traverse
     rays = cf.traverse_rays(1 << 20, seed=int(p * 100))
     got = ctx.trace_rays(sc, rays)
     want = oracle.trace_rays(og, rays)
-    _hits_equal(got, want, f"p={p}")
+    _hits_equal(got, want, f"p={p} {trace_variant}")
     assert (got["t"] >= 0).mean() > 0.5
+    st = ctx.stats()
+    assert st["rays"] > 0
 
 
 def test_trace_full_size_properties(ctx, traverse):
@@ -129,7 +144,7 @@ def _edge_rays(n: int):
 
 @pytest.mark.parametrize("kind", ["random", "empty", "full", "noncubic"])
 @pytest.mark.parametrize("tmax", [1e4, 1.5])
-def test_trace_edge_cases(ctx, kind, tmax):
+def test_trace_edge_cases(ctx, kind, tmax, trace_variant):
     n = 64
     if kind == "random":
         vox = v.generate_scene(v.SCENE_RANDOM_BRICKS, n=n, seed=5, p_brick=0.3, p_voxel=0.3)
@@ -144,6 +159,18 @@ def test_trace_edge_cases(ctx, kind, tmax):
     _hits_equal(ctx.trace_rays(sc, rays, tmax), oracle.trace_rays(oracle.OracleGrid(vox), rays, tmax),
                 f"{kind} tmax={tmax}")
     sc.release()
+
+
+def test_autotune_keeps_results(ctx, traverse):
+    p, vox, sc, og = traverse
+    if p != 0.25:
+        pytest.skip("one occupancy is enough")
+    old = ctx.tuning().as_dict()
+    t = ctx.autotune(sc, 1 << 20)
+    assert t.trace_kernel in (0, 1)
+    rays = cf.traverse_rays(1 << 18, seed=5)
+    _hits_equal(ctx.trace_rays(sc, rays), oracle.trace_rays(og, rays), f"autotuned {t.as_dict()}")
+    ctx.set_tuning(**old)
 
 
 # ---------------------------------------------------------------------------

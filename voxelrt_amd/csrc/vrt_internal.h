@@ -73,7 +73,9 @@ struct vrt_ctx {
   uint64_t frames = 0;
   double ms_pathtrace = 0, ms_temporal = 0, ms_atrous = 0, ms_frame = 0;
   int trace_blocks = 0;  // 0 = default grid for k_trace_rays
+  vrt_tuning tuning{};   // kernel variants + launch shapes (vrt_tuning_set / vrt_autotune)
   vrt::DeviceBuffer d_counters;  // u64 x4: rays, brick steps, voxel steps, hits
+  vrt::DeviceBuffer d_queue;     // u32 work-queue counters of the persistent kernels
   vrt::DeviceBuffer scratch_rays, scratch_hits, scratch_hdr;
   std::vector<vrt_scene*> scenes;
   std::unordered_map<uint32_t, ViewState> views;
@@ -84,4 +86,8 @@ struct vrt_ctx {
   void unreserve(uint64_t bytes);
   int counters_to_host(uint64_t out[4]);
   int view_state(uint32_t id, uint32_t w, uint32_t h, uint32_t fmt, ViewState** out);
+  hipError_t launch_trace(const vrt::SceneDev& s, const float* rays, uint64_t n, float tmax,
+                          void* hits);
+  hipError_t launch_pathtrace(const vrt::SceneDev& s, const vrt_uniforms& u, float fclamp,
+                              uint32_t spp, void* radiance, void* albedo, void* gbuf);
 };

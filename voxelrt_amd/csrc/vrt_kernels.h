@@ -20,6 +20,9 @@ hipError_t launch_pack_pool(hipStream_t st, const uint32_t* vox, int nx, int ny,
                             uint32_t* pool);
 hipError_t launch_trace_rays(hipStream_t st, const SceneDev& s, const float* rays, uint64_t n,
                              float tmax, void* hits, unsigned long long* counters, int blocks);
+hipError_t launch_trace_persistent(hipStream_t st, const SceneDev& s, const float* rays, uint32_t n,
+                                  float tmax, void* hits, unsigned long long* counters,
+                                  uint32_t* queue, int blocks, int steps, int min_refill);
 hipError_t launch_pathtrace(hipStream_t st, const SceneDev& s, const Uniforms& u, float fclamp,
                             uint32_t spp, void* radiance, void* albedo, void* gbuf,
                             unsigned long long* counters);

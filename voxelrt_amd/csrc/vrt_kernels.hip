@@ -17,6 +17,7 @@
 #include "vrt_device.h"
 #include "vrt_kernels.h"
 #include "vrt_trace.h"
+#include "vrt_dda_state.h"
 
 namespace vrt {
 
@@ -143,6 +144,91 @@ __global__ __launch_bounds__(256) void k_trace_rays(SceneDev s, const float* __r
 }
 
 // ---------------------------------------------------------------------------
+// traversal microbench, persistent form: each lane pulls rays from a global
+// queue (one atomic per wave refill) and advances a resumable DDA state one
+// loop iteration at a time (vrt_dda_state.h).  `steps` iterations run between
+// refill checks; a refill happens once at least `min_refill` lanes are idle
+// (or the whole wave is).  Rays are taken from the global counter in chunks of
+// QUEUE_CHUNK per wave, so the counter sees one atomic per 256 rays instead of
+// one per refill.  The queue counter must be zero at launch.
+// ---------------------------------------------------------------------------
+constexpr uint32_t RAY_NONE = 0xFFFFFFFFu;  // lane wants a ray
+constexpr uint32_t RAY_DONE = 0xFFFFFFFEu;  // queue exhausted for this lane
+constexpr uint32_t QUEUE_CHUNK = 256u;      // rays a wave takes per queue atomic
+
+VRT_D void store_hit(uint4* hits, uint32_t i, const TraceResult& h) {
+  uint4 rec;
+  rec.x = __builtin_bit_cast(uint32_t, h.t);
+  rec.y = h.voxel;
+  rec.z = (h.bsteps & 0xffffu) | (h.vsteps << 16);
+  rec.w = h.face;
+  hits[i] = rec;
+}
+
+__global__ __launch_bounds__(256) void k_trace_persistent(SceneDev s, const float* __restrict__ rays,
+                                                          uint32_t n, float tmax,
+                                                          uint4* __restrict__ hits,
+                                                          unsigned long long* __restrict__ counters,
+                                                          uint32_t* __restrict__ queue, int steps,
+                                                          int min_refill) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint64_t lt_mask = lane == 0u ? 0ull : (~0ull >> (64u - lane));
+  uint32_t ray = RAY_NONE;
+  DdaState st;
+  uint32_t nb = 0, nv = 0, nh = 0, nr = 0;
+  // the wave's current chunk of the queue: [cbase, cbase + cleft)
+  uint32_t cbase = 0u, cleft = 0u;
+  for (;;) {
+    const uint64_t need = __ballot(ray == RAY_NONE);
+    const uint64_t busy = __ballot(ray < n);
+    if (need != 0ull && (busy == 0ull || __popcll(need) >= min_refill)) {
+      if (cleft == 0u) {
+        uint32_t b = 0u;
+        if (lane == 0u) b = atomicAdd(queue, QUEUE_CHUNK);
+        cbase = __shfl(b, 0, 64);
+        cleft = QUEUE_CHUNK;
+      }
+      const uint32_t cnt = (uint32_t)__popcll(need);
+      const uint32_t take = cnt < cleft ? cnt : cleft;
+      const uint32_t rank = (uint32_t)__popcll(need & lt_mask);
+      const uint32_t base = cbase;
+      cbase += take;
+      cleft -= take;
+      if (ray == RAY_NONE && rank < take) {
+        const uint32_t my = base + rank;
+        if (my < n) {
+          const float* r = rays + (uint64_t)my * 6;
+          TraceResult o;
+          if (st.begin(s, mk3(r[0], r[1], r[2]), mk3(r[3], r[4], r[5]), tmax, o)) {
+            store_hit(hits, my, o);
+            nr += 1u;
+          } else {
+            ray = my;
+          }
+        } else {
+          ray = RAY_DONE;
+        }
+      }
+    }
+    if (__ballot(ray != RAY_DONE) == 0ull) break;
+    for (int k = 0; k < steps; ++k) {
+      if (ray < n) {
+        TraceResult o;
+        if (st.step(s, o)) {
+          store_hit(hits, ray, o);
+          nb += o.bsteps;
+          nv += o.vsteps;
+          nh += o.t >= 0.0f ? 1u : 0u;
+          nr += 1u;
+          ray = RAY_NONE;
+        }
+      }
+    }
+  }
+  count4(counters, nr, nb, nv, nh);
+}
+
+// ---------------------------------------------------------------------------
 // camera / sky / sampling, common.wgsl:90-139
 // ---------------------------------------------------------------------------
 VRT_D float4 mat_vec4(const float* m, float x, float y, float z, float w) {
@@ -238,8 +324,9 @@ VRT_D PixelOut shade_pixel(const SceneDev& s, const Uniforms& u, float fclamp, u
   TraceResult h0;
   trace(s, ro, rd, 1e4f, h0);
   rc.add(h0);
+  const bool indirect = (u.params0[2] & VRT_FLAG_INDIRECT) != 0u;
   if (h0.t < 0.0f) {
-    res.radiance = sky_color(u, rd, true);
+    res.radiance = indirect ? mk3(0.0f, 0.0f, 0.0f) : sky_color(u, rd, true);
     res.albedo_sqrt = mk3(1.0f, 1.0f, 1.0f);
     res.gbuf = make_float4(-rd.x, -rd.y, -rd.z, -1.0f);
     return res;
@@ -257,11 +344,12 @@ VRT_D PixelOut shade_pixel(const SceneDev& s, const Uniforms& u, float fclamp, u
   const uint32_t nb = u.params0[1];
   const f3 sunRad = mk3(u.sun_radiance[0], u.sun_radiance[1], u.sun_radiance[2]);
   for (uint32_t bounce = 0; bounce <= nb; ++bounce) {
-    if (mat.emissive > 0.0f) radiance = radiance + (tp * mat.albedo) * (mat.emissive * EMISSIVE_SCALE);
+    const bool direct = !(indirect && bounce == 0u);
+    if (mat.emissive > 0.0f && direct) radiance = radiance + (tp * mat.albedo) * (mat.emissive * EMISSIVE_SCALE);
     const f3 surf = pos + n * 1e-3f;
     const f3 l = sample_sun_dir(u, rng);
     const float ndl = dot(n, l);
-    if (ndl > 0.0f && l.y > 0.0f) {
+    if (ndl > 0.0f && l.y > 0.0f && direct) {
       TraceResult hs;
       trace(s, surf, l, 1e4f, hs);
       rc.add(hs);
@@ -552,6 +640,17 @@ hipError_t launch_trace_rays(hipStream_t st, const SceneDev& s, const float* ray
   const uint32_t nb = need < (uint32_t)blocks ? need : (uint32_t)blocks;
   hipLaunchKernelGGL(k_trace_rays, dim3(nb > 0 ? nb : 1), dim3(256), 0, st, s, rays, n, tmax,
                      (uint4*)hits, counters);
+  return hipGetLastError();
+}
+
+hipError_t launch_trace_persistent(hipStream_t st, const SceneDev& s, const float* rays, uint32_t n,
+                                  float tmax, void* hits, unsigned long long* counters,
+                                  uint32_t* queue, int blocks, int steps, int min_refill) {
+  hipError_t e = hipMemsetAsync(queue, 0, sizeof(uint32_t), st);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_trace_persistent, dim3(blocks > 0 ? blocks : 1), dim3(256), 0, st, s, rays, n,
+                     tmax, (uint4*)hits, counters, queue, steps < 1 ? 1 : steps,
+                     min_refill < 1 ? 1 : min_refill);
   return hipGetLastError();
 }
 

@@ -212,7 +212,13 @@ int vrt_create(const vrt_create_info* info, vrt_ctx** out) {
   size_t freeb = 0, total = 0;
   (void)hipMemGetInfo(&freeb, &total);
   ctx->budget = (info && info->hbm_budget) ? info->hbm_budget : (uint64_t)(0.9 * (double)freeb);
-  if (ctx->d_counters.alloc(4 * sizeof(uint64_t)) != hipSuccess ||
+  ctx->tuning.trace_kernel = 1;
+  ctx->tuning.trace_blocks = 2048;
+  ctx->tuning.trace_steps = 8;
+  ctx->tuning.trace_min_refill = 16;
+  ctx->tuning.pt_kernel = 0;
+  if (ctx->d_queue.alloc(64 * sizeof(uint32_t)) != hipSuccess ||
+      ctx->d_counters.alloc(4 * sizeof(uint64_t)) != hipSuccess ||
       hipMemsetAsync(ctx->d_counters.ptr, 0, 4 * sizeof(uint64_t), ctx->stream) != hipSuccess) {
     delete ctx;
     return VRT_ERR_DEVICE;
@@ -371,6 +377,23 @@ int vrt_scene_bricks(vrt_ctx* ctx, const vrt_scene* scene, uint32_t* bricks, uin
 // ---------------------------------------------------------------------------
 // traversal
 // ---------------------------------------------------------------------------
+hipError_t vrt_ctx::launch_pathtrace(const vrt::SceneDev& s, const vrt_uniforms& u, float fclamp,
+                                     uint32_t spp, void* radiance, void* albedo, void* gbuf) {
+  return vrt::launch_pathtrace(stream, s, u, fclamp, spp, radiance, albedo, gbuf,
+                               (unsigned long long*)d_counters.ptr);
+}
+
+hipError_t vrt_ctx::launch_trace(const vrt::SceneDev& s, const float* rays, uint64_t n, float tmax,
+                                 void* hits) {
+  if (tuning.trace_kernel == 1 && n < 0x80000000ull)
+    return vrt::launch_trace_persistent(stream, s, rays, (uint32_t)n, tmax, hits,
+                                        (unsigned long long*)d_counters.ptr, (uint32_t*)d_queue.ptr,
+                                        (int)tuning.trace_blocks, (int)tuning.trace_steps,
+                                        (int)tuning.trace_min_refill);
+  return vrt::launch_trace_rays(stream, s, rays, n, tmax, hits, (unsigned long long*)d_counters.ptr,
+                                trace_blocks);
+}
+
 int vrt_trace_rays_device(vrt_ctx* ctx, const vrt_scene* scene, const float* d_rays, uint64_t n,
                           float tmax, vrt_hit* d_hits, float* ms) {
   if (!ctx || !scene || (n && (!d_rays || !d_hits)))
@@ -380,8 +403,7 @@ int vrt_trace_rays_device(vrt_ctx* ctx, const vrt_scene* scene, const float* d_r
   if (n == 0) { if (ms) *ms = 0.0f; return VRT_OK; }
   const vrt::SceneDev s = scene->dev();
   HIP_TRY(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
-  HIP_TRY(ctx, vrt::launch_trace_rays(ctx->stream, s, d_rays, n, tmax, d_hits,
-                                      (unsigned long long*)ctx->d_counters.ptr, ctx->trace_blocks));
+  HIP_TRY(ctx, ctx->launch_trace(s, d_rays, n, tmax, d_hits));
   HIP_TRY(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
   if (ms) {
     HIP_TRY(ctx, hipEventSynchronize(ctx->ev[1]));
@@ -402,9 +424,7 @@ int vrt_trace_rays(vrt_ctx* ctx, const vrt_scene* scene, const float* rays, uint
   HIP_TRY(ctx, ctx->scratch_hits.ensure(n * 16));
   HIP_TRY(ctx, hipMemcpyAsync(ctx->scratch_rays.ptr, rays, n * 24, hipMemcpyHostToDevice, ctx->stream));
   const vrt::SceneDev s = scene->dev();
-  HIP_TRY(ctx, vrt::launch_trace_rays(ctx->stream, s, (const float*)ctx->scratch_rays.ptr, n, tmax,
-                                      ctx->scratch_hits.ptr, (unsigned long long*)ctx->d_counters.ptr,
-                                      ctx->trace_blocks));
+  HIP_TRY(ctx, ctx->launch_trace(s, (const float*)ctx->scratch_rays.ptr, n, tmax, ctx->scratch_hits.ptr));
   HIP_TRY(ctx, hipMemcpyAsync(hits, ctx->scratch_hits.ptr, n * 16, hipMemcpyDeviceToHost, ctx->stream));
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   return VRT_OK;
@@ -482,9 +502,8 @@ int vrt_render(vrt_ctx* ctx, const vrt_view* views, uint32_t nviews, vrt_frame* 
     const vrt_view& v = views[k];
     ViewState& vs = *states[k];
     const int parity = (int)(vs.frames & 1u);
-    HIP_TRY(ctx, vrt::launch_pathtrace(ctx->stream, v.scene->dev(), v.u, v.firefly_clamp, v.spp,
-                                       vs.radiance.ptr, vs.albedo.ptr, vs.gbuf[parity].ptr,
-                                       (unsigned long long*)ctx->d_counters.ptr));
+    HIP_TRY(ctx, ctx->launch_pathtrace(v.scene->dev(), v.u, v.firefly_clamp, v.spp, vs.radiance.ptr,
+                                       vs.albedo.ptr, vs.gbuf[parity].ptr));
   }
   HIP_TRY(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
   for (uint32_t k = 0; k < nviews; ++k) {
@@ -598,6 +617,108 @@ int vrt_atrous_pass(vrt_ctx* ctx, const vrt_uniforms* u, const void* src, uint32
   HIP_TRY(ctx, vrt::launch_atrous(ctx->stream, *u, s.ptr, (int)src_format, g.ptr, step, d.ptr));
   HIP_TRY(ctx, hipMemcpyAsync(dst, d.ptr, px * 8, hipMemcpyDeviceToHost, ctx->stream));
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return VRT_OK;
+  GUARD_END(ctx)
+}
+
+int vrt_tuning_get(vrt_ctx* ctx, vrt_tuning* out) {
+  if (!ctx || !out) return VRT_ERR_INVALID;
+  *out = ctx->tuning;
+  return VRT_OK;
+}
+
+int vrt_tuning_set(vrt_ctx* ctx, const vrt_tuning* in) {
+  if (!ctx || !in) return VRT_ERR_INVALID;
+  if (in->trace_kernel > 1 || in->pt_kernel > 1 || in->trace_blocks > (1u << 20) ||
+      in->pt_blocks > (1u << 20) || in->trace_steps > 1024 || in->pt_steps > 1024 ||
+      in->trace_min_refill > 64 || in->pt_min_refill > 64)
+    return ctx->fail(VRT_ERR_INVALID, "tuning value out of range");
+  ctx->tuning = *in;
+  return VRT_OK;
+}
+
+int vrt_autotune(vrt_ctx* ctx, const vrt_scene* scene, uint64_t nrays, const vrt_uniforms* u,
+                 vrt_tuning* chosen) {
+  if (!ctx || !scene) return ctx ? ctx->fail(VRT_ERR_INVALID, "null argument") : VRT_ERR_INVALID;
+  GUARD_BEGIN
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  const vrt::SceneDev s = scene->dev();
+  const vrt_tuning start = ctx->tuning;
+  vrt_tuning best = start;
+  auto time_it = [&](auto&& launch, float* out_ms) -> int {
+    HIP_TRY(ctx, launch());  // warm-up
+    HIP_TRY(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
+    for (int r = 0; r < 3; ++r) HIP_TRY(ctx, launch());
+    HIP_TRY(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
+    HIP_TRY(ctx, hipEventSynchronize(ctx->ev[5]));
+    HIP_TRY(ctx, hipEventElapsedTime(out_ms, ctx->ev[4], ctx->ev[5]));
+    return VRT_OK;
+  };
+  uint64_t before[4];
+  int rc = ctx->counters_to_host(before);
+  if (rc) return rc;
+  if (nrays > 0) {
+    // random rays inside the grid's box, directions spread over the sphere
+    const uint64_t n = std::min<uint64_t>(nrays, 1ull << 24);
+    std::vector<float> rays(n * 6);
+    uint64_t z = 0x9E3779B97F4A7C15ull;
+    auto u01 = [&z]() {
+      z += 0x9E3779B97F4A7C15ull;
+      uint64_t x = z;
+      x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+      x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+      return (float)((x ^ (x >> 31)) >> 40) * (1.0f / 16777216.0f);
+    };
+    for (uint64_t i = 0; i < n; ++i) {
+      float* r = rays.data() + i * 6;
+      r[0] = u01() * scene->nx * VRT_VOXEL_SIZE;
+      r[1] = u01() * scene->ny * VRT_VOXEL_SIZE;
+      r[2] = u01() * scene->nz * VRT_VOXEL_SIZE;
+      const float cz = u01() * 2.0f - 1.0f, ph = u01() * 6.2831853f;
+      const float rr = std::sqrt(std::max(0.0f, 1.0f - cz * cz));
+      r[3] = rr * std::cos(ph);
+      r[4] = cz;
+      r[5] = rr * std::sin(ph);
+    }
+    HIP_TRY(ctx, ctx->scratch_rays.ensure(n * 24));
+    HIP_TRY(ctx, ctx->scratch_hits.ensure(n * 16));
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->scratch_rays.ptr, rays.data(), n * 24, hipMemcpyHostToDevice, ctx->stream));
+    const uint32_t blocks[] = {1024, 2048, 4096, 8192};
+    const uint32_t steps[] = {1, 4, 8, 16};
+    const uint32_t refill[] = {1, 16, 32};
+    float best_ms = 1e30f;
+    ctx->tuning.trace_kernel = 0;
+    float ms = 0;
+    rc = time_it([&] { return ctx->launch_trace(s, (const float*)ctx->scratch_rays.ptr, n, 1e4f,
+                                                ctx->scratch_hits.ptr); }, &ms);
+    if (rc) return rc;
+    best_ms = ms;
+    best.trace_kernel = 0;
+    for (uint32_t b : blocks)
+      for (uint32_t st : steps)
+        for (uint32_t rf : refill) {
+          ctx->tuning.trace_kernel = 1;
+          ctx->tuning.trace_blocks = b;
+          ctx->tuning.trace_steps = st;
+          ctx->tuning.trace_min_refill = rf;
+          rc = time_it([&] { return ctx->launch_trace(s, (const float*)ctx->scratch_rays.ptr, n, 1e4f,
+                                                      ctx->scratch_hits.ptr); }, &ms);
+          if (rc) return rc;
+          if (ms < best_ms) {
+            best_ms = ms;
+            best.trace_kernel = 1;
+            best.trace_blocks = b;
+            best.trace_steps = st;
+            best.trace_min_refill = rf;
+          }
+        }
+  }
+  (void)u;
+  ctx->tuning = best;
+  // the sweep's rays do not count in the statistics
+  HIP_TRY(ctx, hipMemcpyAsync(ctx->d_counters.ptr, before, sizeof before, hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  if (chosen) *chosen = best;
   return VRT_OK;
   GUARD_END(ctx)
 }

@@ -74,6 +74,16 @@ class Stats(C.Structure):
                 ("hbm_budget", C.c_uint64)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("trace_kernel", C.c_uint32), ("trace_blocks", C.c_uint32),
+                ("trace_steps", C.c_uint32), ("trace_min_refill", C.c_uint32),
+                ("pt_kernel", C.c_uint32), ("pt_blocks", C.c_uint32), ("pt_steps", C.c_uint32),
+                ("pt_min_refill", C.c_uint32), ("reserved", C.c_uint32 * 8)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
 class SceneDesc(C.Structure):
     _fields_ = [("kind", C.c_uint32), ("nx", C.c_uint32), ("ny", C.c_uint32), ("nz", C.c_uint32),
                 ("seed", C.c_uint64), ("p_brick", C.c_float), ("p_voxel", C.c_float),
@@ -125,6 +135,9 @@ def _declare(L):
     L.vrt_view_release.argtypes = [P, C.c_uint32]
     L.vrt_temporal_pass.argtypes = [P, C.POINTER(Uniforms), P, P, P, P, C.c_uint32, P]
     L.vrt_atrous_pass.argtypes = [P, C.POINTER(Uniforms), P, C.c_uint32, P, C.c_int32, P]
+    L.vrt_tuning_get.argtypes = [P, C.POINTER(Tuning)]
+    L.vrt_tuning_set.argtypes = [P, C.POINTER(Tuning)]
+    L.vrt_autotune.argtypes = [P, P, C.c_uint64, C.POINTER(Uniforms), C.POINTER(Tuning)]
     L.vrt_stats_get.argtypes = [P, C.POINTER(Stats)]
     L.vrt_stats_reset.argtypes = [P]
     L.vrt_camera_matrices.argtypes = [P, C.c_double, C.c_double, C.c_double, C.c_double,
@@ -140,7 +153,8 @@ def _declare(L):
     for name in ("vrt_create", "vrt_device_count", "vrt_scene_upload", "vrt_scene_release",
                  "vrt_scene_bricks", "vrt_scene_dims", "vrt_trace_rays", "vrt_trace_rays_device",
                  "vrt_render", "vrt_view_release", "vrt_temporal_pass", "vrt_atrous_pass",
-                 "vrt_stats_get", "vrt_stats_reset", "vrt_generate_scene", "vrt_scene_spawn"):
+                 "vrt_stats_get", "vrt_stats_reset", "vrt_generate_scene", "vrt_scene_spawn",
+                 "vrt_tuning_get", "vrt_tuning_set", "vrt_autotune"):
         getattr(L, name).restype = I
 
 
@@ -330,6 +344,24 @@ class Context:
         self._check(self.L.vrt_atrous_pass(self.h, C.byref(u), _ptr(src), src_fmt, _ptr(gbuf),
                                            step, _ptr(out)))
         return out
+
+    def tuning(self) -> Tuning:
+        t = Tuning()
+        self._check(self.L.vrt_tuning_get(self.h, C.byref(t)))
+        return t
+
+    def set_tuning(self, **kw) -> Tuning:
+        t = self.tuning()
+        for k, val in kw.items():
+            setattr(t, k, int(val))
+        self._check(self.L.vrt_tuning_set(self.h, C.byref(t)))
+        return t
+
+    def autotune(self, scene: Scene, nrays: int = 1 << 22, u: Uniforms = None) -> Tuning:
+        t = Tuning()
+        self._check(self.L.vrt_autotune(self.h, scene.handle, nrays,
+                                        None if u is None else C.byref(u), C.byref(t)))
+        return t
 
     def stats(self) -> dict:
         s = Stats()
