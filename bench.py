@@ -142,6 +142,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-frames", type=int, default=8)
     ap.add_argument("--workload", default="frame_1080", choices=["frame_1080", "traverse"])
+    ap.add_argument("--pt-kernel", default="auto", choices=["auto", "0", "1"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (--gpus {args.gpus} "
@@ -167,20 +168,27 @@ def main():
         r.set_pose(*cf.world_orbit(desc, i % cf.ORBIT_FRAMES))
         r.frame()
 
+    # launch shapes: the project's autotuner on the first orbit view (outside
+    # the timed region), or a fixed kernel
+    if args.pt_kernel == "auto":
+        r.set_pose(*cf.world_orbit(desc, 0))
+        tuned = ctx.autotune(sc, 0, r.view().u).as_dict()
+    else:
+        tuned = ctx.set_tuning(pt_kernel=int(args.pt_kernel)).as_dict()
+
     for i in range(args.warmup):
         step(i)
-    ctx.stats()  # stream sync: barrier before the timed region
-    s0 = ctx.stats()
-    pt_ms = tp_ms = at_ms = 0.0
+    s0 = ctx.stats()  # stream sync: barrier before the timed region
     t0 = time.perf_counter()
     for k in range(args.steps):
         step(args.warmup + k)
-        st = ctx.stats()
-        pt_ms += st["gpu_ms_pathtrace"]
-        tp_ms += st["gpu_ms_temporal"]
-        at_ms += st["gpu_ms_atrous"]
-    s1 = ctx.stats()  # stream sync: barrier after
+    s1 = ctx.stats()  # stream sync: barrier after (also reads the frames' HIP events)
     wall = time.perf_counter() - t0
+    assert s1["timed_calls"] - s0["timed_calls"] == args.steps
+    pt_ms = s1["sum_ms_pathtrace"] - s0["sum_ms_pathtrace"]
+    tp_ms = s1["sum_ms_temporal"] - s0["sum_ms_temporal"]
+    at_ms = s1["sum_ms_atrous"] - s0["sum_ms_atrous"]
+    fr_ms = s1["sum_ms_frame"] - s0["sum_ms_frame"]
 
     K = args.steps
     rays = s1["rays"] - s0["rays"]
@@ -205,8 +213,9 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": _pmc_traffic("pathtrace_frame1080_pmc.txt"),
                      "alg_bytes_per_launch": int(alg_bytes), "avg_launch_ms": round(pt_avg_ms, 4)},
+        "tuning": {k: tuned[k] for k in ("pt_kernel", "pt_blocks", "pt_steps", "pt_min_refill", "pt_chunk")},
         "breakdown": {"gpu_ms_pathtrace": round(pt_avg_ms, 4), "gpu_ms_temporal": round(tp_ms / K, 4),
-                      "gpu_ms_atrous3": round(at_ms / K, 4),
+                      "gpu_ms_atrous3": round(at_ms / K, 4), "gpu_ms_frame": round(fr_ms / K, 4),
                       "rays_per_frame": rays // K, "brick_steps_per_ray": round(bsteps / max(rays, 1), 2),
                       "voxel_steps_per_ray": round(vsteps / max(rays, 1), 2)},
     }

@@ -151,6 +151,13 @@ typedef struct vrt_stats {
   double   gpu_ms_frame;       /* pathtrace start .. last a-trous end */
   uint64_t hbm_resident;       /* bytes held by scenes + view state */
   uint64_t hbm_budget;
+  /* sums over every vrt_render call since create / reset (HIP events on the
+   * context's stream, read back lazily so rendering never waits on them) */
+  uint64_t timed_calls;
+  double   sum_ms_pathtrace;
+  double   sum_ms_temporal;
+  double   sum_ms_atrous;
+  double   sum_ms_frame;
 } vrt_stats;
 
 /* ---- context -------------------------------------------------------------- */
@@ -215,7 +222,8 @@ typedef struct vrt_tuning {
   uint32_t pt_blocks;
   uint32_t pt_steps;
   uint32_t pt_min_refill;
-  uint32_t reserved[8];
+  uint32_t pt_chunk;          /* pixels a wave takes per queue atomic */
+  uint32_t reserved[7];
 } vrt_tuning;
 int vrt_tuning_get(vrt_ctx* ctx, vrt_tuning* out);
 int vrt_tuning_set(vrt_ctx* ctx, const vrt_tuning* in);

@@ -184,6 +184,20 @@ def frame128(ctx):
     sc.release()
 
 
+PT_VARIANTS = [dict(pt_kernel=0),
+               dict(pt_kernel=1, pt_blocks=1024, pt_steps=8, pt_min_refill=32, pt_chunk=64),
+               dict(pt_kernel=1, pt_blocks=97, pt_steps=1, pt_min_refill=1, pt_chunk=64),
+               dict(pt_kernel=1, pt_blocks=2048, pt_steps=16, pt_min_refill=64, pt_chunk=256)]
+
+
+@pytest.fixture(params=range(len(PT_VARIANTS)), ids=["lane", "persist", "persist-s1", "persist-s16"])
+def pt_variant(request, ctx):
+    old = ctx.tuning().as_dict()
+    ctx.set_tuning(**PT_VARIANTS[request.param])
+    yield PT_VARIANTS[request.param]
+    ctx.set_tuning(**old)
+
+
 def _render_one(ctx, sc, view_id, u, fclamp, spp, w, h, accum_format=0):
     view = v.View(sc.handle, u, fclamp, view_id, spp, accum_format)
     outs = {"radiance": np.zeros((h, w, 4), np.uint16), "albedo": np.zeros((h, w, 4), np.uint8),
@@ -193,7 +207,7 @@ def _render_one(ctx, sc, view_id, u, fclamp, spp, w, h, accum_format=0):
     return outs, fr[0]
 
 
-def test_pathtrace_frame128_bitexact(ctx, frame128):
+def test_pathtrace_frame128_bitexact(ctx, frame128, pt_variant):
     vox, sc, og = frame128
     pos, yaw, pitch = cf.frame_128_pose()
     p = cf.FRAME_128
@@ -209,7 +223,7 @@ def test_pathtrace_frame128_bitexact(ctx, frame128):
 
 
 @pytest.mark.parametrize("bounces,spp,fclamp,frame", [(4, 3, 0.0, 7), (0, 1, 48.0, 123456789), (6, 2, 10.0, 3)])
-def test_pathtrace_variants_bitexact(ctx, frame128, bounces, spp, fclamp, frame):
+def test_pathtrace_variants_bitexact(ctx, frame128, bounces, spp, fclamp, frame, pt_variant):
     """Russian roulette (bounce >= 2), several samples, clamp off/on, seeds."""
     vox, sc, og = frame128
     pos, yaw, pitch = cf.frame_128_pose()
@@ -267,7 +281,37 @@ def test_filter_passes_bitexact(ctx, frame128, fmt):
         src = want_d
 
 
-def test_frame_sequence_bitexact(ctx):
+def test_pathtrace_indirect_only_bitexact(ctx, frame128, pt_variant):
+    """VRT_FLAG_INDIRECT (used by the VoxelBench adapter)."""
+    vox, sc, og = frame128
+    pos, yaw, pitch = cf.frame_128_pose()
+    w, h = 128, 72
+    u, _ = v.make_uniforms(pos, yaw, pitch, w, h, frame=11, bounces=2, flags=4)
+    outs, fr = _render_one(ctx, sc, 903, u, 48.0, 2, w, h)
+    rad, alb, gb, cnt = oracle.pathtrace(og, to_oracle_uniforms(u), 48.0, 2)
+    assert_bits_equal("radiance", outs["radiance"], rad)
+    assert_bits_equal("gbuf", outs["gbuf"], gb)
+    assert fr.rays == cnt.rays
+    # with one sample, a primary miss contributes nothing (no sky)
+    rad, alb, gb, cnt = oracle.pathtrace(og, to_oracle_uniforms(u), 48.0, 1)
+    sky = gb[..., 3] < 0
+    assert sky.any() and np.all(f16(rad)[sky][..., :3] == 0)
+
+
+def test_autotune_pathtrace_keeps_results(ctx, frame128):
+    vox, sc, og = frame128
+    pos, yaw, pitch = cf.frame_128_pose()
+    w, h = 160, 96
+    u, _ = v.make_uniforms(pos, yaw, pitch, w, h, frame=2, bounces=3)
+    old = ctx.tuning().as_dict()
+    t = ctx.autotune(sc, 0, u)
+    outs, fr = _render_one(ctx, sc, 904, u, 48.0, 1, w, h)
+    rad, alb, gb, cnt = oracle.pathtrace(og, to_oracle_uniforms(u), 48.0, 1)
+    assert_bits_equal(f"radiance (tuned {t.as_dict()})", outs["radiance"], rad)
+    ctx.set_tuning(**old)
+
+
+def test_frame_sequence_bitexact(ctx, pt_variant):
     """Five frames of an orbit through the whole chain with temporal and
     denoise on: every target of every frame matches the oracle chain."""
     desc = dict(cf.FRAME_1080_SCENE)

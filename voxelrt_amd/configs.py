@@ -15,6 +15,7 @@
 """
 from __future__ import annotations
 
+import functools
 import math
 
 import numpy as np
@@ -78,10 +79,16 @@ def frame_1080_scene():
     return generate_scene(**FRAME_1080_SCENE)
 
 
+@functools.lru_cache(maxsize=64)
+def _spawn(items: tuple):
+    pos, _, _ = scene_spawn(**dict(items))
+    return tuple(float(x) for x in pos)
+
+
 def world_orbit(desc: dict, i: int, n: int = ORBIT_FRAMES):
     """Pose i of an n-frame orbit around the room of a WORLD scene."""
     from .renderer import orbit_pose
-    spawn, _, _ = scene_spawn(**desc)
+    spawn = _spawn(tuple(sorted(desc.items())))
     N = desc["n"]
     room = desc.get("room", 0) or N // 4
     center = (spawn[0], spawn[1], (N * 0.55) * VOXEL_SIZE)

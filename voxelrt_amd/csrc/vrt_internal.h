@@ -71,7 +71,18 @@ struct vrt_ctx {
   uint64_t budget = 0, resident = 0;
   uint64_t clock = 0;
   uint64_t frames = 0;
-  double ms_pathtrace = 0, ms_temporal = 0, ms_atrous = 0, ms_frame = 0;
+  double ms_pathtrace = 0, ms_temporal = 0, ms_atrous = 0, ms_frame = 0;  // last render call
+  // per-render-call timing events (pathtrace start, temporal start, a-trous
+  // start, end), read back lazily by harvest()
+  static constexpr uint32_t RING = 64;
+  struct TimingSlot {
+    hipEvent_t e[4] = {};
+  };
+  TimingSlot ring[RING];
+  uint64_t ring_head = 0, ring_count = 0;
+  uint64_t timed_calls = 0;
+  double sum_pathtrace = 0, sum_temporal = 0, sum_atrous = 0, sum_frame = 0;
+  int harvest(uint32_t max_slots);  // oldest pending first; waits for each
   int trace_blocks = 0;  // 0 = default grid for k_trace_rays
   vrt_tuning tuning{};   // kernel variants + launch shapes (vrt_tuning_set / vrt_autotune)
   vrt::DeviceBuffer d_counters;  // u64 x4: rays, brick steps, voxel steps, hits

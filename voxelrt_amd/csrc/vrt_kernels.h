@@ -26,6 +26,10 @@ hipError_t launch_trace_persistent(hipStream_t st, const SceneDev& s, const floa
 hipError_t launch_pathtrace(hipStream_t st, const SceneDev& s, const Uniforms& u, float fclamp,
                             uint32_t spp, void* radiance, void* albedo, void* gbuf,
                             unsigned long long* counters);
+hipError_t launch_pathtrace_persistent(hipStream_t st, const SceneDev& s, const Uniforms& u, float fclamp,
+                                       uint32_t spp, void* radiance, void* albedo, void* gbuf,
+                                       unsigned long long* counters, uint32_t* queue, int blocks,
+                                       int steps, int min_refill, int chunk);
 hipError_t launch_temporal(hipStream_t st, const Uniforms& u, const void* radiance,
                            const void* gcur, const void* aprev, const void* gprev, void* aout,
                            int fmt);

@@ -602,6 +602,8 @@ __global__ __launch_bounds__(256) void k_hdr(uint32_t npx, const void* __restric
 
 }  // namespace vrt
 
+#include "vrt_pathtrace_persistent.h"
+
 // ---------------------------------------------------------------------------
 // host-side launchers (declared in vrt_kernels.h)
 // ---------------------------------------------------------------------------
@@ -661,6 +663,19 @@ hipError_t launch_pathtrace(hipStream_t st, const SceneDev& s, const Uniforms& u
   const uint32_t tiles = div_up(w, 16) * div_up(h, 16);
   hipLaunchKernelGGL(k_pathtrace, dim3(tiles), dim3(256), 0, st, s, u, fclamp, spp,
                      (ushort4*)radiance, (uchar4*)albedo, (float4*)gbuf, counters);
+  return hipGetLastError();
+}
+
+hipError_t launch_pathtrace_persistent(hipStream_t st, const SceneDev& s, const Uniforms& u, float fclamp,
+                                       uint32_t spp, void* radiance, void* albedo, void* gbuf,
+                                       unsigned long long* counters, uint32_t* queue, int blocks,
+                                       int steps, int min_refill, int chunk) {
+  hipError_t e = hipMemsetAsync(queue, 0, sizeof(uint32_t), st);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_pathtrace_persistent, dim3(blocks > 0 ? blocks : 1), dim3(256), 0, st, s, u, fclamp,
+                     spp, (ushort4*)radiance, (uchar4*)albedo, (float4*)gbuf, counters, queue,
+                     steps < 1 ? 1 : steps, min_refill < 1 ? 1 : min_refill,
+                     (uint32_t)(chunk < 64 ? 64 : chunk));
   return hipGetLastError();
 }
 

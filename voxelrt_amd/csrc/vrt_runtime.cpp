@@ -209,6 +209,12 @@ int vrt_create(const vrt_create_info* info, vrt_ctx** out) {
       delete ctx;
       return VRT_ERR_DEVICE;
     }
+  for (auto& slot : ctx->ring)
+    for (auto& e : slot.e)
+      if (hipEventCreate(&e) != hipSuccess) {
+        delete ctx;
+        return VRT_ERR_DEVICE;
+      }
   size_t freeb = 0, total = 0;
   (void)hipMemGetInfo(&freeb, &total);
   ctx->budget = (info && info->hbm_budget) ? info->hbm_budget : (uint64_t)(0.9 * (double)freeb);
@@ -217,6 +223,10 @@ int vrt_create(const vrt_create_info* info, vrt_ctx** out) {
   ctx->tuning.trace_steps = 8;
   ctx->tuning.trace_min_refill = 16;
   ctx->tuning.pt_kernel = 0;
+  ctx->tuning.pt_blocks = 1024;
+  ctx->tuning.pt_steps = 8;
+  ctx->tuning.pt_min_refill = 32;
+  ctx->tuning.pt_chunk = 64;
   if (ctx->d_queue.alloc(64 * sizeof(uint32_t)) != hipSuccess ||
       ctx->d_counters.alloc(4 * sizeof(uint64_t)) != hipSuccess ||
       hipMemsetAsync(ctx->d_counters.ptr, 0, 4 * sizeof(uint64_t), ctx->stream) != hipSuccess) {
@@ -379,6 +389,11 @@ int vrt_scene_bricks(vrt_ctx* ctx, const vrt_scene* scene, uint32_t* bricks, uin
 // ---------------------------------------------------------------------------
 hipError_t vrt_ctx::launch_pathtrace(const vrt::SceneDev& s, const vrt_uniforms& u, float fclamp,
                                      uint32_t spp, void* radiance, void* albedo, void* gbuf) {
+  if (tuning.pt_kernel == 1)
+    return vrt::launch_pathtrace_persistent(stream, s, u, fclamp, spp, radiance, albedo, gbuf,
+                                            (unsigned long long*)d_counters.ptr, (uint32_t*)d_queue.ptr + 1,
+                                            (int)tuning.pt_blocks, (int)tuning.pt_steps,
+                                            (int)tuning.pt_min_refill, (int)tuning.pt_chunk);
   return vrt::launch_pathtrace(stream, s, u, fclamp, spp, radiance, albedo, gbuf,
                                (unsigned long long*)d_counters.ptr);
 }
@@ -497,7 +512,16 @@ int vrt_render(vrt_ctx* ctx, const vrt_view* views, uint32_t nviews, vrt_frame* 
     int rc = ctx->counters_to_host(before);
     if (rc) return rc;
   }
-  HIP_TRY(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+  // timing events go into a ring and are read later (vrt_stats_get), so a
+  // frame without host outputs never waits for the device
+  if (ctx->ring_count == vrt_ctx::RING) {
+    int rc = ctx->harvest(1);
+    if (rc) return rc;
+  }
+  hipEvent_t* ev = ctx->ring[(ctx->ring_head) % vrt_ctx::RING].e;
+  ctx->ring_head++;
+  ctx->ring_count++;
+  HIP_TRY(ctx, hipEventRecord(ev[0], ctx->stream));
   for (uint32_t k = 0; k < nviews; ++k) {
     const vrt_view& v = views[k];
     ViewState& vs = *states[k];
@@ -505,7 +529,7 @@ int vrt_render(vrt_ctx* ctx, const vrt_view* views, uint32_t nviews, vrt_frame* 
     HIP_TRY(ctx, ctx->launch_pathtrace(v.scene->dev(), v.u, v.firefly_clamp, v.spp, vs.radiance.ptr,
                                        vs.albedo.ptr, vs.gbuf[parity].ptr));
   }
-  HIP_TRY(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
+  HIP_TRY(ctx, hipEventRecord(ev[1], ctx->stream));
   for (uint32_t k = 0; k < nviews; ++k) {
     const vrt_view& v = views[k];
     ViewState& vs = *states[k];
@@ -514,7 +538,7 @@ int vrt_render(vrt_ctx* ctx, const vrt_view* views, uint32_t nviews, vrt_frame* 
                                       vs.accum[1 - parity].ptr, vs.gbuf[1 - parity].ptr,
                                       vs.accum[parity].ptr, (int)vs.fmt));
   }
-  HIP_TRY(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
+  HIP_TRY(ctx, hipEventRecord(ev[2], ctx->stream));
   for (uint32_t k = 0; k < nviews; ++k) {
     const vrt_view& v = views[k];
     ViewState& vs = *states[k];
@@ -526,23 +550,18 @@ int vrt_render(vrt_ctx* ctx, const vrt_view* views, uint32_t nviews, vrt_frame* 
     HIP_TRY(ctx, vrt::launch_atrous(ctx->stream, v.u, vs.den[0].ptr, 0, vs.gbuf[parity].ptr, 2, vs.den[1].ptr));
     HIP_TRY(ctx, vrt::launch_atrous(ctx->stream, v.u, vs.den[1].ptr, 0, vs.gbuf[parity].ptr, 4, vs.den[0].ptr));
   }
-  HIP_TRY(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
+  HIP_TRY(ctx, hipEventRecord(ev[3], ctx->stream));
   for (uint32_t k = 0; k < nviews && outs; ++k) {
     ViewState& vs = *states[k];
     const int parity = (int)(vs.frames & 1u);
     int rc = copy_out(ctx, vs, views[k], &outs[k], parity, (views[k].u.params0[2] & VRT_FLAG_DENOISE) != 0);
     if (rc) return rc;
   }
-  HIP_TRY(ctx, hipEventSynchronize(ctx->ev[3]));
-  float t01 = 0, t12 = 0, t23 = 0, t03 = 0;
-  HIP_TRY(ctx, hipEventElapsedTime(&t01, ctx->ev[0], ctx->ev[1]));
-  HIP_TRY(ctx, hipEventElapsedTime(&t12, ctx->ev[1], ctx->ev[2]));
-  HIP_TRY(ctx, hipEventElapsedTime(&t23, ctx->ev[2], ctx->ev[3]));
-  HIP_TRY(ctx, hipEventElapsedTime(&t03, ctx->ev[0], ctx->ev[3]));
-  ctx->ms_pathtrace = t01;
-  ctx->ms_temporal = t12;
-  ctx->ms_atrous = t23;
-  ctx->ms_frame = t03;
+  if (outs) {
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    int rc = ctx->harvest(vrt_ctx::RING);
+    if (rc) return rc;
+  }
   for (uint32_t k = 0; k < nviews; ++k) states[k]->frames++;
   ctx->frames += nviews;
   if (want_counts) {
@@ -631,7 +650,7 @@ int vrt_tuning_set(vrt_ctx* ctx, const vrt_tuning* in) {
   if (!ctx || !in) return VRT_ERR_INVALID;
   if (in->trace_kernel > 1 || in->pt_kernel > 1 || in->trace_blocks > (1u << 20) ||
       in->pt_blocks > (1u << 20) || in->trace_steps > 1024 || in->pt_steps > 1024 ||
-      in->trace_min_refill > 64 || in->pt_min_refill > 64)
+      in->trace_min_refill > 64 || in->pt_min_refill > 64 || in->pt_chunk > (1u << 16))
     return ctx->fail(VRT_ERR_INVALID, "tuning value out of range");
   ctx->tuning = *in;
   return VRT_OK;
@@ -713,7 +732,47 @@ int vrt_autotune(vrt_ctx* ctx, const vrt_scene* scene, uint64_t nrays, const vrt
           }
         }
   }
-  (void)u;
+  if (u != nullptr) {
+    // path tracer on the given view (its size, pose and bounce count)
+    const uint64_t px = (uint64_t)u->params1[0] * (uint64_t)u->params1[1];
+    if (px == 0 || px > (1ull << 26)) return ctx->fail(VRT_ERR_INVALID, "autotune view size");
+    vrt::DeviceBuffer rad, alb, gb;
+    HIP_TRY(ctx, rad.alloc(px * 8));
+    HIP_TRY(ctx, alb.alloc(px * 4));
+    HIP_TRY(ctx, gb.alloc(px * 16));
+    auto launch = [&] { return ctx->launch_pathtrace(s, *u, 48.0f, 1, rad.ptr, alb.ptr, gb.ptr); };
+    ctx->tuning = best;
+    ctx->tuning.pt_kernel = 0;
+    float ms = 0;
+    rc = time_it(launch, &ms);
+    if (rc) return rc;
+    float best_ms = ms;
+    best.pt_kernel = 0;
+    const uint32_t blocks[] = {512, 1024, 2048};
+    const uint32_t steps[] = {4, 8, 16};
+    const uint32_t refill[] = {16, 32, 48};
+    const uint32_t chunks[] = {64, 256};
+    for (uint32_t b : blocks)
+      for (uint32_t st : steps)
+        for (uint32_t rf : refill)
+          for (uint32_t ch : chunks) {
+            ctx->tuning.pt_kernel = 1;
+            ctx->tuning.pt_blocks = b;
+            ctx->tuning.pt_steps = st;
+            ctx->tuning.pt_min_refill = rf;
+            ctx->tuning.pt_chunk = ch;
+            rc = time_it(launch, &ms);
+            if (rc) return rc;
+            if (ms < best_ms) {
+              best_ms = ms;
+              best.pt_kernel = 1;
+              best.pt_blocks = b;
+              best.pt_steps = st;
+              best.pt_min_refill = rf;
+              best.pt_chunk = ch;
+            }
+          }
+  }
   ctx->tuning = best;
   // the sweep's rays do not count in the statistics
   HIP_TRY(ctx, hipMemcpyAsync(ctx->d_counters.ptr, before, sizeof before, hipMemcpyHostToDevice, ctx->stream));
@@ -730,7 +789,14 @@ int vrt_stats_get(vrt_ctx* ctx, vrt_stats* out) {
   uint64_t c[4];
   int rc = ctx->counters_to_host(c);
   if (rc) return rc;
+  rc = ctx->harvest(vrt_ctx::RING);
+  if (rc) return rc;
   std::memset(out, 0, sizeof(*out));
+  out->timed_calls = ctx->timed_calls;
+  out->sum_ms_pathtrace = ctx->sum_pathtrace;
+  out->sum_ms_temporal = ctx->sum_temporal;
+  out->sum_ms_atrous = ctx->sum_atrous;
+  out->sum_ms_frame = ctx->sum_frame;
   out->rays = c[0];
   out->brick_steps = c[1];
   out->voxel_steps = c[2];
@@ -753,11 +819,40 @@ int vrt_stats_reset(vrt_ctx* ctx) {
   HIP_TRY(ctx, hipMemsetAsync(ctx->d_counters.ptr, 0, 4 * sizeof(uint64_t), ctx->stream));
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   ctx->frames = 0;
+  {
+    int rc = ctx->harvest(vrt_ctx::RING);
+    if (rc) return rc;
+  }
+  ctx->timed_calls = 0;
+  ctx->sum_pathtrace = ctx->sum_temporal = ctx->sum_atrous = ctx->sum_frame = 0.0;
   return VRT_OK;
   GUARD_END(ctx)
 }
 
 }  // extern "C"
+
+int vrt_ctx::harvest(uint32_t max_slots) {
+  while (ring_count > 0 && max_slots-- > 0) {
+    const TimingSlot& sl = ring[(ring_head - ring_count) % RING];
+    HIP_TRY(this, hipEventSynchronize(sl.e[3]));
+    float t01 = 0, t12 = 0, t23 = 0, t03 = 0;
+    HIP_TRY(this, hipEventElapsedTime(&t01, sl.e[0], sl.e[1]));
+    HIP_TRY(this, hipEventElapsedTime(&t12, sl.e[1], sl.e[2]));
+    HIP_TRY(this, hipEventElapsedTime(&t23, sl.e[2], sl.e[3]));
+    HIP_TRY(this, hipEventElapsedTime(&t03, sl.e[0], sl.e[3]));
+    ms_pathtrace = t01;
+    ms_temporal = t12;
+    ms_atrous = t23;
+    ms_frame = t03;
+    sum_pathtrace += t01;
+    sum_temporal += t12;
+    sum_atrous += t23;
+    sum_frame += t03;
+    ++timed_calls;
+    --ring_count;
+  }
+  return VRT_OK;
+}
 
 vrt_ctx::~vrt_ctx() {
   for (vrt_scene* s : scenes) delete s;
@@ -765,5 +860,8 @@ vrt_ctx::~vrt_ctx() {
   views.clear();
   for (auto& e : ev)
     if (e) (void)hipEventDestroy(e);
+  for (auto& slot : ring)
+    for (auto& e : slot.e)
+      if (e) (void)hipEventDestroy(e);
   if (stream) (void)hipStreamDestroy(stream);
 }

@@ -71,14 +71,17 @@ class Stats(C.Structure):
                 ("hits", C.c_uint64), ("frames", C.c_uint64), ("gpu_ms_pathtrace", C.c_double),
                 ("gpu_ms_temporal", C.c_double), ("gpu_ms_atrous", C.c_double),
                 ("gpu_ms_frame", C.c_double), ("hbm_resident", C.c_uint64),
-                ("hbm_budget", C.c_uint64)]
+                ("hbm_budget", C.c_uint64), ("timed_calls", C.c_uint64),
+                ("sum_ms_pathtrace", C.c_double), ("sum_ms_temporal", C.c_double),
+                ("sum_ms_atrous", C.c_double), ("sum_ms_frame", C.c_double)]
 
 
 class Tuning(C.Structure):
     _fields_ = [("trace_kernel", C.c_uint32), ("trace_blocks", C.c_uint32),
                 ("trace_steps", C.c_uint32), ("trace_min_refill", C.c_uint32),
                 ("pt_kernel", C.c_uint32), ("pt_blocks", C.c_uint32), ("pt_steps", C.c_uint32),
-                ("pt_min_refill", C.c_uint32), ("reserved", C.c_uint32 * 8)]
+                ("pt_min_refill", C.c_uint32), ("pt_chunk", C.c_uint32),
+                ("reserved", C.c_uint32 * 7)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
