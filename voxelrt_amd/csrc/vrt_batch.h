@@ -1,0 +1,43 @@
+// vrt_batch.h -- the batched form: many views, of different scenes, sizes,
+// bounce counts and history formats, rendered by one launch per pass.
+//
+// The host writes one ViewDesc per view into a device table.  Every pass
+// tiles each view into 16x16 pixel tiles; tile0 is the view's first tile in
+// the batch (an exclusive prefix over the views' tile counts).  A block finds
+// its view by binary search over tile0 -- the index is uniform across the
+// block, so the descriptor is read with scalar loads -- and then runs exactly
+// the per-pixel code of the single-view kernels: results are bit-identical
+// to rendering each view alone.
+#pragma once
+#include "vrt_trace.h"
+
+namespace vrt {
+
+struct ViewDesc {
+  SceneDev s;
+  vrt_uniforms u;
+  float fclamp;
+  uint32_t spp, fmt, tile0;
+  uint32_t w, h, tiles_x, denoise;
+  void* radiance;    // rgba16f
+  void* albedo;      // rgba8unorm
+  void* gbuf_cur;    // rgba32f
+  void* gbuf_prev;
+  void* accum_prev;  // rgba16f | rgba32f
+  void* accum_cur;
+  void* den0;        // rgba16f
+  void* den1;
+};
+
+// Index of the view owning tile `b` (views sorted by tile0, tile0[0] = 0).
+VRT_D uint32_t find_view(const ViewDesc* __restrict__ views, uint32_t nviews, uint32_t b) {
+  uint32_t lo = 0, hi = nviews;  // invariant: views[lo].tile0 <= b < views[hi].tile0
+  while (hi - lo > 1u) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (views[mid].tile0 <= b) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+}  // namespace vrt

@@ -83,6 +83,15 @@ struct vrt_ctx {
   uint64_t timed_calls = 0;
   double sum_pathtrace = 0, sum_temporal = 0, sum_atrous = 0, sum_frame = 0;
   int harvest(uint32_t max_slots);  // oldest pending first; waits for each
+  // batched form: pinned staging ring for the view-descriptor table and its
+  // device copy (vrt_batch.h)
+  static constexpr uint32_t STAGES = 4;
+  void* desc_host[STAGES] = {};
+  hipEvent_t desc_done[STAGES] = {};
+  uint32_t desc_cap = 0;  // views per staging buffer
+  uint32_t desc_next = 0;
+  vrt::DeviceBuffer desc_dev;
+  int render_batch(const vrt_view* views, uint32_t nviews, ViewState** states, hipEvent_t* ev);
   int trace_blocks = 0;  // 0 = default grid for k_trace_rays
   vrt_tuning tuning{};   // kernel variants + launch shapes (vrt_tuning_set / vrt_autotune)
   vrt::DeviceBuffer d_counters;  // u64 x4: rays, brick steps, voxel steps, hits

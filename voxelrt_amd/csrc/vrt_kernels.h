@@ -30,6 +30,11 @@ hipError_t launch_pathtrace_persistent(hipStream_t st, const SceneDev& s, const 
                                        uint32_t spp, void* radiance, void* albedo, void* gbuf,
                                        unsigned long long* counters, uint32_t* queue, int blocks,
                                        int steps, int min_refill, int chunk);
+struct ViewDesc;
+hipError_t launch_pathtrace_batch(hipStream_t st, const ViewDesc* views, uint32_t nviews, uint32_t tiles,
+                                  unsigned long long* counters);
+hipError_t launch_temporal_batch(hipStream_t st, const ViewDesc* views, uint32_t nviews, uint32_t tiles);
+hipError_t launch_atrous_batch(hipStream_t st, const ViewDesc* views, uint32_t nviews, uint32_t tiles, int pass);
 hipError_t launch_temporal(hipStream_t st, const Uniforms& u, const void* radiance,
                            const void* gcur, const void* aprev, const void* gprev, void* aout,
                            int fmt);

@@ -18,6 +18,7 @@
 #include "vrt_kernels.h"
 #include "vrt_trace.h"
 #include "vrt_dda_state.h"
+#include "vrt_batch.h"
 
 namespace vrt {
 
@@ -383,24 +384,20 @@ VRT_D PixelOut shade_pixel(const SceneDev& s, const Uniforms& u, float fclamp, u
 
 // Pixel p of a launch covering TILE x TILE tiles, row-major tiles: keeps the
 // 64 lanes of a wave on a compact 16x4 patch (coherent primary rays).
-VRT_D bool tile_pixel(uint32_t w, uint32_t h, uint32_t& px, uint32_t& py) {
+VRT_D bool tile_pixel(uint32_t w, uint32_t h, uint32_t tile, uint32_t& px, uint32_t& py) {
   const uint32_t tiles_x = (w + 15u) >> 4;
-  const uint32_t tile = blockIdx.x;
   px = (tile % tiles_x) * 16u + (threadIdx.x & 15u);
   py = (tile / tiles_x) * 16u + (threadIdx.x >> 4);
   return px < w && py < h;
 }
 
-__global__ __launch_bounds__(256) void k_pathtrace(SceneDev s, Uniforms u, float fclamp,
-                                                   uint32_t spp, ushort4* __restrict__ radiance,
-                                                   uchar4* __restrict__ albedo,
-                                                   float4* __restrict__ gbuf,
-                                                   unsigned long long* __restrict__ counters) {
+// One 16x16 tile of pathtrace.wgsl:248-330 (one pixel per lane).
+VRT_D void pathtrace_tile(const SceneDev& s, const Uniforms& u, float fclamp, uint32_t spp, uint32_t tile,
+                          ushort4* __restrict__ radiance, uchar4* __restrict__ albedo,
+                          float4* __restrict__ gbuf, RayCount& rc) {
   const uint32_t w = (uint32_t)u.params1[0], h = (uint32_t)u.params1[1];
   uint32_t px, py;
-  const bool inb = tile_pixel(w, h, px, py);
-  RayCount rc{0u, 0u, 0u, 0u};
-  if (inb) {
+  if (tile_pixel(w, h, tile, px, py)) {
     const uint64_t i = (uint64_t)py * w + px;
     f3 acc = mk3(0.0f, 0.0f, 0.0f);
     PixelOut first;
@@ -416,6 +413,15 @@ __global__ __launch_bounds__(256) void k_pathtrace(SceneDev s, Uniforms u, float
                             (uint8_t)f32_to_unorm8(first.albedo_sqrt.z), 255);
     gbuf[i] = first.gbuf;
   }
+}
+
+__global__ __launch_bounds__(256) void k_pathtrace(SceneDev s, Uniforms u, float fclamp,
+                                                   uint32_t spp, ushort4* __restrict__ radiance,
+                                                   uchar4* __restrict__ albedo,
+                                                   float4* __restrict__ gbuf,
+                                                   unsigned long long* __restrict__ counters) {
+  RayCount rc{0u, 0u, 0u, 0u};
+  pathtrace_tile(s, u, fclamp, spp, blockIdx.x, radiance, albedo, gbuf, rc);
   count4(counters, rc.rays, rc.bsteps, rc.vsteps, rc.hits);
 }
 
@@ -438,14 +444,12 @@ VRT_D void store_accum(void* p, uint64_t i, float4 v) {
 }
 
 template <int FMT>
-__global__ __launch_bounds__(256) void k_temporal(Uniforms u, const ushort4* __restrict__ radiance,
-                                                  const float4* __restrict__ gcur,
-                                                  const void* __restrict__ aprev,
-                                                  const float4* __restrict__ gprev,
-                                                  void* __restrict__ aout) {
+VRT_D void temporal_tile(const Uniforms& u, uint32_t tile, const ushort4* __restrict__ radiance,
+                         const float4* __restrict__ gcur, const void* __restrict__ aprev,
+                         const float4* __restrict__ gprev, void* __restrict__ aout) {
   const uint32_t w = (uint32_t)u.params1[0], h = (uint32_t)u.params1[1];
   uint32_t px, py;
-  if (!tile_pixel(w, h, px, py)) return;
+  if (!tile_pixel(w, h, tile, px, py)) return;
   const uint64_t i = (uint64_t)py * w + px;
   const ushort4 rv = radiance[i];
   const f3 cur = mk3(f16_to_f32(rv.x), f16_to_f32(rv.y), f16_to_f32(rv.z));
@@ -506,23 +510,38 @@ __global__ __launch_bounds__(256) void k_temporal(Uniforms u, const ushort4* __r
   store_accum<FMT>(aout, i, make_float4(color.x, color.y, color.z, history));
 }
 
+template <int FMT>
+__global__ __launch_bounds__(256) void k_temporal(Uniforms u, const ushort4* __restrict__ radiance,
+                                                  const float4* __restrict__ gcur,
+                                                  const void* __restrict__ aprev,
+                                                  const float4* __restrict__ gprev,
+                                                  void* __restrict__ aout) {
+  temporal_tile<FMT>(u, blockIdx.x, radiance, gcur, aprev, gprev, aout);
+}
+
 // ---------------------------------------------------------------------------
 // atrous.wgsl:26-83.  A 16x16 tile stages its (16+4*step)^2 neighbourhood of
 // colour (as f32x4) and G-buffer in LDS; every tap then reads LDS.
 // ---------------------------------------------------------------------------
+template <int STEP>
+struct AtrousLds {
+  static constexpr int T = 16, HALO = 2 * STEP, S = T + 2 * HALO;
+  float4 c[S * S];
+  float4 g[S * S];
+};
+
 template <int FMT, int STEP>
-__global__ __launch_bounds__(256) void k_atrous(Uniforms u, const void* __restrict__ src,
-                                                const float4* __restrict__ gbuf,
-                                                ushort4* __restrict__ dst) {
+VRT_D void atrous_tile(const Uniforms& u, uint32_t tile, const void* __restrict__ src,
+                       const float4* __restrict__ gbuf, ushort4* __restrict__ dst, AtrousLds<STEP>& lds) {
   constexpr int T = 16;
   constexpr int HALO = 2 * STEP;
   constexpr int S = T + 2 * HALO;
-  __shared__ float4 sc[S * S];
-  __shared__ float4 sg[S * S];
+  float4* sc = lds.c;
+  float4* sg = lds.g;
   const int w = (int)u.params1[0], h = (int)u.params1[1];
   const int tiles_x = (w + T - 1) / T;
-  const int ox = (int)(blockIdx.x % (uint32_t)tiles_x) * T - HALO;
-  const int oy = (int)(blockIdx.x / (uint32_t)tiles_x) * T - HALO;
+  const int ox = (int)(tile % (uint32_t)tiles_x) * T - HALO;
+  const int oy = (int)(tile / (uint32_t)tiles_x) * T - HALO;
   for (int k = (int)threadIdx.x; k < S * S; k += 256) {
     const int lx = k % S, ly = k / S;
     const int gx = ox + lx, gy = oy + ly;
@@ -581,6 +600,60 @@ __global__ __launch_bounds__(256) void k_atrous(Uniforms u, const void* __restri
   }
   const f3 o = sum / vmax(wSum, 1e-6f);
   dst[i] = make_ushort4(f32_to_f16(o.x), f32_to_f16(o.y), f32_to_f16(o.z), f32_to_f16(center.w));
+}
+
+template <int FMT, int STEP>
+__global__ __launch_bounds__(256) void k_atrous(Uniforms u, const void* __restrict__ src,
+                                                const float4* __restrict__ gbuf,
+                                                ushort4* __restrict__ dst) {
+  __shared__ AtrousLds<STEP> lds;
+  atrous_tile<FMT, STEP>(u, blockIdx.x, src, gbuf, dst, lds);
+}
+
+// ---------------------------------------------------------------------------
+// batched form (vrt_batch.h): one launch per pass over every view's tiles
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_pathtrace_batch(const ViewDesc* __restrict__ views, uint32_t nviews,
+                                                         unsigned long long* __restrict__ counters) {
+  const uint32_t vi = find_view(views, nviews, blockIdx.x);
+  const ViewDesc& d = views[vi];
+  RayCount rc{0u, 0u, 0u, 0u};
+  pathtrace_tile(d.s, d.u, d.fclamp, d.spp, blockIdx.x - d.tile0, (ushort4*)d.radiance, (uchar4*)d.albedo,
+                 (float4*)d.gbuf_cur, rc);
+  count4(counters, rc.rays, rc.bsteps, rc.vsteps, rc.hits);
+}
+
+__global__ __launch_bounds__(256) void k_temporal_batch(const ViewDesc* __restrict__ views, uint32_t nviews) {
+  const uint32_t vi = find_view(views, nviews, blockIdx.x);
+  const ViewDesc& d = views[vi];
+  const uint32_t tile = blockIdx.x - d.tile0;
+  if (d.fmt == 1u)
+    temporal_tile<1>(d.u, tile, (const ushort4*)d.radiance, (const float4*)d.gbuf_cur, d.accum_prev,
+                     (const float4*)d.gbuf_prev, d.accum_cur);
+  else
+    temporal_tile<0>(d.u, tile, (const ushort4*)d.radiance, (const float4*)d.gbuf_cur, d.accum_prev,
+                     (const float4*)d.gbuf_prev, d.accum_cur);
+}
+
+// pass 0: accum -> den0 (step 1), 1: den0 -> den1 (step 2), 2: den1 -> den0
+// (step 4), main.js:502-507.  Views without denoise skip every pass.
+template <int PASS>
+__global__ __launch_bounds__(256) void k_atrous_batch(const ViewDesc* __restrict__ views, uint32_t nviews) {
+  constexpr int STEP = PASS == 0 ? 1 : (PASS == 1 ? 2 : 4);
+  __shared__ AtrousLds<STEP> lds;
+  const uint32_t vi = find_view(views, nviews, blockIdx.x);
+  const ViewDesc& d = views[vi];
+  if (!d.denoise) return;
+  const uint32_t tile = blockIdx.x - d.tile0;
+  if (PASS == 0) {
+    if (d.fmt == 1u)
+      atrous_tile<1, STEP>(d.u, tile, d.accum_cur, (const float4*)d.gbuf_cur, (ushort4*)d.den0, lds);
+    else
+      atrous_tile<0, STEP>(d.u, tile, d.accum_cur, (const float4*)d.gbuf_cur, (ushort4*)d.den0, lds);
+  } else {
+    atrous_tile<0, STEP>(d.u, tile, PASS == 1 ? d.den0 : d.den1, (const float4*)d.gbuf_cur,
+                         (ushort4*)(PASS == 1 ? d.den1 : d.den0), lds);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -676,6 +749,24 @@ hipError_t launch_pathtrace_persistent(hipStream_t st, const SceneDev& s, const 
                      spp, (ushort4*)radiance, (uchar4*)albedo, (float4*)gbuf, counters, queue,
                      steps < 1 ? 1 : steps, min_refill < 1 ? 1 : min_refill,
                      (uint32_t)(chunk < 64 ? 64 : chunk));
+  return hipGetLastError();
+}
+
+hipError_t launch_pathtrace_batch(hipStream_t st, const ViewDesc* views, uint32_t nviews, uint32_t tiles,
+                                  unsigned long long* counters) {
+  hipLaunchKernelGGL(k_pathtrace_batch, dim3(tiles), dim3(256), 0, st, views, nviews, counters);
+  return hipGetLastError();
+}
+
+hipError_t launch_temporal_batch(hipStream_t st, const ViewDesc* views, uint32_t nviews, uint32_t tiles) {
+  hipLaunchKernelGGL(k_temporal_batch, dim3(tiles), dim3(256), 0, st, views, nviews);
+  return hipGetLastError();
+}
+
+hipError_t launch_atrous_batch(hipStream_t st, const ViewDesc* views, uint32_t nviews, uint32_t tiles, int pass) {
+  if (pass == 0) hipLaunchKernelGGL(k_atrous_batch<0>, dim3(tiles), dim3(256), 0, st, views, nviews);
+  else if (pass == 1) hipLaunchKernelGGL(k_atrous_batch<1>, dim3(tiles), dim3(256), 0, st, views, nviews);
+  else hipLaunchKernelGGL(k_atrous_batch<2>, dim3(tiles), dim3(256), 0, st, views, nviews);
   return hipGetLastError();
 }
 

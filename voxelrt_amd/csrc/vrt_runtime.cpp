@@ -22,6 +22,7 @@
 #include "vrt_internal.h"
 #include "vrt_kernels.h"
 #include "vrt_trace.h"
+#include "vrt_batch.h"
 
 namespace vrt {
 hipError_t exclusive_scan_u32(hipStream_t st, void* d_tmp, size_t* tmp_bytes, const uint32_t* in,
@@ -521,6 +522,11 @@ int vrt_render(vrt_ctx* ctx, const vrt_view* views, uint32_t nviews, vrt_frame* 
   hipEvent_t* ev = ctx->ring[(ctx->ring_head) % vrt_ctx::RING].e;
   ctx->ring_head++;
   ctx->ring_count++;
+  if (nviews > 1) {
+    // batched form: one launch per pass for all views
+    int rc = ctx->render_batch(views, nviews, states.data(), ev);
+    if (rc) return rc;
+  } else {
   HIP_TRY(ctx, hipEventRecord(ev[0], ctx->stream));
   for (uint32_t k = 0; k < nviews; ++k) {
     const vrt_view& v = views[k];
@@ -551,6 +557,7 @@ int vrt_render(vrt_ctx* ctx, const vrt_view* views, uint32_t nviews, vrt_frame* 
     HIP_TRY(ctx, vrt::launch_atrous(ctx->stream, v.u, vs.den[1].ptr, 0, vs.gbuf[parity].ptr, 4, vs.den[0].ptr));
   }
   HIP_TRY(ctx, hipEventRecord(ev[3], ctx->stream));
+  }
   for (uint32_t k = 0; k < nviews && outs; ++k) {
     ViewState& vs = *states[k];
     const int parity = (int)(vs.frames & 1u);
@@ -831,6 +838,71 @@ int vrt_stats_reset(vrt_ctx* ctx) {
 
 }  // extern "C"
 
+// Batched form: fill one ViewDesc per view (vrt_batch.h) in a pinned staging
+// buffer, copy the table to the device, and launch each pass once over the
+// tiles of every view.  A staging buffer is rewritten only after its previous
+// copy has executed (event per buffer), so the host can run ahead of the
+// device by up to STAGES calls.
+int vrt_ctx::render_batch(const vrt_view* views, uint32_t nviews, ViewState** states, hipEvent_t* ev) {
+  if (nviews > desc_cap) {
+    HIP_TRY(this, hipStreamSynchronize(stream));
+    for (uint32_t k = 0; k < STAGES; ++k) {
+      if (desc_host[k]) (void)hipHostFree(desc_host[k]);
+      desc_host[k] = nullptr;
+    }
+    const uint32_t cap = std::max<uint32_t>(nviews, 64);
+    for (uint32_t k = 0; k < STAGES; ++k) {
+      HIP_TRY(this, hipHostMalloc(&desc_host[k], cap * sizeof(vrt::ViewDesc), hipHostMallocDefault));
+      if (!desc_done[k]) HIP_TRY(this, hipEventCreateWithFlags(&desc_done[k], hipEventDisableTiming));
+    }
+    HIP_TRY(this, desc_dev.alloc(cap * sizeof(vrt::ViewDesc)));
+    desc_cap = cap;
+  }
+  const uint32_t stage = desc_next++ % STAGES;
+  HIP_TRY(this, hipEventSynchronize(desc_done[stage]));  // its previous copy has run
+  vrt::ViewDesc* d = (vrt::ViewDesc*)desc_host[stage];
+  uint32_t tiles = 0;
+  for (uint32_t k = 0; k < nviews; ++k) {
+    const vrt_view& v = views[k];
+    ViewState& vs = *states[k];
+    const int parity = (int)(vs.frames & 1u);
+    std::memset(&d[k], 0, sizeof(d[k]));
+    d[k].s = v.scene->dev();
+    d[k].u = v.u;
+    d[k].fclamp = v.firefly_clamp;
+    d[k].spp = v.spp;
+    d[k].fmt = vs.fmt;
+    d[k].w = vs.w;
+    d[k].h = vs.h;
+    d[k].tiles_x = (vs.w + 15u) / 16u;
+    d[k].tile0 = tiles;
+    d[k].denoise = (v.u.params0[2] & VRT_FLAG_DENOISE) ? 1u : 0u;
+    d[k].radiance = vs.radiance.ptr;
+    d[k].albedo = vs.albedo.ptr;
+    d[k].gbuf_cur = vs.gbuf[parity].ptr;
+    d[k].gbuf_prev = vs.gbuf[1 - parity].ptr;
+    d[k].accum_cur = vs.accum[parity].ptr;
+    d[k].accum_prev = vs.accum[1 - parity].ptr;
+    d[k].den0 = vs.den[0].ptr;
+    d[k].den1 = vs.den[1].ptr;
+    tiles += d[k].tiles_x * ((vs.h + 15u) / 16u);
+  }
+  const vrt::ViewDesc* dd = (const vrt::ViewDesc*)desc_dev.ptr;
+  HIP_TRY(this, hipMemcpyAsync(desc_dev.ptr, d, nviews * sizeof(vrt::ViewDesc), hipMemcpyHostToDevice, stream));
+  HIP_TRY(this, hipEventRecord(desc_done[stage], stream));
+  HIP_TRY(this, hipEventRecord(ev[0], stream));
+  HIP_TRY(this, vrt::launch_pathtrace_batch(stream, dd, nviews, tiles, (unsigned long long*)d_counters.ptr));
+  HIP_TRY(this, hipEventRecord(ev[1], stream));
+  HIP_TRY(this, vrt::launch_temporal_batch(stream, dd, nviews, tiles));
+  HIP_TRY(this, hipEventRecord(ev[2], stream));
+  bool any_denoise = false;
+  for (uint32_t k = 0; k < nviews; ++k) any_denoise |= (views[k].u.params0[2] & VRT_FLAG_DENOISE) != 0;
+  if (any_denoise)
+    for (int pass = 0; pass < 3; ++pass) HIP_TRY(this, vrt::launch_atrous_batch(stream, dd, nviews, tiles, pass));
+  HIP_TRY(this, hipEventRecord(ev[3], stream));
+  return VRT_OK;
+}
+
 int vrt_ctx::harvest(uint32_t max_slots) {
   while (ring_count > 0 && max_slots-- > 0) {
     const TimingSlot& sl = ring[(ring_head - ring_count) % RING];
@@ -863,5 +935,9 @@ vrt_ctx::~vrt_ctx() {
   for (auto& slot : ring)
     for (auto& e : slot.e)
       if (e) (void)hipEventDestroy(e);
+  for (uint32_t k = 0; k < STAGES; ++k) {
+    if (desc_host[k]) (void)hipHostFree(desc_host[k]);
+    if (desc_done[k]) (void)hipEventDestroy(desc_done[k]);
+  }
   if (stream) (void)hipStreamDestroy(stream);
 }
