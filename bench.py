@@ -88,11 +88,11 @@ def traverse(args):
     d_rays = torch.from_numpy(rays).to("cuda:0")
     d_hits = torch.empty(n_rays * 4, dtype=torch.int32, device="cuda:0")
     torch.cuda.synchronize()
-    variants = {"lane": dict(trace_kernel=0),
+    variants = {"lane": dict(trace_kernel=0), "lane_flat": dict(trace_kernel=2),
                 "persistent": dict(trace_kernel=1, trace_blocks=2048, trace_steps=8, trace_min_refill=16)}
     for p in (0.05, 0.25, 0.6):
         sc = ctx.upload(cf.traverse_scene(p))
-        for name in ("lane", "persistent", "autotuned"):
+        for name in ("lane", "lane_flat", "persistent", "autotuned"):
             if name == "autotuned":
                 tuned = ctx.autotune(sc, 1 << 22).as_dict()
             else:
@@ -105,33 +105,223 @@ def traverse(args):
 
 def _traverse_one(args, ctx, sc, p, name, d_rays, d_hits, n_rays, tuning):
     import numpy as np
-    if True:
-        for _ in range(args.warmup):
-            ctx.trace_rays_device(sc, d_rays.data_ptr(), n_rays, d_hits.data_ptr())
+    for _ in range(args.warmup):
+        ctx.trace_rays_device(sc, d_rays.data_ptr(), n_rays, d_hits.data_ptr())
+    s0 = ctx.stats()
+    ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ms.append(ctx.trace_rays_device(sc, d_rays.data_ptr(), n_rays, d_hits.data_ptr()))
+    s1 = ctx.stats()
+    wall = time.perf_counter() - t0
+    K = args.steps
+    bs = (s1["brick_steps"] - s0["brick_steps"]) / K
+    vs = (s1["voxel_steps"] - s0["voxel_steps"]) / K
+    hits = (s1["hits"] - s0["hits"]) / K
+    alg = 40 * n_rays + 4 * bs + 4 * vs + 4 * hits
+    kms = float(np.mean(ms))
+    gbs = alg / (kms * 1e-3) / 1e9
+    print(json.dumps({"metric": "traversal Mrays/s (256^3, 2^24 rays)", "p": p, "variant": name,
+                      "tuning": tuning,
+                      "value": round(n_rays / (kms * 1e-3) / 1e6, 1), "unit": "Mrays/s",
+                      "kernel_ms": round(kms, 4), "wall_ms_per_launch": round(wall / K * 1e3, 4),
+                      "brick_steps_per_ray": round(bs / n_rays, 3),
+                      "voxel_steps_per_ray": round(vs / n_rays, 3), "hit_frac": round(hits / n_rays, 4),
+                      "roofline": {"bound": "hbm", "kernel": "k_trace_rays", "achieved": round(gbs, 1),
+                                   "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                   "frac": round(gbs / HBM_PEAK_GBS, 4),
+                                   "alg_bytes_per_launch": int(alg)}}), flush=True)
+
+
+def batch(args):
+    """Batched form: 8 scenes of {64,128,256,512}^3 and 64 views of uneven
+    size (256x256 .. 1920x1080, aspect 1:1 .. 21:9) and bounce count (1..6),
+    temporal + a-trous on.  'plain' renders the views one call each (one
+    launch per pass per view); 'batched' renders all 64 in one call (one
+    launch per pass).  Aggregate Mrays/s over the timed frames."""
+    import voxelrt_amd as v
+    from voxelrt_amd import configs as cf
+
+    descs, specs = cf.batch_views(64)
+    ctx = v.Context(0)
+    t0 = time.perf_counter()
+    scenes = [ctx.upload(v.generate_scene(**d)) for d in descs]
+    upload_s = time.perf_counter() - t0
+    prev = {}
+
+    def views_for(frame, base_id):
+        views = []
+        for k, sp in enumerate(specs):
+            d = descs[sp["scene"]]
+            pos, yaw, pitch = cf.world_orbit(d, (sp["orbit"] + frame) % cf.ORBIT_FRAMES)
+            key = base_id + k
+            pvp, ppos = prev.get(key, (None, None))
+            u, vp = v.make_uniforms(pos, yaw, pitch, sp["width"], sp["height"], frame=sp["seed"] + frame,
+                                    bounces=sp["bounces"], flags=3, prev_view_proj=pvp, prev_cam_pos=ppos)
+            prev[key] = (vp, pos)
+            views.append(v.View(scenes[sp["scene"]].handle, u, 48.0, key, 1, 0))
+        return views
+
+    results = {}
+    pixels = sum(sp["width"] * sp["height"] for sp in specs)
+    for mode in ("plain", "batched"):
+        base = 10000 if mode == "plain" else 20000
+        for f in range(args.warmup):
+            vs = views_for(f, base)
+            if mode == "plain":
+                for view in vs:
+                    ctx.render([view])
+            else:
+                ctx.render(vs)
         s0 = ctx.stats()
-        ms = []
         t0 = time.perf_counter()
-        for _ in range(args.steps):
-            ms.append(ctx.trace_rays_device(sc, d_rays.data_ptr(), n_rays, d_hits.data_ptr()))
+        for f in range(args.steps):
+            vs = views_for(args.warmup + f, base)
+            if mode == "plain":
+                for view in vs:
+                    ctx.render([view])
+            else:
+                ctx.render(vs)
         s1 = ctx.stats()
         wall = time.perf_counter() - t0
-        K = args.steps
-        bs = (s1["brick_steps"] - s0["brick_steps"]) / K
-        vs = (s1["voxel_steps"] - s0["voxel_steps"]) / K
-        hits = (s1["hits"] - s0["hits"]) / K
-        alg = 40 * n_rays + 4 * bs + 4 * vs + 4 * hits
-        kms = float(np.mean(ms))
-        gbs = alg / (kms * 1e-3) / 1e9
-        print(json.dumps({"metric": "traversal Mrays/s (256^3, 2^24 rays)", "p": p, "variant": name,
-                          "tuning": tuning,
-                          "value": round(n_rays / (kms * 1e-3) / 1e6, 1), "unit": "Mrays/s",
-                          "kernel_ms": round(kms, 4), "wall_ms_per_launch": round(wall / K * 1e3, 4),
-                          "brick_steps_per_ray": round(bs / n_rays, 3),
-                          "voxel_steps_per_ray": round(vs / n_rays, 3), "hit_frac": round(hits / n_rays, 4),
-                          "roofline": {"bound": "hbm", "kernel": "k_trace_rays", "achieved": round(gbs, 1),
-                                       "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                                       "frac": round(gbs / HBM_PEAK_GBS, 4),
-                                       "alg_bytes_per_launch": int(alg)}}), flush=True)
+        rays = s1["rays"] - s0["rays"]
+        results[mode] = {"mrays_s": round(rays / wall / 1e6, 1), "ms_per_batch": round(wall / args.steps * 1e3, 3),
+                         "gpu_ms_per_batch": round((s1["sum_ms_frame"] - s0["sum_ms_frame"]) / args.steps, 3),
+                         "rays_per_batch": rays // args.steps}
+    out = {"metric": "batched aggregate Mrays/s vs plain per-view (64 views, 8 scenes)", "unit": "Mrays/s",
+           "value": results["batched"]["mrays_s"], "plain": results["plain"], "batched": results["batched"],
+           "speedup": round(results["batched"]["mrays_s"] / max(results["plain"]["mrays_s"], 1e-9), 2),
+           "views": len(specs), "pixels_per_batch": pixels, "scene_upload_s": round(upload_s, 2),
+           "steps": args.steps, "warmup": args.warmup}
+    print(json.dumps(out), flush=True)
+    for sc in scenes:
+        sc.release()
+    ctx.close()
+
+
+def _pct(xs, q):
+    import numpy as np
+    return round(float(np.percentile(np.asarray(xs, dtype=np.float64), q)), 3) if len(xs) else None
+
+
+def serve(args):
+    """Served form vs plain serial rendering on one arrival trace.
+
+    16 scenes (64^3 .. 1024^3), 32 camera streams (360p .. 1440p, spp 1..4,
+    bounces 1..4, temporal + denoise), `--requests` frames.  Four runs, each
+    with fresh view histories:
+      plain_saturated   every request rendered alone with vrt_render, back to
+                        back (the plain per-view kernel's capacity)
+      served_saturated  every request submitted at t=0 (continuous batching)
+      plain_poisson     Poisson arrivals at load x plain capacity, one request
+                        at a time in arrival order (latency = done - arrival)
+      served_poisson    the same arrivals through vrt_submit / vrt_wait
+    Every run reads each frame's HDR back to the host."""
+    import threading
+
+    import numpy as np
+
+    import voxelrt_amd as v
+    from voxelrt_amd import configs as cf
+
+    descs, cams, order = cf.serve_workload(args.requests)
+    ctx = v.Context(0)
+    ctx.serve_configure(streams=args.serve_streams, max_batch_views=args.serve_views,
+                        max_batch_pixels=args.serve_pixels)
+    t0 = time.perf_counter()
+    scenes = [ctx.upload(v.generate_scene(**d)) for d in descs]
+    upload_s = time.perf_counter() - t0
+    hdr = [np.zeros((c["height"], c["width"], 3), np.float32) for c in cams]
+
+    def views_for(seq, base):
+        prev, frame_of, out = {}, {}, []
+        for s in seq:
+            cam = cams[s]
+            f = frame_of.get(s, 0)
+            frame_of[s] = f + 1
+            pos, yaw, pitch = cf.world_orbit(descs[cam["scene"]], (cam["orbit"] + f) % cf.ORBIT_FRAMES)
+            pvp, ppos = prev.get(s, (None, None))
+            u, vp = v.make_uniforms(pos, yaw, pitch, cam["width"], cam["height"], frame=cam["seed"] + f,
+                                    bounces=cam["bounces"], flags=3, prev_view_proj=pvp, prev_cam_pos=ppos)
+            prev[s] = (vp, pos)
+            out.append(v.View(scenes[cam["scene"]].handle, u, 48.0, base + s, cam["spp"], 0))
+        return out
+
+    warm = list(range(len(cams)))  # one frame per stream: allocates every view's state
+
+    def run_plain(base, arrivals=None):
+        views = views_for(warm + order, base)
+        for k in range(len(warm)):
+            ctx.render([views[k]], [{"hdr": hdr[warm[k]]}])
+        views = views[len(warm):]
+        lat, service, rays = [], [], 0
+        start = time.perf_counter()
+        for k, view in enumerate(views):
+            if arrivals is not None:
+                wait = start + arrivals[k] - time.perf_counter()
+                if wait > 0:
+                    time.sleep(wait)
+            ts = time.perf_counter()
+            fr = ctx.render([view], [{"hdr": hdr[order[k]]}])
+            te = time.perf_counter()
+            rays += fr[0].rays
+            service.append((te - ts) * 1e3)
+            lat.append((te - start - (arrivals[k] if arrivals is not None else 0.0)) * 1e3)
+        span = time.perf_counter() - start
+        return dict(mrays_s=round(rays / span / 1e6, 1), span_s=round(span, 3), rays=rays,
+                    p50_ms=_pct(lat, 50), p99_ms=_pct(lat, 99), mean_service_ms=round(float(np.mean(service)), 3))
+
+    def run_served(base, arrivals=None):
+        views = views_for(warm + order, base)
+        for k in range(len(warm)):
+            ctx.wait(ctx.submit(views[k]), hdr[warm[k]])
+        views = views[len(warm):]
+        tickets = [None] * len(views)
+        ready = [threading.Event() for _ in views]
+        start = time.perf_counter()
+
+        def submitter():
+            for k, view in enumerate(views):
+                if arrivals is not None:
+                    wait = start + arrivals[k] - time.perf_counter()
+                    if wait > 0:
+                        time.sleep(wait)
+                tickets[k] = ctx.submit(view)
+                ready[k].set()
+
+        th = threading.Thread(target=submitter)
+        th.start()
+        lat, rays, sizes = [], 0, []
+        for k in range(len(views)):
+            ready[k].wait()
+            res = ctx.wait(tickets[k], hdr[order[k]])
+            rays += res["rays"]
+            lat.append(res["timing"]["latency_ms"])
+            sizes.append(res["timing"]["batch_views"])
+        span = time.perf_counter() - start
+        th.join()
+        return dict(mrays_s=round(rays / span / 1e6, 1), span_s=round(span, 3), rays=rays,
+                    p50_ms=_pct(lat, 50), p99_ms=_pct(lat, 99), mean_batch=round(float(np.mean(sizes)), 2))
+
+    res = {}
+    res["plain_saturated"] = run_plain(100000)
+    res["served_saturated"] = run_served(200000)
+    rate = args.load * 1e3 / res["plain_saturated"]["mean_service_ms"]  # requests / s
+    gaps = np.random.default_rng(5).exponential(1.0 / rate, len(order))
+    arrivals = np.cumsum(gaps) - gaps[0]
+    res["plain_poisson"] = run_plain(300000, arrivals)
+    res["served_poisson"] = run_served(400000, arrivals)
+    st = ctx.serve_stats()
+    out = {"metric": "served aggregate Mrays/s and p50/p99 frame latency vs plain serial rendering",
+           "unit": "Mrays/s", "value": res["served_saturated"]["mrays_s"],
+           "speedup_saturated": round(res["served_saturated"]["mrays_s"] / res["plain_saturated"]["mrays_s"], 2),
+           "poisson_rate_per_s": round(rate, 2), "load_vs_plain": args.load,
+           "runs": res, "serve_stats": st, "requests": len(order), "scenes": len(descs),
+           "camera_streams": len(cams), "scene_upload_s": round(upload_s, 2),
+           "serve_config": {"streams": args.serve_streams, "max_batch_views": args.serve_views,
+                            "max_batch_pixels": args.serve_pixels}}
+    print(json.dumps(out), flush=True)
+    ctx.close()
 
 
 def main():
@@ -141,8 +331,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-frames", type=int, default=8)
-    ap.add_argument("--workload", default="frame_1080", choices=["frame_1080", "traverse"])
-    ap.add_argument("--pt-kernel", default="auto", choices=["auto", "0", "1"])
+    ap.add_argument("--workload", default="frame_1080", choices=["frame_1080", "traverse", "batch", "serve"])
+    ap.add_argument("--requests", type=int, default=256)
+    ap.add_argument("--load", type=float, default=0.8, help="Poisson rate as a fraction of plain capacity")
+    ap.add_argument("--serve-streams", type=int, default=2)
+    ap.add_argument("--serve-views", type=int, default=64)
+    ap.add_argument("--serve-pixels", type=int, default=16 << 20)
+    ap.add_argument("--pt-kernel", default="auto", choices=["auto", "0", "1", "2"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (--gpus {args.gpus} "
@@ -151,6 +346,10 @@ def main():
 
     if args.workload == "traverse":
         return traverse(args)
+    if args.workload == "batch":
+        return batch(args)
+    if args.workload == "serve":
+        return serve(args)
 
     import voxelrt_amd as v
     from voxelrt_amd import configs as cf

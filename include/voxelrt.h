@@ -214,16 +214,23 @@ int vrt_atrous_pass(vrt_ctx* ctx, const vrt_uniforms* u, const void* src,
  * given scene and keeps the fastest in the context.  No reference
  * counterpart: WebGPU dispatches fixed 8x8 workgroups (main.js:728-734). */
 typedef struct vrt_tuning {
-  uint32_t trace_kernel;      /* 0 = one ray per lane, 1 = persistent ray queue */
+  uint32_t trace_kernel;      /* 0 = one ray per lane, 1 = persistent ray queue,
+                                 2 = one ray per lane, single-stream DDA,
+                                 3 = one ray per lane, occupancy staged in LDS */
   uint32_t trace_blocks;      /* persistent grid (256-thread blocks), 0 = default */
   uint32_t trace_steps;       /* DDA iterations between queue refill checks */
   uint32_t trace_min_refill;  /* idle lanes (of 64) that trigger a refill */
-  uint32_t pt_kernel;         /* path tracer: 0 = one pixel per lane, 1 = persistent */
+  uint32_t pt_kernel;         /* path tracer: 0 = one pixel per lane, 1 = persistent,
+                                 2 = one pixel per lane, single-stream DDA,
+                                 3 = one pixel per lane, occupancy staged in LDS */
   uint32_t pt_blocks;
   uint32_t pt_steps;
   uint32_t pt_min_refill;
   uint32_t pt_chunk;          /* pixels a wave takes per queue atomic */
-  uint32_t reserved[7];
+  uint32_t pt_waves;          /* one-pixel-per-lane kernels: minimum waves per
+                                 SIMD the register allocation must allow:
+                                 0 = compiler's choice, 4 or 5 */
+  uint32_t reserved[6];
 } vrt_tuning;
 int vrt_tuning_get(vrt_ctx* ctx, vrt_tuning* out);
 int vrt_tuning_set(vrt_ctx* ctx, const vrt_tuning* in);
@@ -236,6 +243,77 @@ int vrt_autotune(vrt_ctx* ctx, const vrt_scene* scene, uint64_t nrays,
 /* ---- stats ------------------------------------------------------------------ */
 int vrt_stats_get(vrt_ctx* ctx, vrt_stats* out);
 int vrt_stats_reset(vrt_ctx* ctx);
+
+/* ---- serving: continuous batching ------------------------------------------- *
+ * A request is one frame of one view (scene, pose, size, spp, bounces, flags).
+ * vrt_submit queues it and returns at once.  A worker thread inside the
+ * context groups queued requests into batched launches (the batched form of
+ * vrt_render) on `streams` device streams: a request that arrives while other
+ * batches are on the GPU joins the next launch as soon as a stream and HBM
+ * free up, and that launch overlaps the running ones on the device.
+ *   - Admission is by HBM footprint: the request's view state (when it is not
+ *     resident yet) plus its result buffer must fit the context's budget.
+ *     View state of views with no queued or running request is evicted
+ *     least-recently-used first (its temporal history restarts, as after
+ *     vrt_view_release).  A request that cannot fit even with everything else
+ *     evicted fails with VRT_ERR_BUDGET.
+ *   - Frames of one view_id run in submission order on one stream, one frame
+ *     of a view per launch (temporal history ping-pong).
+ *   - Results stay on the device until vrt_poll / vrt_wait copy them out:
+ *     hdr (main.js:628-659) and the request's own ray / step counters.
+ *   - Every pixel's result is the same as vrt_render of that view alone.
+ * Scenes referenced by queued or running requests cannot be released.
+ * No reference counterpart: the reference renders one view per
+ * requestAnimationFrame tick (main.js:680-803). */
+typedef uint64_t vrt_ticket;
+
+typedef struct vrt_request {
+  vrt_view view;
+  uint32_t flags;           /* reserved, 0 */
+  uint32_t _pad;
+} vrt_request;
+
+typedef struct vrt_serve_config {
+  uint32_t streams;         /* device streams batches overlap on, 1..4 (default 2) */
+  uint32_t max_batch_views; /* requests per launch, 1..1024 (default 64) */
+  uint64_t max_batch_pixels;/* pixels x spp per launch (default 16M) */
+} vrt_serve_config;
+
+/* Host steady-clock times in ms since vrt_create. */
+typedef struct vrt_timing {
+  double   submit_ms;
+  double   launch_ms;       /* the batch holding the request was enqueued */
+  double   done_ms;         /* the worker saw that batch complete */
+  double   latency_ms;      /* done_ms - submit_ms */
+  double   gpu_ms;          /* HIP-event span of that batch */
+  uint32_t batch_views;     /* requests in that batch */
+  uint32_t stream;          /* stream index it ran on */
+  uint64_t batch_id;
+} vrt_timing;
+
+typedef struct vrt_serve_stats {
+  uint64_t submitted, completed, cancelled, failed;
+  uint64_t batches, batched_requests;
+  uint64_t evictions;
+  uint64_t queued, running;        /* right now */
+  uint64_t max_running_batches;    /* most batches seen on the device at once */
+} vrt_serve_stats;
+
+/* Configure before the first vrt_submit (NULL = defaults). */
+int vrt_serve_configure(vrt_ctx* ctx, const vrt_serve_config* cfg);
+int vrt_submit(vrt_ctx* ctx, const vrt_request* req, vrt_ticket* out);
+/* VRT_ERR_NOT_READY while queued or running.  On VRT_OK the results are
+ * copied to `out` (hdr when non-NULL, counters always; other vrt_frame
+ * fields are ignored), `timing` is filled when non-NULL, and the ticket is
+ * retired.  A failed request returns its error once and is retired. */
+int vrt_poll(vrt_ctx* ctx, vrt_ticket t, vrt_frame* out, vrt_timing* timing);
+/* As vrt_poll, blocking up to timeout_ms (0 = no limit). */
+int vrt_wait(vrt_ctx* ctx, vrt_ticket t, uint32_t timeout_ms, vrt_frame* out,
+             vrt_timing* timing);
+/* Cancel a request that has not been launched yet (VRT_ERR_INVALID once it
+ * runs or finished).  The ticket is retired. */
+int vrt_cancel(vrt_ctx* ctx, vrt_ticket t);
+int vrt_serve_stats_get(vrt_ctx* ctx, vrt_serve_stats* out);
 
 /* ---- host helpers (no device work) ------------------------------------------ *
  * Camera matrices exactly as src/camera.js:47-56 + src/math.js (double math,

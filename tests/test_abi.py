@@ -36,6 +36,21 @@ def test_struct_layouts():
     assert C.sizeof(v.Uniforms) == 304  # common.wgsl:13-35
     assert v.HIT_DTYPE.itemsize == 16
     assert C.sizeof(v.lib.View) == 8 + 304 + 4 * 4
+    assert C.sizeof(v.lib.Request) == C.sizeof(v.lib.View) + 8
+    assert C.sizeof(v.lib.ServeConfig) == 16
+    assert C.sizeof(v.lib.Timing) == 5 * 8 + 2 * 4 + 8
+    assert C.sizeof(v.lib.ServeStats) == 10 * 8
+
+
+def test_serving_entry_points_reject_null_context():
+    L = v.load()
+    t = C.c_uint64(7)
+    assert L.vrt_submit(None, None, C.byref(t)) == -1
+    assert L.vrt_poll(None, 1, None, None) == -1
+    assert L.vrt_wait(None, 1, 1, None, None) == -1
+    assert L.vrt_cancel(None, 1) == -1
+    assert L.vrt_serve_stats_get(None, None) == -1
+    assert L.vrt_serve_configure(None, None) == -1
 
 
 def test_error_paths_without_device():

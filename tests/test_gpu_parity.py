@@ -74,12 +74,14 @@ def traverse(request, ctx):
     sc.release()
 
 
-TRACE_VARIANTS = [dict(trace_kernel=0), dict(trace_kernel=1, trace_blocks=2048, trace_steps=8, trace_min_refill=16),
+TRACE_VARIANTS = [dict(trace_kernel=0), dict(trace_kernel=2), dict(trace_kernel=3),
+                  dict(trace_kernel=1, trace_blocks=2048, trace_steps=8, trace_min_refill=16),
                   dict(trace_kernel=1, trace_blocks=512, trace_steps=1, trace_min_refill=1),
                   dict(trace_kernel=1, trace_blocks=4096, trace_steps=16, trace_min_refill=64)]
 
 
-@pytest.fixture(params=range(len(TRACE_VARIANTS)), ids=["lane", "persist", "persist-s1", "persist-s16"])
+@pytest.fixture(params=range(len(TRACE_VARIANTS)),
+                ids=["lane", "lane-flat", "lane-lds", "persist", "persist-s1", "persist-s16"])
 def trace_variant(request, ctx):
     old = ctx.tuning().as_dict()
     ctx.set_tuning(**TRACE_VARIANTS[request.param])
@@ -167,7 +169,7 @@ def test_autotune_keeps_results(ctx, traverse):
         pytest.skip("one occupancy is enough")
     old = ctx.tuning().as_dict()
     t = ctx.autotune(sc, 1 << 20)
-    assert t.trace_kernel in (0, 1)
+    assert t.trace_kernel in (0, 1, 2, 3)
     rays = cf.traverse_rays(1 << 18, seed=5)
     _hits_equal(ctx.trace_rays(sc, rays), oracle.trace_rays(og, rays), f"autotuned {t.as_dict()}")
     ctx.set_tuning(**old)
@@ -184,13 +186,16 @@ def frame128(ctx):
     sc.release()
 
 
-PT_VARIANTS = [dict(pt_kernel=0),
+PT_VARIANTS = [dict(pt_kernel=0), dict(pt_kernel=0, pt_waves=5), dict(pt_kernel=2, pt_waves=4),
+               dict(pt_kernel=3), dict(pt_kernel=3, pt_waves=4), dict(pt_kernel=3, pt_waves=5),
                dict(pt_kernel=1, pt_blocks=1024, pt_steps=8, pt_min_refill=32, pt_chunk=64),
                dict(pt_kernel=1, pt_blocks=97, pt_steps=1, pt_min_refill=1, pt_chunk=64),
                dict(pt_kernel=1, pt_blocks=2048, pt_steps=16, pt_min_refill=64, pt_chunk=256)]
 
 
-@pytest.fixture(params=range(len(PT_VARIANTS)), ids=["lane", "persist", "persist-s1", "persist-s16"])
+@pytest.fixture(params=range(len(PT_VARIANTS)),
+                ids=["lane", "lane-w5", "lane-flat-w4", "lane-lds", "lane-lds-w4", "lane-lds-w5", "persist",
+                     "persist-s1", "persist-s16"])
 def pt_variant(request, ctx):
     old = ctx.tuning().as_dict()
     ctx.set_tuning(**PT_VARIANTS[request.param])
@@ -329,3 +334,52 @@ def test_frame_sequence_bitexact(ctx, pt_variant):
             assert_bits_equal(f"frame {i} {k}", got[k], want[k])
     ctx.release_view(950)
     sc.release()
+
+
+# ---------------------------------------------------------------------------
+# batched form: every view of a mixed batch equals the same view alone
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("tuning", [dict(pt_kernel=0, pt_waves=0), dict(pt_kernel=3, pt_waves=5),
+                                    dict(pt_kernel=2, pt_waves=4)], ids=["lane", "lds-w5", "flat-w4"])
+def test_batched_views_match_single_views(ctx, tuning):
+    descs = [cf.world_desc(64, 1), cf.world_desc(128, 2), dict(cf.world_desc(64, 3), ny=32)]
+    scenes = [ctx.upload(v.generate_scene(**d)) for d in descs]
+    specs = [  # scene, w, h, bounces, flags, fmt, spp
+        (0, 96, 64, 1, 3, 0, 1), (1, 160, 90, 4, 3, 1, 1), (2, 64, 64, 2, 1, 0, 2),
+        (1, 33, 17, 6, 0, 0, 1), (0, 128, 40, 3, 3, 1, 1)]
+    names = ("radiance", "albedo", "gbuf", "accum", "denoised")
+
+    def make_views(frame, base_id):
+        views = []
+        for k, (si, w, h, nb, flags, fmt, spp) in enumerate(specs):
+            d = descs[si]
+            pos, yaw, pitch = cf.world_orbit(d, frame * 5 + k, 64)
+            u, _ = v.make_uniforms(pos, yaw, pitch, w, h, frame=frame + 100 * k, bounces=nb, flags=flags)
+            views.append(v.View(scenes[si].handle, u, 48.0, base_id + k, spp, fmt))
+        return views
+
+    def alloc(w, h, fmt):
+        return {"radiance": np.zeros((h, w, 4), np.uint16), "albedo": np.zeros((h, w, 4), np.uint8),
+                "gbuf": np.zeros((h, w, 4), np.float32),
+                "accum": np.zeros((h, w, 4), np.uint16 if fmt == 0 else np.float32),
+                "denoised": np.zeros((h, w, 4), np.uint16)}
+
+    old = ctx.tuning().as_dict()
+    ctx.set_tuning(**tuning)
+    for frame in range(3):
+        batch = make_views(frame, 2000)
+        outs_b = [alloc(s[1], s[2], s[5]) for s in specs]
+        ctx.render(batch, outs_b)
+        for k, view in enumerate(make_views(frame, 3000)):
+            outs_s = [alloc(specs[k][1], specs[k][2], specs[k][5])]
+            ctx.render([view], outs_s)
+            for n in names:
+                if n == "denoised" and not (specs[k][4] & 2):
+                    continue
+                assert_bits_equal(f"frame {frame} view {k} {n}", outs_b[k][n], outs_s[0][n])
+    ctx.set_tuning(**old)
+    for k in range(len(specs)):
+        ctx.release_view(2000 + k)
+        ctx.release_view(3000 + k)
+    for sc in scenes:
+        sc.release()

@@ -124,3 +124,26 @@ def batch_views(count: int = 64, seed: int = 11):
         views.append(dict(scene=s, width=w, height=h, bounces=int(rng.integers(1, 7)),
                           orbit=int(rng.integers(0, ORBIT_FRAMES)), seed=int(k * 7717)))
     return descs, views
+
+
+SERVE_GRIDS = (64, 128, 256, 512, 64, 128, 256, 512, 64, 128, 256, 512, 128, 256, 512, 1024)
+SERVE_HEIGHTS = (360, 540, 720, 1080, 1440)
+
+
+def serve_workload(requests: int = 256, streams: int = 32, seed: int = 21, grids=SERVE_GRIDS):
+    """Serving workload: 16 scenes (64^3 .. 1024^3), `streams` camera streams
+    (each a fixed scene, 16:9 resolution 360p..1440p, spp 1..4, bounces 1..4,
+    temporal + denoise, orbiting one pose per frame) and `requests` frames
+    drawn from the streams in a seeded order.  Arrival times are assigned by
+    the caller (Poisson at a chosen rate).  Returns (scene descs, streams,
+    request list of stream indices)."""
+    rng = np.random.default_rng(seed)
+    descs = [world_desc(g, 300 + i) for i, g in enumerate(grids)]
+    cams = []
+    for k in range(streams):
+        h = SERVE_HEIGHTS[int(rng.integers(0, len(SERVE_HEIGHTS)))]
+        cams.append(dict(scene=int(rng.integers(0, len(descs))), width=(h * 16) // 9, height=h,
+                         spp=int(rng.integers(1, 5)), bounces=int(rng.integers(1, 5)),
+                         orbit=int(rng.integers(0, ORBIT_FRAMES)), seed=int(k * 131)))
+    order = [int(x) for x in rng.integers(0, streams, requests)]
+    return descs, cams, order

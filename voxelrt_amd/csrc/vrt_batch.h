@@ -27,6 +27,8 @@ struct ViewDesc {
   void* accum_cur;
   void* den0;        // rgba16f
   void* den1;
+  unsigned long long* vcount;  // this view's rays / brick steps / voxel steps / hits, or null
+  float* hdr;                  // rgb f32 HDR output (main.js:628-659), or null
 };
 
 // Index of the view owning tile `b` (views sorted by tile0, tile0[0] = 0).

@@ -2,7 +2,9 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -39,6 +41,28 @@ struct DeviceBuffer {
   void release();
 };
 
+// One stream's batched-form launch state: a pinned staging ring for the
+// view-descriptor table (vrt_batch.h) and its device copy.  A staging buffer
+// is rewritten only after its previous copy has executed (event per buffer),
+// so the host can run ahead of the device by up to STAGES batches.
+struct BatchLane {
+  static constexpr uint32_t STAGES = 4;
+  hipStream_t stream = nullptr;  // owned when own_stream
+  bool own_stream = false;
+  void* desc_host[STAGES] = {};
+  hipEvent_t desc_done[STAGES] = {};
+  uint32_t desc_cap = 0;  // views per staging buffer
+  uint32_t desc_next = 0;
+  DeviceBuffer desc_dev;
+  BatchLane() = default;
+  BatchLane(const BatchLane&) = delete;
+  BatchLane& operator=(const BatchLane&) = delete;
+  ~BatchLane();
+  hipError_t ensure(uint32_t nviews);
+};
+
+struct Server;  // serving loop (vrt_serve.cpp)
+
 }  // namespace vrt
 
 struct vrt_scene {
@@ -47,8 +71,10 @@ struct vrt_scene {
   uint64_t bytes = 0;
   uint64_t key = 0;
   uint64_t lru = 0;
+  uint64_t uses = 0;  // queued + running serve requests
   vrt::DeviceBuffer d_bricks;  // one word per 8^3 brick: 0 or 1 + pool slot
   vrt::DeviceBuffer d_pool;    // nocc x 528 words (mask + voxels)
+  vrt::DeviceBuffer d_bits;    // one occupancy bit per brick (LDS-staged by the kernels)
   vrt::SceneDev dev() const;
 };
 
@@ -59,6 +85,7 @@ struct ViewState {
   uint32_t w = 0, h = 0, fmt = 0;
   uint64_t bytes = 0;
   uint64_t frames = 0;  // frames rendered: parity = frames & 1 (main.js:695)
+  uint64_t lru = 0;     // context clock at last use
   vrt::DeviceBuffer radiance, albedo;
   vrt::DeviceBuffer gbuf[2], accum[2], den[2];
 };
@@ -68,6 +95,7 @@ struct vrt_ctx {
   hipStream_t stream = nullptr;
   hipEvent_t ev[8] = {};
   std::string err;
+  std::string err_ret;  // vrt_last_error's copy
   uint64_t budget = 0, resident = 0;
   uint64_t clock = 0;
   uint64_t frames = 0;
@@ -83,15 +111,19 @@ struct vrt_ctx {
   uint64_t timed_calls = 0;
   double sum_pathtrace = 0, sum_temporal = 0, sum_atrous = 0, sum_frame = 0;
   int harvest(uint32_t max_slots);  // oldest pending first; waits for each
-  // batched form: pinned staging ring for the view-descriptor table and its
-  // device copy (vrt_batch.h)
-  static constexpr uint32_t STAGES = 4;
-  void* desc_host[STAGES] = {};
-  hipEvent_t desc_done[STAGES] = {};
-  uint32_t desc_cap = 0;  // views per staging buffer
-  uint32_t desc_next = 0;
-  vrt::DeviceBuffer desc_dev;
-  int render_batch(const vrt_view* views, uint32_t nviews, ViewState** states, hipEvent_t* ev);
+  // batched form of vrt_render, on the context's stream
+  vrt::BatchLane lane0;
+  // Enqueue the batched passes for nviews views on `lane`: pathtrace ->
+  // temporal -> a-trous x3 (views with denoise) -> hdr (views whose hdr
+  // pointer is set).  ev[0..3]: pass boundaries.  vcount/hdr: per-view
+  // device counters (4 x u64) and HDR outputs, either array may be null.
+  int render_batch(vrt::BatchLane& lane, const vrt_view* views, uint32_t nviews, ViewState** states,
+                   hipEvent_t* ev, unsigned long long* const* vcount, float* const* hdr);
+  // serialises the C ABI with the serving worker thread
+  std::mutex mu;
+  std::unique_ptr<vrt::Server> server;
+  // make the context's stream wait for every running serve batch
+  int order_after_serve();
   int trace_blocks = 0;  // 0 = default grid for k_trace_rays
   vrt_tuning tuning{};   // kernel variants + launch shapes (vrt_tuning_set / vrt_autotune)
   vrt::DeviceBuffer d_counters;  // u64 x4: rays, brick steps, voxel steps, hits
@@ -105,7 +137,13 @@ struct vrt_ctx {
   int reserve(uint64_t bytes);
   void unreserve(uint64_t bytes);
   int counters_to_host(uint64_t out[4]);
-  int view_state(uint32_t id, uint32_t w, uint32_t h, uint32_t fmt, ViewState** out);
+  int validate_view(const vrt_view& v);
+  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+  double now_ms() const {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+  int view_state(uint32_t id, uint32_t w, uint32_t h, uint32_t fmt, ViewState** out,
+                 hipStream_t st = nullptr);  // st: stream for the initial clears (default: the context's)
   hipError_t launch_trace(const vrt::SceneDev& s, const float* rays, uint64_t n, float tmax,
                           void* hits);
   hipError_t launch_pathtrace(const vrt::SceneDev& s, const vrt_uniforms& u, float fclamp,

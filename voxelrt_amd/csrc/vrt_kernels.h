@@ -11,29 +11,37 @@ namespace vrt {
 using Uniforms = vrt_uniforms;
 struct SceneDev;
 
+// Trace variant ids for the one-ray/pixel-per-lane kernels (launch_* `tv`).
+constexpr int TV_NESTED = 0, TV_FLAT = 1, TV_LDS = 2;
+// Largest occupancy bitmap staged in LDS (words): 48 KiB, 393216 bricks
+// (512^3 = 262144 bricks fits).  Larger scenes fall back to TV_NESTED.
+constexpr uint32_t LDS_BIT_WORDS_MAX = 12288;
+
 hipError_t launch_brick_masks(hipStream_t st, const uint32_t* vox, int nx, int ny, int bgx,
                               int bgy, int bz0, int nbz, uint32_t* masks, uint32_t* flags);
 hipError_t launch_brick_words(hipStream_t st, const uint32_t* flags, const uint32_t* slots,
                               uint64_t nb, uint32_t* bricks);
+hipError_t launch_brick_bits(hipStream_t st, const uint32_t* bricks, uint64_t nb, uint32_t* bits);
 hipError_t launch_pack_pool(hipStream_t st, const uint32_t* vox, int nx, int ny, int bgx, int bgy,
                             int bz0, int nbz, const uint32_t* masks, const uint32_t* bricks,
                             uint32_t* pool);
 hipError_t launch_trace_rays(hipStream_t st, const SceneDev& s, const float* rays, uint64_t n,
-                             float tmax, void* hits, unsigned long long* counters, int blocks);
+                             float tmax, void* hits, unsigned long long* counters, int blocks, int tv);
 hipError_t launch_trace_persistent(hipStream_t st, const SceneDev& s, const float* rays, uint32_t n,
                                   float tmax, void* hits, unsigned long long* counters,
                                   uint32_t* queue, int blocks, int steps, int min_refill);
 hipError_t launch_pathtrace(hipStream_t st, const SceneDev& s, const Uniforms& u, float fclamp,
                             uint32_t spp, void* radiance, void* albedo, void* gbuf,
-                            unsigned long long* counters);
+                            unsigned long long* counters, int tv, int waves);
 hipError_t launch_pathtrace_persistent(hipStream_t st, const SceneDev& s, const Uniforms& u, float fclamp,
                                        uint32_t spp, void* radiance, void* albedo, void* gbuf,
                                        unsigned long long* counters, uint32_t* queue, int blocks,
                                        int steps, int min_refill, int chunk);
 struct ViewDesc;
 hipError_t launch_pathtrace_batch(hipStream_t st, const ViewDesc* views, uint32_t nviews, uint32_t tiles,
-                                  unsigned long long* counters);
+                                  unsigned long long* counters, int tv, int waves, uint32_t max_bit_words);
 hipError_t launch_temporal_batch(hipStream_t st, const ViewDesc* views, uint32_t nviews, uint32_t tiles);
+hipError_t launch_hdr_batch(hipStream_t st, const ViewDesc* views, uint32_t nviews, uint32_t tiles);
 hipError_t launch_atrous_batch(hipStream_t st, const ViewDesc* views, uint32_t nviews, uint32_t tiles, int pass);
 hipError_t launch_temporal(hipStream_t st, const Uniforms& u, const void* radiance,
                            const void* gcur, const void* aprev, const void* gprev, void* aout,

@@ -17,7 +17,7 @@
 #include "vrt_device.h"
 #include "vrt_kernels.h"
 #include "vrt_trace.h"
-#include "vrt_dda_state.h"
+#include "vrt_dda_flat.h"
 #include "vrt_batch.h"
 
 namespace vrt {
@@ -87,6 +87,19 @@ __global__ __launch_bounds__(256) void k_brick_words(const uint32_t* __restrict_
   bricks[i] = flags[i] ? slots[i] + 1u : 0u;
 }
 
+// bits: one occupancy bit per brick, 64 bricks per wave (one ballot).  The
+// buffer holds whole 64-bit groups, so the tail bits of the last group are 0.
+__global__ __launch_bounds__(256) void k_brick_bits(const uint32_t* __restrict__ bricks, uint64_t nb,
+                                                    uint32_t* __restrict__ bits) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if ((i & ~63ull) >= nb) return;  // whole wave past the last group
+  const uint64_t b = __ballot(i < nb && bricks[i < nb ? i : 0] != 0u);
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint64_t g = i >> 6;  // 64-bit group of this wave
+  if (lane == 0u) bits[2 * g] = (uint32_t)b;
+  if (lane == 1u) bits[2 * g + 1] = (uint32_t)(b >> 32);
+}
+
 // One wave per brick of the slab; occupied bricks copy mask + voxels.
 __global__ __launch_bounds__(256) void k_pack_pool(const uint32_t* __restrict__ vox, int nx, int ny,
                                                    int bgx, int bgy, int bz0, int nbz,
@@ -114,13 +127,49 @@ __global__ __launch_bounds__(256) void k_pack_pool(const uint32_t* __restrict__ 
   }
 }
 
+// Trace variants of the one-ray-per-lane kernels (all give identical results):
+//   TV_NESTED  nested brick/voxel loops reading HBM/L2 (vrt_trace.h)
+//   TV_FLAT    single-stream DDA (vrt_dda_flat.h)
+//   TV_LDS     nested loops on the LDS-staged occupancy bitmap + per-lane
+//              brick mask slot (LdsMem); launched with lds_bytes() of dynamic LDS
+// (ids in vrt_kernels.h)
+
+template <int TV>
+VRT_D void trace_sel(const SceneDev& s, const LdsMem& L, f3 ro, f3 rd, float tmax, TraceResult& h) {
+  if constexpr (TV == TV_FLAT)
+    trace_flat(s, ro, rd, tmax, h);
+  else if constexpr (TV == TV_LDS)
+    trace_m(s, L, ro, rd, tmax, h);
+  else
+    trace(s, ro, rd, tmax, h);
+}
+
+// Dynamic LDS of a TV_LDS block: 16 mask words per lane (word-major), then the
+// scene's occupancy bitmap.  Every thread of the block must call this.
+template <int TV>
+VRT_D LdsMem lds_setup(const SceneDev& s, uint32_t* lds) {
+  LdsMem L{nullptr, nullptr, 0u};
+  if constexpr (TV == TV_LDS) {
+    uint32_t* bits = lds + 16u * blockDim.x;
+    for (uint32_t i = threadIdx.x; i < s.bit_words; i += blockDim.x) bits[i] = s.bits[i];
+    __syncthreads();
+    L.bits = bits;
+    L.mask = lds + threadIdx.x;
+    L.stride = blockDim.x;
+  }
+  return L;
+}
+
 // ---------------------------------------------------------------------------
 // traversal microbench: one ray per lane, grid-stride
 // ---------------------------------------------------------------------------
+template <int TV>
 __global__ __launch_bounds__(256) void k_trace_rays(SceneDev s, const float* __restrict__ rays,
                                                     uint64_t n, float tmax,
                                                     uint4* __restrict__ hits,
                                                     unsigned long long* __restrict__ counters) {
+  extern __shared__ uint32_t lds_dyn[];
+  const LdsMem L = lds_setup<TV>(s, lds_dyn);
   uint32_t nb = 0, nv = 0, nh = 0, nr = 0;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   // every lane of a wave runs the same number of outer iterations so the
@@ -129,7 +178,7 @@ __global__ __launch_bounds__(256) void k_trace_rays(SceneDev s, const float* __r
   for (uint64_t i = base0; i < n; i += stride) {
     const float* r = rays + i * 6;
     TraceResult h;
-    trace(s, mk3(r[0], r[1], r[2]), mk3(r[3], r[4], r[5]), tmax, h);
+    trace_sel<TV>(s, L, mk3(r[0], r[1], r[2]), mk3(r[3], r[4], r[5]), tmax, h);
     uint4 rec;
     rec.x = __builtin_bit_cast(uint32_t, h.t);
     rec.y = h.voxel;
@@ -147,7 +196,7 @@ __global__ __launch_bounds__(256) void k_trace_rays(SceneDev s, const float* __r
 // ---------------------------------------------------------------------------
 // traversal microbench, persistent form: each lane pulls rays from a global
 // queue (one atomic per wave refill) and advances a resumable DDA state one
-// loop iteration at a time (vrt_dda_state.h).  `steps` iterations run between
+// loop iteration at a time (vrt_dda_flat.h).  `steps` iterations run between
 // refill checks; a refill happens once at least `min_refill` lanes are idle
 // (or the whole wave is).  Rays are taken from the global counter in chunks of
 // QUEUE_CHUNK per wave, so the counter sees one atomic per 256 rays instead of
@@ -175,7 +224,7 @@ __global__ __launch_bounds__(256) void k_trace_persistent(SceneDev s, const floa
   const uint32_t lane = threadIdx.x & 63u;
   const uint64_t lt_mask = lane == 0u ? 0ull : (~0ull >> (64u - lane));
   uint32_t ray = RAY_NONE;
-  DdaState st;
+  DdaFlat st;
   uint32_t nb = 0, nv = 0, nh = 0, nr = 0;
   // the wave's current chunk of the queue: [cbase, cbase + cleft)
   uint32_t cbase = 0u, cleft = 0u;
@@ -311,7 +360,8 @@ struct RayCount {
   }
 };
 
-VRT_D PixelOut shade_pixel(const SceneDev& s, const Uniforms& u, float fclamp, uint32_t px,
+template <int TV>
+VRT_D PixelOut shade_pixel(const SceneDev& s, const LdsMem& L, const Uniforms& u, float fclamp, uint32_t px,
                            uint32_t py, uint32_t frame, uint32_t w, uint32_t h, RayCount& rc) {
   PixelOut res;
   uint32_t rng = init_rng(px, py, frame);
@@ -323,7 +373,7 @@ VRT_D PixelOut shade_pixel(const SceneDev& s, const Uniforms& u, float fclamp, u
   f3 rd = camera_ray(u, uvx, uvy);
 
   TraceResult h0;
-  trace(s, ro, rd, 1e4f, h0);
+  trace_sel<TV>(s, L, ro, rd, 1e4f, h0);
   rc.add(h0);
   const bool indirect = (u.params0[2] & VRT_FLAG_INDIRECT) != 0u;
   if (h0.t < 0.0f) {
@@ -352,7 +402,7 @@ VRT_D PixelOut shade_pixel(const SceneDev& s, const Uniforms& u, float fclamp, u
     const float ndl = dot(n, l);
     if (ndl > 0.0f && l.y > 0.0f && direct) {
       TraceResult hs;
-      trace(s, surf, l, 1e4f, hs);
+      trace_sel<TV>(s, L, surf, l, 1e4f, hs);
       rc.add(hs);
       if (!(hs.t >= 0.0f)) radiance = radiance + (((tp * mat.albedo) * INV_PI_F) * ndl) * sunRad;
     }
@@ -365,7 +415,7 @@ VRT_D PixelOut shade_pixel(const SceneDev& s, const Uniforms& u, float fclamp, u
       tp = tp / p;
     }
     TraceResult hb;
-    trace(s, surf, rd, 1e4f, hb);
+    trace_sel<TV>(s, L, surf, rd, 1e4f, hb);
     rc.add(hb);
     if (hb.t < 0.0f) {
       radiance = radiance + tp * sky_color(u, rd, false);
@@ -392,7 +442,8 @@ VRT_D bool tile_pixel(uint32_t w, uint32_t h, uint32_t tile, uint32_t& px, uint3
 }
 
 // One 16x16 tile of pathtrace.wgsl:248-330 (one pixel per lane).
-VRT_D void pathtrace_tile(const SceneDev& s, const Uniforms& u, float fclamp, uint32_t spp, uint32_t tile,
+template <int TV>
+VRT_D void pathtrace_tile(const SceneDev& s, const LdsMem& L, const Uniforms& u, float fclamp, uint32_t spp, uint32_t tile,
                           ushort4* __restrict__ radiance, uchar4* __restrict__ albedo,
                           float4* __restrict__ gbuf, RayCount& rc) {
   const uint32_t w = (uint32_t)u.params1[0], h = (uint32_t)u.params1[1];
@@ -402,7 +453,7 @@ VRT_D void pathtrace_tile(const SceneDev& s, const Uniforms& u, float fclamp, ui
     f3 acc = mk3(0.0f, 0.0f, 0.0f);
     PixelOut first;
     for (uint32_t smp = 0; smp < spp; ++smp) {
-      PixelOut r = shade_pixel(s, u, fclamp, px, py, u.params0[0] * spp + smp, w, h, rc);
+      PixelOut r = shade_pixel<TV>(s, L, u, fclamp, px, py, u.params0[0] * spp + smp, w, h, rc);
       if (smp == 0) first = r;
       acc = acc + r.radiance;
     }
@@ -415,13 +466,20 @@ VRT_D void pathtrace_tile(const SceneDev& s, const Uniforms& u, float fclamp, ui
   }
 }
 
-__global__ __launch_bounds__(256) void k_pathtrace(SceneDev s, Uniforms u, float fclamp,
+// WV: minimum waves per SIMD the register allocation must allow (1 = no
+// constraint).  The path tracer's natural allocation (~142 VGPRs) leaves 3
+// waves per SIMD to hide the DDA's dependent loads; 4 or 5 trade a few
+// spills for more waves in flight.
+template <int TV, int WV>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WV))) void k_pathtrace(SceneDev s, Uniforms u, float fclamp,
                                                    uint32_t spp, ushort4* __restrict__ radiance,
                                                    uchar4* __restrict__ albedo,
                                                    float4* __restrict__ gbuf,
                                                    unsigned long long* __restrict__ counters) {
+  extern __shared__ uint32_t lds_dyn[];
+  const LdsMem L = lds_setup<TV>(s, lds_dyn);
   RayCount rc{0u, 0u, 0u, 0u};
-  pathtrace_tile(s, u, fclamp, spp, blockIdx.x, radiance, albedo, gbuf, rc);
+  pathtrace_tile<TV>(s, L, u, fclamp, spp, blockIdx.x, radiance, albedo, gbuf, rc);
   count4(counters, rc.rays, rc.bsteps, rc.vsteps, rc.hits);
 }
 
@@ -613,14 +671,18 @@ __global__ __launch_bounds__(256) void k_atrous(Uniforms u, const void* __restri
 // ---------------------------------------------------------------------------
 // batched form (vrt_batch.h): one launch per pass over every view's tiles
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_pathtrace_batch(const ViewDesc* __restrict__ views, uint32_t nviews,
+template <int TV, int WV>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WV))) void k_pathtrace_batch(const ViewDesc* __restrict__ views, uint32_t nviews,
                                                          unsigned long long* __restrict__ counters) {
+  extern __shared__ uint32_t lds_dyn[];
   const uint32_t vi = find_view(views, nviews, blockIdx.x);
   const ViewDesc& d = views[vi];
+  const LdsMem L = lds_setup<TV>(d.s, lds_dyn);
   RayCount rc{0u, 0u, 0u, 0u};
-  pathtrace_tile(d.s, d.u, d.fclamp, d.spp, blockIdx.x - d.tile0, (ushort4*)d.radiance, (uchar4*)d.albedo,
+  pathtrace_tile<TV>(d.s, L, d.u, d.fclamp, d.spp, blockIdx.x - d.tile0, (ushort4*)d.radiance, (uchar4*)d.albedo,
                  (float4*)d.gbuf_cur, rc);
   count4(counters, rc.rays, rc.bsteps, rc.vsteps, rc.hits);
+  count4(d.vcount, rc.rays, rc.bsteps, rc.vsteps, rc.hits);
 }
 
 __global__ __launch_bounds__(256) void k_temporal_batch(const ViewDesc* __restrict__ views, uint32_t nviews) {
@@ -673,6 +735,24 @@ __global__ __launch_bounds__(256) void k_hdr(uint32_t npx, const void* __restric
   hdr[i * 3 + 2] = (float)((double)il.z * ab * ab);
 }
 
+// HDR of every view that asks for it (d.hdr != null): the denoised target
+// when denoise is on, else the accumulation, as k_hdr.
+__global__ __launch_bounds__(256) void k_hdr_batch(const ViewDesc* __restrict__ views, uint32_t nviews) {
+  const uint32_t vi = find_view(views, nviews, blockIdx.x);
+  const ViewDesc& d = views[vi];
+  if (d.hdr == nullptr) return;
+  uint32_t px, py;
+  if (!tile_pixel(d.w, d.h, blockIdx.x - d.tile0, px, py)) return;
+  const uint64_t i = (uint64_t)py * d.w + px;
+  const float4 il = d.denoise ? load_accum<0>(d.den0, i)
+                              : (d.fmt == 1u ? load_accum<1>(d.accum_cur, i) : load_accum<0>(d.accum_cur, i));
+  const uchar4 a = ((const uchar4*)d.albedo)[i];
+  const double ar = (double)a.x / 255.0, ag = (double)a.y / 255.0, ab = (double)a.z / 255.0;
+  d.hdr[i * 3 + 0] = (float)((double)il.x * ar * ar);
+  d.hdr[i * 3 + 1] = (float)((double)il.y * ag * ag);
+  d.hdr[i * 3 + 2] = (float)((double)il.z * ab * ab);
+}
+
 }  // namespace vrt
 
 #include "vrt_pathtrace_persistent.h"
@@ -683,6 +763,11 @@ __global__ __launch_bounds__(256) void k_hdr(uint32_t npx, const void* __restric
 namespace vrt {
 
 static inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+// TV_LDS launches: 16 mask words per lane (256-thread blocks) + the bitmap,
+// kept within 64 KiB of LDS per block.
+static inline bool lds_fits(const SceneDev& s) { return s.bit_words <= LDS_BIT_WORDS_MAX; }
+static inline uint32_t lds_bytes(const SceneDev& s) { return (16u * 256u + s.bit_words) * 4u; }
 
 hipError_t launch_brick_masks(hipStream_t st, const uint32_t* vox, int nx, int ny, int bgx,
                               int bgy, int bz0, int nbz, uint32_t* masks, uint32_t* flags) {
@@ -699,6 +784,11 @@ hipError_t launch_brick_words(hipStream_t st, const uint32_t* flags, const uint3
   return hipGetLastError();
 }
 
+hipError_t launch_brick_bits(hipStream_t st, const uint32_t* bricks, uint64_t nb, uint32_t* bits) {
+  hipLaunchKernelGGL(k_brick_bits, dim3(div_up(nb, 256)), dim3(256), 0, st, bricks, nb, bits);
+  return hipGetLastError();
+}
+
 hipError_t launch_pack_pool(hipStream_t st, const uint32_t* vox, int nx, int ny, int bgx, int bgy,
                             int bz0, int nbz, const uint32_t* masks, const uint32_t* bricks,
                             uint32_t* pool) {
@@ -709,12 +799,18 @@ hipError_t launch_pack_pool(hipStream_t st, const uint32_t* vox, int nx, int ny,
 }
 
 hipError_t launch_trace_rays(hipStream_t st, const SceneDev& s, const float* rays, uint64_t n,
-                             float tmax, void* hits, unsigned long long* counters, int blocks) {
+                             float tmax, void* hits, unsigned long long* counters, int blocks, int tv) {
   if (blocks <= 0) blocks = 256 * 16;
   const uint32_t need = div_up(n, 256);
   const uint32_t nb = need < (uint32_t)blocks ? need : (uint32_t)blocks;
-  hipLaunchKernelGGL(k_trace_rays, dim3(nb > 0 ? nb : 1), dim3(256), 0, st, s, rays, n, tmax,
-                     (uint4*)hits, counters);
+  const dim3 g(nb > 0 ? nb : 1);
+  if (tv == TV_LDS && lds_fits(s))
+    hipLaunchKernelGGL(k_trace_rays<TV_LDS>, g, dim3(256), lds_bytes(s), st, s, rays, n, tmax, (uint4*)hits,
+                       counters);
+  else if (tv == TV_FLAT)
+    hipLaunchKernelGGL(k_trace_rays<TV_FLAT>, g, dim3(256), 0, st, s, rays, n, tmax, (uint4*)hits, counters);
+  else
+    hipLaunchKernelGGL(k_trace_rays<TV_NESTED>, g, dim3(256), 0, st, s, rays, n, tmax, (uint4*)hits, counters);
   return hipGetLastError();
 }
 
@@ -731,11 +827,26 @@ hipError_t launch_trace_persistent(hipStream_t st, const SceneDev& s, const floa
 
 hipError_t launch_pathtrace(hipStream_t st, const SceneDev& s, const Uniforms& u, float fclamp,
                             uint32_t spp, void* radiance, void* albedo, void* gbuf,
-                            unsigned long long* counters) {
+                            unsigned long long* counters, int tv, int waves) {
   const uint32_t w = (uint32_t)u.params1[0], h = (uint32_t)u.params1[1];
   const uint32_t tiles = div_up(w, 16) * div_up(h, 16);
-  hipLaunchKernelGGL(k_pathtrace, dim3(tiles), dim3(256), 0, st, s, u, fclamp, spp,
-                     (ushort4*)radiance, (uchar4*)albedo, (float4*)gbuf, counters);
+  if (tv == TV_LDS && !lds_fits(s)) tv = TV_NESTED;
+  const uint32_t shmem = tv == TV_LDS ? lds_bytes(s) : 0u;
+  auto go = [&](auto kernel) {
+    hipLaunchKernelGGL(kernel, dim3(tiles), dim3(256), shmem, st, s, u, fclamp, spp, (ushort4*)radiance,
+                       (uchar4*)albedo, (float4*)gbuf, counters);
+  };
+  switch (tv * 8 + (waves == 4 || waves == 5 ? waves : 1)) {
+    case TV_LDS * 8 + 4: go(k_pathtrace<TV_LDS, 4>); break;
+    case TV_LDS * 8 + 5: go(k_pathtrace<TV_LDS, 5>); break;
+    case TV_LDS * 8 + 1: go(k_pathtrace<TV_LDS, 1>); break;
+    case TV_FLAT * 8 + 4: go(k_pathtrace<TV_FLAT, 4>); break;
+    case TV_FLAT * 8 + 5: go(k_pathtrace<TV_FLAT, 5>); break;
+    case TV_FLAT * 8 + 1: go(k_pathtrace<TV_FLAT, 1>); break;
+    case TV_NESTED * 8 + 4: go(k_pathtrace<TV_NESTED, 4>); break;
+    case TV_NESTED * 8 + 5: go(k_pathtrace<TV_NESTED, 5>); break;
+    default: go(k_pathtrace<TV_NESTED, 1>); break;
+  }
   return hipGetLastError();
 }
 
@@ -753,8 +864,28 @@ hipError_t launch_pathtrace_persistent(hipStream_t st, const SceneDev& s, const 
 }
 
 hipError_t launch_pathtrace_batch(hipStream_t st, const ViewDesc* views, uint32_t nviews, uint32_t tiles,
-                                  unsigned long long* counters) {
-  hipLaunchKernelGGL(k_pathtrace_batch, dim3(tiles), dim3(256), 0, st, views, nviews, counters);
+                                  unsigned long long* counters, int tv, int waves, uint32_t max_bit_words) {
+  if (tv == TV_LDS && max_bit_words > LDS_BIT_WORDS_MAX) tv = TV_NESTED;
+  const uint32_t shmem = tv == TV_LDS ? (16u * 256u + max_bit_words) * 4u : 0u;
+  auto go = [&](auto kernel) {
+    hipLaunchKernelGGL(kernel, dim3(tiles), dim3(256), shmem, st, views, nviews, counters);
+  };
+  switch (tv * 8 + (waves == 4 || waves == 5 ? waves : 1)) {
+    case TV_LDS * 8 + 4: go(k_pathtrace_batch<TV_LDS, 4>); break;
+    case TV_LDS * 8 + 5: go(k_pathtrace_batch<TV_LDS, 5>); break;
+    case TV_LDS * 8 + 1: go(k_pathtrace_batch<TV_LDS, 1>); break;
+    case TV_FLAT * 8 + 4: go(k_pathtrace_batch<TV_FLAT, 4>); break;
+    case TV_FLAT * 8 + 5: go(k_pathtrace_batch<TV_FLAT, 5>); break;
+    case TV_FLAT * 8 + 1: go(k_pathtrace_batch<TV_FLAT, 1>); break;
+    case TV_NESTED * 8 + 4: go(k_pathtrace_batch<TV_NESTED, 4>); break;
+    case TV_NESTED * 8 + 5: go(k_pathtrace_batch<TV_NESTED, 5>); break;
+    default: go(k_pathtrace_batch<TV_NESTED, 1>); break;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_hdr_batch(hipStream_t st, const ViewDesc* views, uint32_t nviews, uint32_t tiles) {
+  hipLaunchKernelGGL(k_hdr_batch, dim3(tiles), dim3(256), 0, st, views, nviews);
   return hipGetLastError();
 }
 

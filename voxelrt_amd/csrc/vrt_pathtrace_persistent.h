@@ -16,7 +16,7 @@
 // Every path consumes its PCG stream in exactly the order of the reference
 // loop, so results equal the one-pixel-per-lane kernel bit for bit.
 #pragma once
-#include "vrt_dda_state.h"
+#include "vrt_dda_flat.h"
 
 namespace vrt {
 
@@ -51,7 +51,7 @@ VRT_D f3 primary_dir(const Uniforms& u, uint32_t spp, PathLane& L, uint32_t w, u
 // ray begin, or to the end of the pixel (phase -> PH_NONE).  Written with one
 // ray-begin site and one sky evaluation so each is emitted once.
 VRT_D void path_transition(const SceneDev& s, const Uniforms& u, float fclamp, uint32_t spp, PathLane& L,
-                           DdaState& st, TraceResult& o, bool have_result, ushort4* __restrict__ radiance_out,
+                           DdaFlat& st, TraceResult& o, bool have_result, ushort4* __restrict__ radiance_out,
                            uchar4* __restrict__ albedo_out, float4* __restrict__ gbuf_out, RayCount& rc,
                            uint32_t w, uint32_t h) {
   const uint32_t nb = u.params0[1];
@@ -203,7 +203,7 @@ __global__ __launch_bounds__(256) void k_pathtrace_persistent(SceneDev s, Unifor
   PathLane L;
   L.phase = PH_NONE;
   L.fin = false;
-  DdaState st;
+  DdaFlat st;
   TraceResult o;
   RayCount rc{0u, 0u, 0u, 0u};
   uint32_t cbase = 0u, cleft = 0u;

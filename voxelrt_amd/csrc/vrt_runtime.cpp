@@ -23,6 +23,7 @@
 #include "vrt_kernels.h"
 #include "vrt_trace.h"
 #include "vrt_batch.h"
+#include "vrt_serve.h"
 
 namespace vrt {
 hipError_t exclusive_scan_u32(hipStream_t st, void* d_tmp, size_t* tmp_bytes, const uint32_t* in,
@@ -112,6 +113,8 @@ vrt::SceneDev vrt_scene::dev() const {
   vrt::SceneDev s;
   s.bricks = (const uint32_t*)d_bricks.ptr;
   s.pool = (const uint32_t*)d_pool.ptr;
+  s.bits = (const uint32_t*)d_bits.ptr;
+  s.bit_words = (uint32_t)((nbricks + 31) / 32);
   s.nx = (int)nx;
   s.ny = (int)ny;
   s.nz = (int)nz;
@@ -127,7 +130,8 @@ vrt::SceneDev vrt_scene::dev() const {
 // ---------------------------------------------------------------------------
 // view state
 // ---------------------------------------------------------------------------
-int vrt_ctx::view_state(uint32_t id, uint32_t w, uint32_t h, uint32_t fmt, ViewState** out) {
+int vrt_ctx::view_state(uint32_t id, uint32_t w, uint32_t h, uint32_t fmt, ViewState** out, hipStream_t st) {
+  if (!st) st = stream;
   auto it = views.find(id);
   if (it != views.end()) {
     ViewState& v = it->second;
@@ -157,8 +161,8 @@ int vrt_ctx::view_state(uint32_t id, uint32_t w, uint32_t h, uint32_t fmt, ViewS
   for (int k = 0; k < 2 && e == hipSuccess; ++k) e = v.den[k].alloc(px * 8);
   // WebGPU zero-initialises textures: history reads before the first write
   // see zeros (and gbuf.w = 0 fails the depth test).
-  for (int k = 0; k < 2 && e == hipSuccess; ++k) e = hipMemsetAsync(v.gbuf[k].ptr, 0, px * 16, stream);
-  for (int k = 0; k < 2 && e == hipSuccess; ++k) e = hipMemsetAsync(v.accum[k].ptr, 0, abytes, stream);
+  for (int k = 0; k < 2 && e == hipSuccess; ++k) e = hipMemsetAsync(v.gbuf[k].ptr, 0, px * 16, st);
+  for (int k = 0; k < 2 && e == hipSuccess; ++k) e = hipMemsetAsync(v.accum[k].ptr, 0, abytes, st);
   if (e != hipSuccess) {
     unreserve(bytes);
     views.erase(id);
@@ -205,6 +209,7 @@ int vrt_create(const vrt_create_info* info, vrt_ctx** out) {
     delete ctx;
     return VRT_ERR_DEVICE;
   }
+  ctx->lane0.stream = ctx->stream;
   for (auto& e : ctx->ev)
     if (hipEventCreate(&e) != hipSuccess) {
       delete ctx;
@@ -245,7 +250,13 @@ void vrt_destroy(vrt_ctx* ctx) {
   delete ctx;
 }
 
-const char* vrt_last_error(const vrt_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+const char* vrt_last_error(const vrt_ctx* ctx) {
+  if (!ctx) return "null context";
+  vrt_ctx* c = const_cast<vrt_ctx*>(ctx);
+  std::lock_guard<std::mutex> lock_(c->mu);
+  c->err_ret = c->err;  // a stable copy: the serving worker may set err later
+  return c->err_ret.c_str();
+}
 
 // ---------------------------------------------------------------------------
 // scenes
@@ -254,6 +265,7 @@ int vrt_scene_upload(vrt_ctx* ctx, const uint32_t* vox, uint32_t nx, uint32_t ny
                      uint64_t scene_key, vrt_scene** out) {
   if (!ctx || !vox || !out) return ctx ? ctx->fail(VRT_ERR_INVALID, "null argument") : VRT_ERR_INVALID;
   GUARD_BEGIN
+  std::lock_guard<std::mutex> lock_(ctx->mu);
   *out = nullptr;
   if (nx == 0 || ny == 0 || nz == 0 || nx % 8 || ny % 8 || nz % 8)
     return ctx->fail(VRT_ERR_INVALID, "grid dims %ux%ux%u must be positive multiples of 8", nx, ny, nz);
@@ -298,12 +310,13 @@ int vrt_scene_upload(vrt_ctx* ctx, const uint32_t* vox, uint32_t nx, uint32_t ny
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   const uint64_t nocc = (uint64_t)last[0] + last[1];
   sc->nocc = nocc;
-  const uint64_t bytes = nb * 4 + std::max<uint64_t>(nocc, 1) * vrt::POOL_WORDS * 4;
+  const uint64_t bytes = nb * 4 + std::max<uint64_t>(nocc, 1) * vrt::POOL_WORDS * 4 + (nb + 63) / 64 * 8;
   int rc = ctx->reserve(bytes);
   if (rc) return rc;
   sc->bytes = bytes;
   hipError_t e = sc->d_bricks.alloc(nb * 4);
   if (e == hipSuccess) e = sc->d_pool.alloc(std::max<uint64_t>(nocc, 1) * vrt::POOL_WORDS * 4);
+  if (e == hipSuccess) e = sc->d_bits.alloc((nb + 63) / 64 * 8);
   if (e != hipSuccess) {
     ctx->unreserve(bytes);
     return ctx->fail(e == hipErrorOutOfMemory ? VRT_ERR_NO_MEMORY : VRT_ERR_DEVICE,
@@ -316,6 +329,8 @@ int vrt_scene_upload(vrt_ctx* ctx, const uint32_t* vox, uint32_t nx, uint32_t ny
   e = vrt::launch_brick_words(ctx->stream, (const uint32_t*)flags.ptr, (const uint32_t*)slots.ptr, nb,
                               (uint32_t*)sc->d_bricks.ptr);
   if (e != hipSuccess) return undo(VRT_ERR_DEVICE, "brick words", e);
+  e = vrt::launch_brick_bits(ctx->stream, (const uint32_t*)sc->d_bricks.ptr, nb, (uint32_t*)sc->d_bits.ptr);
+  if (e != hipSuccess) return undo(VRT_ERR_DEVICE, "brick bits", e);
   // pass 2: pack occupied bricks into the pool
   for (int bz0 = 0; bz0 < bgz; bz0 += slab) {
     const int nbz = std::min(slab, bgz - bz0);
@@ -342,8 +357,12 @@ int vrt_scene_upload(vrt_ctx* ctx, const uint32_t* vox, uint32_t nx, uint32_t ny
 int vrt_scene_release(vrt_ctx* ctx, vrt_scene* scene) {
   if (!ctx || !scene) return VRT_ERR_INVALID;
   GUARD_BEGIN
+  std::lock_guard<std::mutex> lock_(ctx->mu);
   auto it = std::find(ctx->scenes.begin(), ctx->scenes.end(), scene);
   if (it == ctx->scenes.end()) return ctx->fail(VRT_ERR_INVALID, "unknown scene handle");
+  if (scene->uses > 0)
+    return ctx->fail(VRT_ERR_INVALID, "scene is used by %llu queued or running requests",
+                     (unsigned long long)scene->uses);
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   ctx->unreserve(scene->bytes);
@@ -363,6 +382,7 @@ int vrt_scene_dims(const vrt_scene* scene, uint32_t dims[3], uint64_t* bytes) {
 int vrt_scene_bricks(vrt_ctx* ctx, const vrt_scene* scene, uint32_t* bricks, uint32_t* masks) {
   if (!ctx || !scene) return VRT_ERR_INVALID;
   GUARD_BEGIN
+  std::lock_guard<std::mutex> lock_(ctx->mu);
   HIP_TRY(ctx, hipSetDevice(ctx->device));
   const uint64_t nb = scene->nbricks;
   std::vector<uint32_t> words(nb);
@@ -388,6 +408,12 @@ int vrt_scene_bricks(vrt_ctx* ctx, const vrt_scene* scene, uint32_t* bricks, uin
 // ---------------------------------------------------------------------------
 // traversal
 // ---------------------------------------------------------------------------
+// vrt_tuning kernel id -> trace variant of the one-per-lane kernels:
+// 0 nested, 2 single-stream (flat), 3 LDS-staged occupancy.
+static int lane_variant(uint32_t kernel) {
+  return kernel == 2 ? vrt::TV_FLAT : (kernel == 3 ? vrt::TV_LDS : vrt::TV_NESTED);
+}
+
 hipError_t vrt_ctx::launch_pathtrace(const vrt::SceneDev& s, const vrt_uniforms& u, float fclamp,
                                      uint32_t spp, void* radiance, void* albedo, void* gbuf) {
   if (tuning.pt_kernel == 1)
@@ -396,7 +422,8 @@ hipError_t vrt_ctx::launch_pathtrace(const vrt::SceneDev& s, const vrt_uniforms&
                                             (int)tuning.pt_blocks, (int)tuning.pt_steps,
                                             (int)tuning.pt_min_refill, (int)tuning.pt_chunk);
   return vrt::launch_pathtrace(stream, s, u, fclamp, spp, radiance, albedo, gbuf,
-                               (unsigned long long*)d_counters.ptr);
+                               (unsigned long long*)d_counters.ptr, lane_variant(tuning.pt_kernel),
+                               (int)tuning.pt_waves);
 }
 
 hipError_t vrt_ctx::launch_trace(const vrt::SceneDev& s, const float* rays, uint64_t n, float tmax,
@@ -407,7 +434,7 @@ hipError_t vrt_ctx::launch_trace(const vrt::SceneDev& s, const float* rays, uint
                                         (int)tuning.trace_blocks, (int)tuning.trace_steps,
                                         (int)tuning.trace_min_refill);
   return vrt::launch_trace_rays(stream, s, rays, n, tmax, hits, (unsigned long long*)d_counters.ptr,
-                                trace_blocks);
+                                trace_blocks, lane_variant(tuning.trace_kernel));
 }
 
 int vrt_trace_rays_device(vrt_ctx* ctx, const vrt_scene* scene, const float* d_rays, uint64_t n,
@@ -415,6 +442,7 @@ int vrt_trace_rays_device(vrt_ctx* ctx, const vrt_scene* scene, const float* d_r
   if (!ctx || !scene || (n && (!d_rays || !d_hits)))
     return ctx ? ctx->fail(VRT_ERR_INVALID, "null argument") : VRT_ERR_INVALID;
   GUARD_BEGIN
+  std::lock_guard<std::mutex> lock_(ctx->mu);
   HIP_TRY(ctx, hipSetDevice(ctx->device));
   if (n == 0) { if (ms) *ms = 0.0f; return VRT_OK; }
   const vrt::SceneDev s = scene->dev();
@@ -434,6 +462,7 @@ int vrt_trace_rays(vrt_ctx* ctx, const vrt_scene* scene, const float* rays, uint
   if (!ctx || !scene || (n && (!rays || !hits)))
     return ctx ? ctx->fail(VRT_ERR_INVALID, "null argument") : VRT_ERR_INVALID;
   GUARD_BEGIN
+  std::lock_guard<std::mutex> lock_(ctx->mu);
   HIP_TRY(ctx, hipSetDevice(ctx->device));
   if (n == 0) return VRT_OK;
   HIP_TRY(ctx, ctx->scratch_rays.ensure(n * 24));
@@ -450,7 +479,8 @@ int vrt_trace_rays(vrt_ctx* ctx, const vrt_scene* scene, const float* rays, uint
 // ---------------------------------------------------------------------------
 // frames
 // ---------------------------------------------------------------------------
-static int validate_view(vrt_ctx* ctx, const vrt_view& v) {
+int vrt_ctx::validate_view(const vrt_view& v) {
+  vrt_ctx* ctx = this;
   if (!v.scene) return ctx->fail(VRT_ERR_INVALID, "view without scene");
   if (std::find(ctx->scenes.begin(), ctx->scenes.end(), v.scene) == ctx->scenes.end())
     return ctx->fail(VRT_ERR_INVALID, "view references an unknown scene");
@@ -495,9 +525,17 @@ static int copy_out(vrt_ctx* ctx, ViewState& vs, const vrt_view& v, vrt_frame* f
 int vrt_render(vrt_ctx* ctx, const vrt_view* views, uint32_t nviews, vrt_frame* outs) {
   if (!ctx || (nviews && !views)) return ctx ? ctx->fail(VRT_ERR_INVALID, "null argument") : VRT_ERR_INVALID;
   GUARD_BEGIN
+  std::lock_guard<std::mutex> lock_(ctx->mu);
   HIP_TRY(ctx, hipSetDevice(ctx->device));
   for (uint32_t k = 0; k < nviews; ++k) {
-    int rc = validate_view(ctx, views[k]);
+    int rc = ctx->validate_view(views[k]);
+    if (rc) return rc;
+    for (uint32_t j = 0; j < k; ++j)
+      if (views[j].view_id == views[k].view_id)
+        return ctx->fail(VRT_ERR_INVALID, "view id %u appears twice in one call", views[k].view_id);
+  }
+  {
+    int rc = ctx->order_after_serve();
     if (rc) return rc;
   }
   std::vector<ViewState*> states(nviews);
@@ -524,7 +562,7 @@ int vrt_render(vrt_ctx* ctx, const vrt_view* views, uint32_t nviews, vrt_frame* 
   ctx->ring_count++;
   if (nviews > 1) {
     // batched form: one launch per pass for all views
-    int rc = ctx->render_batch(views, nviews, states.data(), ev);
+    int rc = ctx->render_batch(ctx->lane0, views, nviews, states.data(), ev, nullptr, nullptr);
     if (rc) return rc;
   } else {
   HIP_TRY(ctx, hipEventRecord(ev[0], ctx->stream));
@@ -569,7 +607,10 @@ int vrt_render(vrt_ctx* ctx, const vrt_view* views, uint32_t nviews, vrt_frame* 
     int rc = ctx->harvest(vrt_ctx::RING);
     if (rc) return rc;
   }
-  for (uint32_t k = 0; k < nviews; ++k) states[k]->frames++;
+  for (uint32_t k = 0; k < nviews; ++k) {
+    states[k]->frames++;
+    states[k]->lru = ++ctx->clock;
+  }
   ctx->frames += nviews;
   if (want_counts) {
     uint64_t after[4];
@@ -590,8 +631,11 @@ int vrt_render(vrt_ctx* ctx, const vrt_view* views, uint32_t nviews, vrt_frame* 
 int vrt_view_release(vrt_ctx* ctx, uint32_t view_id) {
   if (!ctx) return VRT_ERR_INVALID;
   GUARD_BEGIN
+  std::lock_guard<std::mutex> lock_(ctx->mu);
   auto it = ctx->views.find(view_id);
   if (it == ctx->views.end()) return ctx->fail(VRT_ERR_INVALID, "unknown view id %u", view_id);
+  if (ctx->server && ctx->server->vuse.count(view_id))
+    return ctx->fail(VRT_ERR_INVALID, "view id %u has queued or running requests", view_id);
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   ctx->unreserve(it->second.bytes);
@@ -606,6 +650,7 @@ int vrt_temporal_pass(vrt_ctx* ctx, const vrt_uniforms* u, const uint16_t* radia
   if (!ctx || !u || !radiance || !gbuf_cur || !accum_prev || !gbuf_prev || !accum_out || accum_format > 1)
     return ctx ? ctx->fail(VRT_ERR_INVALID, "bad argument") : VRT_ERR_INVALID;
   GUARD_BEGIN
+  std::lock_guard<std::mutex> lock_(ctx->mu);
   HIP_TRY(ctx, hipSetDevice(ctx->device));
   const uint64_t px = (uint64_t)u->params1[0] * (uint64_t)u->params1[1];
   const uint64_t ab = px * (accum_format ? 16 : 8);
@@ -631,6 +676,7 @@ int vrt_atrous_pass(vrt_ctx* ctx, const vrt_uniforms* u, const void* src, uint32
   if (!ctx || !u || !src || !gbuf || !dst || src_format > 1 || (step != 1 && step != 2 && step != 4))
     return ctx ? ctx->fail(VRT_ERR_INVALID, "bad argument (step must be 1, 2 or 4)") : VRT_ERR_INVALID;
   GUARD_BEGIN
+  std::lock_guard<std::mutex> lock_(ctx->mu);
   HIP_TRY(ctx, hipSetDevice(ctx->device));
   const uint64_t px = (uint64_t)u->params1[0] * (uint64_t)u->params1[1];
   const uint64_t sb = px * (src_format ? 16 : 8);
@@ -649,15 +695,18 @@ int vrt_atrous_pass(vrt_ctx* ctx, const vrt_uniforms* u, const void* src, uint32
 
 int vrt_tuning_get(vrt_ctx* ctx, vrt_tuning* out) {
   if (!ctx || !out) return VRT_ERR_INVALID;
+  std::lock_guard<std::mutex> lock_(ctx->mu);
   *out = ctx->tuning;
   return VRT_OK;
 }
 
 int vrt_tuning_set(vrt_ctx* ctx, const vrt_tuning* in) {
   if (!ctx || !in) return VRT_ERR_INVALID;
-  if (in->trace_kernel > 1 || in->pt_kernel > 1 || in->trace_blocks > (1u << 20) ||
+  std::lock_guard<std::mutex> lock_(ctx->mu);
+  if (in->trace_kernel > 3 || in->pt_kernel > 3 || in->trace_blocks > (1u << 20) ||
       in->pt_blocks > (1u << 20) || in->trace_steps > 1024 || in->pt_steps > 1024 ||
-      in->trace_min_refill > 64 || in->pt_min_refill > 64 || in->pt_chunk > (1u << 16))
+      in->trace_min_refill > 64 || in->pt_min_refill > 64 || in->pt_chunk > (1u << 16) ||
+      !(in->pt_waves == 0 || in->pt_waves == 4 || in->pt_waves == 5))
     return ctx->fail(VRT_ERR_INVALID, "tuning value out of range");
   ctx->tuning = *in;
   return VRT_OK;
@@ -667,6 +716,7 @@ int vrt_autotune(vrt_ctx* ctx, const vrt_scene* scene, uint64_t nrays, const vrt
                  vrt_tuning* chosen) {
   if (!ctx || !scene) return ctx ? ctx->fail(VRT_ERR_INVALID, "null argument") : VRT_ERR_INVALID;
   GUARD_BEGIN
+  std::lock_guard<std::mutex> lock_(ctx->mu);
   HIP_TRY(ctx, hipSetDevice(ctx->device));
   const vrt::SceneDev s = scene->dev();
   const vrt_tuning start = ctx->tuning;
@@ -720,6 +770,16 @@ int vrt_autotune(vrt_ctx* ctx, const vrt_scene* scene, uint64_t nrays, const vrt
     if (rc) return rc;
     best_ms = ms;
     best.trace_kernel = 0;
+    for (uint32_t k : {2u, 3u}) {
+      ctx->tuning.trace_kernel = k;
+      rc = time_it([&] { return ctx->launch_trace(s, (const float*)ctx->scratch_rays.ptr, n, 1e4f,
+                                                  ctx->scratch_hits.ptr); }, &ms);
+      if (rc) return rc;
+      if (ms < best_ms) {
+        best_ms = ms;
+        best.trace_kernel = k;
+      }
+    }
     for (uint32_t b : blocks)
       for (uint32_t st : steps)
         for (uint32_t rf : refill) {
@@ -750,11 +810,28 @@ int vrt_autotune(vrt_ctx* ctx, const vrt_scene* scene, uint64_t nrays, const vrt
     auto launch = [&] { return ctx->launch_pathtrace(s, *u, 48.0f, 1, rad.ptr, alb.ptr, gb.ptr); };
     ctx->tuning = best;
     ctx->tuning.pt_kernel = 0;
+    ctx->tuning.pt_waves = 0;
     float ms = 0;
     rc = time_it(launch, &ms);
     if (rc) return rc;
     float best_ms = ms;
     best.pt_kernel = 0;
+    best.pt_waves = 0;
+    // one-pixel-per-lane variants x register budgets (waves per SIMD)
+    for (uint32_t k : {0u, 2u, 3u})
+      for (uint32_t wv : {0u, 4u, 5u}) {
+        if (k == 0 && wv == 0) continue;  // measured above
+        ctx->tuning.pt_kernel = k;
+        ctx->tuning.pt_waves = wv;
+        rc = time_it(launch, &ms);
+        if (rc) return rc;
+        if (ms < best_ms) {
+          best_ms = ms;
+          best.pt_kernel = k;
+          best.pt_waves = wv;
+        }
+      }
+    ctx->tuning.pt_waves = best.pt_waves;
     const uint32_t blocks[] = {512, 1024, 2048};
     const uint32_t steps[] = {4, 8, 16};
     const uint32_t refill[] = {16, 32, 48};
@@ -792,6 +869,7 @@ int vrt_autotune(vrt_ctx* ctx, const vrt_scene* scene, uint64_t nrays, const vrt
 int vrt_stats_get(vrt_ctx* ctx, vrt_stats* out) {
   if (!ctx || !out) return VRT_ERR_INVALID;
   GUARD_BEGIN
+  std::lock_guard<std::mutex> lock_(ctx->mu);
   HIP_TRY(ctx, hipSetDevice(ctx->device));
   uint64_t c[4];
   int rc = ctx->counters_to_host(c);
@@ -822,7 +900,12 @@ int vrt_stats_get(vrt_ctx* ctx, vrt_stats* out) {
 int vrt_stats_reset(vrt_ctx* ctx) {
   if (!ctx) return VRT_ERR_INVALID;
   GUARD_BEGIN
+  std::lock_guard<std::mutex> lock_(ctx->mu);
   HIP_TRY(ctx, hipSetDevice(ctx->device));
+  {
+    int rc = ctx->order_after_serve();
+    if (rc) return rc;
+  }
   HIP_TRY(ctx, hipMemsetAsync(ctx->d_counters.ptr, 0, 4 * sizeof(uint64_t), ctx->stream));
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   ctx->frames = 0;
@@ -838,30 +921,54 @@ int vrt_stats_reset(vrt_ctx* ctx) {
 
 }  // extern "C"
 
-// Batched form: fill one ViewDesc per view (vrt_batch.h) in a pinned staging
-// buffer, copy the table to the device, and launch each pass once over the
-// tiles of every view.  A staging buffer is rewritten only after its previous
-// copy has executed (event per buffer), so the host can run ahead of the
-// device by up to STAGES calls.
-int vrt_ctx::render_batch(const vrt_view* views, uint32_t nviews, ViewState** states, hipEvent_t* ev) {
-  if (nviews > desc_cap) {
-    HIP_TRY(this, hipStreamSynchronize(stream));
-    for (uint32_t k = 0; k < STAGES; ++k) {
-      if (desc_host[k]) (void)hipHostFree(desc_host[k]);
-      desc_host[k] = nullptr;
-    }
-    const uint32_t cap = std::max<uint32_t>(nviews, 64);
-    for (uint32_t k = 0; k < STAGES; ++k) {
-      HIP_TRY(this, hipHostMalloc(&desc_host[k], cap * sizeof(vrt::ViewDesc), hipHostMallocDefault));
-      if (!desc_done[k]) HIP_TRY(this, hipEventCreateWithFlags(&desc_done[k], hipEventDisableTiming));
-    }
-    HIP_TRY(this, desc_dev.alloc(cap * sizeof(vrt::ViewDesc)));
-    desc_cap = cap;
+namespace vrt {
+
+BatchLane::~BatchLane() {
+  for (uint32_t k = 0; k < STAGES; ++k) {
+    if (desc_host[k]) (void)hipHostFree(desc_host[k]);
+    if (desc_done[k]) (void)hipEventDestroy(desc_done[k]);
   }
-  const uint32_t stage = desc_next++ % STAGES;
-  HIP_TRY(this, hipEventSynchronize(desc_done[stage]));  // its previous copy has run
-  vrt::ViewDesc* d = (vrt::ViewDesc*)desc_host[stage];
-  uint32_t tiles = 0;
+  if (own_stream && stream) (void)hipStreamDestroy(stream);
+}
+
+hipError_t BatchLane::ensure(uint32_t nviews) {
+  if (nviews <= desc_cap) return hipSuccess;
+  hipError_t e = hipStreamSynchronize(stream);
+  if (e != hipSuccess) return e;
+  for (uint32_t k = 0; k < STAGES; ++k) {
+    if (desc_host[k]) (void)hipHostFree(desc_host[k]);
+    desc_host[k] = nullptr;
+  }
+  desc_cap = 0;
+  const uint32_t cap = std::max<uint32_t>(nviews, 64);
+  for (uint32_t k = 0; k < STAGES; ++k) {
+    e = hipHostMalloc(&desc_host[k], cap * sizeof(ViewDesc), hipHostMallocDefault);
+    if (e != hipSuccess) return e;
+    if (!desc_done[k]) {
+      e = hipEventCreateWithFlags(&desc_done[k], hipEventDisableTiming);
+      if (e != hipSuccess) return e;
+    }
+  }
+  e = desc_dev.alloc(cap * sizeof(ViewDesc));
+  if (e != hipSuccess) return e;
+  desc_cap = cap;
+  return hipSuccess;
+}
+
+}  // namespace vrt
+
+// Batched form: fill one ViewDesc per view (vrt_batch.h) in the lane's pinned
+// staging buffer, copy the table to the device, and launch each pass once
+// over the tiles of every view.
+int vrt_ctx::render_batch(vrt::BatchLane& lane, const vrt_view* views, uint32_t nviews, ViewState** states,
+                          hipEvent_t* ev, unsigned long long* const* vcount, float* const* hdr) {
+  HIP_TRY(this, lane.ensure(nviews));
+  const hipStream_t st = lane.stream;
+  const uint32_t stage = lane.desc_next++ % vrt::BatchLane::STAGES;
+  HIP_TRY(this, hipEventSynchronize(lane.desc_done[stage]));  // its previous copy has run
+  vrt::ViewDesc* d = (vrt::ViewDesc*)lane.desc_host[stage];
+  uint32_t tiles = 0, max_bit_words = 0;
+  bool any_denoise = false, any_hdr = false;
   for (uint32_t k = 0; k < nviews; ++k) {
     const vrt_view& v = views[k];
     ViewState& vs = *states[k];
@@ -885,21 +992,27 @@ int vrt_ctx::render_batch(const vrt_view* views, uint32_t nviews, ViewState** st
     d[k].accum_prev = vs.accum[1 - parity].ptr;
     d[k].den0 = vs.den[0].ptr;
     d[k].den1 = vs.den[1].ptr;
+    d[k].vcount = vcount ? vcount[k] : nullptr;
+    d[k].hdr = hdr ? hdr[k] : nullptr;
+    max_bit_words = std::max(max_bit_words, d[k].s.bit_words);
+    any_denoise |= d[k].denoise != 0u;
+    any_hdr |= d[k].hdr != nullptr;
     tiles += d[k].tiles_x * ((vs.h + 15u) / 16u);
   }
-  const vrt::ViewDesc* dd = (const vrt::ViewDesc*)desc_dev.ptr;
-  HIP_TRY(this, hipMemcpyAsync(desc_dev.ptr, d, nviews * sizeof(vrt::ViewDesc), hipMemcpyHostToDevice, stream));
-  HIP_TRY(this, hipEventRecord(desc_done[stage], stream));
-  HIP_TRY(this, hipEventRecord(ev[0], stream));
-  HIP_TRY(this, vrt::launch_pathtrace_batch(stream, dd, nviews, tiles, (unsigned long long*)d_counters.ptr));
-  HIP_TRY(this, hipEventRecord(ev[1], stream));
-  HIP_TRY(this, vrt::launch_temporal_batch(stream, dd, nviews, tiles));
-  HIP_TRY(this, hipEventRecord(ev[2], stream));
-  bool any_denoise = false;
-  for (uint32_t k = 0; k < nviews; ++k) any_denoise |= (views[k].u.params0[2] & VRT_FLAG_DENOISE) != 0;
+  const vrt::ViewDesc* dd = (const vrt::ViewDesc*)lane.desc_dev.ptr;
+  HIP_TRY(this, hipMemcpyAsync(lane.desc_dev.ptr, d, nviews * sizeof(vrt::ViewDesc), hipMemcpyHostToDevice, st));
+  HIP_TRY(this, hipEventRecord(lane.desc_done[stage], st));
+  HIP_TRY(this, hipEventRecord(ev[0], st));
+  HIP_TRY(this, vrt::launch_pathtrace_batch(st, dd, nviews, tiles, (unsigned long long*)d_counters.ptr,
+                                            lane_variant(tuning.pt_kernel), (int)tuning.pt_waves,
+                                            max_bit_words));
+  HIP_TRY(this, hipEventRecord(ev[1], st));
+  HIP_TRY(this, vrt::launch_temporal_batch(st, dd, nviews, tiles));
+  HIP_TRY(this, hipEventRecord(ev[2], st));
   if (any_denoise)
-    for (int pass = 0; pass < 3; ++pass) HIP_TRY(this, vrt::launch_atrous_batch(stream, dd, nviews, tiles, pass));
-  HIP_TRY(this, hipEventRecord(ev[3], stream));
+    for (int pass = 0; pass < 3; ++pass) HIP_TRY(this, vrt::launch_atrous_batch(st, dd, nviews, tiles, pass));
+  HIP_TRY(this, hipEventRecord(ev[3], st));
+  if (any_hdr) HIP_TRY(this, vrt::launch_hdr_batch(st, dd, nviews, tiles));
   return VRT_OK;
 }
 
@@ -927,6 +1040,7 @@ int vrt_ctx::harvest(uint32_t max_slots) {
 }
 
 vrt_ctx::~vrt_ctx() {
+  server.reset();  // joins the serving worker; its streams drain first
   for (vrt_scene* s : scenes) delete s;
   scenes.clear();
   views.clear();
@@ -935,9 +1049,6 @@ vrt_ctx::~vrt_ctx() {
   for (auto& slot : ring)
     for (auto& e : slot.e)
       if (e) (void)hipEventDestroy(e);
-  for (uint32_t k = 0; k < STAGES; ++k) {
-    if (desc_host[k]) (void)hipHostFree(desc_host[k]);
-    if (desc_done[k]) (void)hipEventDestroy(desc_done[k]);
-  }
+  // lane0 borrows the context's stream (own_stream is false)
   if (stream) (void)hipStreamDestroy(stream);
 }

@@ -22,6 +22,8 @@ constexpr int POOL_WORDS = 16 + 512;
 struct SceneDev {
   const uint32_t* bricks;
   const uint32_t* pool;
+  const uint32_t* bits;  // occupancy bit per brick: bit (i & 31) of word i >> 5
+  uint32_t bit_words;    // ceil(bricks / 32)
   int nx, ny, nz;
   int bgx, bgy, bgz;
   float voxel_size, inv_voxel;
@@ -50,9 +52,49 @@ VRT_D int floor_clamp(float x, int lo, int hi) {
   return (int)vclamp(vfloor(x), (float)lo, (float)hi);
 }
 
+// Where the DDA reads occupancy from.  GlobalMem: the brick word per coarse
+// step and the mask word per fine step straight from HBM/L2 (the access
+// pattern of voxel.wgsl).  LdsMem: the coarse test reads the scene's
+// occupancy bitmap staged in LDS (one bit per brick); on entering an occupied
+// brick its 64-byte mask is copied once into this lane's LDS slot, so every
+// fine step reads LDS.  Same cells tested in the same order, so hits, normals
+// and step counts are identical.
+struct GlobalMem {
+  VRT_D bool brick(const SceneDev& s, uint32_t bi, uint32_t& bw) const {
+    bw = s.bricks[bi];
+    return bw != 0u;
+  }
+  VRT_D void enter(const uint32_t*) const {}
+  VRT_D uint32_t mask_word(const uint32_t* blk, uint32_t w) const { return blk[w]; }
+};
+
+struct LdsMem {
+  const uint32_t* bits;  // LDS copy of SceneDev::bits
+  uint32_t* mask;        // this lane's mask slot: word w at mask[w * stride]
+  uint32_t stride;       // lanes in the block (word-major layout: no bank conflicts)
+  VRT_D bool brick(const SceneDev& s, uint32_t bi, uint32_t& bw) const {
+    if (((bits[bi >> 5] >> (bi & 31u)) & 1u) == 0u) return false;
+    bw = s.bricks[bi];
+    return true;
+  }
+  VRT_D void enter(const uint32_t* blk) const {
+    const uint4* q = (const uint4*)blk;  // pool entries are 2112 B: 16-byte aligned
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const uint4 v = q[k];
+      mask[(4 * k + 0) * stride] = v.x;
+      mask[(4 * k + 1) * stride] = v.y;
+      mask[(4 * k + 2) * stride] = v.z;
+      mask[(4 * k + 3) * stride] = v.w;
+    }
+  }
+  VRT_D uint32_t mask_word(const uint32_t*, uint32_t w) const { return mask[w * stride]; }
+};
+
 // voxel.wgsl:73-153.  Non-cubic grids: clip against [0,n]^3 per axis; the
 // brick-loop bound 3*BGRID+2 becomes bgx+bgy+bgz+2 (identical for cubes).
-VRT_D void trace(const SceneDev& s, f3 roW, f3 rd, float maxTWorld, TraceResult& o) {
+template <class Mem>
+VRT_D void trace_m(const SceneDev& s, const Mem& mem, f3 roW, f3 rd, float maxTWorld, TraceResult& o) {
   o.t = -1.0f;
   o.mat = 0u;
   o.n = mk3(-rd.x, -rd.y, -rd.z);
@@ -105,9 +147,10 @@ VRT_D void trace(const SceneDev& s, f3 roW, f3 rd, float maxTWorld, TraceResult&
     if (t > tExit) break;
     const uint32_t bi = (uint32_t)bpos[0] + (uint32_t)bpos[1] * bstride_y + (uint32_t)bpos[2] * bstride_z;
     ++bsteps;
-    const uint32_t bw = s.bricks[bi];
-    if (bw != 0u) {
+    uint32_t bw;
+    if (mem.brick(s, bi, bw)) {
       const uint32_t* blk = s.pool + (uint64_t)(bw - 1u) * POOL_WORDS;
+      mem.enter(blk);
       int bmin[3] = {bpos[0] * 8, bpos[1] * 8, bpos[2] * 8};
       int vpos[3];
       float tMaxV[3];
@@ -123,7 +166,7 @@ VRT_D void trace(const SceneDev& s, f3 roW, f3 rd, float maxTWorld, TraceResult&
       for (int j = 0; j < 24; ++j) {
         ++vsteps;
         const uint32_t bit = (uint32_t)((vpos[0] - bmin[0]) + (vpos[1] - bmin[1]) * 8 + (vpos[2] - bmin[2]) * 64);
-        const uint32_t word = blk[bit >> 5];
+        const uint32_t word = mem.mask_word(blk, bit >> 5);
         if ((word >> (bit & 31u)) & 1u) {
           const uint32_t m = blk[16 + bit];
           o.bsteps = bsteps;
@@ -166,6 +209,10 @@ VRT_D void trace(const SceneDev& s, f3 roW, f3 rd, float maxTWorld, TraceResult&
   }
   o.bsteps = bsteps;
   o.vsteps = vsteps;
+}
+
+VRT_D void trace(const SceneDev& s, f3 roW, f3 rd, float maxTWorld, TraceResult& o) {
+  trace_m(s, GlobalMem{}, roW, rd, maxTWorld, o);
 }
 
 }  // namespace vrt

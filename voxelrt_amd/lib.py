@@ -80,11 +80,34 @@ class Tuning(C.Structure):
     _fields_ = [("trace_kernel", C.c_uint32), ("trace_blocks", C.c_uint32),
                 ("trace_steps", C.c_uint32), ("trace_min_refill", C.c_uint32),
                 ("pt_kernel", C.c_uint32), ("pt_blocks", C.c_uint32), ("pt_steps", C.c_uint32),
-                ("pt_min_refill", C.c_uint32), ("pt_chunk", C.c_uint32),
-                ("reserved", C.c_uint32 * 7)]
+                ("pt_min_refill", C.c_uint32), ("pt_chunk", C.c_uint32), ("pt_waves", C.c_uint32),
+                ("reserved", C.c_uint32 * 6)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+class Request(C.Structure):
+    _fields_ = [("view", View), ("flags", C.c_uint32), ("_pad", C.c_uint32)]
+
+
+class ServeConfig(C.Structure):
+    _fields_ = [("streams", C.c_uint32), ("max_batch_views", C.c_uint32), ("max_batch_pixels", C.c_uint64)]
+
+
+class Timing(C.Structure):
+    _fields_ = [("submit_ms", C.c_double), ("launch_ms", C.c_double), ("done_ms", C.c_double),
+                ("latency_ms", C.c_double), ("gpu_ms", C.c_double), ("batch_views", C.c_uint32),
+                ("stream", C.c_uint32), ("batch_id", C.c_uint64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ServeStats(C.Structure):
+    _fields_ = [(k, C.c_uint64) for k in ("submitted", "completed", "cancelled", "failed", "batches",
+                                          "batched_requests", "evictions", "queued", "running",
+                                          "max_running_batches")]
 
 
 class SceneDesc(C.Structure):
@@ -153,6 +176,15 @@ def _declare(L):
     L.vrt_generate_scene.argtypes = [C.POINTER(SceneDesc), P]
     L.vrt_scene_spawn.argtypes = [C.POINTER(SceneDesc), P, C.POINTER(C.c_double),
                                   C.POINTER(C.c_double)]
+    L.vrt_serve_configure.argtypes = [P, C.POINTER(ServeConfig)]
+    L.vrt_submit.argtypes = [P, C.POINTER(Request), C.POINTER(C.c_uint64)]
+    L.vrt_poll.argtypes = [P, C.c_uint64, C.POINTER(Frame), C.POINTER(Timing)]
+    L.vrt_wait.argtypes = [P, C.c_uint64, C.c_uint32, C.POINTER(Frame), C.POINTER(Timing)]
+    L.vrt_cancel.argtypes = [P, C.c_uint64]
+    L.vrt_serve_stats_get.argtypes = [P, C.POINTER(ServeStats)]
+    for name in ("vrt_serve_configure", "vrt_submit", "vrt_poll", "vrt_wait", "vrt_cancel",
+                 "vrt_serve_stats_get"):
+        getattr(L, name).restype = I
     for name in ("vrt_create", "vrt_device_count", "vrt_scene_upload", "vrt_scene_release",
                  "vrt_scene_bricks", "vrt_scene_dims", "vrt_trace_rays", "vrt_trace_rays_device",
                  "vrt_render", "vrt_view_release", "vrt_temporal_pass", "vrt_atrous_pass",
@@ -373,3 +405,44 @@ class Context:
 
     def reset_stats(self):
         self._check(self.L.vrt_stats_reset(self.h))
+
+    # ---- serving (continuous batching) ----
+    def serve_configure(self, streams: int = 2, max_batch_views: int = 64,
+                        max_batch_pixels: int = 16 << 20):
+        cfg = ServeConfig(streams, max_batch_views, max_batch_pixels)
+        self._check(self.L.vrt_serve_configure(self.h, C.byref(cfg)))
+
+    def submit(self, view: View) -> int:
+        req = Request(view, 0, 0)
+        t = C.c_uint64()
+        self._check(self.L.vrt_submit(self.h, C.byref(req), C.byref(t)))
+        return t.value
+
+    def _take(self, rc, fr, tm, hdr):
+        if rc == -5:
+            return None
+        self._check(rc)
+        return {"hdr": hdr, "rays": fr.rays, "brick_steps": fr.brick_steps,
+                "voxel_steps": fr.voxel_steps, "timing": tm.as_dict()}
+
+    def poll(self, ticket: int, hdr: np.ndarray = None):
+        """None while the request is queued or running; else a dict with the
+        hdr array (when given), counters and timing.  The ticket is retired."""
+        fr, tm = Frame(), Timing()
+        if hdr is not None:
+            fr.hdr = _ptr(hdr).value
+        return self._take(self.L.vrt_poll(self.h, ticket, C.byref(fr), C.byref(tm)), fr, tm, hdr)
+
+    def wait(self, ticket: int, hdr: np.ndarray = None, timeout_ms: int = 0):
+        fr, tm = Frame(), Timing()
+        if hdr is not None:
+            fr.hdr = _ptr(hdr).value
+        return self._take(self.L.vrt_wait(self.h, ticket, timeout_ms, C.byref(fr), C.byref(tm)), fr, tm, hdr)
+
+    def cancel(self, ticket: int):
+        self._check(self.L.vrt_cancel(self.h, ticket))
+
+    def serve_stats(self) -> dict:
+        s = ServeStats()
+        self._check(self.L.vrt_serve_stats_get(self.h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in ServeStats._fields_}
