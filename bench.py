@@ -88,11 +88,12 @@ def traverse(args):
     d_rays = torch.from_numpy(rays).to("cuda:0")
     d_hits = torch.empty(n_rays * 4, dtype=torch.int32, device="cuda:0")
     torch.cuda.synchronize()
-    variants = {"lane": dict(trace_kernel=0), "lane_flat": dict(trace_kernel=2),
+    variants = {"lane": dict(trace_kernel=0), "lane_flat": dict(trace_kernel=2), "lane_lds": dict(trace_kernel=3),
+                "lane_fast": dict(trace_kernel=4),
                 "persistent": dict(trace_kernel=1, trace_blocks=2048, trace_steps=8, trace_min_refill=16)}
     for p in (0.05, 0.25, 0.6):
         sc = ctx.upload(cf.traverse_scene(p))
-        for name in ("lane", "lane_flat", "persistent", "autotuned"):
+        for name in ("lane", "lane_lds", "lane_fast", "persistent", "autotuned"):
             if name == "autotuned":
                 tuned = ctx.autotune(sc, 1 << 22).as_dict()
             else:
@@ -337,7 +338,8 @@ def main():
     ap.add_argument("--serve-streams", type=int, default=2)
     ap.add_argument("--serve-views", type=int, default=64)
     ap.add_argument("--serve-pixels", type=int, default=16 << 20)
-    ap.add_argument("--pt-kernel", default="auto", choices=["auto", "0", "1", "2"])
+    ap.add_argument("--pt-kernel", default="auto", choices=["auto", "0", "1", "2", "3", "4"])
+    ap.add_argument("--pt-waves", type=int, default=0, choices=[0, 4, 5])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (--gpus {args.gpus} "
@@ -373,7 +375,7 @@ def main():
         r.set_pose(*cf.world_orbit(desc, 0))
         tuned = ctx.autotune(sc, 0, r.view().u).as_dict()
     else:
-        tuned = ctx.set_tuning(pt_kernel=int(args.pt_kernel)).as_dict()
+        tuned = ctx.set_tuning(pt_kernel=int(args.pt_kernel), pt_waves=args.pt_waves).as_dict()
 
     for i in range(args.warmup):
         step(i)
@@ -412,7 +414,8 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": _pmc_traffic("pathtrace_frame1080_pmc.txt"),
                      "alg_bytes_per_launch": int(alg_bytes), "avg_launch_ms": round(pt_avg_ms, 4)},
-        "tuning": {k: tuned[k] for k in ("pt_kernel", "pt_blocks", "pt_steps", "pt_min_refill", "pt_chunk")},
+        "tuning": {k: tuned[k] for k in ("pt_kernel", "pt_waves", "pt_blocks", "pt_steps", "pt_min_refill",
+                                          "pt_chunk")},
         "breakdown": {"gpu_ms_pathtrace": round(pt_avg_ms, 4), "gpu_ms_temporal": round(tp_ms / K, 4),
                       "gpu_ms_atrous3": round(at_ms / K, 4), "gpu_ms_frame": round(fr_ms / K, 4),
                       "rays_per_frame": rays // K, "brick_steps_per_ray": round(bsteps / max(rays, 1), 2),

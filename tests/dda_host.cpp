@@ -1,0 +1,114 @@
+// Host build of the library's device DDA code (voxelrt_amd/csrc), for CPU
+// tests: the scene layout (brick words, pool, bitmap, prefix counts) is built
+// here from a dense grid the way vrt_scene_upload builds it on the device, and
+// each trace variant runs on the CPU over a ray list.  Test infrastructure
+// only; compiled by tests/test_dda_host.py.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "../voxelrt_amd/csrc/vrt_dda_fast.h"
+#include "../voxelrt_amd/csrc/vrt_dda_flat.h"
+#include "../voxelrt_amd/csrc/vrt_trace.h"
+
+namespace {
+
+struct HostScene {
+  std::vector<uint32_t> bricks, pool, bits, prefix;
+  vrt::SceneDev dev{};
+};
+
+void build(const uint32_t* vox, int nx, int ny, int nz, HostScene& h) {
+  const int bgx = nx / 8, bgy = ny / 8, bgz = nz / 8;
+  const uint64_t nb = (uint64_t)bgx * bgy * bgz;
+  h.bricks.assign(nb, 0u);
+  h.pool.clear();
+  uint32_t slot = 0;
+  for (int bz = 0; bz < bgz; ++bz)
+    for (int by = 0; by < bgy; ++by)
+      for (int bx = 0; bx < bgx; ++bx) {
+        uint32_t mask[16] = {0};
+        uint32_t words[512];
+        bool any = false;
+        for (int lz = 0; lz < 8; ++lz)
+          for (int ly = 0; ly < 8; ++ly)
+            for (int lx = 0; lx < 8; ++lx) {
+              const uint64_t vi = (uint64_t)(bx * 8 + lx) + (uint64_t)(by * 8 + ly) * nx +
+                                  (uint64_t)(bz * 8 + lz) * (uint64_t)nx * ny;
+              const uint32_t bit = (uint32_t)(lx + 8 * ly + 64 * lz);
+              words[bit] = vox[vi];
+              if (vox[vi]) {
+                mask[bit >> 5] |= 1u << (bit & 31u);
+                any = true;
+              }
+            }
+        const uint64_t bi = (uint64_t)bx + (uint64_t)by * bgx + (uint64_t)bz * bgx * bgy;
+        if (!any) continue;
+        h.bricks[bi] = ++slot;
+        h.pool.insert(h.pool.end(), mask, mask + 16);
+        h.pool.insert(h.pool.end(), words, words + 512);
+      }
+  if (h.pool.empty()) h.pool.assign(vrt::POOL_WORDS, 0u);
+  const uint64_t nw = (nb + 31) / 32;
+  h.bits.assign(nw, 0u);
+  for (uint64_t i = 0; i < nb; ++i)
+    if (h.bricks[i]) h.bits[i >> 5] |= 1u << (i & 31u);
+  h.prefix.assign(nw, 0u);
+  for (uint64_t w = 1; w < nw; ++w) h.prefix[w] = h.prefix[w - 1] + (uint32_t)__builtin_popcount(h.bits[w - 1]);
+  vrt::SceneDev& s = h.dev;
+  s.bricks = h.bricks.data();
+  s.pool = h.pool.data();
+  s.bits = h.bits.data();
+  s.prefix = h.prefix.data();
+  s.bit_words = (uint32_t)nw;
+  s.nx = nx;
+  s.ny = ny;
+  s.nz = nz;
+  s.bgx = bgx;
+  s.bgy = bgy;
+  s.bgz = bgz;
+  s.voxel_size = 0.0625f;
+  s.inv_voxel = 1.0f / 0.0625f;
+  s.zero = 0.0f;
+}
+
+}  // namespace
+
+// variant: 0 nested (HBM), 1 flat, 2 nested with LDS occupancy, 3 single loop
+// with both levels in LDS.  hits: n records of 4 u32 (t bits, voxel,
+// brick_steps | voxel_steps << 16, face), as vrt_trace_rays writes them.
+extern "C" int dda_host_trace(const uint32_t* vox, int nx, int ny, int nz, const float* rays, uint64_t n,
+                              float tmax, int variant, uint32_t* hits) {
+  if (nx % 8 || ny % 8 || nz % 8 || nx <= 0 || ny <= 0 || nz <= 0) return -1;
+  HostScene h;
+  build(vox, nx, ny, nz, h);
+  // one block's LDS as the kernels lay it out: 256 mask slots, bitmap, prefix
+  const uint32_t nw = h.dev.bit_words;
+  std::vector<uint32_t> lds(16 * 256 + 2 * (size_t)nw, 0u);
+  std::memcpy(lds.data() + 16 * 256, h.bits.data(), nw * 4);
+  std::memcpy(lds.data() + 16 * 256 + nw, h.prefix.data(), nw * 4);
+  const uint32_t lane = 37;  // any slot
+  vrt::LdsMem lm{lds.data() + 16 * 256, lds.data() + lane, 256u};
+  vrt::LdsScene ls{lds.data(), lane, 16u * 256u, 16u * 256u + nw};
+  for (uint64_t i = 0; i < n; ++i) {
+    const float* r = rays + 6 * i;
+    const vrt::f3 ro = vrt::mk3(r[0], r[1], r[2]), rd = vrt::mk3(r[3], r[4], r[5]);
+    vrt::TraceResult o;
+    switch (variant) {
+      case 0: vrt::trace(h.dev, ro, rd, tmax, o); break;
+      case 1: vrt::trace_flat(h.dev, ro, rd, tmax, o); break;
+      case 2: vrt::trace_m(h.dev, lm, ro, rd, tmax, o); break;
+      case 3: vrt::trace_fast(h.dev, ls, ro, rd, tmax, o); break;
+      default: return -1;
+    }
+    uint32_t tb;
+    std::memcpy(&tb, &o.t, 4);
+    hits[4 * i + 0] = tb;
+    hits[4 * i + 1] = o.voxel;
+    hits[4 * i + 2] = (o.bsteps & 0xffffu) | (o.vsteps << 16);
+    hits[4 * i + 3] = o.face;
+  }
+  return 0;
+}

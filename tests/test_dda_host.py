@@ -1,0 +1,93 @@
+"""The library's DDA variants run on the CPU (a host build of the device
+headers, tests/dda_host.cpp) against the oracle: nested loops, single-stream,
+LDS-staged occupancy and the single-loop LDS form must return the same hit
+voxel, face, both step counts and the bits of t for every ray.  This checks
+the variants' logic without a GPU (the GPU tests check the same on the
+device)."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+import voxelrt_amd as v
+from voxelrt_amd import configs as cf
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+VARIANTS = {"nested": 0, "flat": 1, "lds": 2, "fast": 3}
+
+
+@pytest.fixture(scope="module")
+def dda(tmp_path_factory):
+    out = str(tmp_path_factory.mktemp("dda") / "dda_host.so")
+    cmd = ["/opt/rocm/bin/hipcc", "-x", "hip", "--cuda-host-only", "-O2", "-std=c++17", "-ffp-contract=off",
+           "-fPIC", "-shared", os.path.join(ROOT, "tests", "dda_host.cpp"), "-o", out]
+    if not os.path.exists(cmd[0]):
+        pytest.skip("hipcc not available")
+    subprocess.run(cmd, check=True, capture_output=True)
+    L = C.CDLL(out)
+    L.dda_host_trace.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_uint64, C.c_float,
+                                 C.c_int, C.c_void_p]
+    L.dda_host_trace.restype = C.c_int
+
+    def run(vox, rays, variant, tmax=1e4):
+        nz, ny, nx = vox.shape
+        vox = np.ascontiguousarray(vox, np.uint32)
+        rays = np.ascontiguousarray(rays, np.float32)
+        hits = np.zeros(len(rays), v.HIT_DTYPE)
+        rc = L.dda_host_trace(vox.ctypes.data, nx, ny, nz, rays.ctypes.data, len(rays), tmax, variant,
+                              hits.ctypes.data)
+        assert rc == 0
+        return hits
+    return run
+
+
+def _equal(got, want, name):
+    for f in ("voxel", "face", "brick_steps", "voxel_steps"):
+        bad = np.flatnonzero(got[f] != want[f])
+        assert len(bad) == 0, f"{name} {f}: {len(bad)} mismatches, first {bad[:5]}"
+    tb = np.flatnonzero(got["t"].view(np.uint32) != want["t"].view(np.uint32))
+    assert len(tb) == 0, f"{name} t: {len(tb)} mismatches, first {tb[:5]}"
+
+
+def _edge_rays(n):
+    """Axis-aligned and diagonal rays from grid faces, edges and corners,
+    including zero direction components."""
+    ext = n * v.VOXEL_SIZE
+    rays = []
+    for o in [(0, 0, 0), (ext, ext, ext), (ext / 2, 0, ext / 2), (0, ext / 3, ext / 7), (ext, ext / 2, 0),
+              (ext / 2, ext / 2, ext / 2), (-1.0, ext / 2, ext / 2), (ext / 4, ext + 2.0, ext / 4)]:
+        for d in [(1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1), (1, 1, 0), (1, -1, 1),
+                  (-1, -1, -1), (0.6, 0.0, -0.8), (1e-9, 1.0, 0.0), (0.0, 0.0, 0.0)]:
+            dd = np.asarray(d, np.float64)
+            nrm = np.linalg.norm(dd)
+            rays.append(list(o) + list(dd / nrm if nrm > 0 else dd))
+    return np.asarray(rays, np.float32)
+
+
+@pytest.mark.parametrize("variant", list(VARIANTS))
+@pytest.mark.parametrize("shape,p", [((64, 64, 64), 0.05), ((64, 64, 64), 0.25), ((64, 64, 64), 0.6),
+                                     ((32, 48, 64), 0.3), ((16, 8, 24), 1.0)])
+def test_variants_match_oracle(dda, variant, shape, p):
+    nz, ny, nx = shape
+    vox = v.generate_scene(v.SCENE_RANDOM_BRICKS, n=nx, ny=ny, nz=nz, seed=int(p * 100) + nx, p_brick=p,
+                           p_voxel=0.5)
+    n = max(shape)
+    rays = np.concatenate([cf.traverse_rays(20000, n=n, seed=3), _edge_rays(n)])
+    # scale origins to the (possibly non-cubic) box
+    rays[:, :3] *= np.array([nx, ny, nz], np.float32) / n
+    want = oracle.trace_rays(oracle.OracleGrid(vox), rays)
+    got = dda(vox, rays, VARIANTS[variant])
+    _equal(got, want, f"{variant} {shape} p={p}")
+
+
+@pytest.mark.parametrize("variant", list(VARIANTS))
+def test_variants_on_world_scene_with_short_tmax(dda, variant):
+    vox = v.generate_scene(**cf.world_desc(64, 4))
+    rays = cf.traverse_rays(20000, n=64, seed=8)
+    og = oracle.OracleGrid(vox)
+    for tmax in (1e4, 0.75):
+        _equal(dda(vox, rays, VARIANTS[variant], tmax), oracle.trace_rays(og, rays, tmax=tmax),
+               f"{variant} tmax={tmax}")

@@ -15,7 +15,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#define VRT_D __device__ __forceinline__
+// Host and device: the host instantiation lets CPU tests run the same DDA code.
+#define VRT_D __host__ __device__ __forceinline__
 
 namespace vrt {
 

@@ -75,6 +75,7 @@ struct vrt_scene {
   vrt::DeviceBuffer d_bricks;  // one word per 8^3 brick: 0 or 1 + pool slot
   vrt::DeviceBuffer d_pool;    // nocc x 528 words (mask + voxels)
   vrt::DeviceBuffer d_bits;    // one occupancy bit per brick (LDS-staged by the kernels)
+  vrt::DeviceBuffer d_prefix;  // per bitmap word: occupied bricks before it
   vrt::SceneDev dev() const;
 };
 

@@ -12,9 +12,10 @@ using Uniforms = vrt_uniforms;
 struct SceneDev;
 
 // Trace variant ids for the one-ray/pixel-per-lane kernels (launch_* `tv`).
-constexpr int TV_NESTED = 0, TV_FLAT = 1, TV_LDS = 2;
+constexpr int TV_NESTED = 0, TV_FLAT = 1, TV_LDS = 2, TV_FAST = 3;
 // Largest occupancy bitmap staged in LDS (words): 48 KiB, 393216 bricks
 // (512^3 = 262144 bricks fits).  Larger scenes fall back to TV_NESTED.
+// TV_FAST stages the bitmap and its prefix counts: half that many words.
 constexpr uint32_t LDS_BIT_WORDS_MAX = 12288;
 
 hipError_t launch_brick_masks(hipStream_t st, const uint32_t* vox, int nx, int ny, int bgx,
@@ -22,6 +23,7 @@ hipError_t launch_brick_masks(hipStream_t st, const uint32_t* vox, int nx, int n
 hipError_t launch_brick_words(hipStream_t st, const uint32_t* flags, const uint32_t* slots,
                               uint64_t nb, uint32_t* bricks);
 hipError_t launch_brick_bits(hipStream_t st, const uint32_t* bricks, uint64_t nb, uint32_t* bits);
+hipError_t launch_popc(hipStream_t st, const uint32_t* words, uint64_t n, uint32_t* counts);
 hipError_t launch_pack_pool(hipStream_t st, const uint32_t* vox, int nx, int ny, int bgx, int bgy,
                             int bz0, int nbz, const uint32_t* masks, const uint32_t* bricks,
                             uint32_t* pool);

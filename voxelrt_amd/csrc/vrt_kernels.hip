@@ -18,6 +18,7 @@
 #include "vrt_kernels.h"
 #include "vrt_trace.h"
 #include "vrt_dda_flat.h"
+#include "vrt_dda_fast.h"
 #include "vrt_batch.h"
 
 namespace vrt {
@@ -100,6 +101,12 @@ __global__ __launch_bounds__(256) void k_brick_bits(const uint32_t* __restrict__
   if (lane == 1u) bits[2 * g + 1] = (uint32_t)(b >> 32);
 }
 
+__global__ __launch_bounds__(256) void k_popc(const uint32_t* __restrict__ words, uint64_t n,
+                                              uint32_t* __restrict__ counts) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) counts[i] = (uint32_t)__popc(words[i]);
+}
+
 // One wave per brick of the slab; occupied bricks copy mask + voxels.
 __global__ __launch_bounds__(256) void k_pack_pool(const uint32_t* __restrict__ vox, int nx, int ny,
                                                    int bgx, int bgy, int bz0, int nbz,
@@ -134,28 +141,46 @@ __global__ __launch_bounds__(256) void k_pack_pool(const uint32_t* __restrict__ 
 //              brick mask slot (LdsMem); launched with lds_bytes() of dynamic LDS
 // (ids in vrt_kernels.h)
 
+//   TV_FAST    single-loop DDA with both levels in LDS (vrt_dda_fast.h):
+//              bitmap + prefix counts + per-lane mask slot
+// What a lane needs to trace in a given variant.
+struct LaneMem {
+  LdsMem m;    // TV_LDS
+  LdsScene f;  // TV_FAST
+};
+
 template <int TV>
-VRT_D void trace_sel(const SceneDev& s, const LdsMem& L, f3 ro, f3 rd, float tmax, TraceResult& h) {
+VRT_D void trace_sel(const SceneDev& s, const LaneMem& L, f3 ro, f3 rd, float tmax, TraceResult& h) {
   if constexpr (TV == TV_FLAT)
     trace_flat(s, ro, rd, tmax, h);
   else if constexpr (TV == TV_LDS)
-    trace_m(s, L, ro, rd, tmax, h);
+    trace_m(s, L.m, ro, rd, tmax, h);
+  else if constexpr (TV == TV_FAST)
+    trace_fast(s, L.f, ro, rd, tmax, h);
   else
     trace(s, ro, rd, tmax, h);
 }
 
-// Dynamic LDS of a TV_LDS block: 16 mask words per lane (word-major), then the
-// scene's occupancy bitmap.  Every thread of the block must call this.
+// Dynamic LDS of a TV_LDS / TV_FAST block of 256 threads: 16 mask words per
+// lane (word-major: word w of lane l at w * 256 + l), then the scene's
+// occupancy bitmap, then (TV_FAST) its prefix counts.  Every thread of the
+// block must call this.
 template <int TV>
-VRT_D LdsMem lds_setup(const SceneDev& s, uint32_t* lds) {
-  LdsMem L{nullptr, nullptr, 0u};
-  if constexpr (TV == TV_LDS) {
-    uint32_t* bits = lds + 16u * blockDim.x;
+VRT_D LaneMem lds_setup(const SceneDev& s, uint32_t* lds) {
+  LaneMem L{{nullptr, nullptr, 0u}, {nullptr, 0u, 0u, 0u}};
+  if constexpr (TV == TV_LDS || TV == TV_FAST) {
+    uint32_t* bits = lds + 16u * 256u;
     for (uint32_t i = threadIdx.x; i < s.bit_words; i += blockDim.x) bits[i] = s.bits[i];
+    if constexpr (TV == TV_FAST)
+      for (uint32_t i = threadIdx.x; i < s.bit_words; i += blockDim.x) bits[s.bit_words + i] = s.prefix[i];
     __syncthreads();
-    L.bits = bits;
-    L.mask = lds + threadIdx.x;
-    L.stride = blockDim.x;
+    L.m.bits = bits;
+    L.m.mask = lds + threadIdx.x;
+    L.m.stride = 256u;
+    L.f.lds = lds;
+    L.f.lane = threadIdx.x;
+    L.f.bits_off = 16u * 256u;
+    L.f.prefix_off = 16u * 256u + s.bit_words;
   }
   return L;
 }
@@ -169,7 +194,7 @@ __global__ __launch_bounds__(256) void k_trace_rays(SceneDev s, const float* __r
                                                     uint4* __restrict__ hits,
                                                     unsigned long long* __restrict__ counters) {
   extern __shared__ uint32_t lds_dyn[];
-  const LdsMem L = lds_setup<TV>(s, lds_dyn);
+  const LaneMem L = lds_setup<TV>(s, lds_dyn);
   uint32_t nb = 0, nv = 0, nh = 0, nr = 0;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   // every lane of a wave runs the same number of outer iterations so the
@@ -361,7 +386,7 @@ struct RayCount {
 };
 
 template <int TV>
-VRT_D PixelOut shade_pixel(const SceneDev& s, const LdsMem& L, const Uniforms& u, float fclamp, uint32_t px,
+VRT_D PixelOut shade_pixel(const SceneDev& s, const LaneMem& L, const Uniforms& u, float fclamp, uint32_t px,
                            uint32_t py, uint32_t frame, uint32_t w, uint32_t h, RayCount& rc) {
   PixelOut res;
   uint32_t rng = init_rng(px, py, frame);
@@ -443,7 +468,7 @@ VRT_D bool tile_pixel(uint32_t w, uint32_t h, uint32_t tile, uint32_t& px, uint3
 
 // One 16x16 tile of pathtrace.wgsl:248-330 (one pixel per lane).
 template <int TV>
-VRT_D void pathtrace_tile(const SceneDev& s, const LdsMem& L, const Uniforms& u, float fclamp, uint32_t spp, uint32_t tile,
+VRT_D void pathtrace_tile(const SceneDev& s, const LaneMem& L, const Uniforms& u, float fclamp, uint32_t spp, uint32_t tile,
                           ushort4* __restrict__ radiance, uchar4* __restrict__ albedo,
                           float4* __restrict__ gbuf, RayCount& rc) {
   const uint32_t w = (uint32_t)u.params1[0], h = (uint32_t)u.params1[1];
@@ -477,7 +502,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WV))) void 
                                                    float4* __restrict__ gbuf,
                                                    unsigned long long* __restrict__ counters) {
   extern __shared__ uint32_t lds_dyn[];
-  const LdsMem L = lds_setup<TV>(s, lds_dyn);
+  const LaneMem L = lds_setup<TV>(s, lds_dyn);
   RayCount rc{0u, 0u, 0u, 0u};
   pathtrace_tile<TV>(s, L, u, fclamp, spp, blockIdx.x, radiance, albedo, gbuf, rc);
   count4(counters, rc.rays, rc.bsteps, rc.vsteps, rc.hits);
@@ -677,7 +702,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WV))) void 
   extern __shared__ uint32_t lds_dyn[];
   const uint32_t vi = find_view(views, nviews, blockIdx.x);
   const ViewDesc& d = views[vi];
-  const LdsMem L = lds_setup<TV>(d.s, lds_dyn);
+  const LaneMem L = lds_setup<TV>(d.s, lds_dyn);
   RayCount rc{0u, 0u, 0u, 0u};
   pathtrace_tile<TV>(d.s, L, d.u, d.fclamp, d.spp, blockIdx.x - d.tile0, (ushort4*)d.radiance, (uchar4*)d.albedo,
                  (float4*)d.gbuf_cur, rc);
@@ -766,8 +791,17 @@ static inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b
 
 // TV_LDS launches: 16 mask words per lane (256-thread blocks) + the bitmap,
 // kept within 64 KiB of LDS per block.
-static inline bool lds_fits(const SceneDev& s) { return s.bit_words <= LDS_BIT_WORDS_MAX; }
-static inline uint32_t lds_bytes(const SceneDev& s) { return (16u * 256u + s.bit_words) * 4u; }
+static inline bool lds_fits(int tv, uint32_t bit_words) {
+  return (tv == TV_FAST ? 2u * bit_words : bit_words) <= LDS_BIT_WORDS_MAX;
+}
+static inline uint32_t lds_bytes(int tv, uint32_t bit_words) {
+  return (16u * 256u + (tv == TV_FAST ? 2u * bit_words : bit_words)) * 4u;
+}
+// Variant actually launched: LDS variants fall back to nested loops when the
+// bitmap does not fit.
+static inline int lds_variant(int tv, uint32_t bit_words) {
+  return (tv == TV_LDS || tv == TV_FAST) && !lds_fits(tv, bit_words) ? TV_NESTED : tv;
+}
 
 hipError_t launch_brick_masks(hipStream_t st, const uint32_t* vox, int nx, int ny, int bgx,
                               int bgy, int bz0, int nbz, uint32_t* masks, uint32_t* flags) {
@@ -789,6 +823,11 @@ hipError_t launch_brick_bits(hipStream_t st, const uint32_t* bricks, uint64_t nb
   return hipGetLastError();
 }
 
+hipError_t launch_popc(hipStream_t st, const uint32_t* words, uint64_t n, uint32_t* counts) {
+  hipLaunchKernelGGL(k_popc, dim3(div_up(n, 256)), dim3(256), 0, st, words, n, counts);
+  return hipGetLastError();
+}
+
 hipError_t launch_pack_pool(hipStream_t st, const uint32_t* vox, int nx, int ny, int bgx, int bgy,
                             int bz0, int nbz, const uint32_t* masks, const uint32_t* bricks,
                             uint32_t* pool) {
@@ -804,9 +843,13 @@ hipError_t launch_trace_rays(hipStream_t st, const SceneDev& s, const float* ray
   const uint32_t need = div_up(n, 256);
   const uint32_t nb = need < (uint32_t)blocks ? need : (uint32_t)blocks;
   const dim3 g(nb > 0 ? nb : 1);
-  if (tv == TV_LDS && lds_fits(s))
-    hipLaunchKernelGGL(k_trace_rays<TV_LDS>, g, dim3(256), lds_bytes(s), st, s, rays, n, tmax, (uint4*)hits,
-                       counters);
+  tv = lds_variant(tv, s.bit_words);
+  if (tv == TV_FAST)
+    hipLaunchKernelGGL(k_trace_rays<TV_FAST>, g, dim3(256), lds_bytes(tv, s.bit_words), st, s, rays, n, tmax,
+                       (uint4*)hits, counters);
+  else if (tv == TV_LDS)
+    hipLaunchKernelGGL(k_trace_rays<TV_LDS>, g, dim3(256), lds_bytes(tv, s.bit_words), st, s, rays, n, tmax,
+                       (uint4*)hits, counters);
   else if (tv == TV_FLAT)
     hipLaunchKernelGGL(k_trace_rays<TV_FLAT>, g, dim3(256), 0, st, s, rays, n, tmax, (uint4*)hits, counters);
   else
@@ -830,13 +873,16 @@ hipError_t launch_pathtrace(hipStream_t st, const SceneDev& s, const Uniforms& u
                             unsigned long long* counters, int tv, int waves) {
   const uint32_t w = (uint32_t)u.params1[0], h = (uint32_t)u.params1[1];
   const uint32_t tiles = div_up(w, 16) * div_up(h, 16);
-  if (tv == TV_LDS && !lds_fits(s)) tv = TV_NESTED;
-  const uint32_t shmem = tv == TV_LDS ? lds_bytes(s) : 0u;
+  tv = lds_variant(tv, s.bit_words);
+  const uint32_t shmem = (tv == TV_LDS || tv == TV_FAST) ? lds_bytes(tv, s.bit_words) : 0u;
   auto go = [&](auto kernel) {
     hipLaunchKernelGGL(kernel, dim3(tiles), dim3(256), shmem, st, s, u, fclamp, spp, (ushort4*)radiance,
                        (uchar4*)albedo, (float4*)gbuf, counters);
   };
   switch (tv * 8 + (waves == 4 || waves == 5 ? waves : 1)) {
+    case TV_FAST * 8 + 4: go(k_pathtrace<TV_FAST, 4>); break;
+    case TV_FAST * 8 + 5: go(k_pathtrace<TV_FAST, 5>); break;
+    case TV_FAST * 8 + 1: go(k_pathtrace<TV_FAST, 1>); break;
     case TV_LDS * 8 + 4: go(k_pathtrace<TV_LDS, 4>); break;
     case TV_LDS * 8 + 5: go(k_pathtrace<TV_LDS, 5>); break;
     case TV_LDS * 8 + 1: go(k_pathtrace<TV_LDS, 1>); break;
@@ -865,12 +911,15 @@ hipError_t launch_pathtrace_persistent(hipStream_t st, const SceneDev& s, const 
 
 hipError_t launch_pathtrace_batch(hipStream_t st, const ViewDesc* views, uint32_t nviews, uint32_t tiles,
                                   unsigned long long* counters, int tv, int waves, uint32_t max_bit_words) {
-  if (tv == TV_LDS && max_bit_words > LDS_BIT_WORDS_MAX) tv = TV_NESTED;
-  const uint32_t shmem = tv == TV_LDS ? (16u * 256u + max_bit_words) * 4u : 0u;
+  tv = lds_variant(tv, max_bit_words);
+  const uint32_t shmem = (tv == TV_LDS || tv == TV_FAST) ? lds_bytes(tv, max_bit_words) : 0u;
   auto go = [&](auto kernel) {
     hipLaunchKernelGGL(kernel, dim3(tiles), dim3(256), shmem, st, views, nviews, counters);
   };
   switch (tv * 8 + (waves == 4 || waves == 5 ? waves : 1)) {
+    case TV_FAST * 8 + 4: go(k_pathtrace_batch<TV_FAST, 4>); break;
+    case TV_FAST * 8 + 5: go(k_pathtrace_batch<TV_FAST, 5>); break;
+    case TV_FAST * 8 + 1: go(k_pathtrace_batch<TV_FAST, 1>); break;
     case TV_LDS * 8 + 4: go(k_pathtrace_batch<TV_LDS, 4>); break;
     case TV_LDS * 8 + 5: go(k_pathtrace_batch<TV_LDS, 5>); break;
     case TV_LDS * 8 + 1: go(k_pathtrace_batch<TV_LDS, 1>); break;

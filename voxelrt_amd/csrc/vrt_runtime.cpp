@@ -114,6 +114,7 @@ vrt::SceneDev vrt_scene::dev() const {
   s.bricks = (const uint32_t*)d_bricks.ptr;
   s.pool = (const uint32_t*)d_pool.ptr;
   s.bits = (const uint32_t*)d_bits.ptr;
+  s.prefix = (const uint32_t*)d_prefix.ptr;
   s.bit_words = (uint32_t)((nbricks + 31) / 32);
   s.nx = (int)nx;
   s.ny = (int)ny;
@@ -310,13 +311,15 @@ int vrt_scene_upload(vrt_ctx* ctx, const uint32_t* vox, uint32_t nx, uint32_t ny
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   const uint64_t nocc = (uint64_t)last[0] + last[1];
   sc->nocc = nocc;
-  const uint64_t bytes = nb * 4 + std::max<uint64_t>(nocc, 1) * vrt::POOL_WORDS * 4 + (nb + 63) / 64 * 8;
+  const uint64_t bytes = nb * 4 + std::max<uint64_t>(nocc, 1) * vrt::POOL_WORDS * 4 + (nb + 63) / 64 * 8 +
+                         (nb + 31) / 32 * 4;
   int rc = ctx->reserve(bytes);
   if (rc) return rc;
   sc->bytes = bytes;
   hipError_t e = sc->d_bricks.alloc(nb * 4);
   if (e == hipSuccess) e = sc->d_pool.alloc(std::max<uint64_t>(nocc, 1) * vrt::POOL_WORDS * 4);
   if (e == hipSuccess) e = sc->d_bits.alloc((nb + 63) / 64 * 8);
+  if (e == hipSuccess) e = sc->d_prefix.alloc((nb + 31) / 32 * 4);
   if (e != hipSuccess) {
     ctx->unreserve(bytes);
     return ctx->fail(e == hipErrorOutOfMemory ? VRT_ERR_NO_MEMORY : VRT_ERR_DEVICE,
@@ -331,6 +334,21 @@ int vrt_scene_upload(vrt_ctx* ctx, const uint32_t* vox, uint32_t nx, uint32_t ny
   if (e != hipSuccess) return undo(VRT_ERR_DEVICE, "brick words", e);
   e = vrt::launch_brick_bits(ctx->stream, (const uint32_t*)sc->d_bricks.ptr, nb, (uint32_t*)sc->d_bits.ptr);
   if (e != hipSuccess) return undo(VRT_ERR_DEVICE, "brick bits", e);
+  {
+    // prefix[w] = exclusive scan of popcount(bits[w]): the pool slot of the
+    // first occupied brick at or after word w (slots follow brick order)
+    const uint64_t nw = (nb + 31) / 32;
+    e = vrt::launch_popc(ctx->stream, (const uint32_t*)sc->d_bits.ptr, nw, (uint32_t*)flags.ptr);
+    if (e != hipSuccess) return undo(VRT_ERR_DEVICE, "bit counts", e);
+    size_t need = 0;
+    e = vrt::exclusive_scan_u32(ctx->stream, nullptr, &need, (const uint32_t*)flags.ptr,
+                                (uint32_t*)sc->d_prefix.ptr, nw);
+    if (e == hipSuccess && need > tmp.bytes) e = tmp.alloc(need);
+    if (e == hipSuccess)
+      e = vrt::exclusive_scan_u32(ctx->stream, tmp.ptr, &need, (const uint32_t*)flags.ptr,
+                                  (uint32_t*)sc->d_prefix.ptr, nw);
+    if (e != hipSuccess) return undo(VRT_ERR_DEVICE, "bit prefix", e);
+  }
   // pass 2: pack occupied bricks into the pool
   for (int bz0 = 0; bz0 < bgz; bz0 += slab) {
     const int nbz = std::min(slab, bgz - bz0);
@@ -409,9 +427,15 @@ int vrt_scene_bricks(vrt_ctx* ctx, const vrt_scene* scene, uint32_t* bricks, uin
 // traversal
 // ---------------------------------------------------------------------------
 // vrt_tuning kernel id -> trace variant of the one-per-lane kernels:
-// 0 nested, 2 single-stream (flat), 3 LDS-staged occupancy.
+// 0 nested, 2 single-stream (flat), 3 LDS-staged occupancy, 4 single loop
+// with both levels in LDS.
 static int lane_variant(uint32_t kernel) {
-  return kernel == 2 ? vrt::TV_FLAT : (kernel == 3 ? vrt::TV_LDS : vrt::TV_NESTED);
+  switch (kernel) {
+    case 2: return vrt::TV_FLAT;
+    case 3: return vrt::TV_LDS;
+    case 4: return vrt::TV_FAST;
+    default: return vrt::TV_NESTED;
+  }
 }
 
 hipError_t vrt_ctx::launch_pathtrace(const vrt::SceneDev& s, const vrt_uniforms& u, float fclamp,
@@ -703,7 +727,7 @@ int vrt_tuning_get(vrt_ctx* ctx, vrt_tuning* out) {
 int vrt_tuning_set(vrt_ctx* ctx, const vrt_tuning* in) {
   if (!ctx || !in) return VRT_ERR_INVALID;
   std::lock_guard<std::mutex> lock_(ctx->mu);
-  if (in->trace_kernel > 3 || in->pt_kernel > 3 || in->trace_blocks > (1u << 20) ||
+  if (in->trace_kernel > 4 || in->pt_kernel > 4 || in->trace_blocks > (1u << 20) ||
       in->pt_blocks > (1u << 20) || in->trace_steps > 1024 || in->pt_steps > 1024 ||
       in->trace_min_refill > 64 || in->pt_min_refill > 64 || in->pt_chunk > (1u << 16) ||
       !(in->pt_waves == 0 || in->pt_waves == 4 || in->pt_waves == 5))
@@ -770,7 +794,7 @@ int vrt_autotune(vrt_ctx* ctx, const vrt_scene* scene, uint64_t nrays, const vrt
     if (rc) return rc;
     best_ms = ms;
     best.trace_kernel = 0;
-    for (uint32_t k : {2u, 3u}) {
+    for (uint32_t k : {2u, 3u, 4u}) {
       ctx->tuning.trace_kernel = k;
       rc = time_it([&] { return ctx->launch_trace(s, (const float*)ctx->scratch_rays.ptr, n, 1e4f,
                                                   ctx->scratch_hits.ptr); }, &ms);
@@ -818,7 +842,7 @@ int vrt_autotune(vrt_ctx* ctx, const vrt_scene* scene, uint64_t nrays, const vrt
     best.pt_kernel = 0;
     best.pt_waves = 0;
     // one-pixel-per-lane variants x register budgets (waves per SIMD)
-    for (uint32_t k : {0u, 2u, 3u})
+    for (uint32_t k : {0u, 2u, 3u, 4u})
       for (uint32_t wv : {0u, 4u, 5u}) {
         if (k == 0 && wv == 0) continue;  // measured above
         ctx->tuning.pt_kernel = k;

@@ -22,8 +22,9 @@ constexpr int POOL_WORDS = 16 + 512;
 struct SceneDev {
   const uint32_t* bricks;
   const uint32_t* pool;
-  const uint32_t* bits;  // occupancy bit per brick: bit (i & 31) of word i >> 5
-  uint32_t bit_words;    // ceil(bricks / 32)
+  const uint32_t* bits;    // occupancy bit per brick: bit (i & 31) of word i >> 5
+  const uint32_t* prefix;  // prefix[w] = set bits in bits[0 .. w-1] (= pool slot of the word's first brick)
+  uint32_t bit_words;      // ceil(bricks / 32)
   int nx, ny, nz;
   int bgx, bgy, bgz;
   float voxel_size, inv_voxel;
