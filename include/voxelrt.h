@@ -148,7 +148,9 @@ typedef struct vrt_stats {
   double   gpu_ms_pathtrace;   /* HIP-event time of the last render call */
   double   gpu_ms_temporal;
   double   gpu_ms_atrous;
-  double   gpu_ms_frame;       /* pathtrace start .. last a-trous end */
+  double   gpu_ms_frame;       /* pathtrace start .. last a-trous end (with
+                                  batch_mode 2 the passes run as one graph:
+                                  pathtrace = frame, temporal = a-trous = 0) */
   uint64_t hbm_resident;       /* bytes held by scenes + view state */
   uint64_t hbm_budget;
   /* sums over every vrt_render call since create / reset (HIP events on the
@@ -232,7 +234,13 @@ typedef struct vrt_tuning {
   uint32_t pt_waves;          /* one-pixel-per-lane kernels: minimum waves per
                                  SIMD the register allocation must allow:
                                  0 = compiler's choice, 4 or 5 */
-  uint32_t reserved[6];
+  uint32_t batch_mode;        /* batched form (nviews > 1, serving):
+                                 0 = one grid launch per pass over all tiles,
+                                 1 = persistent path tracer on a shared tile
+                                     queue + fixed-grid filter passes,
+                                 2 = as 1, replayed from a captured hipGraph */
+  uint32_t batch_blocks;      /* persistent path-tracer grid, 0 = 8 per CU */
+  uint32_t reserved[4];
 } vrt_tuning;
 int vrt_tuning_get(vrt_ctx* ctx, vrt_tuning* out);
 int vrt_tuning_set(vrt_ctx* ctx, const vrt_tuning* in);

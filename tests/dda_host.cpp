@@ -16,7 +16,7 @@
 namespace {
 
 struct HostScene {
-  std::vector<uint32_t> bricks, pool, bits, prefix;
+  std::vector<uint32_t> bricks, pool, bits, prefix, pbits, pprefix;
   vrt::SceneDev dev{};
 };
 
@@ -57,7 +57,30 @@ void build(const uint32_t* vox, int nx, int ny, int nz, HostScene& h) {
     if (h.bricks[i]) h.bits[i >> 5] |= 1u << (i & 31u);
   h.prefix.assign(nw, 0u);
   for (uint64_t w = 1; w < nw; ++w) h.prefix[w] = h.prefix[w - 1] + (uint32_t)__builtin_popcount(h.bits[w - 1]);
+  // padded layout: bit index bx | by << sh1 | bz << sh2
+  auto lg = [](int v) {
+    uint32_t k = 0;
+    while ((1 << k) < v) ++k;
+    return k;
+  };
+  const uint32_t sh1 = lg(bgx), sh2 = sh1 + lg(bgy), sh3 = sh2 + lg(bgz);
+  const uint64_t pw = ((1ull << sh3) + 31) / 32;
+  h.pbits.assign(pw, 0u);
+  for (int bz = 0; bz < bgz; ++bz)
+    for (int by = 0; by < bgy; ++by)
+      for (int bx = 0; bx < bgx; ++bx)
+        if (h.bricks[(uint64_t)bx + (uint64_t)by * bgx + (uint64_t)bz * bgx * bgy]) {
+          const uint64_t i = (uint64_t)bx | ((uint64_t)by << sh1) | ((uint64_t)bz << sh2);
+          h.pbits[i >> 5] |= 1u << (i & 31u);
+        }
+  h.pprefix.assign(pw, 0u);
+  for (uint64_t w = 1; w < pw; ++w) h.pprefix[w] = h.pprefix[w - 1] + (uint32_t)__builtin_popcount(h.pbits[w - 1]);
   vrt::SceneDev& s = h.dev;
+  s.pbits = h.pbits.data();
+  s.pprefix = h.pprefix.data();
+  s.pwords = (uint32_t)pw;
+  s.psh1 = sh1;
+  s.psh2 = sh2;
   s.bricks = h.bricks.data();
   s.pool = h.pool.data();
   s.bits = h.bits.data();
@@ -85,13 +108,15 @@ extern "C" int dda_host_trace(const uint32_t* vox, int nx, int ny, int nz, const
   HostScene h;
   build(vox, nx, ny, nz, h);
   // one block's LDS as the kernels lay it out: 256 mask slots, bitmap, prefix
-  const uint32_t nw = h.dev.bit_words;
-  std::vector<uint32_t> lds(16 * 256 + 2 * (size_t)nw, 0u);
-  std::memcpy(lds.data() + 16 * 256, h.bits.data(), nw * 4);
-  std::memcpy(lds.data() + 16 * 256 + nw, h.prefix.data(), nw * 4);
+  const uint32_t pw = h.dev.pwords;
+  std::vector<uint32_t> lds(16 * 256 + 2 * (size_t)pw, 0u);
+  std::memcpy(lds.data() + 16 * 256, h.pbits.data(), pw * 4);
+  std::memcpy(lds.data() + 16 * 256 + pw, h.pprefix.data(), pw * 4);
   const uint32_t lane = 37;  // any slot
-  vrt::LdsMem lm{lds.data() + 16 * 256, lds.data() + lane, 256u};
-  vrt::LdsScene ls{lds.data(), lane, 16u * 256u, 16u * 256u + nw};
+  std::vector<uint32_t> lds2(16 * 256 + (size_t)h.dev.bit_words, 0u);
+  std::memcpy(lds2.data() + 16 * 256, h.bits.data(), h.dev.bit_words * 4);
+  vrt::LdsMem lm{lds2.data() + 16 * 256, lds2.data() + lane, 256u};
+  vrt::LdsScene ls{lds.data(), lane, 16u * 256u, 16u * 256u + pw};
   for (uint64_t i = 0; i < n; ++i) {
     const float* r = rays + 6 * i;
     const vrt::f3 ro = vrt::mk3(r[0], r[1], r[2]), rd = vrt::mk3(r[3], r[4], r[5]);

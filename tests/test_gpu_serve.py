@@ -59,9 +59,11 @@ def _alone(ctx, scenes, descs, specs, base):
     return out
 
 
-def test_served_equals_alone(world):
+@pytest.mark.parametrize("mode", [0, 2], ids=["grid", "graph"])
+def test_served_equals_alone(world, mode):
     descs, vox = world
     with v.Context(0) as ctx:
+        ctx.set_tuning(pt_kernel=4, pt_waves=5, batch_mode=mode)
         ctx.serve_configure(streams=2, max_batch_views=8)
         scenes = [ctx.upload(x) for x in vox]
         specs = _request_specs(40, seed=5)

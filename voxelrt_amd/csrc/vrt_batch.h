@@ -31,6 +31,18 @@ struct ViewDesc {
   float* hdr;                  // rgb f32 HDR output (main.js:628-659), or null
 };
 
+// Header of the device descriptor table, at offset 0; the ViewDesc array
+// follows at BATCH_HEAD_BYTES.  `queue` is the shared tile queue of the
+// persistent path-trace launch; the host uploads it as 0 with every table.
+struct BatchHead {
+  uint32_t nviews;
+  uint32_t tiles;
+  uint32_t queue;
+  uint32_t pad[13];
+};
+constexpr uint32_t BATCH_HEAD_BYTES = 64;
+static_assert(sizeof(BatchHead) == BATCH_HEAD_BYTES, "BatchHead layout");
+
 // Index of the view owning tile `b` (views sorted by tile0, tile0[0] = 0).
 VRT_D uint32_t find_view(const ViewDesc* __restrict__ views, uint32_t nviews, uint32_t b) {
   uint32_t lo = 0, hi = nviews;  // invariant: views[lo].tile0 <= b < views[hi].tile0

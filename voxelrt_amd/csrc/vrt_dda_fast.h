@@ -3,11 +3,13 @@
 //
 // Occupancy lives in LDS (one array per block, see LdsScene): a 16-word slot
 // per lane into which the 512-bit mask of the brick being walked is copied on
-// entry, the scene's brick bitmap (one bit per brick) and per-word prefix
-// counts of set bits.  A brick's pool slot is the number of occupied bricks
-// before it (slots are assigned by an exclusive scan in brick order), i.e.
-// prefix[w] + popcount of the lower bits of its bitmap word -- no dependent
-// HBM read of the brick word.
+// entry, the scene's brick bitmap over the power-of-two padded brick grid
+// (SceneDev::pbits, one bit per brick) and per-word prefix counts of set bits.
+// The coarse cell index is the packed position bx | by << psh1 | bz << psh2,
+// which is also its bit index, so the brick coordinates are bit fields.  A
+// brick's pool slot is the number of occupied bricks before it (slots follow
+// brick order, and padding bits are 0), i.e. prefix[w] + popcount of the
+// lower bits of its bitmap word -- no dependent HBM read of the brick word.
 //
 // One iteration = test the current cell + advance the current level by one
 // cell.  The level never appears in that code: it only sets registers that
@@ -35,7 +37,7 @@ struct LdsScene {
   uint32_t* lds;
   uint32_t lane;        // this lane's slot base (0..255)
   uint32_t bits_off;    // 4096
-  uint32_t prefix_off;  // bits_off + bit_words
+  uint32_t prefix_off;  // bits_off + SceneDev::pwords
 };
 
 // Cells left along each axis before leaving [0, n) from p moving by sd,
@@ -85,7 +87,8 @@ VRT_D void trace_fast(const SceneDev& s, const LdsScene& L, f3 roW, f3 rd, float
 
   uint32_t ax = (tmin[0] >= tmin[1] && tmin[0] >= tmin[2]) ? 0u : (tmin[1] >= tmin[2] ? 1u : 2u);
   float t = tEnter;
-  const uint32_t sy = (uint32_t)s.bgx, sz = (uint32_t)s.bgx * (uint32_t)s.bgy;
+  const uint32_t sh1 = s.psh1, sh2 = s.psh2;
+  const uint32_t mx = (1u << sh1) - 1u, my = (1u << (sh2 - sh1)) - 1u;
   const uint32_t dec[3] = {sd[0] != 0 ? 1u : 0u, sd[1] != 0 ? 256u : 0u, sd[2] != 0 ? 65536u : 0u};
 
   // current level (starts coarse)
@@ -103,9 +106,9 @@ VRT_D void trace_fast(const SceneDev& s, const LdsScene& L, f3 roW, f3 rd, float
       tm[c] = (((float)bp[c] + ge[c]) * 8.0f - ro[c]) * inv[c];
     }
     di[0] = sd[0];
-    di[1] = sd[1] * (int)sy;
-    di[2] = sd[2] * (int)sz;
-    idx = (uint32_t)bp[0] + (uint32_t)bp[1] * sy + (uint32_t)bp[2] * sz;
+    di[1] = sd[1] * (int)(1u << sh1);
+    di[2] = sd[2] * (int)(1u << sh2);
+    idx = (uint32_t)bp[0] | ((uint32_t)bp[1] << sh1) | ((uint32_t)bp[2] << sh2);
     cnt = pack_left(bp, BG, sd);
   }
   uint32_t iter = (uint32_t)(s.bgx + s.bgy + s.bgz + 2);
@@ -132,7 +135,7 @@ VRT_D void trace_fast(const SceneDev& s, const LdsScene& L, f3 roW, f3 rd, float
         for (int c = 0; c < 3; ++c) nn[c] = ((uint32_t)c == ax ? -1.0f : -0.0f) * (float)sd[c];
         o.n = mk3(nn[0], nn[1], nn[2]);
         o.mat = s.pool[(uint64_t)blk + 16u + idx];
-        const uint32_t bz = idxB / sz, rem = idxB - bz * sz, by = rem / sy, bx = rem - by * sy;
+        const uint32_t bx = idxB & mx, by = (idxB >> sh1) & my, bz = idxB >> sh2;
         const uint32_t vx = bx * 8u + (idx & 7u), vy = by * 8u + ((idx >> 3) & 7u), vz = bz * 8u + (idx >> 6);
         o.voxel = vx + vy * (uint32_t)s.nx + vz * (uint32_t)s.nx * (uint32_t)s.ny;
         const float nv = ax == 0u ? nn[0] : (ax == 1u ? nn[1] : nn[2]);
@@ -152,7 +155,7 @@ VRT_D void trace_fast(const SceneDev& s, const LdsScene& L, f3 roW, f3 rd, float
         L.lds[((4 * k + 2) << 8) + L.lane] = v.z;
         L.lds[((4 * k + 3) << 8) + L.lane] = v.w;
       }
-      const uint32_t bz = idx / sz, rem = idx - bz * sz, by = rem / sy, bx = rem - by * sy;
+      const uint32_t bx = idx & mx, by = (idx >> sh1) & my, bz = idx >> sh2;
       const int bmin[3] = {(int)bx * 8, (int)by * 8, (int)bz * 8};
       const float ts = t + 1e-4f;
       int lp[3];
@@ -210,8 +213,8 @@ VRT_D void trace_fast(const SceneDev& s, const LdsScene& L, f3 roW, f3 rd, float
         d[c] = adl[c] * 8.0f;
       }
       di[0] = sd[0];
-      di[1] = sd[1] * (int)sy;
-      di[2] = sd[2] * (int)sz;
+      di[1] = sd[1] * (int)(1u << sh1);
+      di[2] = sd[2] * (int)(1u << sh2);
       idx = idxB;
       cnt = cntB;
       iter = iterB;

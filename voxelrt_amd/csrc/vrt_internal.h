@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <chrono>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -53,12 +54,15 @@ struct BatchLane {
   hipEvent_t desc_done[STAGES] = {};
   uint32_t desc_cap = 0;  // views per staging buffer
   uint32_t desc_next = 0;
-  DeviceBuffer desc_dev;
+  DeviceBuffer desc_dev;  // BatchHead, then the ViewDesc array (vrt_batch.h)
+  // instantiated graphs of the persistent batched passes, by launch shape
+  std::map<uint64_t, hipGraphExec_t> graphs;
   BatchLane() = default;
   BatchLane(const BatchLane&) = delete;
   BatchLane& operator=(const BatchLane&) = delete;
   ~BatchLane();
   hipError_t ensure(uint32_t nviews);
+  void drop_graphs();
 };
 
 struct Server;  // serving loop (vrt_serve.cpp)
@@ -76,6 +80,8 @@ struct vrt_scene {
   vrt::DeviceBuffer d_pool;    // nocc x 528 words (mask + voxels)
   vrt::DeviceBuffer d_bits;    // one occupancy bit per brick (LDS-staged by the kernels)
   vrt::DeviceBuffer d_prefix;  // per bitmap word: occupied bricks before it
+  vrt::DeviceBuffer d_pbits, d_pprefix;  // the same over the power-of-two padded brick grid
+  uint32_t psh1 = 0, psh2 = 0, psh3 = 0;  // padded bit index = bx | by << psh1 | bz << psh2; 2^psh3 bits
   vrt::SceneDev dev() const;
 };
 
@@ -126,6 +132,7 @@ struct vrt_ctx {
   // make the context's stream wait for every running serve batch
   int order_after_serve();
   int trace_blocks = 0;  // 0 = default grid for k_trace_rays
+  int cus = 256;         // compute units of the device (persistent grids)
   vrt_tuning tuning{};   // kernel variants + launch shapes (vrt_tuning_set / vrt_autotune)
   vrt::DeviceBuffer d_counters;  // u64 x4: rays, brick steps, voxel steps, hits
   vrt::DeviceBuffer d_queue;     // u32 work-queue counters of the persistent kernels

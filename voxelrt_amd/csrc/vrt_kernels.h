@@ -15,7 +15,7 @@ struct SceneDev;
 constexpr int TV_NESTED = 0, TV_FLAT = 1, TV_LDS = 2, TV_FAST = 3;
 // Largest occupancy bitmap staged in LDS (words): 48 KiB, 393216 bricks
 // (512^3 = 262144 bricks fits).  Larger scenes fall back to TV_NESTED.
-// TV_FAST stages the bitmap and its prefix counts: half that many words.
+// TV_FAST stages the padded bitmap and its prefix counts within the same limit.
 constexpr uint32_t LDS_BIT_WORDS_MAX = 12288;
 
 hipError_t launch_brick_masks(hipStream_t st, const uint32_t* vox, int nx, int ny, int bgx,
@@ -23,6 +23,8 @@ hipError_t launch_brick_masks(hipStream_t st, const uint32_t* vox, int nx, int n
 hipError_t launch_brick_words(hipStream_t st, const uint32_t* flags, const uint32_t* slots,
                               uint64_t nb, uint32_t* bricks);
 hipError_t launch_brick_bits(hipStream_t st, const uint32_t* bricks, uint64_t nb, uint32_t* bits);
+hipError_t launch_brick_pbits(hipStream_t st, const uint32_t* bricks, int bgx, int bgy, int bgz, uint32_t sh1,
+                              uint32_t sh2, uint64_t nbits, uint32_t* bits);
 hipError_t launch_popc(hipStream_t st, const uint32_t* words, uint64_t n, uint32_t* counts);
 hipError_t launch_pack_pool(hipStream_t st, const uint32_t* vox, int nx, int ny, int bgx, int bgy,
                             int bz0, int nbz, const uint32_t* masks, const uint32_t* bricks,
@@ -41,9 +43,19 @@ hipError_t launch_pathtrace_persistent(hipStream_t st, const SceneDev& s, const 
                                        int steps, int min_refill, int chunk);
 struct ViewDesc;
 hipError_t launch_pathtrace_batch(hipStream_t st, const ViewDesc* views, uint32_t nviews, uint32_t tiles,
-                                  unsigned long long* counters, int tv, int waves, uint32_t max_bit_words);
+                                  unsigned long long* counters, int tv, int waves, uint32_t max_bit_words,
+                                  uint32_t max_pwords);
 hipError_t launch_temporal_batch(hipStream_t st, const ViewDesc* views, uint32_t nviews, uint32_t tiles);
 hipError_t launch_hdr_batch(hipStream_t st, const ViewDesc* views, uint32_t nviews, uint32_t tiles);
+struct BatchHead;
+// Persistent batched form: the path tracer on a fixed grid of pt_blocks pulling
+// tiles from head->queue, then temporal / a-trous x3 (denoise) / hdr on a
+// fixed grid-stride grid of gs_blocks.  No argument depends on the batch, so
+// the sequence can be captured once into a graph and replayed.
+hipError_t launch_batch_persistent(hipStream_t st, BatchHead* head, const ViewDesc* views,
+                                   unsigned long long* counters, int tv, int waves, uint32_t max_bit_words,
+                                   uint32_t max_pwords, uint32_t pt_blocks, uint32_t gs_blocks, bool denoise,
+                                   bool hdr);
 hipError_t launch_atrous_batch(hipStream_t st, const ViewDesc* views, uint32_t nviews, uint32_t tiles, int pass);
 hipError_t launch_temporal(hipStream_t st, const Uniforms& u, const void* radiance,
                            const void* gcur, const void* aprev, const void* gprev, void* aout,

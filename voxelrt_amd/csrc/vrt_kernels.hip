@@ -101,6 +101,24 @@ __global__ __launch_bounds__(256) void k_brick_bits(const uint32_t* __restrict__
   if (lane == 1u) bits[2 * g + 1] = (uint32_t)(b >> 32);
 }
 
+// Padded bitmap: bit i of the power-of-two brick grid, i = bx | by << sh1 |
+// bz << sh2; bricks outside the real grid read as empty.  64 bits per wave.
+__global__ __launch_bounds__(256) void k_brick_pbits(const uint32_t* __restrict__ bricks, int bgx, int bgy, int bgz,
+                                                     uint32_t sh1, uint32_t sh2, uint64_t nbits,
+                                                     uint32_t* __restrict__ bits) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if ((i & ~63ull) >= nbits) return;  // whole wave past the last group
+  const uint32_t x = (uint32_t)(i & ((1ull << sh1) - 1)), y = (uint32_t)((i >> sh1) & ((1ull << (sh2 - sh1)) - 1));
+  const uint64_t z = i >> sh2;
+  const bool in = i < nbits && x < (uint32_t)bgx && y < (uint32_t)bgy && z < (uint64_t)bgz;
+  const uint64_t bi = in ? (uint64_t)x + (uint64_t)y * bgx + z * (uint64_t)bgx * bgy : 0;
+  const uint64_t b = __ballot(in && bricks[bi] != 0u);
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint64_t g = i >> 6;
+  if (lane == 0u) bits[2 * g] = (uint32_t)b;
+  if (lane == 1u && 64 * g + 32 < nbits) bits[2 * g + 1] = (uint32_t)(b >> 32);
+}
+
 __global__ __launch_bounds__(256) void k_popc(const uint32_t* __restrict__ words, uint64_t n,
                                               uint32_t* __restrict__ counts) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -168,19 +186,25 @@ VRT_D void trace_sel(const SceneDev& s, const LaneMem& L, f3 ro, f3 rd, float tm
 template <int TV>
 VRT_D LaneMem lds_setup(const SceneDev& s, uint32_t* lds) {
   LaneMem L{{nullptr, nullptr, 0u}, {nullptr, 0u, 0u, 0u}};
-  if constexpr (TV == TV_LDS || TV == TV_FAST) {
+  if constexpr (TV == TV_LDS) {
     uint32_t* bits = lds + 16u * 256u;
     for (uint32_t i = threadIdx.x; i < s.bit_words; i += blockDim.x) bits[i] = s.bits[i];
-    if constexpr (TV == TV_FAST)
-      for (uint32_t i = threadIdx.x; i < s.bit_words; i += blockDim.x) bits[s.bit_words + i] = s.prefix[i];
     __syncthreads();
     L.m.bits = bits;
     L.m.mask = lds + threadIdx.x;
     L.m.stride = 256u;
+  }
+  if constexpr (TV == TV_FAST) {
+    uint32_t* bits = lds + 16u * 256u;
+    for (uint32_t i = threadIdx.x; i < s.pwords; i += blockDim.x) {
+      bits[i] = s.pbits[i];
+      bits[s.pwords + i] = s.pprefix[i];
+    }
+    __syncthreads();
     L.f.lds = lds;
     L.f.lane = threadIdx.x;
     L.f.bits_off = 16u * 256u;
-    L.f.prefix_off = 16u * 256u + s.bit_words;
+    L.f.prefix_off = 16u * 256u + s.pwords;
   }
   return L;
 }
@@ -697,23 +721,58 @@ __global__ __launch_bounds__(256) void k_atrous(Uniforms u, const void* __restri
 // batched form (vrt_batch.h): one launch per pass over every view's tiles
 // ---------------------------------------------------------------------------
 template <int TV, int WV>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WV))) void k_pathtrace_batch(const ViewDesc* __restrict__ views, uint32_t nviews,
-                                                         unsigned long long* __restrict__ counters) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WV))) void k_pathtrace_batch(
+    const ViewDesc* __restrict__ views, uint32_t nviews, unsigned long long* __restrict__ counters) {
   extern __shared__ uint32_t lds_dyn[];
   const uint32_t vi = find_view(views, nviews, blockIdx.x);
   const ViewDesc& d = views[vi];
   const LaneMem L = lds_setup<TV>(d.s, lds_dyn);
   RayCount rc{0u, 0u, 0u, 0u};
   pathtrace_tile<TV>(d.s, L, d.u, d.fclamp, d.spp, blockIdx.x - d.tile0, (ushort4*)d.radiance, (uchar4*)d.albedo,
-                 (float4*)d.gbuf_cur, rc);
+                     (float4*)d.gbuf_cur, rc);
   count4(counters, rc.rays, rc.bsteps, rc.vsteps, rc.hits);
   count4(d.vcount, rc.rays, rc.bsteps, rc.vsteps, rc.hits);
 }
 
-__global__ __launch_bounds__(256) void k_temporal_batch(const ViewDesc* __restrict__ views, uint32_t nviews) {
-  const uint32_t vi = find_view(views, nviews, blockIdx.x);
+// Persistent form: a fixed grid of blocks pulls tiles of every view from the
+// table's shared queue (one atomic per tile), so views of any size and cost
+// share the machine until the last tile; LDS-staged occupancy is reloaded
+// only when consecutive tiles belong to different scenes.  The grid does not
+// depend on the batch, so the launch can be replayed from a captured graph.
+template <int TV, int WV>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WV))) void k_pathtrace_batch_q(
+    BatchHead* __restrict__ head, const ViewDesc* __restrict__ views, unsigned long long* __restrict__ counters) {
+  extern __shared__ uint32_t lds_dyn[];
+  __shared__ uint32_t s_tile;
+  const uint32_t tiles = head->tiles, nviews = head->nviews;
+  const void* loaded = nullptr;  // scene whose occupancy is in LDS
+  LaneMem L{{nullptr, nullptr, 0u}, {nullptr, 0u, 0u, 0u}};
+  for (;;) {
+    __syncthreads();  // every lane is done with the previous tile (s_tile, LDS)
+    if (threadIdx.x == 0u) s_tile = atomicAdd(&head->queue, 1u);
+    __syncthreads();
+    const uint32_t b = s_tile;
+    if (b >= tiles) break;
+    const uint32_t vi = find_view(views, nviews, b);
+    const ViewDesc& d = views[vi];
+    if (TV == TV_LDS || TV == TV_FAST) {
+      if (d.s.pool != loaded) {
+        L = lds_setup<TV>(d.s, lds_dyn);
+        loaded = d.s.pool;
+      }
+    }
+    RayCount rc{0u, 0u, 0u, 0u};
+    pathtrace_tile<TV>(d.s, L, d.u, d.fclamp, d.spp, b - d.tile0, (ushort4*)d.radiance, (uchar4*)d.albedo,
+                       (float4*)d.gbuf_cur, rc);
+    count4(counters, rc.rays, rc.bsteps, rc.vsteps, rc.hits);
+    count4(d.vcount, rc.rays, rc.bsteps, rc.vsteps, rc.hits);
+  }
+}
+
+VRT_D void temporal_batch_tile(const ViewDesc* __restrict__ views, uint32_t nviews, uint32_t b) {
+  const uint32_t vi = find_view(views, nviews, b);
   const ViewDesc& d = views[vi];
-  const uint32_t tile = blockIdx.x - d.tile0;
+  const uint32_t tile = b - d.tile0;
   if (d.fmt == 1u)
     temporal_tile<1>(d.u, tile, (const ushort4*)d.radiance, (const float4*)d.gbuf_cur, d.accum_prev,
                      (const float4*)d.gbuf_prev, d.accum_cur);
@@ -721,17 +780,25 @@ __global__ __launch_bounds__(256) void k_temporal_batch(const ViewDesc* __restri
     temporal_tile<0>(d.u, tile, (const ushort4*)d.radiance, (const float4*)d.gbuf_cur, d.accum_prev,
                      (const float4*)d.gbuf_prev, d.accum_cur);
 }
+__global__ __launch_bounds__(256) void k_temporal_batch(const ViewDesc* __restrict__ views, uint32_t nviews) {
+  temporal_batch_tile(views, nviews, blockIdx.x);
+}
+// Grid-stride forms for the captured graph: the tile count comes from the table.
+__global__ __launch_bounds__(256) void k_temporal_batch_gs(const BatchHead* __restrict__ head,
+                                                           const ViewDesc* __restrict__ views) {
+  for (uint32_t b = blockIdx.x; b < head->tiles; b += gridDim.x) temporal_batch_tile(views, head->nviews, b);
+}
 
 // pass 0: accum -> den0 (step 1), 1: den0 -> den1 (step 2), 2: den1 -> den0
 // (step 4), main.js:502-507.  Views without denoise skip every pass.
 template <int PASS>
-__global__ __launch_bounds__(256) void k_atrous_batch(const ViewDesc* __restrict__ views, uint32_t nviews) {
+VRT_D void atrous_batch_tile(const ViewDesc* __restrict__ views, uint32_t nviews, uint32_t b,
+                             AtrousLds<PASS == 0 ? 1 : (PASS == 1 ? 2 : 4)>& lds) {
   constexpr int STEP = PASS == 0 ? 1 : (PASS == 1 ? 2 : 4);
-  __shared__ AtrousLds<STEP> lds;
-  const uint32_t vi = find_view(views, nviews, blockIdx.x);
+  const uint32_t vi = find_view(views, nviews, b);
   const ViewDesc& d = views[vi];
   if (!d.denoise) return;
-  const uint32_t tile = blockIdx.x - d.tile0;
+  const uint32_t tile = b - d.tile0;
   if (PASS == 0) {
     if (d.fmt == 1u)
       atrous_tile<1, STEP>(d.u, tile, d.accum_cur, (const float4*)d.gbuf_cur, (ushort4*)d.den0, lds);
@@ -740,6 +807,20 @@ __global__ __launch_bounds__(256) void k_atrous_batch(const ViewDesc* __restrict
   } else {
     atrous_tile<0, STEP>(d.u, tile, PASS == 1 ? d.den0 : d.den1, (const float4*)d.gbuf_cur,
                          (ushort4*)(PASS == 1 ? d.den1 : d.den0), lds);
+  }
+}
+template <int PASS>
+__global__ __launch_bounds__(256) void k_atrous_batch(const ViewDesc* __restrict__ views, uint32_t nviews) {
+  __shared__ AtrousLds<PASS == 0 ? 1 : (PASS == 1 ? 2 : 4)> lds;
+  atrous_batch_tile<PASS>(views, nviews, blockIdx.x, lds);
+}
+template <int PASS>
+__global__ __launch_bounds__(256) void k_atrous_batch_gs(const BatchHead* __restrict__ head,
+                                                         const ViewDesc* __restrict__ views) {
+  __shared__ AtrousLds<PASS == 0 ? 1 : (PASS == 1 ? 2 : 4)> lds;
+  for (uint32_t b = blockIdx.x; b < head->tiles; b += gridDim.x) {
+    atrous_batch_tile<PASS>(views, head->nviews, b, lds);
+    __syncthreads();  // the next tile rewrites the LDS tile
   }
 }
 
@@ -762,12 +843,12 @@ __global__ __launch_bounds__(256) void k_hdr(uint32_t npx, const void* __restric
 
 // HDR of every view that asks for it (d.hdr != null): the denoised target
 // when denoise is on, else the accumulation, as k_hdr.
-__global__ __launch_bounds__(256) void k_hdr_batch(const ViewDesc* __restrict__ views, uint32_t nviews) {
-  const uint32_t vi = find_view(views, nviews, blockIdx.x);
+VRT_D void hdr_batch_tile(const ViewDesc* __restrict__ views, uint32_t nviews, uint32_t b) {
+  const uint32_t vi = find_view(views, nviews, b);
   const ViewDesc& d = views[vi];
   if (d.hdr == nullptr) return;
   uint32_t px, py;
-  if (!tile_pixel(d.w, d.h, blockIdx.x - d.tile0, px, py)) return;
+  if (!tile_pixel(d.w, d.h, b - d.tile0, px, py)) return;
   const uint64_t i = (uint64_t)py * d.w + px;
   const float4 il = d.denoise ? load_accum<0>(d.den0, i)
                               : (d.fmt == 1u ? load_accum<1>(d.accum_cur, i) : load_accum<0>(d.accum_cur, i));
@@ -776,6 +857,13 @@ __global__ __launch_bounds__(256) void k_hdr_batch(const ViewDesc* __restrict__ 
   d.hdr[i * 3 + 0] = (float)((double)il.x * ar * ar);
   d.hdr[i * 3 + 1] = (float)((double)il.y * ag * ag);
   d.hdr[i * 3 + 2] = (float)((double)il.z * ab * ab);
+}
+__global__ __launch_bounds__(256) void k_hdr_batch(const ViewDesc* __restrict__ views, uint32_t nviews) {
+  hdr_batch_tile(views, nviews, blockIdx.x);
+}
+__global__ __launch_bounds__(256) void k_hdr_batch_gs(const BatchHead* __restrict__ head,
+                                                      const ViewDesc* __restrict__ views) {
+  for (uint32_t b = blockIdx.x; b < head->tiles; b += gridDim.x) hdr_batch_tile(views, head->nviews, b);
 }
 
 }  // namespace vrt
@@ -791,16 +879,18 @@ static inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b
 
 // TV_LDS launches: 16 mask words per lane (256-thread blocks) + the bitmap,
 // kept within 64 KiB of LDS per block.
-static inline bool lds_fits(int tv, uint32_t bit_words) {
-  return (tv == TV_FAST ? 2u * bit_words : bit_words) <= LDS_BIT_WORDS_MAX;
+// LDS words a variant stages beyond the mask slots: TV_LDS the bitmap,
+// TV_FAST the padded bitmap and its prefix counts.
+static inline uint32_t lds_words(int tv, uint32_t bit_words, uint32_t pwords) {
+  return tv == TV_FAST ? 2u * pwords : (tv == TV_LDS ? bit_words : 0u);
 }
-static inline uint32_t lds_bytes(int tv, uint32_t bit_words) {
-  return (16u * 256u + (tv == TV_FAST ? 2u * bit_words : bit_words)) * 4u;
+static inline uint32_t lds_bytes(int tv, uint32_t bit_words, uint32_t pwords) {
+  return tv == TV_FAST || tv == TV_LDS ? (16u * 256u + lds_words(tv, bit_words, pwords)) * 4u : 0u;
 }
 // Variant actually launched: LDS variants fall back to nested loops when the
 // bitmap does not fit.
-static inline int lds_variant(int tv, uint32_t bit_words) {
-  return (tv == TV_LDS || tv == TV_FAST) && !lds_fits(tv, bit_words) ? TV_NESTED : tv;
+static inline int lds_variant(int tv, uint32_t bit_words, uint32_t pwords) {
+  return (tv == TV_LDS || tv == TV_FAST) && lds_words(tv, bit_words, pwords) > LDS_BIT_WORDS_MAX ? TV_NESTED : tv;
 }
 
 hipError_t launch_brick_masks(hipStream_t st, const uint32_t* vox, int nx, int ny, int bgx,
@@ -820,6 +910,13 @@ hipError_t launch_brick_words(hipStream_t st, const uint32_t* flags, const uint3
 
 hipError_t launch_brick_bits(hipStream_t st, const uint32_t* bricks, uint64_t nb, uint32_t* bits) {
   hipLaunchKernelGGL(k_brick_bits, dim3(div_up(nb, 256)), dim3(256), 0, st, bricks, nb, bits);
+  return hipGetLastError();
+}
+
+hipError_t launch_brick_pbits(hipStream_t st, const uint32_t* bricks, int bgx, int bgy, int bgz, uint32_t sh1,
+                              uint32_t sh2, uint64_t nbits, uint32_t* bits) {
+  hipLaunchKernelGGL(k_brick_pbits, dim3(div_up(nbits, 256)), dim3(256), 0, st, bricks, bgx, bgy, bgz, sh1, sh2,
+                     nbits, bits);
   return hipGetLastError();
 }
 
@@ -843,13 +940,12 @@ hipError_t launch_trace_rays(hipStream_t st, const SceneDev& s, const float* ray
   const uint32_t need = div_up(n, 256);
   const uint32_t nb = need < (uint32_t)blocks ? need : (uint32_t)blocks;
   const dim3 g(nb > 0 ? nb : 1);
-  tv = lds_variant(tv, s.bit_words);
+  tv = lds_variant(tv, s.bit_words, s.pwords);
+  const uint32_t shmem = lds_bytes(tv, s.bit_words, s.pwords);
   if (tv == TV_FAST)
-    hipLaunchKernelGGL(k_trace_rays<TV_FAST>, g, dim3(256), lds_bytes(tv, s.bit_words), st, s, rays, n, tmax,
-                       (uint4*)hits, counters);
+    hipLaunchKernelGGL(k_trace_rays<TV_FAST>, g, dim3(256), shmem, st, s, rays, n, tmax, (uint4*)hits, counters);
   else if (tv == TV_LDS)
-    hipLaunchKernelGGL(k_trace_rays<TV_LDS>, g, dim3(256), lds_bytes(tv, s.bit_words), st, s, rays, n, tmax,
-                       (uint4*)hits, counters);
+    hipLaunchKernelGGL(k_trace_rays<TV_LDS>, g, dim3(256), shmem, st, s, rays, n, tmax, (uint4*)hits, counters);
   else if (tv == TV_FLAT)
     hipLaunchKernelGGL(k_trace_rays<TV_FLAT>, g, dim3(256), 0, st, s, rays, n, tmax, (uint4*)hits, counters);
   else
@@ -873,8 +969,8 @@ hipError_t launch_pathtrace(hipStream_t st, const SceneDev& s, const Uniforms& u
                             unsigned long long* counters, int tv, int waves) {
   const uint32_t w = (uint32_t)u.params1[0], h = (uint32_t)u.params1[1];
   const uint32_t tiles = div_up(w, 16) * div_up(h, 16);
-  tv = lds_variant(tv, s.bit_words);
-  const uint32_t shmem = (tv == TV_LDS || tv == TV_FAST) ? lds_bytes(tv, s.bit_words) : 0u;
+  tv = lds_variant(tv, s.bit_words, s.pwords);
+  const uint32_t shmem = lds_bytes(tv, s.bit_words, s.pwords);
   auto go = [&](auto kernel) {
     hipLaunchKernelGGL(kernel, dim3(tiles), dim3(256), shmem, st, s, u, fclamp, spp, (ushort4*)radiance,
                        (uchar4*)albedo, (float4*)gbuf, counters);
@@ -910,9 +1006,10 @@ hipError_t launch_pathtrace_persistent(hipStream_t st, const SceneDev& s, const 
 }
 
 hipError_t launch_pathtrace_batch(hipStream_t st, const ViewDesc* views, uint32_t nviews, uint32_t tiles,
-                                  unsigned long long* counters, int tv, int waves, uint32_t max_bit_words) {
-  tv = lds_variant(tv, max_bit_words);
-  const uint32_t shmem = (tv == TV_LDS || tv == TV_FAST) ? lds_bytes(tv, max_bit_words) : 0u;
+                                  unsigned long long* counters, int tv, int waves, uint32_t max_bit_words,
+                                  uint32_t max_pwords) {
+  tv = lds_variant(tv, max_bit_words, max_pwords);
+  const uint32_t shmem = lds_bytes(tv, max_bit_words, max_pwords);
   auto go = [&](auto kernel) {
     hipLaunchKernelGGL(kernel, dim3(tiles), dim3(256), shmem, st, views, nviews, counters);
   };
@@ -930,6 +1027,40 @@ hipError_t launch_pathtrace_batch(hipStream_t st, const ViewDesc* views, uint32_
     case TV_NESTED * 8 + 5: go(k_pathtrace_batch<TV_NESTED, 5>); break;
     default: go(k_pathtrace_batch<TV_NESTED, 1>); break;
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_persistent(hipStream_t st, BatchHead* head, const ViewDesc* views,
+                                   unsigned long long* counters, int tv, int waves, uint32_t max_bit_words,
+                                   uint32_t max_pwords, uint32_t pt_blocks, uint32_t gs_blocks, bool denoise,
+                                   bool hdr) {
+  tv = lds_variant(tv, max_bit_words, max_pwords);
+  const uint32_t shmem = lds_bytes(tv, max_bit_words, max_pwords);
+  const dim3 gp(pt_blocks > 0 ? pt_blocks : 1), gg(gs_blocks > 0 ? gs_blocks : 1);
+  auto go = [&](auto kernel) { hipLaunchKernelGGL(kernel, gp, dim3(256), shmem, st, head, views, counters); };
+  switch (tv * 8 + (waves == 4 || waves == 5 ? waves : 1)) {
+    case TV_FAST * 8 + 4: go(k_pathtrace_batch_q<TV_FAST, 4>); break;
+    case TV_FAST * 8 + 5: go(k_pathtrace_batch_q<TV_FAST, 5>); break;
+    case TV_FAST * 8 + 1: go(k_pathtrace_batch_q<TV_FAST, 1>); break;
+    case TV_LDS * 8 + 4: go(k_pathtrace_batch_q<TV_LDS, 4>); break;
+    case TV_LDS * 8 + 5: go(k_pathtrace_batch_q<TV_LDS, 5>); break;
+    case TV_LDS * 8 + 1: go(k_pathtrace_batch_q<TV_LDS, 1>); break;
+    case TV_FLAT * 8 + 4: go(k_pathtrace_batch_q<TV_FLAT, 4>); break;
+    case TV_FLAT * 8 + 5: go(k_pathtrace_batch_q<TV_FLAT, 5>); break;
+    case TV_FLAT * 8 + 1: go(k_pathtrace_batch_q<TV_FLAT, 1>); break;
+    case TV_NESTED * 8 + 4: go(k_pathtrace_batch_q<TV_NESTED, 4>); break;
+    case TV_NESTED * 8 + 5: go(k_pathtrace_batch_q<TV_NESTED, 5>); break;
+    default: go(k_pathtrace_batch_q<TV_NESTED, 1>); break;
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_temporal_batch_gs, gg, dim3(256), 0, st, head, views);
+  if (denoise) {
+    hipLaunchKernelGGL(k_atrous_batch_gs<0>, gg, dim3(256), 0, st, head, views);
+    hipLaunchKernelGGL(k_atrous_batch_gs<1>, gg, dim3(256), 0, st, head, views);
+    hipLaunchKernelGGL(k_atrous_batch_gs<2>, gg, dim3(256), 0, st, head, views);
+  }
+  if (hdr) hipLaunchKernelGGL(k_hdr_batch_gs, gg, dim3(256), 0, st, head, views);
   return hipGetLastError();
 }
 

@@ -115,6 +115,11 @@ vrt::SceneDev vrt_scene::dev() const {
   s.pool = (const uint32_t*)d_pool.ptr;
   s.bits = (const uint32_t*)d_bits.ptr;
   s.prefix = (const uint32_t*)d_prefix.ptr;
+  s.pbits = (const uint32_t*)d_pbits.ptr;
+  s.pprefix = (const uint32_t*)d_pprefix.ptr;
+  s.pwords = psh3 >= 5 ? (1u << (psh3 - 5)) : 1u;
+  s.psh1 = psh1;
+  s.psh2 = psh2;
   s.bit_words = (uint32_t)((nbricks + 31) / 32);
   s.nx = (int)nx;
   s.ny = (int)ny;
@@ -211,6 +216,11 @@ int vrt_create(const vrt_create_info* info, vrt_ctx** out) {
     return VRT_ERR_DEVICE;
   }
   ctx->lane0.stream = ctx->stream;
+  {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) == hipSuccess && cus > 0)
+      ctx->cus = cus;
+  }
   for (auto& e : ctx->ev)
     if (hipEventCreate(&e) != hipSuccess) {
       delete ctx;
@@ -311,8 +321,18 @@ int vrt_scene_upload(vrt_ctx* ctx, const uint32_t* vox, uint32_t nx, uint32_t ny
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   const uint64_t nocc = (uint64_t)last[0] + last[1];
   sc->nocc = nocc;
+  // padded bitmap over a power-of-two brick grid: ceil(log2) bits per axis
+  auto lg = [](uint32_t v) {
+    uint32_t k = 0;
+    while ((1u << k) < v) ++k;
+    return k;
+  };
+  sc->psh1 = lg((uint32_t)bgx);
+  sc->psh2 = sc->psh1 + lg((uint32_t)bgy);
+  sc->psh3 = sc->psh2 + lg((uint32_t)bgz);
+  const uint64_t pbits_n = 1ull << sc->psh3;
   const uint64_t bytes = nb * 4 + std::max<uint64_t>(nocc, 1) * vrt::POOL_WORDS * 4 + (nb + 63) / 64 * 8 +
-                         (nb + 31) / 32 * 4;
+                         (nb + 31) / 32 * 4 + (pbits_n + 63) / 64 * 8 + (pbits_n + 31) / 32 * 4;
   int rc = ctx->reserve(bytes);
   if (rc) return rc;
   sc->bytes = bytes;
@@ -320,6 +340,8 @@ int vrt_scene_upload(vrt_ctx* ctx, const uint32_t* vox, uint32_t nx, uint32_t ny
   if (e == hipSuccess) e = sc->d_pool.alloc(std::max<uint64_t>(nocc, 1) * vrt::POOL_WORDS * 4);
   if (e == hipSuccess) e = sc->d_bits.alloc((nb + 63) / 64 * 8);
   if (e == hipSuccess) e = sc->d_prefix.alloc((nb + 31) / 32 * 4);
+  if (e == hipSuccess) e = sc->d_pbits.alloc(((pbits_n + 63) / 64) * 8);
+  if (e == hipSuccess) e = sc->d_pprefix.alloc((pbits_n + 31) / 32 * 4);
   if (e != hipSuccess) {
     ctx->unreserve(bytes);
     return ctx->fail(e == hipErrorOutOfMemory ? VRT_ERR_NO_MEMORY : VRT_ERR_DEVICE,
@@ -348,6 +370,20 @@ int vrt_scene_upload(vrt_ctx* ctx, const uint32_t* vox, uint32_t nx, uint32_t ny
       e = vrt::exclusive_scan_u32(ctx->stream, tmp.ptr, &need, (const uint32_t*)flags.ptr,
                                   (uint32_t*)sc->d_prefix.ptr, nw);
     if (e != hipSuccess) return undo(VRT_ERR_DEVICE, "bit prefix", e);
+    // the padded layout
+    const uint64_t pn = 1ull << sc->psh3, pw = (pn + 31) / 32;
+    e = vrt::launch_brick_pbits(ctx->stream, (const uint32_t*)sc->d_bricks.ptr, bgx, bgy, bgz, sc->psh1, sc->psh2,
+                                pn, (uint32_t*)sc->d_pbits.ptr);
+    if (e == hipSuccess) e = vrt::launch_popc(ctx->stream, (const uint32_t*)sc->d_pbits.ptr, pw, (uint32_t*)flags.ptr);
+    if (e != hipSuccess) return undo(VRT_ERR_DEVICE, "padded bits", e);
+    need = 0;
+    e = vrt::exclusive_scan_u32(ctx->stream, nullptr, &need, (const uint32_t*)flags.ptr,
+                                (uint32_t*)sc->d_pprefix.ptr, pw);
+    if (e == hipSuccess && need > tmp.bytes) e = tmp.alloc(need);
+    if (e == hipSuccess)
+      e = vrt::exclusive_scan_u32(ctx->stream, tmp.ptr, &need, (const uint32_t*)flags.ptr,
+                                  (uint32_t*)sc->d_pprefix.ptr, pw);
+    if (e != hipSuccess) return undo(VRT_ERR_DEVICE, "padded prefix", e);
   }
   // pass 2: pack occupied bricks into the pool
   for (int bz0 = 0; bz0 < bgz; bz0 += slab) {
@@ -730,7 +766,8 @@ int vrt_tuning_set(vrt_ctx* ctx, const vrt_tuning* in) {
   if (in->trace_kernel > 4 || in->pt_kernel > 4 || in->trace_blocks > (1u << 20) ||
       in->pt_blocks > (1u << 20) || in->trace_steps > 1024 || in->pt_steps > 1024 ||
       in->trace_min_refill > 64 || in->pt_min_refill > 64 || in->pt_chunk > (1u << 16) ||
-      !(in->pt_waves == 0 || in->pt_waves == 4 || in->pt_waves == 5))
+      !(in->pt_waves == 0 || in->pt_waves == 4 || in->pt_waves == 5) || in->batch_mode > 2 ||
+      in->batch_blocks > (1u << 16))
     return ctx->fail(VRT_ERR_INVALID, "tuning value out of range");
   ctx->tuning = *in;
   return VRT_OK;
@@ -947,7 +984,13 @@ int vrt_stats_reset(vrt_ctx* ctx) {
 
 namespace vrt {
 
+void BatchLane::drop_graphs() {
+  for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+  graphs.clear();
+}
+
 BatchLane::~BatchLane() {
+  drop_graphs();
   for (uint32_t k = 0; k < STAGES; ++k) {
     if (desc_host[k]) (void)hipHostFree(desc_host[k]);
     if (desc_done[k]) (void)hipEventDestroy(desc_done[k]);
@@ -964,16 +1007,17 @@ hipError_t BatchLane::ensure(uint32_t nviews) {
     desc_host[k] = nullptr;
   }
   desc_cap = 0;
+  drop_graphs();  // they hold the old table's address
   const uint32_t cap = std::max<uint32_t>(nviews, 64);
   for (uint32_t k = 0; k < STAGES; ++k) {
-    e = hipHostMalloc(&desc_host[k], cap * sizeof(ViewDesc), hipHostMallocDefault);
+    e = hipHostMalloc(&desc_host[k], BATCH_HEAD_BYTES + cap * sizeof(ViewDesc), hipHostMallocDefault);
     if (e != hipSuccess) return e;
     if (!desc_done[k]) {
       e = hipEventCreateWithFlags(&desc_done[k], hipEventDisableTiming);
       if (e != hipSuccess) return e;
     }
   }
-  e = desc_dev.alloc(cap * sizeof(ViewDesc));
+  e = desc_dev.alloc(BATCH_HEAD_BYTES + cap * sizeof(ViewDesc));
   if (e != hipSuccess) return e;
   desc_cap = cap;
   return hipSuccess;
@@ -990,8 +1034,9 @@ int vrt_ctx::render_batch(vrt::BatchLane& lane, const vrt_view* views, uint32_t 
   const hipStream_t st = lane.stream;
   const uint32_t stage = lane.desc_next++ % vrt::BatchLane::STAGES;
   HIP_TRY(this, hipEventSynchronize(lane.desc_done[stage]));  // its previous copy has run
-  vrt::ViewDesc* d = (vrt::ViewDesc*)lane.desc_host[stage];
-  uint32_t tiles = 0, max_bit_words = 0;
+  vrt::BatchHead* hh = (vrt::BatchHead*)lane.desc_host[stage];
+  vrt::ViewDesc* d = (vrt::ViewDesc*)((char*)lane.desc_host[stage] + vrt::BATCH_HEAD_BYTES);
+  uint32_t tiles = 0, max_bit_words = 0, max_pwords = 0;
   bool any_denoise = false, any_hdr = false;
   for (uint32_t k = 0; k < nviews; ++k) {
     const vrt_view& v = views[k];
@@ -1019,17 +1064,62 @@ int vrt_ctx::render_batch(vrt::BatchLane& lane, const vrt_view* views, uint32_t 
     d[k].vcount = vcount ? vcount[k] : nullptr;
     d[k].hdr = hdr ? hdr[k] : nullptr;
     max_bit_words = std::max(max_bit_words, d[k].s.bit_words);
+    max_pwords = std::max(max_pwords, d[k].s.pwords);
     any_denoise |= d[k].denoise != 0u;
     any_hdr |= d[k].hdr != nullptr;
     tiles += d[k].tiles_x * ((vs.h + 15u) / 16u);
   }
-  const vrt::ViewDesc* dd = (const vrt::ViewDesc*)lane.desc_dev.ptr;
-  HIP_TRY(this, hipMemcpyAsync(lane.desc_dev.ptr, d, nviews * sizeof(vrt::ViewDesc), hipMemcpyHostToDevice, st));
+  std::memset(hh, 0, sizeof *hh);
+  hh->nviews = nviews;
+  hh->tiles = tiles;
+  hh->queue = 0;
+  vrt::BatchHead* dh = (vrt::BatchHead*)lane.desc_dev.ptr;
+  const vrt::ViewDesc* dd = (const vrt::ViewDesc*)((char*)lane.desc_dev.ptr + vrt::BATCH_HEAD_BYTES);
+  HIP_TRY(this, hipMemcpyAsync(lane.desc_dev.ptr, lane.desc_host[stage],
+                               vrt::BATCH_HEAD_BYTES + nviews * sizeof(vrt::ViewDesc), hipMemcpyHostToDevice, st));
   HIP_TRY(this, hipEventRecord(lane.desc_done[stage], st));
   HIP_TRY(this, hipEventRecord(ev[0], st));
+  if (tuning.batch_mode == 1 || tuning.batch_mode == 2) {
+    const int tv = lane_variant(tuning.pt_kernel), wv = (int)tuning.pt_waves;
+    const uint32_t ptb = tuning.batch_blocks ? tuning.batch_blocks : (uint32_t)cus * 8u;
+    const uint32_t gsb = (uint32_t)cus * 8u;
+    auto enqueue = [&] {
+      return vrt::launch_batch_persistent(st, dh, dd, (unsigned long long*)d_counters.ptr, tv, wv, max_bit_words,
+                                          max_pwords, ptb, gsb, any_denoise, any_hdr);
+    };
+    if (tuning.batch_mode == 1) {
+      HIP_TRY(this, enqueue());
+    } else {
+      // one graph per launch shape; the table it reads is rewritten per batch
+      const uint64_t key = (uint64_t)tv | (uint64_t)wv << 4 | (uint64_t)any_denoise << 8 | (uint64_t)any_hdr << 9 |
+                           (uint64_t)max_bit_words << 10 | (uint64_t)max_pwords << 34 | (uint64_t)(ptb & 0xffff) << 48;
+      auto it = lane.graphs.find(key);
+      if (it == lane.graphs.end()) {
+        if (lane.graphs.size() >= 16) lane.drop_graphs();
+        hipGraph_t g = nullptr;
+        HIP_TRY(this, hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+        const hipError_t le = enqueue();
+        const hipError_t ce = hipStreamEndCapture(st, &g);
+        if (le != hipSuccess || ce != hipSuccess) {
+          if (g) (void)hipGraphDestroy(g);
+          return fail(VRT_ERR_DEVICE, "graph capture: %s", hipGetErrorString(le != hipSuccess ? le : ce));
+        }
+        hipGraphExec_t ex = nullptr;
+        const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ie != hipSuccess) return fail(VRT_ERR_DEVICE, "graph instantiate: %s", hipGetErrorString(ie));
+        it = lane.graphs.emplace(key, ex).first;
+      }
+      HIP_TRY(this, hipGraphLaunch(it->second, st));
+    }
+    HIP_TRY(this, hipEventRecord(ev[1], st));
+    HIP_TRY(this, hipEventRecord(ev[2], st));
+    HIP_TRY(this, hipEventRecord(ev[3], st));
+    return VRT_OK;
+  }
   HIP_TRY(this, vrt::launch_pathtrace_batch(st, dd, nviews, tiles, (unsigned long long*)d_counters.ptr,
                                             lane_variant(tuning.pt_kernel), (int)tuning.pt_waves,
-                                            max_bit_words));
+                                            max_bit_words, max_pwords));
   HIP_TRY(this, hipEventRecord(ev[1], st));
   HIP_TRY(this, vrt::launch_temporal_batch(st, dd, nviews, tiles));
   HIP_TRY(this, hipEventRecord(ev[2], st));

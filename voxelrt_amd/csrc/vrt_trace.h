@@ -25,6 +25,13 @@ struct SceneDev {
   const uint32_t* bits;    // occupancy bit per brick: bit (i & 31) of word i >> 5
   const uint32_t* prefix;  // prefix[w] = set bits in bits[0 .. w-1] (= pool slot of the word's first brick)
   uint32_t bit_words;      // ceil(bricks / 32)
+  // The same bitmap over a power-of-two padded brick grid, indexed by the
+  // packed position bx | by << psh1 | bz << psh2 (padding bits are 0), and
+  // its prefix counts: the layout of the single-loop DDA (vrt_dda_fast.h).
+  const uint32_t* pbits;
+  const uint32_t* pprefix;
+  uint32_t pwords;
+  uint32_t psh1, psh2;
   int nx, ny, nz;
   int bgx, bgy, bgz;
   float voxel_size, inv_voxel;
