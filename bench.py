@@ -30,13 +30,14 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "Mrays/s & ms/frame @1080p 256³; batched/served p50/p99 latency vs plain"
 
 
-def _pmc_traffic(name: str):
-    """HBM bytes per launch from a committed rocprofv3 --pmc summary, if any
-    (profiles/<name>: a line 'traffic_bytes_per_launch: <int>')."""
+def _pmc_traffic(name: str, key: str):
+    """HBM bytes per launch of one kernel variant from a committed rocprofv3
+    --pmc summary, if it has that variant (profiles/<name>: a line
+    'traffic_bytes_per_launch[<key>]: <int>'), else None."""
     path = os.path.join(ROOT, "profiles", name)
     try:
         for line in open(path):
-            if line.startswith("traffic_bytes_per_launch:"):
+            if line.startswith(f"traffic_bytes_per_launch[{key}]:"):
                 return int(float(line.split(":", 1)[1]))
     except OSError:
         pass
@@ -428,7 +429,8 @@ def main():
                    "width": w, "height": h, "grid": 256, "spp": 1, "bounces": 2},
         "roofline": {"bound": "hbm", "kernel": "k_pathtrace", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": _pmc_traffic("pathtrace_frame1080_pmc.txt"),
+                     "traffic": _pmc_traffic("pathtrace_frame1080_pmc.txt",
+                                             f"pt_kernel={tuned['pt_kernel']},pt_waves={tuned['pt_waves']}"),
                      "alg_bytes_per_launch": int(alg_bytes), "avg_launch_ms": round(pt_avg_ms, 4)},
         "tuning": {k: tuned[k] for k in ("pt_kernel", "pt_waves", "pt_blocks", "pt_steps", "pt_min_refill",
                                           "pt_chunk")},
