@@ -136,6 +136,9 @@ typedef struct vrt_frame {
   uint64_t  rays;       /* rays traced for this view (primary+shadow+bounce) */
   uint64_t  brick_steps;
   uint64_t  voxel_steps;
+  uint8_t*  ldr;        /* rgba8 present.wgsl:40-55 output: the same illum
+                           times albedo, times exposure (params1.z), ACES fit,
+                           gamma 1/2.2 (the canvas capture, main.js:571-626) */
 } vrt_frame;
 
 /* Aggregate counters since vrt_create or the last vrt_stats_reset. */

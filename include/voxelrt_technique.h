@@ -33,7 +33,8 @@ struct MethodHIP : Technique {
   double prevPos[3];
   bool havePrev = false;
   std::vector<float> hdr;
-  std::vector<uint32_t> packed;
+  std::vector<uint32_t> packed;  // the grid as uploaded (padded dims px, py, pz)
+  vrt_uniforms last_u{};         // uniforms of the last frame() (for inspection / replay)
 
   const char* name() override { return "HIP"; }
 
@@ -124,6 +125,7 @@ struct MethodHIP : Technique {
     v.view_id = 1;
     v.spp = 1;
     v.accum_format = VRT_ACCUM_RGBA32F;
+    last_u = v.u;
     vrt_frame f{};
     f.hdr = hdr.data();
     if (vrt_render(ctx, &v, 1, &f) != VRT_OK) {

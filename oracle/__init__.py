@@ -124,6 +124,7 @@ def _declare(L):
     L.orc_temporal.argtypes = [C.POINTER(Uniforms), P, P, P, P, C.c_uint32, P, C.c_int]
     L.orc_atrous.argtypes = [C.POINTER(Uniforms), P, C.c_uint32, P, C.c_int, P, C.c_int]
     L.orc_hdr.argtypes = [C.c_int, C.c_int, P, C.c_uint32, P, P]
+    L.orc_present.argtypes = [C.c_int, C.c_int, P, C.c_uint32, P, C.c_float, P]
 
 
 def _ptr(a: np.ndarray):
@@ -307,6 +308,14 @@ def hdr(illum, illum_fmt, albedo):
     h, w = albedo.shape[:2]
     out = np.zeros((h, w, 3), np.float32)
     lib().orc_hdr(w, h, _ptr(illum), illum_fmt, _ptr(albedo), _ptr(out))
+    return out
+
+
+def present(illum, illum_fmt, albedo, exposure=1.0):
+    """present.wgsl:40-55 -> (h, w, 4) uint8 (alpha 255)."""
+    h, w = albedo.shape[:2]
+    out = np.zeros((h, w, 4), np.uint8)
+    lib().orc_present(w, h, _ptr(illum), illum_fmt, _ptr(albedo), C.c_float(exposure), _ptr(out))
     return out
 
 

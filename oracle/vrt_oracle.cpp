@@ -905,4 +905,20 @@ void orc_hdr(int w, int h, const void* illum, uint32_t illum_format, const uint8
     }
 }
 
+void orc_present(int w, int h, const void* illum, uint32_t illum_format, const uint8_t* albedo, float exposure,
+                 uint8_t* rgba) {
+  const float inv_gamma = (float)(1.0 / 2.2);  // WGSL folds 1.0 / 2.2 as an abstract float
+  for (int64_t i = 0; i < (int64_t)w * h; ++i) {
+    for (int c = 0; c < 3; ++c) {
+      const float s = (float)albedo[i * 4 + c] / 255.0f;  // rgba8unorm sample
+      const float x = (load_tex(illum, illum_format, (uint64_t)i * 4 + c) * (s * s)) * exposure;
+      const float num = x * (2.51f * x + 0.03f);
+      const float den = x * (2.43f * x + 0.59f) + 0.14f;
+      const float t = fclamp_(num / den, 0.0f, 1.0f);
+      rgba[i * 4 + c] = f32_to_unorm8(pow_own(t, inv_gamma));
+    }
+    rgba[i * 4 + 3] = 255;
+  }
+}
+
 }  // extern "C"

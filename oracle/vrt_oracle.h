@@ -119,6 +119,11 @@ void orc_atrous(const vrt_uniforms* u, const void* src, uint32_t src_format,
 void orc_hdr(int w, int h, const void* illum, uint32_t illum_format,
              const uint8_t* albedo, float* hdr);
 
+/* present.wgsl:40-55: c = illum * (alb8/255)^2 * exposure, ACES fit,
+ * pow(c, 1/2.2), stored rgba8unorm (alpha 255), FP32 like the shader. */
+void orc_present(int w, int h, const void* illum, uint32_t illum_format,
+                 const uint8_t* albedo, float exposure, uint8_t* rgba);
+
 #ifdef __cplusplus
 }
 #endif

@@ -44,8 +44,10 @@ class OracleChain:
             d0 = oracle.atrous(u, d1, gb, 4, 0, threads=self.threads)
             out["denoised"] = d0
             out["hdr"] = oracle.hdr(d0, 0, alb)
+            out["ldr"] = oracle.present(d0, 0, alb, u.params1[2])
         else:
             out["hdr"] = oracle.hdr(acc, self.fmt, alb)
+            out["ldr"] = oracle.present(acc, self.fmt, alb, u.params1[2])
         self.frames += 1
         return out
 

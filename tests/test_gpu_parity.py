@@ -330,9 +330,9 @@ def test_frame_sequence_bitexact(ctx, pt_variant):
     chain = OracleChain(og, w, h, 0)
     for i in range(5):
         r.set_pose(*cf.world_orbit(desc, i, 64))
-        got = r.frame(("radiance", "albedo", "gbuf", "accum", "denoised", "hdr"))
+        got = r.frame(("radiance", "albedo", "gbuf", "accum", "denoised", "hdr", "ldr"))
         want = chain.frame(to_oracle_uniforms(r.last_view.u), 48.0)
-        for k in ("radiance", "albedo", "gbuf", "accum", "denoised", "hdr"):
+        for k in ("radiance", "albedo", "gbuf", "accum", "denoised", "hdr", "ldr"):
             assert_bits_equal(f"frame {i} {k}", got[k], want[k])
     ctx.release_view(950)
     sc.release()

@@ -133,3 +133,45 @@ def test_atrous_constant_image_fixed_point(step):
     gb[:3, :5, 3] = -1.0  # a sky patch is copied, and skipped as a tap
     out = oracle.atrous(u, src, gb, step, 0)
     assert np.array_equal(out, src)
+
+
+def _present_f64(illum, alb8, exposure):
+    """present.wgsl:40-55 in float64 with numpy's pow (independent of the
+    oracle's FP32 code and own pow)."""
+    s = alb8[..., :3].astype(np.float64) / 255.0
+    x = illum[..., :3].astype(np.float64) * s * s * exposure
+    t = np.clip((x * (2.51 * x + 0.03)) / (x * (2.43 * x + 0.59) + 0.14), 0.0, 1.0)
+    return np.floor(np.power(t, 1.0 / 2.2) * 255.0 + 0.5)
+
+
+@pytest.mark.parametrize("exposure", [1.0, 0.35, 4.0])
+def test_present_matches_float64_formula(exposure):
+    rng = np.random.default_rng(7)
+    h, w = 64, 96
+    illum = np.exp(rng.uniform(-8, 5, (h, w, 4))).astype(np.float32)
+    illum[0, :8, :3] = 0.0                   # black
+    illum[1, :8, :3] = 1e6                   # saturates to 1 -> 255
+    alb = rng.integers(0, 256, (h, w, 4), dtype=np.uint8)
+    alb[:2, :8, :3] = 255
+    got = oracle.present(illum, 1, alb, exposure)
+    want = _present_f64(illum, alb, exposure)
+    assert (got[..., 3] == 255).all()
+    assert np.abs(got[..., :3].astype(np.int64) - want.astype(np.int64)).max() <= 1
+    assert (got[0, :8, :3] == 0).all() and (got[1, :8, :3] == 255).all()
+    # monotone in illumination for a fixed albedo
+    ramp = np.linspace(0, 20, 4096, dtype=np.float32)
+    il = np.zeros((1, 4096, 4), np.float32)
+    il[0, :, 0] = ramp
+    a = np.full((1, 4096, 4), 200, np.uint8)
+    r = oracle.present(il, 1, a, exposure)[0, :, 0].astype(np.int64)
+    assert (np.diff(r) >= 0).all()
+
+
+def test_present_reads_f16_history():
+    rng = np.random.default_rng(3)
+    h, w = 16, 16
+    f = rng.uniform(0, 3, (h, w, 4)).astype(np.float16)
+    alb = rng.integers(0, 256, (h, w, 4), dtype=np.uint8)
+    a = oracle.present(f.view(np.uint16), 0, alb, 1.0)
+    b = oracle.present(f.astype(np.float32), 1, alb, 1.0)
+    assert np.array_equal(a, b)

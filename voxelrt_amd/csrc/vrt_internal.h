@@ -136,7 +136,7 @@ struct vrt_ctx {
   vrt_tuning tuning{};   // kernel variants + launch shapes (vrt_tuning_set / vrt_autotune)
   vrt::DeviceBuffer d_counters;  // u64 x4: rays, brick steps, voxel steps, hits
   vrt::DeviceBuffer d_queue;     // u32 work-queue counters of the persistent kernels
-  vrt::DeviceBuffer scratch_rays, scratch_hits, scratch_hdr;
+  vrt::DeviceBuffer scratch_rays, scratch_hits, scratch_hdr, scratch_ldr;
   std::vector<vrt_scene*> scenes;
   std::unordered_map<uint32_t, ViewState> views;
 

@@ -62,6 +62,8 @@ hipError_t launch_temporal(hipStream_t st, const Uniforms& u, const void* radian
                            int fmt);
 hipError_t launch_atrous(hipStream_t st, const Uniforms& u, const void* src, int src_fmt,
                          const void* gbuf, int step, void* dst);
+hipError_t launch_present(hipStream_t st, uint32_t npx, const void* illum, int fmt, const void* albedo,
+                          float exposure, void* out);
 hipError_t launch_hdr(hipStream_t st, uint32_t npx, const void* illum, int fmt, const void* albedo,
                       float* hdr);
 

@@ -841,6 +841,31 @@ __global__ __launch_bounds__(256) void k_hdr(uint32_t npx, const void* __restric
   hdr[i * 3 + 2] = (float)((double)il.z * ab * ab);
 }
 
+// present.wgsl:40-55 in FP32 with this project's pow: illum * (alb8/255)^2 *
+// exposure, ACES fit, gamma 1/2.2, stored rgba8unorm.
+template <int FMT>
+__global__ __launch_bounds__(256) void k_present(uint32_t npx, const void* __restrict__ illum,
+                                                 const uchar4* __restrict__ albedo, float exposure,
+                                                 uchar4* __restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npx) return;
+  const float4 il = load_accum<FMT>(illum, i);
+  const uchar4 a = albedo[i];
+  const float inv_gamma = (float)(1.0 / 2.2);
+  const float ic[3] = {il.x, il.y, il.z};
+  const uint8_t ab[3] = {a.x, a.y, a.z};
+  uint32_t o[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const float s = (float)ab[c] / 255.0f;
+    const float x = (ic[c] * (s * s)) * exposure;
+    const float num = x * (2.51f * x + 0.03f);
+    const float den = x * (2.43f * x + 0.59f) + 0.14f;
+    o[c] = f32_to_unorm8(pow_r(vclamp(num / den, 0.0f, 1.0f), inv_gamma));
+  }
+  out[i] = make_uchar4((uint8_t)o[0], (uint8_t)o[1], (uint8_t)o[2], 255);
+}
+
 // HDR of every view that asks for it (d.hdr != null): the denoised target
 // when denoise is on, else the accumulation, as k_hdr.
 VRT_D void hdr_batch_tile(const ViewDesc* __restrict__ views, uint32_t nviews, uint32_t b) {
@@ -1121,6 +1146,17 @@ hipError_t launch_atrous(hipStream_t st, const Uniforms& u, const void* src, int
   if (step != 1 && step != 2 && step != 4) return hipErrorInvalidValue;
   if (src_fmt == 1) atrous_fmt<1>(st, u, src, gbuf, step, dst, tiles);
   else atrous_fmt<0>(st, u, src, gbuf, step, dst, tiles);
+  return hipGetLastError();
+}
+
+hipError_t launch_present(hipStream_t st, uint32_t npx, const void* illum, int fmt, const void* albedo,
+                          float exposure, void* out) {
+  if (fmt == 1)
+    hipLaunchKernelGGL(k_present<1>, dim3(div_up(npx, 256)), dim3(256), 0, st, npx, illum, (const uchar4*)albedo,
+                       exposure, (uchar4*)out);
+  else
+    hipLaunchKernelGGL(k_present<0>, dim3(div_up(npx, 256)), dim3(256), 0, st, npx, illum, (const uchar4*)albedo,
+                       exposure, (uchar4*)out);
   return hipGetLastError();
 }
 

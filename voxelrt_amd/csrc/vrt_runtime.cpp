@@ -578,6 +578,15 @@ static int copy_out(vrt_ctx* ctx, ViewState& vs, const vrt_view& v, vrt_frame* f
                                  (float*)ctx->scratch_hdr.ptr));
     HIP_TRY(ctx, hipMemcpyAsync(fo->hdr, ctx->scratch_hdr.ptr, px * 12, hipMemcpyDeviceToHost, ctx->stream));
   }
+  if (fo->ldr) {
+    // present.wgsl:40-55 on the same illumination as the HDR capture
+    const void* illum = denoised ? vs.den[0].ptr : vs.accum[parity].ptr;
+    const int fmt = denoised ? 0 : (int)vs.fmt;
+    HIP_TRY(ctx, ctx->scratch_ldr.ensure(px * 4));
+    HIP_TRY(ctx, vrt::launch_present(ctx->stream, (uint32_t)px, illum, fmt, vs.albedo.ptr, v.u.params1[2],
+                                     ctx->scratch_ldr.ptr));
+    HIP_TRY(ctx, hipMemcpyAsync(fo->ldr, ctx->scratch_ldr.ptr, px * 4, hipMemcpyDeviceToHost, ctx->stream));
+  }
   (void)v;
   return VRT_OK;
 }

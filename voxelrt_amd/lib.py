@@ -63,7 +63,8 @@ class View(C.Structure):
 class Frame(C.Structure):
     _fields_ = [("radiance", C.c_void_p), ("albedo", C.c_void_p), ("gbuf", C.c_void_p),
                 ("accum", C.c_void_p), ("denoised", C.c_void_p), ("hdr", C.c_void_p),
-                ("rays", C.c_uint64), ("brick_steps", C.c_uint64), ("voxel_steps", C.c_uint64)]
+                ("rays", C.c_uint64), ("brick_steps", C.c_uint64), ("voxel_steps", C.c_uint64),
+                ("ldr", C.c_void_p)]
 
 
 class Stats(C.Structure):
@@ -355,7 +356,7 @@ class Context:
         if outs is not None:
             farr = (Frame * nv)()
             for k, o in enumerate(outs):
-                for name in ("radiance", "albedo", "gbuf", "accum", "denoised", "hdr"):
+                for name in ("radiance", "albedo", "gbuf", "accum", "denoised", "hdr", "ldr"):
                     a = o.get(name) if o else None
                     if a is not None:
                         keep.append(a)

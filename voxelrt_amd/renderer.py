@@ -86,13 +86,23 @@ class Renderer:
         return arrays
 
     def capture(self, hdr: bool = True):
-        """Render a frame and return its HDR image (main.js:628-659)."""
-        return self.frame(("hdr",) if hdr else ("denoised",))
+        """Render a frame and return what main.js's capture() returns
+        (main.js:571-659): the presented rgb bytes (present.wgsl:40-55), the
+        mean / std of their per-pixel (r+g+b)/3, and with hdr=True the linear
+        HDR image."""
+        out = self.frame(("ldr", "hdr") if hdr else ("ldr",))
+        rgb = out["ldr"][..., :3]
+        lum = rgb.astype(np.float64).sum(-1) / 3.0
+        res = {"rgb": rgb, "mean": float(lum.mean()), "std": float(lum.std()), "w": self.width, "h": self.height}
+        if hdr:
+            res["hdr"] = out["hdr"]
+        return res
 
     def _alloc(self, names):
         h, w = self.height, self.width
         shapes = {"radiance": ((h, w, 4), np.uint16), "albedo": ((h, w, 4), np.uint8),
                   "gbuf": ((h, w, 4), np.float32),
                   "accum": ((h, w, 4), np.uint16 if self.accum_format == 0 else np.float32),
-                  "denoised": ((h, w, 4), np.uint16), "hdr": ((h, w, 3), np.float32)}
+                  "denoised": ((h, w, 4), np.uint16), "hdr": ((h, w, 3), np.float32),
+                  "ldr": ((h, w, 4), np.uint8)}
         return {n: np.zeros(*shapes[n]) for n in names}
