@@ -93,7 +93,10 @@ def traverse(args):
                 "lane_fast": dict(trace_kernel=4),
                 "persistent": dict(trace_kernel=1, trace_blocks=2048, trace_steps=8, trace_min_refill=16)}
     for p in (0.05, 0.25, 0.6):
-        sc = ctx.upload(cf.traverse_scene(p))
+        vox = cf.traverse_scene(p)
+        sc = ctx.upload(vox)
+        if not args.no_cpu:
+            print(json.dumps(_traverse_cpu(vox, rays, p)), flush=True)
         for name in ("lane", "lane_lds", "lane_fast", "persistent", "autotuned"):
             if name == "autotuned":
                 tuned = ctx.autotune(sc, 1 << 22).as_dict()
@@ -103,6 +106,40 @@ def traverse(args):
                           tuned if name == "autotuned" else variants[name])
         sc.release()
     ctx.close()
+
+
+def _traverse_cpu(vox, rays, p, sample=1 << 20):
+    """CPU baselines on the first `sample` rays of the same workload: the
+    oracle's FP32 DDA (all host cores and one core) and the reference's own
+    VoxelBench brick DDA (core.cpp:52-130, double t, 4^3 bricks) compiled from
+    its sources in oracle/_ref, on all host cores."""
+    import numpy as np
+
+    import oracle
+
+    r = rays[:sample]
+    g = oracle.OracleGrid(vox)
+    cores = os.cpu_count() or 1
+    out = {"metric": "traversal CPU baseline Mrays/s (256^3)", "p": p, "sample": f"first {len(r)} of the 2^24 rays",
+           "cpu_baseline": []}
+    for threads in (cores, 1):
+        rr = r if threads > 1 else r[: len(r) // 8]
+        t0 = time.perf_counter()
+        oracle.trace_rays(g, rr, threads=threads)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"].append({"value": round(len(rr) / dt / 1e6, 2), "unit": "Mrays/s", "cores": threads,
+                                    "kind": "port", "sample": f"{len(rr)} rays, oracle FP32 DDA (voxel.wgsl)"})
+    if oracle.ref_lib() is not None:
+        rv = np.ascontiguousarray(r.copy())
+        rv[:, :3] /= 0.0625  # VoxelBench works in voxel units
+        oracle.vb_trace(vox, rv[:1], naive=False, threads=cores)  # builds its grid + brickmap once
+        t0 = time.perf_counter()
+        oracle.vb_trace(vox, rv, naive=False, threads=cores)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"].append({"value": round(len(rv) / dt / 1e6, 2), "unit": "Mrays/s", "cores": cores,
+                                    "kind": "reference",
+                                    "sample": f"{len(rv)} rays, VoxelBench trace() (core.cpp:52-130), occupancy only"})
+    return out
 
 
 def _traverse_one(args, ctx, sc, p, name, d_rays, d_hits, n_rays, tuning):

@@ -322,17 +322,24 @@ def present(illum, illum_fmt, albedo, exposure=1.0):
 # ---------------------------------------------------------------------------
 # VoxelBench (reference sources) traversal
 # ---------------------------------------------------------------------------
+_vb_grid_key = None
+
+
 def vb_trace(occ: np.ndarray, rays_vox: np.ndarray, naive: bool, tmax: float = 1e9,
              threads: int = 0):
     """Trace rays (voxel units) with VoxelBench's trace/traceNaive on an
     occupancy grid shaped (nz, ny, nx).  Returns (vox_xyz_face[n,4], t[n])."""
+    global _vb_grid_key
     R = ref_lib()
     if R is None:
         raise RuntimeError("oracle/_ref/libvoxelbench_core.so not built")
-    occ = np.ascontiguousarray(occ != 0, np.uint8)
-    nz, ny, nx = occ.shape
-    if R.vb_set_grid(_ptr(occ), nx, ny, nz) != 0:
-        raise ValueError("VoxelBench dims must be multiples of 4")
+    key = (id(occ), occ.shape, occ.__array_interface__["data"][0])
+    if key != _vb_grid_key:  # the VoxelBench grid + brickmap is rebuilt only for a new grid
+        occ8 = np.ascontiguousarray(occ != 0, np.uint8)
+        nz, ny, nx = occ8.shape
+        if R.vb_set_grid(_ptr(occ8), nx, ny, nz) != 0:
+            raise ValueError("VoxelBench dims must be multiples of 4")
+        _vb_grid_key = key
     rays_vox = np.ascontiguousarray(rays_vox, np.float32).reshape(-1, 6)
     out = np.zeros((len(rays_vox), 4), np.int32)
     t = np.zeros(len(rays_vox), np.float32)
