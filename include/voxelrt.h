@@ -177,6 +177,19 @@ int  vrt_device_count(int32_t* count);
  * masks there (the build scene.js:411-433 does on the CPU). */
 int vrt_scene_upload(vrt_ctx* ctx, const uint32_t* vox, uint32_t nx, uint32_t ny,
                      uint32_t nz, uint64_t scene_key, vrt_scene** out);
+/* Editable scenes (VRT_SCENE_EDITABLE) also keep the dense grid resident
+ * (4 B per voxel, counted in the HBM budget); vrt_scene_edit writes a box of
+ * voxels (sx*sy*sz words, x fastest) into it and rebuilds the bricks, masks,
+ * pool and occupancy bitmaps on the device (a full rebuild: slots follow
+ * brick order, so a change of occupancy anywhere moves later slots).  Work
+ * already enqueued that reads the scene completes first; frames launched
+ * after the call see the edit.  Replaces VoxelBench's setVox + brick
+ * rebuild on a SceneEvent geomEdit (core.h:78-82, scenes.h:8-12). */
+#define VRT_SCENE_EDITABLE 1u
+int vrt_scene_upload_ex(vrt_ctx* ctx, const uint32_t* vox, uint32_t nx, uint32_t ny,
+                        uint32_t nz, uint64_t scene_key, uint32_t flags, vrt_scene** out);
+int vrt_scene_edit(vrt_ctx* ctx, vrt_scene* scene, const uint32_t* vox, uint32_t x0,
+                   uint32_t y0, uint32_t z0, uint32_t sx, uint32_t sy, uint32_t sz);
 int vrt_scene_release(vrt_ctx* ctx, vrt_scene* scene);
 /* Read back the device-built brick words (nbricks) and masks (nbricks*16). */
 int vrt_scene_bricks(vrt_ctx* ctx, const vrt_scene* scene, uint32_t* bricks,

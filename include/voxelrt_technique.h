@@ -59,19 +59,56 @@ struct MethodHIP : Technique {
     return w ? w : 1u;  // solid voxels must stay non-zero
   }
 
+  void pack(std::vector<uint32_t>& dst) const {
+    dst.assign((size_t)px * py * pz, 0u);
+    for (int z = 0; z < NZ; ++z)
+      for (int y = 0; y < NY; ++y)
+        for (int x = 0; x < NX; ++x)
+          dst[(size_t)x + (size_t)y * px + (size_t)z * px * py] = packMaterial(grid[gi(x, y, z)]);
+  }
+
   void upload() {
     if (scene) vrt_scene_release(ctx, scene);
     scene = nullptr;
     px = (uint32_t)((NX + 7) / 8 * 8);
     py = (uint32_t)((NY + 7) / 8 * 8);
     pz = (uint32_t)((NZ + 7) / 8 * 8);
-    packed.assign((size_t)px * py * pz, 0u);
-    for (int z = 0; z < NZ; ++z)
-      for (int y = 0; y < NY; ++y)
-        for (int x = 0; x < NX; ++x)
-          packed[(size_t)x + (size_t)y * px + (size_t)z * px * py] = packMaterial(grid[gi(x, y, z)]);
-    if (vrt_scene_upload(ctx, packed.data(), px, py, pz, 0, &scene) != VRT_OK)
+    pack(packed);
+    // editable: geometry events rebuild on the device from a box edit
+    if (vrt_scene_upload_ex(ctx, packed.data(), px, py, pz, 0, VRT_SCENE_EDITABLE, &scene) != VRT_OK)
       std::fprintf(stderr, "MethodHIP: scene upload failed: %s\n", vrt_last_error(ctx));
+  }
+
+  // A geometry event (scenes.h:8-12): send the bounding box of the changed
+  // voxels to the device, which rebuilds bricks, masks and pool there.
+  void edit() {
+    std::vector<uint32_t> next;
+    pack(next);
+    uint32_t lo[3] = {px, py, pz}, hi[3] = {0, 0, 0};
+    bool any = false;
+    for (uint32_t z = 0; z < pz; ++z)
+      for (uint32_t y = 0; y < py; ++y)
+        for (uint32_t x = 0; x < px; ++x) {
+          const size_t i = (size_t)x + (size_t)y * px + (size_t)z * px * py;
+          if (next[i] == packed[i]) continue;
+          const uint32_t c[3] = {x, y, z};
+          for (int k = 0; k < 3; ++k) {
+            lo[k] = std::min(lo[k], c[k]);
+            hi[k] = std::max(hi[k], c[k]);
+          }
+          any = true;
+        }
+    packed.swap(next);
+    if (!any) return;
+    const uint32_t sx = hi[0] - lo[0] + 1, sy = hi[1] - lo[1] + 1, sz = hi[2] - lo[2] + 1;
+    std::vector<uint32_t> box((size_t)sx * sy * sz);
+    for (uint32_t z = 0; z < sz; ++z)
+      for (uint32_t y = 0; y < sy; ++y)
+        for (uint32_t x = 0; x < sx; ++x)
+          box[(size_t)x + (size_t)y * sx + (size_t)z * sx * sy] =
+              packed[(size_t)(lo[0] + x) + (size_t)(lo[1] + y) * px + (size_t)(lo[2] + z) * px * py];
+    if (vrt_scene_edit(ctx, scene, box.data(), lo[0], lo[1], lo[2], sx, sy, sz) != VRT_OK)
+      std::fprintf(stderr, "MethodHIP: scene edit failed: %s\n", vrt_last_error(ctx));
   }
 
   void init() override {
@@ -93,7 +130,7 @@ struct MethodHIP : Technique {
   }
 
   void onEvent(const SceneEvent& e) override {
-    if (e.geomEdit && ctx) upload();  // lighting-only events: SKY is read every frame
+    if (e.geomEdit && ctx && scene) edit();  // lighting-only events: SKY is read every frame
   }
 
   long frame(long rayBudget) override {

@@ -391,3 +391,49 @@ def test_batched_views_match_single_views(ctx, tuning):
         ctx.release_view(3000 + k)
     for sc in scenes:
         sc.release()
+
+
+# ---------------------------------------------------------------------------
+# editable scenes: device rebuild after a box edit
+# ---------------------------------------------------------------------------
+def test_scene_edits_rebuild_on_device(ctx):
+    """Carve a hole, fill a block and recolour voxels of an editable scene:
+    after each edit the device bricks/masks, traced hits and a rendered frame
+    equal the oracle's on the edited dense grid, and the pool grows when the
+    occupied-brick count does."""
+    desc = cf.world_desc(64, 7)
+    vox = v.generate_scene(**desc)
+    sc = ctx.upload(vox, editable=True)
+    rng = np.random.default_rng(11)
+    edits = [((8, 0, 8), np.zeros((20, 24, 30), np.uint32)),                          # carve
+             ((30, 30, 30), np.full((10, 17, 9), 0x10406080, np.uint32)),             # fill into empty air
+             ((0, 0, 0), rng.integers(0, 2, (64, 64, 64), dtype=np.uint32) * 0x01808080)]  # rewrite all
+    rays = cf.traverse_rays(1 << 15, n=64, seed=12)
+    for k, (origin, box) in enumerate(edits):
+        x0, y0, z0 = origin
+        sz, sy, sx = box.shape
+        vox[z0:z0 + sz, y0:y0 + sy, x0:x0 + sx] = box
+        sc.edit(box, origin)
+        words, masks = sc.bricks()
+        ob, om = oracle.build_bricks(vox)
+        assert np.array_equal(words != 0, ob != 0), k
+        assert np.array_equal(masks, om), k
+        og = oracle.OracleGrid(vox)
+        for tk in (0, 4):
+            old = ctx.tuning().as_dict()
+            ctx.set_tuning(trace_kernel=tk)
+            _hits_equal(ctx.trace_rays(sc, rays), oracle.trace_rays(og, rays), f"edit {k} trace_kernel {tk}")
+            ctx.set_tuning(**old)
+        pos, yaw, pitch = cf.world_orbit(desc, k, 64)
+        u, _ = v.make_uniforms(pos, yaw, pitch, 96, 64, frame=k, bounces=2, flags=0)
+        got, _ = _render_one(ctx, sc, 960 + k, u, 48.0, 1, 96, 64)
+        rad, alb, gb, _ = oracle.pathtrace(og, to_oracle_uniforms(u), 48.0, 1)
+        assert_bits_equal(f"edit {k} radiance", got["radiance"], rad)
+        assert_bits_equal(f"edit {k} gbuf", got["gbuf"], gb)
+    with pytest.raises(v.VrtError):
+        sc.edit(np.zeros((4, 4, 4), np.uint32), (62, 0, 0))  # outside the grid
+    sc.release()
+    plain = ctx.upload(vox)
+    with pytest.raises(v.VrtError):
+        plain.edit(np.zeros((1, 1, 1), np.uint32), (0, 0, 0))  # not editable
+    plain.release()

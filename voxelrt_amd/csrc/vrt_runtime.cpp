@@ -272,41 +272,38 @@ const char* vrt_last_error(const vrt_ctx* ctx) {
 // ---------------------------------------------------------------------------
 // scenes
 // ---------------------------------------------------------------------------
-int vrt_scene_upload(vrt_ctx* ctx, const uint32_t* vox, uint32_t nx, uint32_t ny, uint32_t nz,
-                     uint64_t scene_key, vrt_scene** out) {
-  if (!ctx || !vox || !out) return ctx ? ctx->fail(VRT_ERR_INVALID, "null argument") : VRT_ERR_INVALID;
-  GUARD_BEGIN
-  std::lock_guard<std::mutex> lock_(ctx->mu);
-  *out = nullptr;
-  if (nx == 0 || ny == 0 || nz == 0 || nx % 8 || ny % 8 || nz % 8)
-    return ctx->fail(VRT_ERR_INVALID, "grid dims %ux%ux%u must be positive multiples of 8", nx, ny, nz);
-  if ((uint64_t)nx * ny * nz > 0xFFFFFFFEull)
-    return ctx->fail(VRT_ERR_INVALID, "grid has more than 2^32-2 voxels");
-  HIP_TRY(ctx, hipSetDevice(ctx->device));
+}  // extern "C"
+
+// Build (or rebuild) a scene's device structures from a dense packed-u32
+// grid: host memory (hvox, staged in slabs of whole brick layers) or device
+// memory (dvox, the editable scene's resident copy).  Pass 1 builds the
+// 512-bit masks and occupancy flags, a device exclusive scan assigns pool
+// slots in brick order, pass 2 packs masks + voxels into the pool, then the
+// bitmaps and their prefix counts are built.  On a rebuild the pool grows
+// when the occupied-brick count needs it; the other buffers depend on the
+// dimensions only.  Ends with the context's stream synchronised.
+static int scene_build(vrt_ctx* ctx, vrt_scene* sc, const uint32_t* hvox, const uint32_t* dvox) {
+  const uint32_t nx = sc->nx, ny = sc->ny, nz = sc->nz;
   const int bgx = (int)nx / 8, bgy = (int)ny / 8, bgz = (int)nz / 8;
   const uint64_t nb = (uint64_t)bgx * bgy * bgz;
   const uint64_t layer_bytes = (uint64_t)nx * ny * 8 * 4;  // one brick layer of voxels
   const uint64_t stage_cap = 512ull << 20;
-  const int slab = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)bgz, stage_cap / layer_bytes));
+  const int slab = dvox ? bgz : (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)bgz, stage_cap / layer_bytes));
   const bool single = slab >= bgz;
-
-  std::unique_ptr<vrt_scene> sc(new vrt_scene());
-  sc->nx = nx; sc->ny = ny; sc->nz = nz;
-  sc->key = scene_key;
-  sc->nbricks = nb;
   vrt::DeviceBuffer stage, masks, flags, slots, tmp;
-  HIP_TRY(ctx, stage.alloc(layer_bytes * slab));
+  if (!dvox) HIP_TRY(ctx, stage.alloc(layer_bytes * slab));
+  const uint32_t* src = dvox ? dvox : (const uint32_t*)stage.ptr;
   HIP_TRY(ctx, masks.alloc(nb * 64));
   HIP_TRY(ctx, flags.alloc(nb * 4));
   HIP_TRY(ctx, slots.alloc(nb * 4));
   // pass 1: masks + occupancy flags
   for (int bz0 = 0; bz0 < bgz; bz0 += slab) {
     const int nbz = std::min(slab, bgz - bz0);
-    HIP_TRY(ctx, hipMemcpyAsync(stage.ptr, vox + (uint64_t)bz0 * 8 * nx * ny, layer_bytes * nbz,
-                                hipMemcpyHostToDevice, ctx->stream));
-    HIP_TRY(ctx, vrt::launch_brick_masks(ctx->stream, (const uint32_t*)stage.ptr, (int)nx, (int)ny,
-                                         bgx, bgy, bz0, nbz, (uint32_t*)masks.ptr,
-                                         (uint32_t*)flags.ptr));
+    if (!dvox)
+      HIP_TRY(ctx, hipMemcpyAsync(stage.ptr, hvox + (uint64_t)bz0 * 8 * nx * ny, layer_bytes * nbz,
+                                  hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, vrt::launch_brick_masks(ctx->stream, src + (dvox ? (uint64_t)bz0 * 8 * nx * ny : 0), (int)nx,
+                                         (int)ny, bgx, bgy, bz0, nbz, (uint32_t*)masks.ptr, (uint32_t*)flags.ptr));
   }
   // slots = exclusive prefix of flags
   size_t tmp_bytes = 0;
@@ -331,37 +328,39 @@ int vrt_scene_upload(vrt_ctx* ctx, const uint32_t* vox, uint32_t nx, uint32_t ny
   sc->psh2 = sc->psh1 + lg((uint32_t)bgy);
   sc->psh3 = sc->psh2 + lg((uint32_t)bgz);
   const uint64_t pbits_n = 1ull << sc->psh3;
-  const uint64_t bytes = nb * 4 + std::max<uint64_t>(nocc, 1) * vrt::POOL_WORDS * 4 + (nb + 63) / 64 * 8 +
-                         (nb + 31) / 32 * 4 + (pbits_n + 63) / 64 * 8 + (pbits_n + 31) / 32 * 4;
-  int rc = ctx->reserve(bytes);
-  if (rc) return rc;
-  sc->bytes = bytes;
-  hipError_t e = sc->d_bricks.alloc(nb * 4);
-  if (e == hipSuccess) e = sc->d_pool.alloc(std::max<uint64_t>(nocc, 1) * vrt::POOL_WORDS * 4);
-  if (e == hipSuccess) e = sc->d_bits.alloc((nb + 63) / 64 * 8);
-  if (e == hipSuccess) e = sc->d_prefix.alloc((nb + 31) / 32 * 4);
-  if (e == hipSuccess) e = sc->d_pbits.alloc(((pbits_n + 63) / 64) * 8);
-  if (e == hipSuccess) e = sc->d_pprefix.alloc((pbits_n + 31) / 32 * 4);
-  if (e != hipSuccess) {
-    ctx->unreserve(bytes);
-    return ctx->fail(e == hipErrorOutOfMemory ? VRT_ERR_NO_MEMORY : VRT_ERR_DEVICE,
-                     "scene allocation failed: %s", hipGetErrorString(e));
+  const uint64_t pool_need = std::max<uint64_t>(nocc, 1) * vrt::POOL_WORDS * 4;
+  // editable scenes keep 25 % headroom in the pool so that edits rarely reallocate
+  const uint64_t pool_bytes = sc->d_pool.bytes >= pool_need ? sc->d_pool.bytes
+                              : (sc->d_dense.ptr ? pool_need + pool_need / 4 : pool_need);
+  const uint64_t bytes = sc->d_dense.bytes + nb * 4 + pool_bytes + (nb + 63) / 64 * 8 + (nb + 31) / 32 * 4 +
+                         (pbits_n + 63) / 64 * 8 + (pbits_n + 31) / 32 * 4;
+  if (bytes > sc->bytes) {
+    int rc = ctx->reserve(bytes - sc->bytes);
+    if (rc) return rc;
+    sc->bytes = bytes;
   }
-  auto undo = [&](int code, const char* what, hipError_t err) {
-    ctx->unreserve(bytes);
-    return ctx->fail(code, "%s: %s", what, hipGetErrorString(err));
-  };
+  hipError_t e = hipSuccess;
+  if (!sc->d_bricks.ptr) e = sc->d_bricks.alloc(nb * 4);
+  if (e == hipSuccess && sc->d_pool.bytes < pool_need) e = sc->d_pool.alloc(pool_bytes);
+  if (e == hipSuccess && !sc->d_bits.ptr) e = sc->d_bits.alloc((nb + 63) / 64 * 8);
+  if (e == hipSuccess && !sc->d_prefix.ptr) e = sc->d_prefix.alloc((nb + 31) / 32 * 4);
+  if (e == hipSuccess && !sc->d_pbits.ptr) e = sc->d_pbits.alloc(((pbits_n + 63) / 64) * 8);
+  if (e == hipSuccess && !sc->d_pprefix.ptr) e = sc->d_pprefix.alloc((pbits_n + 31) / 32 * 4);
+  if (e != hipSuccess)
+    return ctx->fail(e == hipErrorOutOfMemory ? VRT_ERR_NO_MEMORY : VRT_ERR_DEVICE, "scene allocation failed: %s",
+                     hipGetErrorString(e));
+  auto err = [&](const char* what, hipError_t er) { return ctx->fail(VRT_ERR_DEVICE, "%s: %s", what, hipGetErrorString(er)); };
   e = vrt::launch_brick_words(ctx->stream, (const uint32_t*)flags.ptr, (const uint32_t*)slots.ptr, nb,
                               (uint32_t*)sc->d_bricks.ptr);
-  if (e != hipSuccess) return undo(VRT_ERR_DEVICE, "brick words", e);
+  if (e != hipSuccess) return err("brick words", e);
   e = vrt::launch_brick_bits(ctx->stream, (const uint32_t*)sc->d_bricks.ptr, nb, (uint32_t*)sc->d_bits.ptr);
-  if (e != hipSuccess) return undo(VRT_ERR_DEVICE, "brick bits", e);
+  if (e != hipSuccess) return err("brick bits", e);
   {
     // prefix[w] = exclusive scan of popcount(bits[w]): the pool slot of the
     // first occupied brick at or after word w (slots follow brick order)
     const uint64_t nw = (nb + 31) / 32;
     e = vrt::launch_popc(ctx->stream, (const uint32_t*)sc->d_bits.ptr, nw, (uint32_t*)flags.ptr);
-    if (e != hipSuccess) return undo(VRT_ERR_DEVICE, "bit counts", e);
+    if (e != hipSuccess) return err("bit counts", e);
     size_t need = 0;
     e = vrt::exclusive_scan_u32(ctx->stream, nullptr, &need, (const uint32_t*)flags.ptr,
                                 (uint32_t*)sc->d_prefix.ptr, nw);
@@ -369,13 +368,13 @@ int vrt_scene_upload(vrt_ctx* ctx, const uint32_t* vox, uint32_t nx, uint32_t ny
     if (e == hipSuccess)
       e = vrt::exclusive_scan_u32(ctx->stream, tmp.ptr, &need, (const uint32_t*)flags.ptr,
                                   (uint32_t*)sc->d_prefix.ptr, nw);
-    if (e != hipSuccess) return undo(VRT_ERR_DEVICE, "bit prefix", e);
+    if (e != hipSuccess) return err("bit prefix", e);
     // the padded layout
     const uint64_t pn = 1ull << sc->psh3, pw = (pn + 31) / 32;
     e = vrt::launch_brick_pbits(ctx->stream, (const uint32_t*)sc->d_bricks.ptr, bgx, bgy, bgz, sc->psh1, sc->psh2,
                                 pn, (uint32_t*)sc->d_pbits.ptr);
     if (e == hipSuccess) e = vrt::launch_popc(ctx->stream, (const uint32_t*)sc->d_pbits.ptr, pw, (uint32_t*)flags.ptr);
-    if (e != hipSuccess) return undo(VRT_ERR_DEVICE, "padded bits", e);
+    if (e != hipSuccess) return err("padded bits", e);
     need = 0;
     e = vrt::exclusive_scan_u32(ctx->stream, nullptr, &need, (const uint32_t*)flags.ptr,
                                 (uint32_t*)sc->d_pprefix.ptr, pw);
@@ -383,28 +382,101 @@ int vrt_scene_upload(vrt_ctx* ctx, const uint32_t* vox, uint32_t nx, uint32_t ny
     if (e == hipSuccess)
       e = vrt::exclusive_scan_u32(ctx->stream, tmp.ptr, &need, (const uint32_t*)flags.ptr,
                                   (uint32_t*)sc->d_pprefix.ptr, pw);
-    if (e != hipSuccess) return undo(VRT_ERR_DEVICE, "padded prefix", e);
+    if (e != hipSuccess) return err("padded prefix", e);
   }
   // pass 2: pack occupied bricks into the pool
   for (int bz0 = 0; bz0 < bgz; bz0 += slab) {
     const int nbz = std::min(slab, bgz - bz0);
-    if (!single) {
-      e = hipMemcpyAsync(stage.ptr, vox + (uint64_t)bz0 * 8 * nx * ny, layer_bytes * nbz,
-                         hipMemcpyHostToDevice, ctx->stream);
-      if (e != hipSuccess) return undo(VRT_ERR_DEVICE, "upload", e);
+    if (!dvox && !single) {
+      e = hipMemcpyAsync(stage.ptr, hvox + (uint64_t)bz0 * 8 * nx * ny, layer_bytes * nbz, hipMemcpyHostToDevice,
+                         ctx->stream);
+      if (e != hipSuccess) return err("upload", e);
     }
-    e = vrt::launch_pack_pool(ctx->stream, (const uint32_t*)stage.ptr, (int)nx, (int)ny, bgx, bgy, bz0,
-                              nbz, (const uint32_t*)masks.ptr, (const uint32_t*)sc->d_bricks.ptr,
+    e = vrt::launch_pack_pool(ctx->stream, src + (dvox ? (uint64_t)bz0 * 8 * nx * ny : 0), (int)nx, (int)ny, bgx,
+                              bgy, bz0, nbz, (const uint32_t*)masks.ptr, (const uint32_t*)sc->d_bricks.ptr,
                               (uint32_t*)sc->d_pool.ptr);
-    if (e != hipSuccess) return undo(VRT_ERR_DEVICE, "pack", e);
+    if (e != hipSuccess) return err("pack", e);
   }
   e = hipStreamSynchronize(ctx->stream);
-  if (e != hipSuccess) return undo(VRT_ERR_DEVICE, "scene build", e);
+  if (e != hipSuccess) return err("scene build", e);
+  return VRT_OK;
+}
+
+extern "C" {
+
+int vrt_scene_upload_ex(vrt_ctx* ctx, const uint32_t* vox, uint32_t nx, uint32_t ny, uint32_t nz,
+                        uint64_t scene_key, uint32_t flags, vrt_scene** out) {
+  if (!ctx || !vox || !out) return ctx ? ctx->fail(VRT_ERR_INVALID, "null argument") : VRT_ERR_INVALID;
+  GUARD_BEGIN
+  std::lock_guard<std::mutex> lock_(ctx->mu);
+  *out = nullptr;
+  if (flags & ~VRT_SCENE_EDITABLE) return ctx->fail(VRT_ERR_INVALID, "unknown scene flags 0x%x", flags);
+  if (nx == 0 || ny == 0 || nz == 0 || nx % 8 || ny % 8 || nz % 8)
+    return ctx->fail(VRT_ERR_INVALID, "grid dims %ux%ux%u must be positive multiples of 8", nx, ny, nz);
+  if ((uint64_t)nx * ny * nz > 0xFFFFFFFEull)
+    return ctx->fail(VRT_ERR_INVALID, "grid has more than 2^32-2 voxels");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  std::unique_ptr<vrt_scene> sc(new vrt_scene());
+  sc->nx = nx;
+  sc->ny = ny;
+  sc->nz = nz;
+  sc->key = scene_key;
+  sc->nbricks = (uint64_t)(nx / 8) * (ny / 8) * (nz / 8);
+  const uint64_t dense = (uint64_t)nx * ny * nz * 4;
+  if (flags & VRT_SCENE_EDITABLE) {
+    int rc = ctx->reserve(dense);
+    if (rc) return rc;
+    sc->bytes = dense;
+    hipError_t e = sc->d_dense.alloc(dense);
+    if (e == hipSuccess) e = hipMemcpyAsync(sc->d_dense.ptr, vox, dense, hipMemcpyHostToDevice, ctx->stream);
+    if (e != hipSuccess) {
+      ctx->unreserve(dense);
+      return ctx->fail(e == hipErrorOutOfMemory ? VRT_ERR_NO_MEMORY : VRT_ERR_DEVICE, "dense grid: %s",
+                       hipGetErrorString(e));
+    }
+  }
+  const int rc = scene_build(ctx, sc.get(), vox, (const uint32_t*)sc->d_dense.ptr);
+  if (rc) {
+    ctx->unreserve(sc->bytes);
+    return rc;
+  }
   sc->lru = ++ctx->clock;
   vrt_scene* raw = sc.release();
   ctx->scenes.push_back(raw);
   *out = raw;
   return VRT_OK;
+  GUARD_END(ctx)
+}
+
+int vrt_scene_upload(vrt_ctx* ctx, const uint32_t* vox, uint32_t nx, uint32_t ny, uint32_t nz,
+                     uint64_t scene_key, vrt_scene** out) {
+  return vrt_scene_upload_ex(ctx, vox, nx, ny, nz, scene_key, 0u, out);
+}
+
+int vrt_scene_edit(vrt_ctx* ctx, vrt_scene* scene, const uint32_t* vox, uint32_t x0, uint32_t y0, uint32_t z0,
+                   uint32_t sx, uint32_t sy, uint32_t sz) {
+  if (!ctx || !scene || !vox) return ctx ? ctx->fail(VRT_ERR_INVALID, "null argument") : VRT_ERR_INVALID;
+  GUARD_BEGIN
+  std::lock_guard<std::mutex> lock_(ctx->mu);
+  if (std::find(ctx->scenes.begin(), ctx->scenes.end(), scene) == ctx->scenes.end())
+    return ctx->fail(VRT_ERR_INVALID, "unknown scene handle");
+  if (!scene->d_dense.ptr) return ctx->fail(VRT_ERR_INVALID, "scene was not uploaded with VRT_SCENE_EDITABLE");
+  if (sx == 0 || sy == 0 || sz == 0) return VRT_OK;
+  if ((uint64_t)x0 + sx > scene->nx || (uint64_t)y0 + sy > scene->ny || (uint64_t)z0 + sz > scene->nz)
+    return ctx->fail(VRT_ERR_INVALID, "edit box %u+%u, %u+%u, %u+%u outside the %ux%ux%u grid", x0, sx, y0, sy, z0,
+                     sz, scene->nx, scene->ny, scene->nz);
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  {
+    // running serve batches may read this scene: the rebuild comes after them
+    int rc = ctx->order_after_serve();
+    if (rc) return rc;
+  }
+  uint32_t* dense = (uint32_t*)scene->d_dense.ptr;
+  for (uint32_t z = 0; z < sz; ++z)
+    HIP_TRY(ctx, hipMemcpy2DAsync(dense + ((uint64_t)(z0 + z) * scene->ny + y0) * scene->nx + x0,
+                                  (size_t)scene->nx * 4, vox + (uint64_t)z * sx * sy, (size_t)sx * 4,
+                                  (size_t)sx * 4, sy, hipMemcpyHostToDevice, ctx->stream));
+  return scene_build(ctx, scene, nullptr, dense);
   GUARD_END(ctx)
 }
 

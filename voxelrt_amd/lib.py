@@ -22,6 +22,7 @@ ACCUM_RGBA32F = 1
 SCENE_RANDOM_BRICKS = 1
 SCENE_WORLD = 2
 SCENE_VOXELRT = 3
+SCENE_EDITABLE = 1
 
 ERRORS = {-1: "VRT_ERR_INVALID", -2: "VRT_ERR_DEVICE", -3: "VRT_ERR_NO_MEMORY",
           -4: "VRT_ERR_BUDGET", -5: "VRT_ERR_NOT_READY", -6: "VRT_ERR_CANCELLED",
@@ -154,6 +155,12 @@ def _declare(L):
     L.vrt_scene_upload.argtypes = [P, P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64,
                                    C.POINTER(P)]
     L.vrt_scene_release.argtypes = [P, P]
+    L.vrt_scene_upload_ex.argtypes = [P, P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint32,
+                                      C.POINTER(P)]
+    L.vrt_scene_edit.argtypes = [P, P, P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                 C.c_uint32]
+    L.vrt_scene_upload_ex.restype = I
+    L.vrt_scene_edit.restype = I
     L.vrt_scene_bricks.argtypes = [P, P, P, P]
     L.vrt_scene_dims.argtypes = [P, P, C.POINTER(C.c_uint64)]
     L.vrt_trace_rays.argtypes = [P, P, P, C.c_uint64, C.c_float, P]
@@ -288,6 +295,14 @@ class Scene:
         self.ctx._check(self.ctx.L.vrt_scene_dims(self.handle, _ptr(dims), C.byref(b)))
         return b.value
 
+    def edit(self, box: np.ndarray, origin):
+        """Write box (sz, sy, sx) uint32 at voxel origin (x0, y0, z0) of an
+        editable scene and rebuild it on the device."""
+        box = np.ascontiguousarray(box, np.uint32)
+        sz, sy, sx = box.shape
+        x0, y0, z0 = (int(c) for c in origin)
+        self.ctx._check(self.ctx.L.vrt_scene_edit(self.ctx.h, self.handle, _ptr(box), x0, y0, z0, sx, sy, sz))
+
     def release(self):
         if self.handle:
             self.ctx._check(self.ctx.L.vrt_scene_release(self.ctx.h, self.handle))
@@ -326,11 +341,12 @@ class Context:
         if rc != 0:
             raise VrtError(rc, self.L.vrt_last_error(self.h).decode())
 
-    def upload(self, vox: np.ndarray, key: int = 0) -> Scene:
+    def upload(self, vox: np.ndarray, key: int = 0, editable: bool = False) -> Scene:
         vox = np.ascontiguousarray(vox, np.uint32)
         nz, ny, nx = vox.shape
         out = C.c_void_p()
-        self._check(self.L.vrt_scene_upload(self.h, _ptr(vox), nx, ny, nz, key, C.byref(out)))
+        self._check(self.L.vrt_scene_upload_ex(self.h, _ptr(vox), nx, ny, nz, key,
+                                               SCENE_EDITABLE if editable else 0, C.byref(out)))
         return Scene(self, out.value, (nx, ny, nz))
 
     def trace_rays(self, scene: Scene, rays: np.ndarray, tmax: float = 1e4) -> np.ndarray:
