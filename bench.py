@@ -91,13 +91,14 @@ def traverse(args):
     torch.cuda.synchronize()
     variants = {"lane": dict(trace_kernel=0), "lane_flat": dict(trace_kernel=2), "lane_lds": dict(trace_kernel=3),
                 "lane_fast": dict(trace_kernel=4),
+                "persistent_fast": dict(trace_kernel=5, trace_blocks=2048, trace_steps=4, trace_min_refill=1),
                 "persistent": dict(trace_kernel=1, trace_blocks=2048, trace_steps=8, trace_min_refill=16)}
     for p in (0.05, 0.25, 0.6):
         vox = cf.traverse_scene(p)
         sc = ctx.upload(vox)
         if not args.no_cpu:
             print(json.dumps(_traverse_cpu(vox, rays, p)), flush=True)
-        for name in ("lane", "lane_lds", "lane_fast", "persistent", "autotuned"):
+        for name in ("lane", "lane_fast", "persistent", "persistent_fast", "autotuned"):
             if name == "autotuned":
                 tuned = ctx.autotune(sc, 1 << 22).as_dict()
             else:
@@ -392,7 +393,7 @@ def main():
     ap.add_argument("--serve-views", type=int, default=64)
     ap.add_argument("--serve-pixels", type=int, default=16 << 20)
     ap.add_argument("--batch-mode", type=int, default=2, choices=[0, 1, 2])
-    ap.add_argument("--pt-kernel", default="auto", choices=["auto", "0", "1", "2", "3", "4"])
+    ap.add_argument("--pt-kernel", default="auto", choices=["auto", "0", "1", "2", "3", "4", "5"])
     ap.add_argument("--pt-waves", type=int, default=0, choices=[0, 4, 5])
     args = ap.parse_args()
     if args.gpus != 1:

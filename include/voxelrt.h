@@ -235,14 +235,17 @@ typedef struct vrt_tuning {
   uint32_t trace_kernel;      /* 0 = one ray per lane, 1 = persistent ray queue,
                                  2 = one ray per lane, single-stream DDA,
                                  3 = one ray per lane, occupancy staged in LDS,
-                                 4 = one ray per lane, single loop, both levels in LDS */
+                                 4 = one ray per lane, single loop, both levels in LDS,
+                                 5 = persistent ray queue, single loop in LDS */
   uint32_t trace_blocks;      /* persistent grid (256-thread blocks), 0 = default */
   uint32_t trace_steps;       /* DDA iterations between queue refill checks */
   uint32_t trace_min_refill;  /* idle lanes (of 64) that trigger a refill */
   uint32_t pt_kernel;         /* path tracer: 0 = one pixel per lane, 1 = persistent,
                                  2 = one pixel per lane, single-stream DDA,
                                  3 = one pixel per lane, occupancy staged in LDS,
-                                 4 = one pixel per lane, single loop, both levels in LDS */
+                                 4 = one pixel per lane, single loop, both levels in LDS,
+                                 5 = as 4 with one trace call site per lane (shadow and
+                                     bounce rays of different lanes share the DDA loop) */
   uint32_t pt_blocks;
   uint32_t pt_steps;
   uint32_t pt_min_refill;

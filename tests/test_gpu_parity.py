@@ -77,11 +77,14 @@ def traverse(request, ctx):
 TRACE_VARIANTS = [dict(trace_kernel=0), dict(trace_kernel=2), dict(trace_kernel=3), dict(trace_kernel=4),
                   dict(trace_kernel=1, trace_blocks=2048, trace_steps=8, trace_min_refill=16),
                   dict(trace_kernel=1, trace_blocks=512, trace_steps=1, trace_min_refill=1),
-                  dict(trace_kernel=1, trace_blocks=4096, trace_steps=16, trace_min_refill=64)]
+                  dict(trace_kernel=1, trace_blocks=4096, trace_steps=16, trace_min_refill=64),
+                  dict(trace_kernel=5, trace_blocks=2048, trace_steps=4, trace_min_refill=1),
+                  dict(trace_kernel=5, trace_blocks=333, trace_steps=1, trace_min_refill=64)]
 
 
 @pytest.fixture(params=range(len(TRACE_VARIANTS)),
-                ids=["lane", "lane-flat", "lane-lds", "lane-fast", "persist", "persist-s1", "persist-s16"])
+                ids=["lane", "lane-flat", "lane-lds", "lane-fast", "persist", "persist-s1", "persist-s16",
+                     "persist-fast", "persist-fast-333"])
 def trace_variant(request, ctx):
     old = ctx.tuning().as_dict()
     ctx.set_tuning(**TRACE_VARIANTS[request.param])
@@ -169,7 +172,7 @@ def test_autotune_keeps_results(ctx, traverse):
         pytest.skip("one occupancy is enough")
     old = ctx.tuning().as_dict()
     t = ctx.autotune(sc, 1 << 20)
-    assert t.trace_kernel in (0, 1, 2, 3, 4)
+    assert t.trace_kernel in (0, 1, 2, 3, 4, 5)
     rays = cf.traverse_rays(1 << 18, seed=5)
     _hits_equal(ctx.trace_rays(sc, rays), oracle.trace_rays(og, rays), f"autotuned {t.as_dict()}")
     ctx.set_tuning(**old)
@@ -188,7 +191,7 @@ def frame128(ctx):
 
 PT_VARIANTS = [dict(pt_kernel=0), dict(pt_kernel=0, pt_waves=5), dict(pt_kernel=2, pt_waves=4),
                dict(pt_kernel=3), dict(pt_kernel=3, pt_waves=4), dict(pt_kernel=3, pt_waves=5),
-               dict(pt_kernel=4), dict(pt_kernel=4, pt_waves=5),
+               dict(pt_kernel=4), dict(pt_kernel=4, pt_waves=5), dict(pt_kernel=5), dict(pt_kernel=5, pt_waves=5),
                dict(pt_kernel=1, pt_blocks=1024, pt_steps=8, pt_min_refill=32, pt_chunk=64),
                dict(pt_kernel=1, pt_blocks=97, pt_steps=1, pt_min_refill=1, pt_chunk=64),
                dict(pt_kernel=1, pt_blocks=2048, pt_steps=16, pt_min_refill=64, pt_chunk=256)]
@@ -196,7 +199,7 @@ PT_VARIANTS = [dict(pt_kernel=0), dict(pt_kernel=0, pt_waves=5), dict(pt_kernel=
 
 @pytest.fixture(params=range(len(PT_VARIANTS)),
                 ids=["lane", "lane-w5", "lane-flat-w4", "lane-lds", "lane-lds-w4", "lane-lds-w5", "lane-fast",
-                     "lane-fast-w5", "persist",
+                     "lane-fast-w5", "lane-fast1", "lane-fast1-w5", "persist",
                      "persist-s1", "persist-s16"])
 def pt_variant(request, ctx):
     old = ctx.tuning().as_dict()
@@ -346,9 +349,10 @@ def test_frame_sequence_bitexact(ctx, pt_variant):
                                     dict(pt_kernel=4, pt_waves=5, batch_mode=1),
                                     dict(pt_kernel=0, pt_waves=0, batch_mode=1, batch_blocks=7),
                                     dict(pt_kernel=4, pt_waves=5, batch_mode=2),
+                                    dict(pt_kernel=5, pt_waves=4, batch_mode=1),
                                     dict(pt_kernel=3, pt_waves=4, batch_mode=2)],
                          ids=["lane", "lds-w5", "flat-w4", "fast-w5", "queue-fast", "queue-7blocks",
-                              "graph-fast", "graph-lds"])
+                              "graph-fast", "queue-fast1-w4", "graph-lds"])
 def test_batched_views_match_single_views(ctx, tuning):
     descs = [cf.world_desc(64, 1), cf.world_desc(128, 2), dict(cf.world_desc(64, 3), ny=32)]
     scenes = [ctx.upload(v.generate_scene(**d)) for d in descs]

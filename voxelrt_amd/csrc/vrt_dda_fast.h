@@ -53,95 +53,129 @@ VRT_D uint32_t pack_left(const int p[3], const int n[3], const int sd[3]) {
   return cnt;
 }
 
-VRT_D void trace_fast(const SceneDev& s, const LdsScene& L, f3 roW, f3 rd, float maxTWorld, TraceResult& o) {
-  o.t = -1.0f;
-  o.mat = 0u;
-  o.n = mk3(-rd.x, -rd.y, -rd.z);
-  o.voxel = 0xFFFFFFFFu;
-  o.face = 255u;
-  o.bsteps = 0u;
-  o.vsteps = 0u;
+// Resumable state of one ray (all in registers once inlined).  begin() sets
+// the ray up; step() runs one iteration and returns DDA_RUN, DDA_MISS or
+// DDA_HIT; finish() writes the result record.  trace_fast() runs them in a
+// loop; the persistent traversal kernel interleaves rays per lane.
+enum : uint32_t { DDA_RUN = 0u, DDA_MISS = 1u, DDA_HIT = 2u };
 
-  const float ro[3] = {roW.x * s.inv_voxel, roW.y * s.inv_voxel, roW.z * s.inv_voxel};
-  const float maxT = maxTWorld * s.inv_voxel;
-  const float rdv[3] = {rd.x, rd.y, rd.z};
-  float inv[3], ge[3], adl[3];
+// Bitwise selects (c ? a : b, bit for bit).  Written as masks so that the
+// compiler cannot turn a choice between two state-array elements into a load
+// through a chosen address, which would keep the DDA state in scratch memory.
+VRT_D float vsel(bool c, float a, float b) {
+  const uint32_t m = 0u - (uint32_t)c;
+  return __builtin_bit_cast(float, (m & __builtin_bit_cast(uint32_t, a)) | (~m & __builtin_bit_cast(uint32_t, b)));
+}
+VRT_D int isel(bool c, int a, int b) {
+  const uint32_t m = 0u - (uint32_t)c;
+  return (int)((m & (uint32_t)a) | (~m & (uint32_t)b));
+}
+
+struct DdaFast {
+  float ro[3], rdv[3], inv[3];
   int sd[3];
-  float tmin[3], tmax[3];
-  const int G[3] = {s.nx, s.ny, s.nz};
-  const int BG[3] = {s.bgx, s.bgy, s.bgz};
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    inv[c] = safe_inv(rdv[c]);
-    adl[c] = vabs(inv[c]);
-    sd[c] = sign_i(rdv[c]);
-    ge[c] = rdv[c] >= 0.0f ? 1.0f : 0.0f;
-    const float t0 = (s.zero - ro[c]) * inv[c];
-    const float t1 = ((float)G[c] - ro[c]) * inv[c];
-    tmin[c] = vmin(t0, t1);
-    tmax[c] = vmax(t0, t1);
-  }
-  const float tEnter = vmax(vmax(tmin[0], tmin[1]), vmax(tmin[2], 0.0f));
-  const float tExit = vmin(vmin(tmax[0], tmax[1]), vmin(tmax[2], maxT));
-  if (tEnter > tExit) return;
-
-  uint32_t ax = (tmin[0] >= tmin[1] && tmin[0] >= tmin[2]) ? 0u : (tmin[1] >= tmin[2] ? 1u : 2u);
-  float t = tEnter;
-  const uint32_t sh1 = s.psh1, sh2 = s.psh2;
-  const uint32_t mx = (1u << sh1) - 1u, my = (1u << (sh2 - sh1)) - 1u;
-  const uint32_t dec[3] = {sd[0] != 0 ? 1u : 0u, sd[1] != 0 ? 256u : 0u, sd[2] != 0 ? 65536u : 0u};
-
-  // current level (starts coarse)
+  float tExit, maxT;
+  // current level
   float tm[3], d[3];
   int di[3];
-  uint32_t idx, cnt;
-  {
+  uint32_t idx, cnt, iter;
+  uint32_t base, wshift, inc;  // LDS word base / stride shift, step-counter increment
+  float tBound, t;             // coarse: leave at t > tExit; fine: never
+  uint32_t ax;
+  // coarse state saved while walking a brick
+  float tmB[3];
+  uint32_t idxB, cntB, iterB, blk;
+  uint32_t steps;  // brick steps | voxel steps << 16
+
+  // voxel.wgsl:79-104.  Returns true when the ray misses the grid at setup
+  // (o is then the miss record).
+  VRT_D bool begin(const SceneDev& s, const LdsScene& L, f3 roW, f3 rd, float maxTWorld, TraceResult& o) {
+    o.t = -1.0f;
+    o.mat = 0u;
+    o.n = mk3(-rd.x, -rd.y, -rd.z);
+    o.voxel = 0xFFFFFFFFu;
+    o.face = 255u;
+    o.bsteps = 0u;
+    o.vsteps = 0u;
+    ro[0] = roW.x * s.inv_voxel;
+    ro[1] = roW.y * s.inv_voxel;
+    ro[2] = roW.z * s.inv_voxel;
+    maxT = maxTWorld * s.inv_voxel;
+    rdv[0] = rd.x;
+    rdv[1] = rd.y;
+    rdv[2] = rd.z;
+    float tmin[3], tmax[3];
+    const int G[3] = {s.nx, s.ny, s.nz};
+    const int BG[3] = {s.bgx, s.bgy, s.bgz};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      inv[c] = safe_inv(rdv[c]);
+      sd[c] = sign_i(rdv[c]);
+      const float t0 = (s.zero - ro[c]) * inv[c];
+      const float t1 = ((float)G[c] - ro[c]) * inv[c];
+      tmin[c] = vmin(t0, t1);
+      tmax[c] = vmax(t0, t1);
+    }
+    const float tEnter = vmax(vmax(tmin[0], tmin[1]), vmax(tmin[2], 0.0f));
+    tExit = vmin(vmin(tmax[0], tmax[1]), vmin(tmax[2], maxT));
+    if (tEnter > tExit) return true;
+    ax = (tmin[0] >= tmin[1] && tmin[0] >= tmin[2]) ? 0u : (tmin[1] >= tmin[2] ? 1u : 2u);
+    t = tEnter;
     const float ts = t + 1e-4f;
     int bp[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const float sp = ro[c] + rdv[c] * ts;
       bp[c] = floor_clamp(sp / 8.0f, 0, BG[c] - 1);
-      d[c] = adl[c] * 8.0f;
-      tm[c] = (((float)bp[c] + ge[c]) * 8.0f - ro[c]) * inv[c];
+      d[c] = vabs(inv[c]) * 8.0f;
+      const float ge = rdv[c] >= 0.0f ? 1.0f : 0.0f;
+      tm[c] = (((float)bp[c] + ge) * 8.0f - ro[c]) * inv[c];
     }
     di[0] = sd[0];
-    di[1] = sd[1] * (int)(1u << sh1);
-    di[2] = sd[2] * (int)(1u << sh2);
-    idx = (uint32_t)bp[0] | ((uint32_t)bp[1] << sh1) | ((uint32_t)bp[2] << sh2);
+    di[1] = sd[1] * (int)(1u << s.psh1);
+    di[2] = sd[2] * (int)(1u << s.psh2);
+    idx = (uint32_t)bp[0] | ((uint32_t)bp[1] << s.psh1) | ((uint32_t)bp[2] << s.psh2);
     cnt = pack_left(bp, BG, sd);
+    iter = (uint32_t)(s.bgx + s.bgy + s.bgz + 2);
+    base = L.bits_off;
+    wshift = 0u;
+    inc = 1u;
+    tBound = tExit;
+    tmB[0] = tmB[1] = tmB[2] = 0.0f;
+    idxB = cntB = iterB = blk = 0u;
+    steps = 0u;
+    return false;
   }
-  uint32_t iter = (uint32_t)(s.bgx + s.bgy + s.bgz + 2);
-  uint32_t base = L.bits_off, wshift = 0u, inc = 1u;  // coarse: bitmap word idx>>5, brick steps
-  float tBound = tExit;                                // coarse: leave at t > tExit; fine: never
-  // coarse state saved while walking a brick
-  float tmB[3] = {0.0f, 0.0f, 0.0f};
-  uint32_t idxB = 0u, cntB = 0u, iterB = 0u, blk = 0u;
-  uint32_t steps = 0u;  // brick steps | voxel steps << 16
 
-  for (;;) {
+  // Advance the current level by one cell (ties x, then y, then z); true
+  // when that leaves the level's box or exhausts its iteration budget.
+  VRT_D bool advance() {
+    const float m0 = tm[0], m1 = tm[1], m2 = tm[2];
+    const int i0 = di[0], i1 = di[1], i2 = di[2];
+    const bool c0 = m0 <= m1 && m0 <= m2;
+    const bool c1 = !c0 && m1 <= m2;
+    t = vsel(c0, m0, vsel(c1, m1, m2));
+    ax = c0 ? 0u : (c1 ? 1u : 2u);
+    const float n0 = m0 + d[0], n1 = m1 + d[1], n2 = m2 + d[2];
+    tm[0] = vsel(c0, n0, m0);
+    tm[1] = vsel(c1, n1, m1);
+    tm[2] = vsel(!c0 && !c1, n2, m2);
+    const int dsel = isel(c0, i0, isel(c1, i1, i2));  // 0 exactly when that axis does not move
+    idx += (uint32_t)dsel;
+    const uint32_t sh = c0 ? 0u : (c1 ? 8u : 16u);
+    const bool out = ((cnt >> sh) & 255u) == 0u || iter == 1u;
+    cnt -= (dsel != 0 ? 1u : 0u) << sh;
+    iter -= 1u;
+    return out;
+  }
+
+  VRT_D uint32_t step(const SceneDev& s, const LdsScene& L) {
     // ---- test the current cell ----
-    if (t > tBound) break;
+    if (t > tBound) return DDA_MISS;
     const uint32_t word = L.lds[base + ((idx >> 5) << wshift)];
     steps += inc;
     if ((word >> (idx & 31u)) & 1u) {
-      if (inc != 1u) {  // fine level: hit (voxel.wgsl:122-128)
-        o.bsteps = steps & 0xffffu;
-        o.vsteps = steps >> 16;
-        if (t > maxT) return;
-        o.t = t * s.voxel_size;
-        float nn[3];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) nn[c] = ((uint32_t)c == ax ? -1.0f : -0.0f) * (float)sd[c];
-        o.n = mk3(nn[0], nn[1], nn[2]);
-        o.mat = s.pool[(uint64_t)blk + 16u + idx];
-        const uint32_t bx = idxB & mx, by = (idxB >> sh1) & my, bz = idxB >> sh2;
-        const uint32_t vx = bx * 8u + (idx & 7u), vy = by * 8u + ((idx >> 3) & 7u), vz = bz * 8u + (idx >> 6);
-        o.voxel = vx + vy * (uint32_t)s.nx + vz * (uint32_t)s.nx * (uint32_t)s.ny;
-        const float nv = ax == 0u ? nn[0] : (ax == 1u ? nn[1] : nn[2]);
-        o.face = nv > 0.0f ? 2u * ax : (nv < 0.0f ? 2u * ax + 1u : 6u);
-        return;
-      }
+      if (inc != 1u) return DDA_HIT;  // fine level (voxel.wgsl:122)
       // coarse level: enter the occupied brick (voxel.wgsl:111-118)
       const uint32_t w = idx >> 5;
       const uint32_t slot = L.lds[L.prefix_off + w] + (uint32_t)__builtin_popcount(word & ((1u << (idx & 31u)) - 1u));
@@ -155,7 +189,8 @@ VRT_D void trace_fast(const SceneDev& s, const LdsScene& L, f3 roW, f3 rd, float
         L.lds[((4 * k + 2) << 8) + L.lane] = v.z;
         L.lds[((4 * k + 3) << 8) + L.lane] = v.w;
       }
-      const uint32_t bx = idx & mx, by = (idx >> sh1) & my, bz = idx >> sh2;
+      const uint32_t mx = (1u << s.psh1) - 1u, my = (1u << (s.psh2 - s.psh1)) - 1u;
+      const uint32_t bx = idx & mx, by = (idx >> s.psh1) & my, bz = idx >> s.psh2;
       const int bmin[3] = {(int)bx * 8, (int)by * 8, (int)bz * 8};
       const float ts = t + 1e-4f;
       int lp[3];
@@ -164,9 +199,10 @@ VRT_D void trace_fast(const SceneDev& s, const LdsScene& L, f3 roW, f3 rd, float
         tmB[c] = tm[c];
         const float p = ro[c] + rdv[c] * ts;
         const int vp = floor_clamp(p, bmin[c], bmin[c] + 7);
-        tm[c] = (((float)vp + ge[c]) - ro[c]) * inv[c];
+        const float ge = rdv[c] >= 0.0f ? 1.0f : 0.0f;
+        tm[c] = (((float)vp + ge) - ro[c]) * inv[c];
         lp[c] = vp - bmin[c];
-        d[c] = adl[c];
+        d[c] = vabs(inv[c]);
       }
       di[0] = sd[0];
       di[1] = sd[1] * 8;
@@ -175,71 +211,69 @@ VRT_D void trace_fast(const SceneDev& s, const LdsScene& L, f3 roW, f3 rd, float
       cntB = cnt;
       iterB = iter;
       idx = (uint32_t)lp[0] + 8u * (uint32_t)lp[1] + 64u * (uint32_t)lp[2];
-      {
-        const int n8[3] = {8, 8, 8};
-        cnt = pack_left(lp, n8, sd);
-      }
+      const int n8[3] = {8, 8, 8};
+      cnt = pack_left(lp, n8, sd);
       iter = 24u;
       base = L.lane;
       wshift = 8u;
       inc = 0x10000u;
       tBound = __builtin_inff();
-      continue;  // test the entry voxel
+      return DDA_RUN;  // the entry voxel is tested next
     }
-    // ---- advance the current level by one cell (ties x, then y, then z) ----
-    bool out;
-    {
-      const bool c0 = tm[0] <= tm[1] && tm[0] <= tm[2];
-      const bool c1 = !c0 && tm[1] <= tm[2];
-      t = c0 ? tm[0] : (c1 ? tm[1] : tm[2]);
-      ax = c0 ? 0u : (c1 ? 1u : 2u);
-      const float n0 = tm[0] + d[0], n1 = tm[1] + d[1], n2 = tm[2] + d[2];
-      tm[0] = c0 ? n0 : tm[0];
-      tm[1] = c1 ? n1 : tm[1];
-      tm[2] = (!c0 && !c1) ? n2 : tm[2];
-      idx += (uint32_t)(c0 ? di[0] : (c1 ? di[1] : di[2]));
-      const uint32_t sh = c0 ? 0u : (c1 ? 8u : 16u);
-      out = ((cnt >> sh) & 255u) == 0u || iter == 1u;
-      cnt -= c0 ? dec[0] : (c1 ? dec[1] : dec[2]);
-      iter -= 1u;
-    }
-    if (out) {
-      if (inc == 1u) break;  // coarse: left the grid / out of brick iterations -> miss
-      // fine: leave the brick, back to the coarse DDA and its advance
-      // (voxel.wgsl:142-150 follows the fine loop)
+    if (!advance()) return DDA_RUN;
+    if (inc == 1u) return DDA_MISS;  // coarse: left the grid / out of brick iterations
+    // fine: leave the brick, back to the coarse DDA and its advance
+    // (voxel.wgsl:142-150 follows the fine loop)
 #pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        tm[c] = tmB[c];
-        d[c] = adl[c] * 8.0f;
-      }
-      di[0] = sd[0];
-      di[1] = sd[1] * (int)(1u << sh1);
-      di[2] = sd[2] * (int)(1u << sh2);
-      idx = idxB;
-      cnt = cntB;
-      iter = iterB;
-      base = L.bits_off;
-      wshift = 0u;
-      inc = 1u;
-      tBound = tExit;
-      const bool c0 = tm[0] <= tm[1] && tm[0] <= tm[2];
-      const bool c1 = !c0 && tm[1] <= tm[2];
-      t = c0 ? tm[0] : (c1 ? tm[1] : tm[2]);
-      ax = c0 ? 0u : (c1 ? 1u : 2u);
-      const float n0 = tm[0] + d[0], n1 = tm[1] + d[1], n2 = tm[2] + d[2];
-      tm[0] = c0 ? n0 : tm[0];
-      tm[1] = c1 ? n1 : tm[1];
-      tm[2] = (!c0 && !c1) ? n2 : tm[2];
-      idx += (uint32_t)(c0 ? di[0] : (c1 ? di[1] : di[2]));
-      const uint32_t sh = c0 ? 0u : (c1 ? 8u : 16u);
-      const bool gone = ((cnt >> sh) & 255u) == 0u || iter == 1u;
-      cnt -= c0 ? dec[0] : (c1 ? dec[1] : dec[2]);
-      iter -= 1u;
-      if (gone) break;
+    for (int c = 0; c < 3; ++c) {
+      tm[c] = tmB[c];
+      d[c] = vabs(inv[c]) * 8.0f;
     }
+    di[0] = sd[0];
+    di[1] = sd[1] * (int)(1u << s.psh1);
+    di[2] = sd[2] * (int)(1u << s.psh2);
+    idx = idxB;
+    cnt = cntB;
+    iter = iterB;
+    base = L.bits_off;
+    wshift = 0u;
+    inc = 1u;
+    tBound = tExit;
+    return advance() ? DDA_MISS : DDA_RUN;
   }
-  o.bsteps = steps & 0xffffu;
-  o.vsteps = steps >> 16;
+
+  // The result record of a finished ray (status DDA_MISS or DDA_HIT).
+  VRT_D void finish(const SceneDev& s, uint32_t status, TraceResult& o) const {
+    o.t = -1.0f;
+    o.mat = 0u;
+    o.n = mk3(-rdv[0], -rdv[1], -rdv[2]);
+    o.voxel = 0xFFFFFFFFu;
+    o.face = 255u;
+    o.bsteps = steps & 0xffffu;
+    o.vsteps = steps >> 16;
+    if (status != DDA_HIT || t > maxT) return;  // a hit beyond maxT reports a miss (voxel.wgsl:124)
+    o.t = t * s.voxel_size;  // voxel.wgsl:122-128
+    float nn[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) nn[c] = ((uint32_t)c == ax ? -1.0f : -0.0f) * (float)sd[c];
+    o.n = mk3(nn[0], nn[1], nn[2]);
+    o.mat = s.pool[(uint64_t)blk + 16u + idx];
+    const uint32_t mx = (1u << s.psh1) - 1u, my = (1u << (s.psh2 - s.psh1)) - 1u;
+    const uint32_t bx = idxB & mx, by = (idxB >> s.psh1) & my, bz = idxB >> s.psh2;
+    const uint32_t vx = bx * 8u + (idx & 7u), vy = by * 8u + ((idx >> 3) & 7u), vz = bz * 8u + (idx >> 6);
+    o.voxel = vx + vy * (uint32_t)s.nx + vz * (uint32_t)s.nx * (uint32_t)s.ny;
+    const float nv = ax == 0u ? nn[0] : (ax == 1u ? nn[1] : nn[2]);
+    o.face = nv > 0.0f ? 2u * ax : (nv < 0.0f ? 2u * ax + 1u : 6u);
+  }
+};
+
+VRT_D void trace_fast(const SceneDev& s, const LdsScene& L, f3 roW, f3 rd, float maxTWorld, TraceResult& o) {
+  DdaFast st;
+  if (st.begin(s, L, roW, rd, maxTWorld, o)) return;
+  uint32_t r;
+  while ((r = st.step(s, L)) == DDA_RUN) {
+  }
+  st.finish(s, r, o);
 }
 
 }  // namespace vrt

@@ -12,7 +12,11 @@ using Uniforms = vrt_uniforms;
 struct SceneDev;
 
 // Trace variant ids for the one-ray/pixel-per-lane kernels (launch_* `tv`).
-constexpr int TV_NESTED = 0, TV_FLAT = 1, TV_LDS = 2, TV_FAST = 3;
+constexpr int TV_NESTED = 0, TV_FLAT = 1, TV_LDS = 2, TV_FAST = 3, TV_FAST1 = 4;
+// Variants that trace with the single-loop DDA on LDS (TV_FAST1: the same
+// DDA, path tracer with one trace call site per lane).
+constexpr bool tv_fast(int tv) { return tv == TV_FAST || tv == TV_FAST1; }
+constexpr bool tv_uses_lds(int tv) { return tv == TV_LDS || tv_fast(tv); }
 // Largest occupancy bitmap staged in LDS (words): 48 KiB, 393216 bricks
 // (512^3 = 262144 bricks fits).  Larger scenes fall back to TV_NESTED.
 // TV_FAST stages the padded bitmap and its prefix counts within the same limit.
@@ -33,7 +37,7 @@ hipError_t launch_trace_rays(hipStream_t st, const SceneDev& s, const float* ray
                              float tmax, void* hits, unsigned long long* counters, int blocks, int tv);
 hipError_t launch_trace_persistent(hipStream_t st, const SceneDev& s, const float* rays, uint32_t n,
                                   float tmax, void* hits, unsigned long long* counters,
-                                  uint32_t* queue, int blocks, int steps, int min_refill);
+                                  uint32_t* queue, int blocks, int steps, int min_refill, bool fast);
 hipError_t launch_pathtrace(hipStream_t st, const SceneDev& s, const Uniforms& u, float fclamp,
                             uint32_t spp, void* radiance, void* albedo, void* gbuf,
                             unsigned long long* counters, int tv, int waves);

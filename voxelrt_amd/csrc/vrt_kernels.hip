@@ -161,6 +161,7 @@ __global__ __launch_bounds__(256) void k_pack_pool(const uint32_t* __restrict__ 
 
 //   TV_FAST    single-loop DDA with both levels in LDS (vrt_dda_fast.h):
 //              bitmap + prefix counts + per-lane mask slot
+//   TV_FAST1   TV_FAST traversal; the path tracer uses shade_pixel_1
 // What a lane needs to trace in a given variant.
 struct LaneMem {
   LdsMem m;    // TV_LDS
@@ -173,7 +174,7 @@ VRT_D void trace_sel(const SceneDev& s, const LaneMem& L, f3 ro, f3 rd, float tm
     trace_flat(s, ro, rd, tmax, h);
   else if constexpr (TV == TV_LDS)
     trace_m(s, L.m, ro, rd, tmax, h);
-  else if constexpr (TV == TV_FAST)
+  else if constexpr (tv_fast(TV))
     trace_fast(s, L.f, ro, rd, tmax, h);
   else
     trace(s, ro, rd, tmax, h);
@@ -194,7 +195,7 @@ VRT_D LaneMem lds_setup(const SceneDev& s, uint32_t* lds) {
     L.m.mask = lds + threadIdx.x;
     L.m.stride = 256u;
   }
-  if constexpr (TV == TV_FAST) {
+  if constexpr (tv_fast(TV)) {
     uint32_t* bits = lds + 16u * 256u;
     for (uint32_t i = threadIdx.x; i < s.pwords; i += blockDim.x) {
       bits[i] = s.pbits[i];
@@ -264,16 +265,41 @@ VRT_D void store_hit(uint4* hits, uint32_t i, const TraceResult& h) {
   hits[i] = rec;
 }
 
+// The resumable DDA a persistent lane runs: single-stream HBM form
+// (vrt_dda_flat.h) or the single-loop LDS form (vrt_dda_fast.h).
+struct FlatRunner {
+  DdaFlat st;
+  VRT_D bool begin(const SceneDev& s, const LaneMem&, f3 ro, f3 rd, float tmax, TraceResult& o) {
+    return st.begin(s, ro, rd, tmax, o);
+  }
+  VRT_D bool step(const SceneDev& s, const LaneMem&, TraceResult& o) { return st.step(s, o); }
+};
+struct FastRunner {
+  DdaFast st;
+  VRT_D bool begin(const SceneDev& s, const LaneMem& L, f3 ro, f3 rd, float tmax, TraceResult& o) {
+    return st.begin(s, L.f, ro, rd, tmax, o);
+  }
+  VRT_D bool step(const SceneDev& s, const LaneMem& L, TraceResult& o) {
+    const uint32_t r = st.step(s, L.f);
+    if (r == DDA_RUN) return false;
+    st.finish(s, r, o);
+    return true;
+  }
+};
+
+template <class Runner, int TV>
 __global__ __launch_bounds__(256) void k_trace_persistent(SceneDev s, const float* __restrict__ rays,
                                                           uint32_t n, float tmax,
                                                           uint4* __restrict__ hits,
                                                           unsigned long long* __restrict__ counters,
                                                           uint32_t* __restrict__ queue, int steps,
                                                           int min_refill) {
+  extern __shared__ uint32_t lds_dyn[];
+  const LaneMem L = lds_setup<TV>(s, lds_dyn);
   const uint32_t lane = threadIdx.x & 63u;
   const uint64_t lt_mask = lane == 0u ? 0ull : (~0ull >> (64u - lane));
   uint32_t ray = RAY_NONE;
-  DdaFlat st;
+  Runner st;
   uint32_t nb = 0, nv = 0, nh = 0, nr = 0;
   // the wave's current chunk of the queue: [cbase, cbase + cleft)
   uint32_t cbase = 0u, cleft = 0u;
@@ -298,7 +324,7 @@ __global__ __launch_bounds__(256) void k_trace_persistent(SceneDev s, const floa
         if (my < n) {
           const float* r = rays + (uint64_t)my * 6;
           TraceResult o;
-          if (st.begin(s, mk3(r[0], r[1], r[2]), mk3(r[3], r[4], r[5]), tmax, o)) {
+          if (st.begin(s, L, mk3(r[0], r[1], r[2]), mk3(r[3], r[4], r[5]), tmax, o)) {
             store_hit(hits, my, o);
             nr += 1u;
           } else {
@@ -313,7 +339,7 @@ __global__ __launch_bounds__(256) void k_trace_persistent(SceneDev s, const floa
     for (int k = 0; k < steps; ++k) {
       if (ray < n) {
         TraceResult o;
-        if (st.step(s, o)) {
+        if (st.step(s, L, o)) {
           store_hit(hits, ray, o);
           nb += o.bsteps;
           nv += o.vsteps;
@@ -481,6 +507,107 @@ VRT_D PixelOut shade_pixel(const SceneDev& s, const LaneMem& L, const Uniforms& 
   return res;
 }
 
+// pathtrace.wgsl:248-330 with one trace call site.  Each lane walks its
+// pixel's rays one at a time -- primary, then per vertex the sun shadow ray
+// and the bounce ray -- and the shading between two traces runs as a
+// transition on the ray just finished.  Lanes of a wave therefore trace rays
+// of different kinds in the same DDA loop instead of waiting at separate
+// call sites.  Every lane performs exactly the operations of shade_pixel in
+// the same order (same RNG draws, same FP32 sequence): results are identical.
+template <int TV>
+VRT_D PixelOut shade_pixel_1(const SceneDev& s, const LaneMem& L, const Uniforms& u, float fclamp, uint32_t px,
+                             uint32_t py, uint32_t frame, uint32_t w, uint32_t h, RayCount& rc) {
+  enum : uint32_t { PRIMARY, SHADOW, BOUNCE, DONE, SKY };
+  PixelOut res;
+  uint32_t rng = init_rng(px, py, frame);
+  const float jx = rand01(rng) - 0.5f;
+  const float jy = rand01(rng) - 0.5f;
+  const float uvx = (((float)px + 0.5f) + jx) / (float)w;
+  const float uvy = (((float)py + 0.5f) + jy) / (float)h;
+  const bool indirect = (u.params0[2] & VRT_FLAG_INDIRECT) != 0u;
+  const uint32_t nb = u.params0[1];
+  const f3 sunRad = mk3(u.sun_radiance[0], u.sun_radiance[1], u.sun_radiance[2]);
+  f3 ro = mk3(u.cam_pos[0], u.cam_pos[1], u.cam_pos[2]);
+  f3 rd = camera_ray(u, uvx, uvy);
+  f3 radiance = mk3(0.0f, 0.0f, 0.0f), tp = mk3(1.0f, 1.0f, 1.0f);
+  f3 pos = ro, n = mk3(0.0f, 0.0f, 0.0f), surf = ro, l = rd, alb1 = mk3(0.0f, 0.0f, 0.0f);
+  Material mat{};
+  float ndl = 0.0f;
+  uint32_t bounce = 0u, phase = PRIMARY;
+  res.albedo_sqrt = mk3(1.0f, 1.0f, 1.0f);
+  res.gbuf = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  while (phase < DONE) {
+    TraceResult hh;
+    trace_sel<TV>(s, L, ro, phase == SHADOW ? l : rd, 1e4f, hh);
+    rc.add(hh);
+    bool vertex = false, after_nee = false;
+    if (phase == PRIMARY) {
+      if (hh.t < 0.0f) {
+        res.radiance = indirect ? mk3(0.0f, 0.0f, 0.0f) : sky_color(u, rd, true);
+        res.gbuf = make_float4(-rd.x, -rd.y, -rd.z, -1.0f);
+        phase = SKY;
+        break;
+      }
+      mat = unpack_material(hh.mat);
+      alb1 = mat.albedo;
+      pos = ro + rd * hh.t;
+      n = hh.n;
+      res.albedo_sqrt = mk3(vsqrt(alb1.x), vsqrt(alb1.y), vsqrt(alb1.z));
+      res.gbuf = make_float4(n.x, n.y, n.z, hh.t);
+      vertex = true;
+    } else if (phase == SHADOW) {
+      if (!(hh.t >= 0.0f)) radiance = radiance + (((tp * mat.albedo) * INV_PI_F) * ndl) * sunRad;
+      after_nee = true;
+    } else {  // BOUNCE
+      if (hh.t < 0.0f) {
+        radiance = radiance + tp * sky_color(u, rd, false);
+        phase = DONE;
+      } else {
+        pos = surf + rd * hh.t;
+        n = hh.n;
+        mat = unpack_material(hh.mat);
+        ++bounce;
+        vertex = true;
+      }
+    }
+    if (vertex) {  // emission + sun NEE at the path vertex
+      const bool direct = !(indirect && bounce == 0u);
+      if (mat.emissive > 0.0f && direct) radiance = radiance + (tp * mat.albedo) * (mat.emissive * EMISSIVE_SCALE);
+      surf = pos + n * 1e-3f;
+      l = sample_sun_dir(u, rng);
+      ndl = dot(n, l);
+      if (ndl > 0.0f && l.y > 0.0f && direct) {
+        phase = SHADOW;
+        ro = surf;
+      } else {
+        after_nee = true;
+      }
+    }
+    if (after_nee) {  // cosine bounce + Russian roulette
+      if (bounce == nb) {
+        phase = DONE;
+      } else {
+        rd = cosine_hemisphere(n, rng);
+        tp = tp * mat.albedo;
+        bool alive = true;
+        if (bounce >= 2u) {
+          const float p = vclamp(vmax(tp.x, vmax(tp.y, tp.z)), 0.05f, 0.95f);
+          if (rand01(rng) > p) alive = false;
+          else tp = tp / p;
+        }
+        phase = alive ? BOUNCE : DONE;
+        ro = surf;
+      }
+    }
+  }
+  if (phase == SKY) return res;
+  if (!(fclamp <= 0.0f))
+    radiance = mk3(vmin(radiance.x, fclamp), vmin(radiance.y, fclamp), vmin(radiance.z, fclamp));
+  const f3 den = mk3(vmax(alb1.x, 1e-3f), vmax(alb1.y, 1e-3f), vmax(alb1.z, 1e-3f));
+  res.radiance = radiance / den;
+  return res;
+}
+
 // Pixel p of a launch covering TILE x TILE tiles, row-major tiles: keeps the
 // 64 lanes of a wave on a compact 16x4 patch (coherent primary rays).
 VRT_D bool tile_pixel(uint32_t w, uint32_t h, uint32_t tile, uint32_t& px, uint32_t& py) {
@@ -502,7 +629,11 @@ VRT_D void pathtrace_tile(const SceneDev& s, const LaneMem& L, const Uniforms& u
     f3 acc = mk3(0.0f, 0.0f, 0.0f);
     PixelOut first;
     for (uint32_t smp = 0; smp < spp; ++smp) {
-      PixelOut r = shade_pixel<TV>(s, L, u, fclamp, px, py, u.params0[0] * spp + smp, w, h, rc);
+      PixelOut r;
+      if constexpr (TV == TV_FAST1)
+        r = shade_pixel_1<TV>(s, L, u, fclamp, px, py, u.params0[0] * spp + smp, w, h, rc);
+      else
+        r = shade_pixel<TV>(s, L, u, fclamp, px, py, u.params0[0] * spp + smp, w, h, rc);
       if (smp == 0) first = r;
       acc = acc + r.radiance;
     }
@@ -755,7 +886,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WV))) void 
     if (b >= tiles) break;
     const uint32_t vi = find_view(views, nviews, b);
     const ViewDesc& d = views[vi];
-    if (TV == TV_LDS || TV == TV_FAST) {
+    if (tv_uses_lds(TV)) {
       if (d.s.pool != loaded) {
         L = lds_setup<TV>(d.s, lds_dyn);
         loaded = d.s.pool;
@@ -907,15 +1038,15 @@ static inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b
 // LDS words a variant stages beyond the mask slots: TV_LDS the bitmap,
 // TV_FAST the padded bitmap and its prefix counts.
 static inline uint32_t lds_words(int tv, uint32_t bit_words, uint32_t pwords) {
-  return tv == TV_FAST ? 2u * pwords : (tv == TV_LDS ? bit_words : 0u);
+  return tv_fast(tv) ? 2u * pwords : (tv == TV_LDS ? bit_words : 0u);
 }
 static inline uint32_t lds_bytes(int tv, uint32_t bit_words, uint32_t pwords) {
-  return tv == TV_FAST || tv == TV_LDS ? (16u * 256u + lds_words(tv, bit_words, pwords)) * 4u : 0u;
+  return tv_uses_lds(tv) ? (16u * 256u + lds_words(tv, bit_words, pwords)) * 4u : 0u;
 }
 // Variant actually launched: LDS variants fall back to nested loops when the
 // bitmap does not fit.
 static inline int lds_variant(int tv, uint32_t bit_words, uint32_t pwords) {
-  return (tv == TV_LDS || tv == TV_FAST) && lds_words(tv, bit_words, pwords) > LDS_BIT_WORDS_MAX ? TV_NESTED : tv;
+  return tv_uses_lds(tv) && lds_words(tv, bit_words, pwords) > LDS_BIT_WORDS_MAX ? TV_NESTED : tv;
 }
 
 hipError_t launch_brick_masks(hipStream_t st, const uint32_t* vox, int nx, int ny, int bgx,
@@ -980,12 +1111,17 @@ hipError_t launch_trace_rays(hipStream_t st, const SceneDev& s, const float* ray
 
 hipError_t launch_trace_persistent(hipStream_t st, const SceneDev& s, const float* rays, uint32_t n,
                                   float tmax, void* hits, unsigned long long* counters,
-                                  uint32_t* queue, int blocks, int steps, int min_refill) {
+                                  uint32_t* queue, int blocks, int steps, int min_refill, bool fast) {
   hipError_t e = hipMemsetAsync(queue, 0, sizeof(uint32_t), st);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_trace_persistent, dim3(blocks > 0 ? blocks : 1), dim3(256), 0, st, s, rays, n,
-                     tmax, (uint4*)hits, counters, queue, steps < 1 ? 1 : steps,
-                     min_refill < 1 ? 1 : min_refill);
+  const dim3 g(blocks > 0 ? blocks : 1);
+  const int sp = steps < 1 ? 1 : steps, mr = min_refill < 1 ? 1 : min_refill;
+  if (fast && lds_variant(TV_FAST, s.bit_words, s.pwords) == TV_FAST)
+    hipLaunchKernelGGL((k_trace_persistent<FastRunner, TV_FAST>), g, dim3(256), lds_bytes(TV_FAST, s.bit_words, s.pwords),
+                       st, s, rays, n, tmax, (uint4*)hits, counters, queue, sp, mr);
+  else
+    hipLaunchKernelGGL((k_trace_persistent<FlatRunner, TV_NESTED>), g, dim3(256), 0, st, s, rays, n, tmax,
+                       (uint4*)hits, counters, queue, sp, mr);
   return hipGetLastError();
 }
 
@@ -1001,6 +1137,9 @@ hipError_t launch_pathtrace(hipStream_t st, const SceneDev& s, const Uniforms& u
                        (uchar4*)albedo, (float4*)gbuf, counters);
   };
   switch (tv * 8 + (waves == 4 || waves == 5 ? waves : 1)) {
+    case TV_FAST1 * 8 + 4: go(k_pathtrace<TV_FAST1, 4>); break;
+    case TV_FAST1 * 8 + 5: go(k_pathtrace<TV_FAST1, 5>); break;
+    case TV_FAST1 * 8 + 1: go(k_pathtrace<TV_FAST1, 1>); break;
     case TV_FAST * 8 + 4: go(k_pathtrace<TV_FAST, 4>); break;
     case TV_FAST * 8 + 5: go(k_pathtrace<TV_FAST, 5>); break;
     case TV_FAST * 8 + 1: go(k_pathtrace<TV_FAST, 1>); break;
@@ -1039,6 +1178,9 @@ hipError_t launch_pathtrace_batch(hipStream_t st, const ViewDesc* views, uint32_
     hipLaunchKernelGGL(kernel, dim3(tiles), dim3(256), shmem, st, views, nviews, counters);
   };
   switch (tv * 8 + (waves == 4 || waves == 5 ? waves : 1)) {
+    case TV_FAST1 * 8 + 4: go(k_pathtrace_batch<TV_FAST1, 4>); break;
+    case TV_FAST1 * 8 + 5: go(k_pathtrace_batch<TV_FAST1, 5>); break;
+    case TV_FAST1 * 8 + 1: go(k_pathtrace_batch<TV_FAST1, 1>); break;
     case TV_FAST * 8 + 4: go(k_pathtrace_batch<TV_FAST, 4>); break;
     case TV_FAST * 8 + 5: go(k_pathtrace_batch<TV_FAST, 5>); break;
     case TV_FAST * 8 + 1: go(k_pathtrace_batch<TV_FAST, 1>); break;
@@ -1064,6 +1206,9 @@ hipError_t launch_batch_persistent(hipStream_t st, BatchHead* head, const ViewDe
   const dim3 gp(pt_blocks > 0 ? pt_blocks : 1), gg(gs_blocks > 0 ? gs_blocks : 1);
   auto go = [&](auto kernel) { hipLaunchKernelGGL(kernel, gp, dim3(256), shmem, st, head, views, counters); };
   switch (tv * 8 + (waves == 4 || waves == 5 ? waves : 1)) {
+    case TV_FAST1 * 8 + 4: go(k_pathtrace_batch_q<TV_FAST1, 4>); break;
+    case TV_FAST1 * 8 + 5: go(k_pathtrace_batch_q<TV_FAST1, 5>); break;
+    case TV_FAST1 * 8 + 1: go(k_pathtrace_batch_q<TV_FAST1, 1>); break;
     case TV_FAST * 8 + 4: go(k_pathtrace_batch_q<TV_FAST, 4>); break;
     case TV_FAST * 8 + 5: go(k_pathtrace_batch_q<TV_FAST, 5>); break;
     case TV_FAST * 8 + 1: go(k_pathtrace_batch_q<TV_FAST, 1>); break;

@@ -536,12 +536,13 @@ int vrt_scene_bricks(vrt_ctx* ctx, const vrt_scene* scene, uint32_t* bricks, uin
 // ---------------------------------------------------------------------------
 // vrt_tuning kernel id -> trace variant of the one-per-lane kernels:
 // 0 nested, 2 single-stream (flat), 3 LDS-staged occupancy, 4 single loop
-// with both levels in LDS.
+// with both levels in LDS, 5 as 4 with one trace call site in the path tracer.
 static int lane_variant(uint32_t kernel) {
   switch (kernel) {
     case 2: return vrt::TV_FLAT;
     case 3: return vrt::TV_LDS;
     case 4: return vrt::TV_FAST;
+    case 5: return vrt::TV_FAST1;
     default: return vrt::TV_NESTED;
   }
 }
@@ -560,11 +561,11 @@ hipError_t vrt_ctx::launch_pathtrace(const vrt::SceneDev& s, const vrt_uniforms&
 
 hipError_t vrt_ctx::launch_trace(const vrt::SceneDev& s, const float* rays, uint64_t n, float tmax,
                                  void* hits) {
-  if (tuning.trace_kernel == 1 && n < 0x80000000ull)
+  if ((tuning.trace_kernel == 1 || tuning.trace_kernel == 5) && n < 0x80000000ull)
     return vrt::launch_trace_persistent(stream, s, rays, (uint32_t)n, tmax, hits,
                                         (unsigned long long*)d_counters.ptr, (uint32_t*)d_queue.ptr,
                                         (int)tuning.trace_blocks, (int)tuning.trace_steps,
-                                        (int)tuning.trace_min_refill);
+                                        (int)tuning.trace_min_refill, tuning.trace_kernel == 5);
   return vrt::launch_trace_rays(stream, s, rays, n, tmax, hits, (unsigned long long*)d_counters.ptr,
                                 trace_blocks, lane_variant(tuning.trace_kernel));
 }
@@ -844,7 +845,7 @@ int vrt_tuning_get(vrt_ctx* ctx, vrt_tuning* out) {
 int vrt_tuning_set(vrt_ctx* ctx, const vrt_tuning* in) {
   if (!ctx || !in) return VRT_ERR_INVALID;
   std::lock_guard<std::mutex> lock_(ctx->mu);
-  if (in->trace_kernel > 4 || in->pt_kernel > 4 || in->trace_blocks > (1u << 20) ||
+  if (in->trace_kernel > 5 || in->pt_kernel > 5 || in->trace_blocks > (1u << 20) ||
       in->pt_blocks > (1u << 20) || in->trace_steps > 1024 || in->pt_steps > 1024 ||
       in->trace_min_refill > 64 || in->pt_min_refill > 64 || in->pt_chunk > (1u << 16) ||
       !(in->pt_waves == 0 || in->pt_waves == 4 || in->pt_waves == 5) || in->batch_mode > 2 ||
@@ -922,10 +923,11 @@ int vrt_autotune(vrt_ctx* ctx, const vrt_scene* scene, uint64_t nrays, const vrt
         best.trace_kernel = k;
       }
     }
+    for (uint32_t pk : {1u, 5u})
     for (uint32_t b : blocks)
       for (uint32_t st : steps)
         for (uint32_t rf : refill) {
-          ctx->tuning.trace_kernel = 1;
+          ctx->tuning.trace_kernel = pk;
           ctx->tuning.trace_blocks = b;
           ctx->tuning.trace_steps = st;
           ctx->tuning.trace_min_refill = rf;
@@ -934,7 +936,7 @@ int vrt_autotune(vrt_ctx* ctx, const vrt_scene* scene, uint64_t nrays, const vrt
           if (rc) return rc;
           if (ms < best_ms) {
             best_ms = ms;
-            best.trace_kernel = 1;
+            best.trace_kernel = pk;
             best.trace_blocks = b;
             best.trace_steps = st;
             best.trace_min_refill = rf;
@@ -960,7 +962,7 @@ int vrt_autotune(vrt_ctx* ctx, const vrt_scene* scene, uint64_t nrays, const vrt
     best.pt_kernel = 0;
     best.pt_waves = 0;
     // one-pixel-per-lane variants x register budgets (waves per SIMD)
-    for (uint32_t k : {0u, 2u, 3u, 4u})
+    for (uint32_t k : {0u, 2u, 3u, 4u, 5u})
       for (uint32_t wv : {0u, 4u, 5u}) {
         if (k == 0 && wv == 0) continue;  // measured above
         ctx->tuning.pt_kernel = k;
