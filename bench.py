@@ -90,15 +90,17 @@ def traverse(args):
     d_hits = torch.empty(n_rays * 4, dtype=torch.int32, device="cuda:0")
     torch.cuda.synchronize()
     variants = {"lane": dict(trace_kernel=0), "lane_flat": dict(trace_kernel=2), "lane_lds": dict(trace_kernel=3),
-                "lane_fast": dict(trace_kernel=4),
-                "persistent_fast": dict(trace_kernel=5, trace_blocks=2048, trace_steps=4, trace_min_refill=1),
+                "lane_fast": dict(trace_kernel=4, dda_batch=1),
+                "lane_fast_b16": dict(trace_kernel=4, dda_batch=16),
+                "persistent_fast_b16": dict(trace_kernel=5, trace_blocks=1024, trace_steps=8, trace_min_refill=16,
+                                            dda_batch=16),
                 "persistent": dict(trace_kernel=1, trace_blocks=2048, trace_steps=8, trace_min_refill=16)}
     for p in (0.05, 0.25, 0.6):
         vox = cf.traverse_scene(p)
         sc = ctx.upload(vox)
         if not args.no_cpu:
             print(json.dumps(_traverse_cpu(vox, rays, p)), flush=True)
-        for name in ("lane", "lane_fast", "persistent", "persistent_fast", "autotuned"):
+        for name in ("lane", "lane_fast", "lane_fast_b16", "persistent", "persistent_fast_b16", "autotuned"):
             if name == "autotuned":
                 tuned = ctx.autotune(sc, 1 << 22).as_dict()
             else:

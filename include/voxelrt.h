@@ -259,7 +259,11 @@ typedef struct vrt_tuning {
                                      queue + fixed-grid filter passes,
                                  2 = as 1, replayed from a captured hipGraph */
   uint32_t batch_blocks;      /* persistent path-tracer grid, 0 = 8 per CU */
-  uint32_t reserved[4];
+  uint32_t dda_batch;         /* traversal kernels 4 and 5: lanes of a wave that wait
+                                 at occupied bricks before they enter them together
+                                 (1..64; 0 = 1, enter at once).  The path tracer
+                                 always enters at once. */
+  uint32_t reserved[3];
 } vrt_tuning;
 int vrt_tuning_get(vrt_ctx* ctx, vrt_tuning* out);
 int vrt_tuning_set(vrt_ctx* ctx, const vrt_tuning* in);

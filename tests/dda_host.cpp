@@ -94,6 +94,7 @@ void build(const uint32_t* vox, int nx, int ny, int nz, HostScene& h) {
   s.bgz = bgz;
   s.voxel_size = 0.0625f;
   s.inv_voxel = 1.0f / 0.0625f;
+  s.entry_batch = 1u;
   s.zero = 0.0f;
 }
 

@@ -75,16 +75,18 @@ def traverse(request, ctx):
 
 
 TRACE_VARIANTS = [dict(trace_kernel=0), dict(trace_kernel=2), dict(trace_kernel=3), dict(trace_kernel=4),
+                  dict(trace_kernel=4, dda_batch=8), dict(trace_kernel=4, dda_batch=64),
                   dict(trace_kernel=1, trace_blocks=2048, trace_steps=8, trace_min_refill=16),
                   dict(trace_kernel=1, trace_blocks=512, trace_steps=1, trace_min_refill=1),
                   dict(trace_kernel=1, trace_blocks=4096, trace_steps=16, trace_min_refill=64),
                   dict(trace_kernel=5, trace_blocks=2048, trace_steps=4, trace_min_refill=1),
-                  dict(trace_kernel=5, trace_blocks=333, trace_steps=1, trace_min_refill=64)]
+                  dict(trace_kernel=5, trace_blocks=333, trace_steps=1, trace_min_refill=64),
+                  dict(trace_kernel=5, trace_blocks=1024, trace_steps=8, trace_min_refill=16, dda_batch=16)]
 
 
 @pytest.fixture(params=range(len(TRACE_VARIANTS)),
-                ids=["lane", "lane-flat", "lane-lds", "lane-fast", "persist", "persist-s1", "persist-s16",
-                     "persist-fast", "persist-fast-333"])
+                ids=["lane", "lane-flat", "lane-lds", "lane-fast", "lane-fast-b8", "lane-fast-b64", "persist",
+                     "persist-s1", "persist-s16", "persist-fast", "persist-fast-333", "persist-fast-b16"])
 def trace_variant(request, ctx):
     old = ctx.tuning().as_dict()
     ctx.set_tuning(**TRACE_VARIANTS[request.param])

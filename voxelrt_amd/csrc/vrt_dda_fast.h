@@ -18,6 +18,15 @@
 // different levels run the same instructions together.  Rare events (entering
 // an occupied brick, leaving it, hit, miss) branch.
 //
+// Entering a brick (copy of its mask into LDS, fine-level setup) is the
+// longest of those branches, and in a wave of 64 lanes some lane enters a
+// brick in most iterations, so the whole wave would pay for it each time with
+// a few lanes active.  step() therefore stops at an occupied brick
+// (DDA_ENTER) and run() lets the lanes that stopped enter together: its inner
+// loop steps the other lanes until SceneDev::entry_batch lanes of the wave
+// wait (a wave-uniform count, so the loop exit is a scalar branch) or no lane
+// is left stepping.  That changes when a lane enters, never what it computes.
+//
 // Exactness: every t value comes from the same FP32 operations in the same
 // order as vrt::trace_m (and the oracle): tMaxB += |invRd|*8, tMaxV += |invRd|,
 // fine tMax recomputed from the origin on entry, ties x, then y, then z.  The
@@ -57,7 +66,7 @@ VRT_D uint32_t pack_left(const int p[3], const int n[3], const int sd[3]) {
 // the ray up; step() runs one iteration and returns DDA_RUN, DDA_MISS or
 // DDA_HIT; finish() writes the result record.  trace_fast() runs them in a
 // loop; the persistent traversal kernel interleaves rays per lane.
-enum : uint32_t { DDA_RUN = 0u, DDA_MISS = 1u, DDA_HIT = 2u };
+enum : uint32_t { DDA_RUN = 0u, DDA_MISS = 1u, DDA_HIT = 2u, DDA_ENTER = 3u };
 
 // Bitwise selects (c ? a : b, bit for bit).  Written as masks so that the
 // compiler cannot turn a choice between two state-array elements into a load
@@ -169,61 +178,57 @@ struct DdaFast {
     return out;
   }
 
-  VRT_D uint32_t step(const SceneDev& s, const LdsScene& L) {
-    // ---- test the current cell ----
-    if (t > tBound) return DDA_MISS;
-    const uint32_t word = L.lds[base + ((idx >> 5) << wshift)];
-    steps += inc;
-    if ((word >> (idx & 31u)) & 1u) {
-      if (inc != 1u) return DDA_HIT;  // fine level (voxel.wgsl:122)
-      // coarse level: enter the occupied brick (voxel.wgsl:111-118)
-      const uint32_t w = idx >> 5;
-      const uint32_t slot = L.lds[L.prefix_off + w] + (uint32_t)__builtin_popcount(word & ((1u << (idx & 31u)) - 1u));
-      blk = slot * (uint32_t)POOL_WORDS;
-      const uint4* q = (const uint4*)(s.pool + blk);
+  // Enter the occupied brick at idx (voxel.wgsl:111-118): its pool slot is
+  // prefix[w] + the set bits below it in bitmap word w; its mask goes into
+  // the lane's LDS slot; the fine level starts at the voxel containing
+  // ro + rd * (t + 1e-4), clamped into the brick, with tMax from the origin.
+  VRT_D void enter(const SceneDev& s, const LdsScene& L) {
+    const uint32_t w = idx >> 5;
+    const uint32_t word = L.lds[L.bits_off + w];
+    const uint32_t slot = L.lds[L.prefix_off + w] + (uint32_t)__builtin_popcount(word & ((1u << (idx & 31u)) - 1u));
+    blk = slot * (uint32_t)POOL_WORDS;
+    const uint4* q = (const uint4*)(s.pool + blk);
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const uint4 v = q[k];
-        L.lds[((4 * k + 0) << 8) + L.lane] = v.x;
-        L.lds[((4 * k + 1) << 8) + L.lane] = v.y;
-        L.lds[((4 * k + 2) << 8) + L.lane] = v.z;
-        L.lds[((4 * k + 3) << 8) + L.lane] = v.w;
-      }
-      const uint32_t mx = (1u << s.psh1) - 1u, my = (1u << (s.psh2 - s.psh1)) - 1u;
-      const uint32_t bx = idx & mx, by = (idx >> s.psh1) & my, bz = idx >> s.psh2;
-      const int bmin[3] = {(int)bx * 8, (int)by * 8, (int)bz * 8};
-      const float ts = t + 1e-4f;
-      int lp[3];
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        tmB[c] = tm[c];
-        const float p = ro[c] + rdv[c] * ts;
-        const int vp = floor_clamp(p, bmin[c], bmin[c] + 7);
-        const float ge = rdv[c] >= 0.0f ? 1.0f : 0.0f;
-        tm[c] = (((float)vp + ge) - ro[c]) * inv[c];
-        lp[c] = vp - bmin[c];
-        d[c] = vabs(inv[c]);
-      }
-      di[0] = sd[0];
-      di[1] = sd[1] * 8;
-      di[2] = sd[2] * 64;
-      idxB = idx;
-      cntB = cnt;
-      iterB = iter;
-      idx = (uint32_t)lp[0] + 8u * (uint32_t)lp[1] + 64u * (uint32_t)lp[2];
-      const int n8[3] = {8, 8, 8};
-      cnt = pack_left(lp, n8, sd);
-      iter = 24u;
-      base = L.lane;
-      wshift = 8u;
-      inc = 0x10000u;
-      tBound = __builtin_inff();
-      return DDA_RUN;  // the entry voxel is tested next
+    for (int k = 0; k < 4; ++k) {
+      const uint4 v = q[k];
+      L.lds[((4 * k + 0) << 8) + L.lane] = v.x;
+      L.lds[((4 * k + 1) << 8) + L.lane] = v.y;
+      L.lds[((4 * k + 2) << 8) + L.lane] = v.z;
+      L.lds[((4 * k + 3) << 8) + L.lane] = v.w;
     }
-    if (!advance()) return DDA_RUN;
-    if (inc == 1u) return DDA_MISS;  // coarse: left the grid / out of brick iterations
-    // fine: leave the brick, back to the coarse DDA and its advance
-    // (voxel.wgsl:142-150 follows the fine loop)
+    const uint32_t mx = (1u << s.psh1) - 1u, my = (1u << (s.psh2 - s.psh1)) - 1u;
+    const uint32_t bx = idx & mx, by = (idx >> s.psh1) & my, bz = idx >> s.psh2;
+    const int bmin[3] = {(int)bx * 8, (int)by * 8, (int)bz * 8};
+    const float ts = t + 1e-4f;
+    int lp[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      tmB[c] = tm[c];
+      const float p = ro[c] + rdv[c] * ts;
+      const int vp = floor_clamp(p, bmin[c], bmin[c] + 7);
+      const float ge = rdv[c] >= 0.0f ? 1.0f : 0.0f;
+      tm[c] = (((float)vp + ge) - ro[c]) * inv[c];
+      lp[c] = vp - bmin[c];
+      d[c] = vabs(inv[c]);
+    }
+    di[0] = sd[0];
+    di[1] = sd[1] * 8;
+    di[2] = sd[2] * 64;
+    idxB = idx;
+    cntB = cnt;
+    iterB = iter;
+    idx = (uint32_t)lp[0] + 8u * (uint32_t)lp[1] + 64u * (uint32_t)lp[2];
+    const int n8[3] = {8, 8, 8};
+    cnt = pack_left(lp, n8, sd);
+    iter = 24u;
+    base = L.lane;
+    wshift = 8u;
+    inc = 0x10000u;
+    tBound = __builtin_inff();
+  }
+
+  // Leave the brick: back to the coarse registers saved on entry.
+  VRT_D void leave(const SceneDev& s, const LdsScene& L) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       tm[c] = tmB[c];
@@ -239,7 +244,53 @@ struct DdaFast {
     wshift = 0u;
     inc = 1u;
     tBound = tExit;
+  }
+
+  // One iteration: test the current cell, then advance.  Returns DDA_ENTER
+  // (without advancing) at an occupied brick of the coarse level; the caller
+  // calls enter() and steps on.
+  VRT_D uint32_t step(const SceneDev& s, const LdsScene& L) {
+    if (t > tBound) return DDA_MISS;
+    const uint32_t word = L.lds[base + ((idx >> 5) << wshift)];
+    steps += inc;
+    if ((word >> (idx & 31u)) & 1u) return inc != 1u ? DDA_HIT : DDA_ENTER;  // voxel.wgsl:122 / 111
+    if (!advance()) return DDA_RUN;
+    if (inc == 1u) return DDA_MISS;  // coarse: left the grid / out of brick iterations
+    // fine: leave the brick, back to the coarse DDA and its advance
+    // (voxel.wgsl:142-150 follows the fine loop)
+    leave(s, L);
     return advance() ? DDA_MISS : DDA_RUN;
+  }
+
+  // Walk to the end (DDA_HIT or DDA_MISS).  Batched: enter bricks in groups
+  // of up to s.entry_batch lanes, and every lane of the wave that walks a ray
+  // calls it; otherwise (and with entry_batch <= 1) enter at once.
+  template <bool Batched>
+  VRT_D uint32_t run(const SceneDev& s, const LdsScene& L) {
+    uint32_t r = DDA_RUN;
+    if (!Batched || s.entry_batch <= 1u) {  // enter at once: no wave-level bookkeeping
+      for (;;) {
+        r = step(s, L);
+        if (r == DDA_ENTER) {
+          enter(s, L);
+          continue;
+        }
+        if (r != DDA_RUN) return r;
+      }
+    }
+    for (;;) {
+      uint32_t waiting = 0u;  // lanes of the wave stopped at a brick (uniform)
+      while (r == DDA_RUN) {
+        r = step(s, L);
+        waiting += (uint32_t)__builtin_popcountll(wave_ballot(r == DDA_ENTER));
+        if (waiting >= s.entry_batch) break;
+      }
+      if (r == DDA_ENTER) {
+        enter(s, L);
+        r = DDA_RUN;
+      }
+      if (r != DDA_RUN) return r;
+    }
   }
 
   // The result record of a finished ray (status DDA_MISS or DDA_HIT).
@@ -267,13 +318,11 @@ struct DdaFast {
   }
 };
 
+template <bool Batched = false>
 VRT_D void trace_fast(const SceneDev& s, const LdsScene& L, f3 roW, f3 rd, float maxTWorld, TraceResult& o) {
   DdaFast st;
   if (st.begin(s, L, roW, rd, maxTWorld, o)) return;
-  uint32_t r;
-  while ((r = st.step(s, L)) == DDA_RUN) {
-  }
-  st.finish(s, r, o);
+  st.finish(s, st.run<Batched>(s, L), o);
 }
 
 }  // namespace vrt

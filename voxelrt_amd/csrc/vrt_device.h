@@ -31,6 +31,16 @@ VRT_D f3 operator*(f3 a, float s) { return mk3(a.x * s, a.y * s, a.z * s); }
 VRT_D f3 operator/(f3 a, f3 b) { return mk3(a.x / b.x, a.y / b.y, a.z / b.z); }
 VRT_D f3 operator/(f3 a, float s) { return mk3(a.x / s, a.y / s, a.z / s); }
 
+// Lanes of the calling wave for which p holds (bit l = lane l).  The host
+// instantiation (CPU tests) runs one lane at a time, which is lane 0.
+VRT_D uint64_t wave_ballot(bool p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __ballot(p);
+#else
+  return p ? 1ull : 0ull;
+#endif
+}
+
 VRT_D float vmin(float a, float b) { return b < a ? b : a; }
 VRT_D float vmax(float a, float b) { return a < b ? b : a; }
 VRT_D float vclamp(float x, float lo, float hi) { return vmin(vmax(x, lo), hi); }

@@ -153,6 +153,11 @@ struct vrt_ctx {
   }
   int view_state(uint32_t id, uint32_t w, uint32_t h, uint32_t fmt, ViewState** out,
                  hipStream_t st = nullptr);  // st: stream for the initial clears (default: the context's)
+  // A scene's device view with this context's launch-time tuning applied.
+  vrt::SceneDev tuned(vrt::SceneDev s) const {
+    s.entry_batch = tuning.dda_batch ? tuning.dda_batch : 1u;
+    return s;
+  }
   hipError_t launch_trace(const vrt::SceneDev& s, const float* rays, uint64_t n, float tmax,
                           void* hits);
   hipError_t launch_pathtrace(const vrt::SceneDev& s, const vrt_uniforms& u, float fclamp,

@@ -228,7 +228,10 @@ __global__ __launch_bounds__(256) void k_trace_rays(SceneDev s, const float* __r
   for (uint64_t i = base0; i < n; i += stride) {
     const float* r = rays + i * 6;
     TraceResult h;
-    trace_sel<TV>(s, L, mk3(r[0], r[1], r[2]), mk3(r[3], r[4], r[5]), tmax, h);
+    if constexpr (tv_fast(TV))  // wave-batched brick entries (SceneDev::entry_batch)
+      trace_fast<true>(s, L.f, mk3(r[0], r[1], r[2]), mk3(r[3], r[4], r[5]), tmax, h);
+    else
+      trace_sel<TV>(s, L, mk3(r[0], r[1], r[2]), mk3(r[3], r[4], r[5]), tmax, h);
     uint4 rec;
     rec.x = __builtin_bit_cast(uint32_t, h.t);
     rec.y = h.voxel;
@@ -273,17 +276,36 @@ struct FlatRunner {
     return st.begin(s, ro, rd, tmax, o);
   }
   VRT_D bool step(const SceneDev& s, const LaneMem&, TraceResult& o) { return st.step(s, o); }
+  VRT_D void settle(const SceneDev&, const LaneMem&, bool) {}
 };
+// A lane that reaches an occupied brick waits (pend) until entry_batch lanes
+// of its wave wait, or every lane walking a ray does; settle(), called by the
+// whole wave after each step, lets them enter together.
 struct FastRunner {
   DdaFast st;
+  bool pend = false;
   VRT_D bool begin(const SceneDev& s, const LaneMem& L, f3 ro, f3 rd, float tmax, TraceResult& o) {
     return st.begin(s, L.f, ro, rd, tmax, o);
   }
   VRT_D bool step(const SceneDev& s, const LaneMem& L, TraceResult& o) {
+    if (pend) return false;
     const uint32_t r = st.step(s, L.f);
     if (r == DDA_RUN) return false;
+    if (r == DDA_ENTER) {
+      pend = true;
+      return false;
+    }
     st.finish(s, r, o);
     return true;
+  }
+  VRT_D void settle(const SceneDev& s, const LaneMem& L, bool walking) {
+    const uint64_t w = wave_ballot(pend);
+    if (w == 0ull) return;
+    const uint32_t nw = (uint32_t)__popcll(w), nr = (uint32_t)__popcll(wave_ballot(walking));
+    if (nw >= (nr < s.entry_batch ? nr : s.entry_batch) && pend) {
+      st.enter(s, L.f);
+      pend = false;
+    }
   }
 };
 
@@ -348,6 +370,7 @@ __global__ __launch_bounds__(256) void k_trace_persistent(SceneDev s, const floa
           ray = RAY_NONE;
         }
       }
+      st.settle(s, L, ray < n);
     }
   }
   count4(counters, nr, nb, nv, nh);

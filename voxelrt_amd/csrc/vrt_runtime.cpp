@@ -129,6 +129,7 @@ vrt::SceneDev vrt_scene::dev() const {
   s.bgz = (int)(nz / 8);
   s.voxel_size = VRT_VOXEL_SIZE;
   s.inv_voxel = 1.0f / VRT_VOXEL_SIZE;
+  s.entry_batch = 1u;
   s.zero = 0.0f;
   return s;
 }
@@ -547,8 +548,9 @@ static int lane_variant(uint32_t kernel) {
   }
 }
 
-hipError_t vrt_ctx::launch_pathtrace(const vrt::SceneDev& s, const vrt_uniforms& u, float fclamp,
+hipError_t vrt_ctx::launch_pathtrace(const vrt::SceneDev& s0, const vrt_uniforms& u, float fclamp,
                                      uint32_t spp, void* radiance, void* albedo, void* gbuf) {
+  const vrt::SceneDev s = tuned(s0);
   if (tuning.pt_kernel == 1)
     return vrt::launch_pathtrace_persistent(stream, s, u, fclamp, spp, radiance, albedo, gbuf,
                                             (unsigned long long*)d_counters.ptr, (uint32_t*)d_queue.ptr + 1,
@@ -559,8 +561,9 @@ hipError_t vrt_ctx::launch_pathtrace(const vrt::SceneDev& s, const vrt_uniforms&
                                (int)tuning.pt_waves);
 }
 
-hipError_t vrt_ctx::launch_trace(const vrt::SceneDev& s, const float* rays, uint64_t n, float tmax,
+hipError_t vrt_ctx::launch_trace(const vrt::SceneDev& s0, const float* rays, uint64_t n, float tmax,
                                  void* hits) {
+  const vrt::SceneDev s = tuned(s0);
   if ((tuning.trace_kernel == 1 || tuning.trace_kernel == 5) && n < 0x80000000ull)
     return vrt::launch_trace_persistent(stream, s, rays, (uint32_t)n, tmax, hits,
                                         (unsigned long long*)d_counters.ptr, (uint32_t*)d_queue.ptr,
@@ -849,7 +852,7 @@ int vrt_tuning_set(vrt_ctx* ctx, const vrt_tuning* in) {
       in->pt_blocks > (1u << 20) || in->trace_steps > 1024 || in->pt_steps > 1024 ||
       in->trace_min_refill > 64 || in->pt_min_refill > 64 || in->pt_chunk > (1u << 16) ||
       !(in->pt_waves == 0 || in->pt_waves == 4 || in->pt_waves == 5) || in->batch_mode > 2 ||
-      in->batch_blocks > (1u << 16))
+      in->batch_blocks > (1u << 16) || in->dda_batch > 64)
     return ctx->fail(VRT_ERR_INVALID, "tuning value out of range");
   ctx->tuning = *in;
   return VRT_OK;
@@ -913,35 +916,47 @@ int vrt_autotune(vrt_ctx* ctx, const vrt_scene* scene, uint64_t nrays, const vrt
     if (rc) return rc;
     best_ms = ms;
     best.trace_kernel = 0;
-    for (uint32_t k : {2u, 3u, 4u}) {
-      ctx->tuning.trace_kernel = k;
-      rc = time_it([&] { return ctx->launch_trace(s, (const float*)ctx->scratch_rays.ptr, n, 1e4f,
-                                                  ctx->scratch_hits.ptr); }, &ms);
-      if (rc) return rc;
-      if (ms < best_ms) {
-        best_ms = ms;
-        best.trace_kernel = k;
-      }
-    }
-    for (uint32_t pk : {1u, 5u})
-    for (uint32_t b : blocks)
-      for (uint32_t st : steps)
-        for (uint32_t rf : refill) {
-          ctx->tuning.trace_kernel = pk;
-          ctx->tuning.trace_blocks = b;
-          ctx->tuning.trace_steps = st;
-          ctx->tuning.trace_min_refill = rf;
-          rc = time_it([&] { return ctx->launch_trace(s, (const float*)ctx->scratch_rays.ptr, n, 1e4f,
-                                                      ctx->scratch_hits.ptr); }, &ms);
-          if (rc) return rc;
-          if (ms < best_ms) {
-            best_ms = ms;
-            best.trace_kernel = pk;
-            best.trace_blocks = b;
-            best.trace_steps = st;
-            best.trace_min_refill = rf;
-          }
+    best.dda_batch = 1;
+    // one-ray-per-lane variants; the single-loop one also over its brick
+    // entry batch
+    for (uint32_t k : {2u, 3u, 4u})
+      for (uint32_t db : {1u, 8u, 16u, 32u}) {
+        if (k != 4u && db != 1u) continue;
+        ctx->tuning.trace_kernel = k;
+        ctx->tuning.dda_batch = db;
+        rc = time_it([&] { return ctx->launch_trace(s, (const float*)ctx->scratch_rays.ptr, n, 1e4f,
+                                                    ctx->scratch_hits.ptr); }, &ms);
+        if (rc) return rc;
+        if (ms < best_ms) {
+          best_ms = ms;
+          best.trace_kernel = k;
+          best.dda_batch = db;
         }
+      }
+    for (uint32_t pk : {1u, 5u})
+      for (uint32_t db : {1u, 16u}) {
+        if (pk == 1u && db != 1u) continue;
+        for (uint32_t b : blocks)
+          for (uint32_t st : steps)
+            for (uint32_t rf : refill) {
+              ctx->tuning.trace_kernel = pk;
+              ctx->tuning.trace_blocks = b;
+              ctx->tuning.trace_steps = st;
+              ctx->tuning.trace_min_refill = rf;
+              ctx->tuning.dda_batch = db;
+              rc = time_it([&] { return ctx->launch_trace(s, (const float*)ctx->scratch_rays.ptr, n, 1e4f,
+                                                          ctx->scratch_hits.ptr); }, &ms);
+              if (rc) return rc;
+              if (ms < best_ms) {
+                best_ms = ms;
+                best.trace_kernel = pk;
+                best.trace_blocks = b;
+                best.trace_steps = st;
+                best.trace_min_refill = rf;
+                best.dda_batch = db;
+              }
+            }
+      }
   }
   if (u != nullptr) {
     // path tracer on the given view (its size, pose and bounce count)
@@ -1126,7 +1141,7 @@ int vrt_ctx::render_batch(vrt::BatchLane& lane, const vrt_view* views, uint32_t 
     ViewState& vs = *states[k];
     const int parity = (int)(vs.frames & 1u);
     std::memset(&d[k], 0, sizeof(d[k]));
-    d[k].s = v.scene->dev();
+    d[k].s = tuned(v.scene->dev());
     d[k].u = v.u;
     d[k].fclamp = v.firefly_clamp;
     d[k].spp = v.spp;

@@ -35,6 +35,11 @@ struct SceneDev {
   int nx, ny, nz;
   int bgx, bgy, bgz;
   float voxel_size, inv_voxel;
+  // Single-loop DDA in the traversal kernels: lanes of a wave that reach an
+  // occupied brick wait until this many of them (or every lane still
+  // walking) wait, then enter their bricks together (vrt_dda_fast.h).
+  // 1 = enter at once.
+  uint32_t entry_batch;
   // Always 0.0f.  Passed at run time so that (zero - x) stays a real
   // subtraction: the gfx950 instruction selector folds a literal (0.0f - x)
   // feeding a multiply into a negated operand, which turns +0 into -0.
