@@ -263,7 +263,11 @@ typedef struct vrt_tuning {
                                  at occupied bricks before they enter them together
                                  (1..64; 0 = 1, enter at once).  The path tracer
                                  always enters at once. */
-  uint32_t reserved[3];
+  uint32_t empty_skip;        /* single-loop DDA (trace 4/5, pt 4): 1 = take the
+                                 coarse steps through runs of empty brick rows in
+                                 bulk (same results, fewer instructions on sparse
+                                 scenes), 0 = step by step */
+  uint32_t reserved[2];
 } vrt_tuning;
 int vrt_tuning_get(vrt_ctx* ctx, vrt_tuning* out);
 int vrt_tuning_set(vrt_ctx* ctx, const vrt_tuning* in);

@@ -5,7 +5,9 @@
 // only; compiled by tests/test_dda_host.py.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -17,6 +19,7 @@ namespace {
 
 struct HostScene {
   std::vector<uint32_t> bricks, pool, bits, prefix, pbits, pprefix;
+  std::vector<uint8_t> pdist;
   vrt::SceneDev dev{};
 };
 
@@ -75,9 +78,33 @@ void build(const uint32_t* vox, int nx, int ny, int nz, HostScene& h) {
         }
   h.pprefix.assign(pw, 0u);
   for (uint64_t w = 1; w < pw; ++w) h.pprefix[w] = h.pprefix[w - 1] + (uint32_t)__builtin_popcount(h.pbits[w - 1]);
+  // row distances: Chebyshev distance over (by, bz) from each row of bricks to
+  // the nearest row holding an occupied brick (brute force), then per padded
+  // bitmap word the minimum over the grid rows it covers
+  std::vector<int> rocc((size_t)bgy * bgz, 0), rdist((size_t)bgy * bgz, 255);
+  for (int bz = 0; bz < bgz; ++bz)
+    for (int by = 0; by < bgy; ++by)
+      for (int bx = 0; bx < bgx; ++bx)
+        if (h.bricks[(uint64_t)bx + (uint64_t)by * bgx + (uint64_t)bz * bgx * bgy]) rocc[by + bz * bgy] = 1;
+  for (int bz = 0; bz < bgz; ++bz)
+    for (int by = 0; by < bgy; ++by)
+      for (int z = 0; z < bgz; ++z)
+        for (int y = 0; y < bgy; ++y)
+          if (rocc[y + z * bgy]) {
+            const int dd = std::max(std::abs(y - by), std::abs(z - bz));
+            if (dd < rdist[by + bz * bgy]) rdist[by + bz * bgy] = dd;
+          }
+  h.pdist.assign((pw + 3) & ~3ull, 255u);
+  for (uint64_t w = 0; w < pw; ++w)
+    for (uint64_t i = w * 32; i < w * 32 + 32; ++i) {
+      const uint64_t y = (i >> sh1) & ((1ull << (sh2 - sh1)) - 1), z = i >> sh2;
+      if (y < (uint64_t)bgy && z < (uint64_t)bgz)
+        h.pdist[w] = (uint8_t)std::min<int>(h.pdist[w], rdist[y + z * bgy]);
+    }
   vrt::SceneDev& s = h.dev;
   s.pbits = h.pbits.data();
   s.pprefix = h.pprefix.data();
+  s.pdist = h.pdist.data();
   s.pwords = (uint32_t)pw;
   s.psh1 = sh1;
   s.psh2 = sh2;
@@ -101,23 +128,28 @@ void build(const uint32_t* vox, int nx, int ny, int nz, HostScene& h) {
 }  // namespace
 
 // variant: 0 nested (HBM), 1 flat, 2 nested with LDS occupancy, 3 single loop
-// with both levels in LDS.  hits: n records of 4 u32 (t bits, voxel,
-// brick_steps | voxel_steps << 16, face), as vrt_trace_rays writes them.
+// with both levels in LDS, 4 the same with the empty-row skip, 5 the batched
+// form of 4 (trace_fast<true>; one lane here).  hits: n records of 4 u32
+// (t bits, voxel, brick_steps | voxel_steps << 16, face), as vrt_trace_rays
+// writes them.  *iters: loop iterations (step() calls) of variants 3 and 4.
 extern "C" int dda_host_trace(const uint32_t* vox, int nx, int ny, int nz, const float* rays, uint64_t n,
-                              float tmax, int variant, uint32_t* hits) {
+                              float tmax, int variant, uint32_t* hits, uint64_t* iters) {
   if (nx % 8 || ny % 8 || nz % 8 || nx <= 0 || ny <= 0 || nz <= 0) return -1;
   HostScene h;
   build(vox, nx, ny, nz, h);
-  // one block's LDS as the kernels lay it out: 256 mask slots, bitmap, prefix
+  // one block's LDS as the kernels lay it out: 256 mask slots, bitmap, prefix,
+  // row distances
   const uint32_t pw = h.dev.pwords;
-  std::vector<uint32_t> lds(16 * 256 + 2 * (size_t)pw, 0u);
+  std::vector<uint32_t> lds(16 * 256 + 2 * (size_t)pw + (pw + 3) / 4, 0u);
   std::memcpy(lds.data() + 16 * 256, h.pbits.data(), pw * 4);
   std::memcpy(lds.data() + 16 * 256 + pw, h.pprefix.data(), pw * 4);
+  std::memcpy(lds.data() + 16 * 256 + 2 * pw, h.pdist.data(), h.pdist.size());
+  uint64_t nit = 0;
   const uint32_t lane = 37;  // any slot
   std::vector<uint32_t> lds2(16 * 256 + (size_t)h.dev.bit_words, 0u);
   std::memcpy(lds2.data() + 16 * 256, h.bits.data(), h.dev.bit_words * 4);
   vrt::LdsMem lm{lds2.data() + 16 * 256, lds2.data() + lane, 256u};
-  vrt::LdsScene ls{lds.data(), lane, 16u * 256u, 16u * 256u + pw};
+  vrt::LdsScene ls{lds.data(), lane, 16u * 256u, 16u * 256u + pw, 16u * 256u + 2u * pw};
   for (uint64_t i = 0; i < n; ++i) {
     const float* r = rays + 6 * i;
     const vrt::f3 ro = vrt::mk3(r[0], r[1], r[2]), rd = vrt::mk3(r[3], r[4], r[5]);
@@ -126,7 +158,24 @@ extern "C" int dda_host_trace(const uint32_t* vox, int nx, int ny, int nz, const
       case 0: vrt::trace(h.dev, ro, rd, tmax, o); break;
       case 1: vrt::trace_flat(h.dev, ro, rd, tmax, o); break;
       case 2: vrt::trace_m(h.dev, lm, ro, rd, tmax, o); break;
-      case 3: vrt::trace_fast(h.dev, ls, ro, rd, tmax, o); break;
+      case 3:
+      case 4: {
+        vrt::DdaFast st;
+        if (st.begin(h.dev, ls, ro, rd, tmax, o)) break;
+        uint32_t r;
+        for (;;) {
+          r = variant == 4 ? st.step<true>(h.dev, ls) : st.step(h.dev, ls);
+          ++nit;
+          if (r == vrt::DDA_ENTER) {
+            st.enter(h.dev, ls);
+            continue;
+          }
+          if (r != vrt::DDA_RUN) break;
+        }
+        st.finish(h.dev, r, o);
+        break;
+      }
+      case 5: vrt::trace_fast<true, true>(h.dev, ls, ro, rd, tmax, o); break;
       default: return -1;
     }
     uint32_t tb;
@@ -136,5 +185,6 @@ extern "C" int dda_host_trace(const uint32_t* vox, int nx, int ny, int nz, const
     hits[4 * i + 2] = (o.bsteps & 0xffffu) | (o.vsteps << 16);
     hits[4 * i + 3] = o.face;
   }
+  if (iters) *iters = nit;
   return 0;
 }

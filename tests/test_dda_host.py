@@ -1,9 +1,13 @@
 """The library's DDA variants run on the CPU (a host build of the device
 headers, tests/dda_host.cpp) against the oracle: nested loops, single-stream,
-LDS-staged occupancy and the single-loop LDS form must return the same hit
-voxel, face, both step counts and the bits of t for every ray.  This checks
-the variants' logic without a GPU (the GPU tests check the same on the
-device)."""
+LDS-staged occupancy and the single-loop LDS form (step by step, with the
+bulk skip of empty brick rows, and its wave-batched form) must return the
+same hit voxel, face, both step counts and the bits of t for every ray.  This
+checks the variants' logic without a GPU (the GPU tests check the same on
+the device).  Sparse scenes (isolated blocks, a floor, an empty grid, grids
+whose padded bitmap words cover several rows) make the skip take long runs;
+a test checks that it does (fewer loop iterations) so that the equality
+checks are not vacuous."""
 import ctypes as C
 import os
 import subprocess
@@ -14,9 +18,10 @@ import pytest
 import oracle
 import voxelrt_amd as v
 from voxelrt_amd import configs as cf
+from tests.sparse import sparse_rays as _sparse_rays, sparse_scenes as _sparse_scenes
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-VARIANTS = {"nested": 0, "flat": 1, "lds": 2, "fast": 3}
+VARIANTS = {"nested": 0, "flat": 1, "lds": 2, "fast": 3, "fast_skip": 4, "fast_batched_skip": 5}
 
 
 @pytest.fixture(scope="module")
@@ -29,17 +34,20 @@ def dda(tmp_path_factory):
     subprocess.run(cmd, check=True, capture_output=True)
     L = C.CDLL(out)
     L.dda_host_trace.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_uint64, C.c_float,
-                                 C.c_int, C.c_void_p]
+                                 C.c_int, C.c_void_p, C.c_void_p]
     L.dda_host_trace.restype = C.c_int
 
-    def run(vox, rays, variant, tmax=1e4):
+    def run(vox, rays, variant, tmax=1e4, iters=None):
         nz, ny, nx = vox.shape
         vox = np.ascontiguousarray(vox, np.uint32)
         rays = np.ascontiguousarray(rays, np.float32)
         hits = np.zeros(len(rays), v.HIT_DTYPE)
+        it = np.zeros(1, np.uint64)
         rc = L.dda_host_trace(vox.ctypes.data, nx, ny, nz, rays.ctypes.data, len(rays), tmax, variant,
-                              hits.ctypes.data)
+                              hits.ctypes.data, it.ctypes.data)
         assert rc == 0
+        if iters is not None:
+            iters.append(int(it[0]))
         return hits
     return run
 
@@ -91,3 +99,30 @@ def test_variants_on_world_scene_with_short_tmax(dda, variant):
     for tmax in (1e4, 0.75):
         _equal(dda(vox, rays, VARIANTS[variant], tmax), oracle.trace_rays(og, rays, tmax=tmax),
                f"{variant} tmax={tmax}")
+
+
+@pytest.mark.parametrize("variant", ["fast", "fast_skip", "fast_batched_skip"])
+@pytest.mark.parametrize("name", [s[0] for s in _sparse_scenes()])
+def test_empty_row_skip_matches_oracle(dda, variant, name):
+    vox = dict(_sparse_scenes())[name]
+    rays = np.concatenate([_sparse_rays(vox, 6000, 11), _edge_rays(max(vox.shape))])
+    og = oracle.OracleGrid(vox)
+    for tmax in (1e4, 1.3):
+        _equal(dda(vox, rays, VARIANTS[variant], tmax), oracle.trace_rays(og, rays, tmax=tmax),
+               f"{variant} {name} tmax={tmax}")
+
+
+def test_empty_row_skip_takes_runs(dda):
+    """The skip really replaces coarse iterations on sparse scenes (and never
+    adds any; none on a full grid, where no row is empty)."""
+    most = {"blocks": 0.7, "floor": 0.8, "empty": 0.7, "wide": 0.97}
+    for name, vox in _sparse_scenes():
+        rays = _sparse_rays(vox, 3000, 5)
+        a, b = [], []
+        dda(vox, rays, VARIANTS["fast"], iters=a)
+        dda(vox, rays, VARIANTS["fast_skip"], iters=b)
+        assert b[0] <= a[0], (name, a, b)
+        if name == "full":
+            assert b[0] == a[0], (name, a, b)
+        if name in most:
+            assert b[0] < most[name] * a[0], (name, a, b)

@@ -13,6 +13,7 @@ import oracle
 import voxelrt_amd as v
 from voxelrt_amd import configs as cf
 from tests.chain import OracleChain, assert_bits_equal, f16, to_oracle_uniforms
+from tests.sparse import sparse_rays, sparse_scenes
 
 pytestmark = pytest.mark.gpu
 
@@ -76,17 +77,21 @@ def traverse(request, ctx):
 
 TRACE_VARIANTS = [dict(trace_kernel=0), dict(trace_kernel=2), dict(trace_kernel=3), dict(trace_kernel=4),
                   dict(trace_kernel=4, dda_batch=8), dict(trace_kernel=4, dda_batch=64),
+                  dict(trace_kernel=4, empty_skip=1), dict(trace_kernel=4, dda_batch=16, empty_skip=1),
                   dict(trace_kernel=1, trace_blocks=2048, trace_steps=8, trace_min_refill=16),
                   dict(trace_kernel=1, trace_blocks=512, trace_steps=1, trace_min_refill=1),
                   dict(trace_kernel=1, trace_blocks=4096, trace_steps=16, trace_min_refill=64),
                   dict(trace_kernel=5, trace_blocks=2048, trace_steps=4, trace_min_refill=1),
                   dict(trace_kernel=5, trace_blocks=333, trace_steps=1, trace_min_refill=64),
-                  dict(trace_kernel=5, trace_blocks=1024, trace_steps=8, trace_min_refill=16, dda_batch=16)]
+                  dict(trace_kernel=5, trace_blocks=1024, trace_steps=8, trace_min_refill=16, dda_batch=16),
+                  dict(trace_kernel=5, trace_blocks=700, trace_steps=3, trace_min_refill=8, dda_batch=16,
+                       empty_skip=1)]
 
 
 @pytest.fixture(params=range(len(TRACE_VARIANTS)),
-                ids=["lane", "lane-flat", "lane-lds", "lane-fast", "lane-fast-b8", "lane-fast-b64", "persist",
-                     "persist-s1", "persist-s16", "persist-fast", "persist-fast-333", "persist-fast-b16"])
+                ids=["lane", "lane-flat", "lane-lds", "lane-fast", "lane-fast-b8", "lane-fast-b64", "lane-fast-skip",
+                     "lane-fast-b16-skip", "persist", "persist-s1", "persist-s16", "persist-fast", "persist-fast-333",
+                     "persist-fast-b16", "persist-fast-skip"])
 def trace_variant(request, ctx):
     old = ctx.tuning().as_dict()
     ctx.set_tuning(**TRACE_VARIANTS[request.param])
@@ -168,6 +173,19 @@ def test_trace_edge_cases(ctx, kind, tmax, trace_variant):
     sc.release()
 
 
+@pytest.mark.parametrize("name", [s[0] for s in sparse_scenes()])
+def test_trace_sparse_scenes(ctx, name, trace_variant):
+    """Scenes with long runs of empty brick rows (where the empty-row skip
+    takes its bulk steps), with rays from inside and outside the grid."""
+    vox = dict(sparse_scenes())[name]
+    sc = ctx.upload(vox)
+    rays = np.concatenate([sparse_rays(vox, 20000, 11), _edge_rays(max(vox.shape))])
+    og = oracle.OracleGrid(vox)
+    for tmax in (1e4, 1.3):
+        _hits_equal(ctx.trace_rays(sc, rays, tmax), oracle.trace_rays(og, rays, tmax), f"{name} tmax={tmax}")
+    sc.release()
+
+
 def test_autotune_keeps_results(ctx, traverse):
     p, vox, sc, og = traverse
     if p != 0.25:
@@ -194,6 +212,7 @@ def frame128(ctx):
 PT_VARIANTS = [dict(pt_kernel=0), dict(pt_kernel=0, pt_waves=5), dict(pt_kernel=2, pt_waves=4),
                dict(pt_kernel=3), dict(pt_kernel=3, pt_waves=4), dict(pt_kernel=3, pt_waves=5),
                dict(pt_kernel=4), dict(pt_kernel=4, pt_waves=5), dict(pt_kernel=5), dict(pt_kernel=5, pt_waves=5),
+               dict(pt_kernel=4, pt_waves=5, empty_skip=1), dict(pt_kernel=4, pt_waves=0, empty_skip=1),
                dict(pt_kernel=1, pt_blocks=1024, pt_steps=8, pt_min_refill=32, pt_chunk=64),
                dict(pt_kernel=1, pt_blocks=97, pt_steps=1, pt_min_refill=1, pt_chunk=64),
                dict(pt_kernel=1, pt_blocks=2048, pt_steps=16, pt_min_refill=64, pt_chunk=256)]
@@ -201,7 +220,8 @@ PT_VARIANTS = [dict(pt_kernel=0), dict(pt_kernel=0, pt_waves=5), dict(pt_kernel=
 
 @pytest.fixture(params=range(len(PT_VARIANTS)),
                 ids=["lane", "lane-w5", "lane-flat-w4", "lane-lds", "lane-lds-w4", "lane-lds-w5", "lane-fast",
-                     "lane-fast-w5", "lane-fast1", "lane-fast1-w5", "persist",
+                     "lane-fast-w5", "lane-fast1", "lane-fast1-w5", "lane-fast-w5-skip", "lane-fast-skip",
+                     "persist",
                      "persist-s1", "persist-s16"])
 def pt_variant(request, ctx):
     old = ctx.tuning().as_dict()
@@ -350,7 +370,7 @@ def test_frame_sequence_bitexact(ctx, pt_variant):
                                     dict(pt_kernel=2, pt_waves=4), dict(pt_kernel=4, pt_waves=5),
                                     dict(pt_kernel=4, pt_waves=5, batch_mode=1),
                                     dict(pt_kernel=0, pt_waves=0, batch_mode=1, batch_blocks=7),
-                                    dict(pt_kernel=4, pt_waves=5, batch_mode=2),
+                                    dict(pt_kernel=4, pt_waves=5, batch_mode=2, empty_skip=1),
                                     dict(pt_kernel=5, pt_waves=4, batch_mode=1),
                                     dict(pt_kernel=3, pt_waves=4, batch_mode=2)],
                          ids=["lane", "lds-w5", "flat-w4", "fast-w5", "queue-fast", "queue-7blocks",
@@ -425,9 +445,9 @@ def test_scene_edits_rebuild_on_device(ctx):
         assert np.array_equal(words != 0, ob != 0), k
         assert np.array_equal(masks, om), k
         og = oracle.OracleGrid(vox)
-        for tk in (0, 4):
+        for tk, skip in ((0, 0), (4, 0), (4, 1)):
             old = ctx.tuning().as_dict()
-            ctx.set_tuning(trace_kernel=tk)
+            ctx.set_tuning(trace_kernel=tk, empty_skip=skip)
             _hits_equal(ctx.trace_rays(sc, rays), oracle.trace_rays(og, rays), f"edit {k} trace_kernel {tk}")
             ctx.set_tuning(**old)
         pos, yaw, pitch = cf.world_orbit(desc, k, 64)

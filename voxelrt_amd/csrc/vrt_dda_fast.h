@@ -27,6 +27,21 @@
 // wait (a wave-uniform count, so the loop exit is a scalar branch) or no lane
 // is left stepping.  That changes when a lane enters, never what it computes.
 //
+// Empty rows (the Skip instantiations).  At an empty coarse cell whose bitmap
+// word has row distance D >= 2, every cell the ray can reach with at most
+// D - 1 steps along y and along z (and any number along x, inside the grid)
+// is empty.  skip_rows() then performs the coarse steps that stay inside that
+// region in bulk: with T below the t of the first step that could leave it
+// (and below tExit), the DDA takes, in some interleaving, exactly the steps
+// whose tMax is < T.  Their order does not change where they lead: each axis'
+// tMax runs through the same additions tMax += tDelta as one step at a time,
+// and the position, the last step's t and axis (largest t; on ties the
+// later axis, as the per-step order x, y, z gives), the cell counts, the
+// iteration budget and the step count follow from the per-axis step counts.
+// No cell among them is occupied, so no test inside the run could have ended
+// the ray, and the state after the run is the step-by-step state: results,
+// including step counts, are unchanged.
+//
 // Exactness: every t value comes from the same FP32 operations in the same
 // order as vrt::trace_m (and the oracle): tMaxB += |invRd|*8, tMaxV += |invRd|,
 // fine tMax recomputed from the origin on entry, ties x, then y, then z.  The
@@ -41,12 +56,14 @@ namespace vrt {
 
 // LDS of a 256-lane block: words [0, 4096) hold the lanes' mask slots
 // (word w of lane l at w * 256 + l), then the bitmap at bits_off, then its
-// prefix counts at prefix_off.
+// prefix counts at prefix_off, then the row distances (SceneDev::pdist, one
+// byte per bitmap word) at dist_off.
 struct LdsScene {
   uint32_t* lds;
   uint32_t lane;        // this lane's slot base (0..255)
   uint32_t bits_off;    // 4096
   uint32_t prefix_off;  // bits_off + SceneDev::pwords
+  uint32_t dist_off;    // prefix_off + SceneDev::pwords
 };
 
 // Cells left along each axis before leaving [0, n) from p moving by sd,
@@ -246,14 +263,69 @@ struct DdaFast {
     tBound = tExit;
   }
 
+  // Coarse level, at an empty cell with row distance D >= 1: take in bulk the
+  // steps whose tMax is below T (see the file comment).  Returns false when
+  // no step qualifies (the caller advances one step as usual).
+  VRT_D bool skip_rows(uint32_t D) {
+    const uint32_t c0 = cnt & 255u, c1 = (cnt >> 8) & 255u, c2 = (cnt >> 16) & 255u;
+    const uint32_t r1 = D - 1u < c1 ? D - 1u : c1, r2 = D - 1u < c2 ? D - 1u : c2;
+    if (iter <= c0 + r1 + r2) return false;  // iteration budget (never binds inside the grid)
+    // T: below the t of step r+1 along each moving axis (half a tDelta of
+    // margin over the rounding of r additions), and not beyond tExit
+    float T = tBound;
+    if (sd[0] != 0) T = vmin(T, tm[0] + ((float)c0 - 0.5f) * d[0]);
+    if (sd[1] != 0) T = vmin(T, tm[1] + ((float)r1 - 0.5f) * d[1]);
+    if (sd[2] != 0) T = vmin(T, tm[2] + ((float)r2 - 0.5f) * d[2]);
+    uint32_t k[3];
+    float last[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      uint32_t n = 0u;
+      float m = tm[c], p = 0.0f;
+      while (m < T) {
+        p = m;
+        m = m + d[c];
+        ++n;
+      }
+      tm[c] = m;
+      k[c] = n;
+      last[c] = p;
+    }
+    const uint32_t K = k[0] + k[1] + k[2];
+    if (K == 0u) return false;
+    // the last step: largest t, ties to the later axis
+    float tl = -1.0f;
+    uint32_t al = 0u;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      if (k[c] != 0u && last[c] >= tl) {
+        tl = last[c];
+        al = (uint32_t)c;
+      }
+    t = tl;
+    ax = al;
+    idx += (uint32_t)((int)k[0] * di[0] + (int)k[1] * di[1] + (int)k[2] * di[2]);
+    cnt -= (di[0] != 0 ? k[0] : 0u) | ((di[1] != 0 ? k[1] : 0u) << 8) | ((di[2] != 0 ? k[2] : 0u) << 16);
+    iter -= K;
+    steps += K - 1u;  // the cells passed over are tested (empty); the last one is tested next
+    return true;
+  }
+
   // One iteration: test the current cell, then advance.  Returns DDA_ENTER
   // (without advancing) at an occupied brick of the coarse level; the caller
-  // calls enter() and steps on.
+  // calls enter() and steps on.  Skip: take runs of empty rows in bulk.
+  template <bool Skip = false>
   VRT_D uint32_t step(const SceneDev& s, const LdsScene& L) {
     if (t > tBound) return DDA_MISS;
     const uint32_t word = L.lds[base + ((idx >> 5) << wshift)];
     steps += inc;
     if ((word >> (idx & 31u)) & 1u) return inc != 1u ? DDA_HIT : DDA_ENTER;  // voxel.wgsl:122 / 111
+    if constexpr (Skip) {
+      if (inc == 1u) {
+        const uint32_t D = (L.lds[L.dist_off + (idx >> 7)] >> (8u * ((idx >> 5) & 3u))) & 255u;
+        if (D >= 2u && skip_rows(D)) return DDA_RUN;  // runs that can also move in y and z
+      }
+    }
     if (!advance()) return DDA_RUN;
     if (inc == 1u) return DDA_MISS;  // coarse: left the grid / out of brick iterations
     // fine: leave the brick, back to the coarse DDA and its advance
@@ -265,23 +337,23 @@ struct DdaFast {
   // Walk to the end (DDA_HIT or DDA_MISS).  Batched: enter bricks in groups
   // of up to s.entry_batch lanes, and every lane of the wave that walks a ray
   // calls it; otherwise (and with entry_batch <= 1) enter at once.
-  template <bool Batched>
+  template <bool Batched, bool Skip>
   VRT_D uint32_t run(const SceneDev& s, const LdsScene& L) {
     uint32_t r = DDA_RUN;
     if (!Batched || s.entry_batch <= 1u) {  // enter at once: no wave-level bookkeeping
-      for (;;) {
-        r = step(s, L);
+      do {
+        r = step<Skip>(s, L);
         if (r == DDA_ENTER) {
           enter(s, L);
-          continue;
+          r = DDA_RUN;
         }
-        if (r != DDA_RUN) return r;
-      }
+      } while (r == DDA_RUN);
+      return r;
     }
     for (;;) {
       uint32_t waiting = 0u;  // lanes of the wave stopped at a brick (uniform)
       while (r == DDA_RUN) {
-        r = step(s, L);
+        r = step<Skip>(s, L);
         waiting += (uint32_t)__builtin_popcountll(wave_ballot(r == DDA_ENTER));
         if (waiting >= s.entry_batch) break;
       }
@@ -318,11 +390,11 @@ struct DdaFast {
   }
 };
 
-template <bool Batched = false>
+template <bool Batched = false, bool Skip = false>
 VRT_D void trace_fast(const SceneDev& s, const LdsScene& L, f3 roW, f3 rd, float maxTWorld, TraceResult& o) {
   DdaFast st;
   if (st.begin(s, L, roW, rd, maxTWorld, o)) return;
-  st.finish(s, st.run<Batched>(s, L), o);
+  st.finish(s, st.run<Batched, Skip>(s, L), o);
 }
 
 }  // namespace vrt

@@ -81,6 +81,7 @@ struct vrt_scene {
   vrt::DeviceBuffer d_bits;    // one occupancy bit per brick (LDS-staged by the kernels)
   vrt::DeviceBuffer d_prefix;  // per bitmap word: occupied bricks before it
   vrt::DeviceBuffer d_pbits, d_pprefix;  // the same over the power-of-two padded brick grid
+  vrt::DeviceBuffer d_pdist;             // row distance per padded bitmap word (bytes)
   vrt::DeviceBuffer d_dense;  // editable scenes: the dense packed grid, rebuilt from on edits
   uint32_t psh1 = 0, psh2 = 0, psh3 = 0;  // padded bit index = bx | by << psh1 | bz << psh2; 2^psh3 bits
   vrt::SceneDev dev() const;

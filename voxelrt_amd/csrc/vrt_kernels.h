@@ -12,10 +12,11 @@ using Uniforms = vrt_uniforms;
 struct SceneDev;
 
 // Trace variant ids for the one-ray/pixel-per-lane kernels (launch_* `tv`).
-constexpr int TV_NESTED = 0, TV_FLAT = 1, TV_LDS = 2, TV_FAST = 3, TV_FAST1 = 4;
+constexpr int TV_NESTED = 0, TV_FLAT = 1, TV_LDS = 2, TV_FAST = 3, TV_FAST1 = 4, TV_FASTSKIP = 5;
 // Variants that trace with the single-loop DDA on LDS (TV_FAST1: the same
 // DDA, path tracer with one trace call site per lane).
-constexpr bool tv_fast(int tv) { return tv == TV_FAST || tv == TV_FAST1; }
+// TV_FASTSKIP: the same DDA taking runs of empty brick rows in bulk.
+constexpr bool tv_fast(int tv) { return tv == TV_FAST || tv == TV_FAST1 || tv == TV_FASTSKIP; }
 constexpr bool tv_uses_lds(int tv) { return tv == TV_LDS || tv_fast(tv); }
 // Largest occupancy bitmap staged in LDS (words): 48 KiB, 393216 bricks
 // (512^3 = 262144 bricks fits).  Larger scenes fall back to TV_NESTED.
@@ -30,6 +31,10 @@ hipError_t launch_brick_bits(hipStream_t st, const uint32_t* bricks, uint64_t nb
 hipError_t launch_brick_pbits(hipStream_t st, const uint32_t* bricks, int bgx, int bgy, int bgz, uint32_t sh1,
                               uint32_t sh2, uint64_t nbits, uint32_t* bits);
 hipError_t launch_popc(hipStream_t st, const uint32_t* words, uint64_t n, uint32_t* counts);
+// Row distances of a scene (SceneDev::pdist) from its brick words; tmp holds
+// 3 * bgy * bgz bytes, pdist (pwords + 3) & ~3 bytes.
+hipError_t launch_row_dist(hipStream_t st, const uint32_t* bricks, int bgx, int bgy, int bgz, uint32_t sh1,
+                           uint32_t sh2, uint64_t pwords, uint8_t* tmp, uint8_t* pdist);
 hipError_t launch_pack_pool(hipStream_t st, const uint32_t* vox, int nx, int ny, int bgx, int bgy,
                             int bz0, int nbz, const uint32_t* masks, const uint32_t* bricks,
                             uint32_t* pool);
@@ -37,7 +42,7 @@ hipError_t launch_trace_rays(hipStream_t st, const SceneDev& s, const float* ray
                              float tmax, void* hits, unsigned long long* counters, int blocks, int tv);
 hipError_t launch_trace_persistent(hipStream_t st, const SceneDev& s, const float* rays, uint32_t n,
                                   float tmax, void* hits, unsigned long long* counters,
-                                  uint32_t* queue, int blocks, int steps, int min_refill, bool fast);
+                                  uint32_t* queue, int blocks, int steps, int min_refill, int tv);
 hipError_t launch_pathtrace(hipStream_t st, const SceneDev& s, const Uniforms& u, float fclamp,
                             uint32_t spp, void* radiance, void* albedo, void* gbuf,
                             unsigned long long* counters, int tv, int waves);

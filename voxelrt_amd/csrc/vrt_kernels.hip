@@ -125,6 +125,63 @@ __global__ __launch_bounds__(256) void k_popc(const uint32_t* __restrict__ words
   if (i < n) counts[i] = (uint32_t)__popc(words[i]);
 }
 
+// Row distances (SceneDev::pdist).  A row is the line of bricks with fixed
+// (by, bz), index r = by + bz * bgy.  occ[r]: some brick of row r is occupied.
+__global__ __launch_bounds__(256) void k_row_occ(const uint32_t* __restrict__ bricks, int bgx, int bgy, int bgz,
+                                                 uint8_t* __restrict__ occ) {
+  const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= (uint64_t)bgy * (uint64_t)bgz) return;
+  const uint32_t* row = bricks + r * (uint64_t)bgx;  // bricks bx + by*bgx + bz*bgx*bgy = bx + r*bgx
+  uint32_t o = 0u;
+  for (int x = 0; x < bgx; ++x) o |= row[x];
+  occ[r] = o != 0u ? 1 : 0;
+}
+
+// dy[r]: distance along y, within the row's z, to the nearest occupied row
+// (255 when there is none or it is farther).
+__global__ __launch_bounds__(256) void k_row_dy(const uint8_t* __restrict__ occ, int bgy, int bgz,
+                                                uint8_t* __restrict__ dy) {
+  const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= (uint64_t)bgy * (uint64_t)bgz) return;
+  const int by = (int)(r % (uint64_t)bgy);
+  const uint8_t* col = occ + (r - (uint64_t)by);
+  int best = 255;
+  for (int y = 0; y < bgy; ++y)
+    if (col[y]) best = min(best, y > by ? y - by : by - y);
+  dy[r] = (uint8_t)best;
+}
+
+// dist[r] = min over z' of max(|bz - z'|, dy[by, z']): the Chebyshev
+// distance over (y, z) to the nearest occupied row, capped at 255.
+__global__ __launch_bounds__(256) void k_row_dist(const uint8_t* __restrict__ dy, int bgy, int bgz,
+                                                  uint8_t* __restrict__ dist) {
+  const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= (uint64_t)bgy * (uint64_t)bgz) return;
+  const int by = (int)(r % (uint64_t)bgy), bz = (int)(r / (uint64_t)bgy);
+  int best = 255;
+  for (int z = 0; z < bgz; ++z) best = min(best, max(z > bz ? z - bz : bz - z, (int)dy[(uint64_t)by + (uint64_t)z * bgy]));
+  dist[r] = (uint8_t)best;
+}
+
+// pdist[w] = min of dist over the grid's rows that padded bitmap word w
+// covers (255 for a word of padding only); nw is pwords rounded up to 4.
+__global__ __launch_bounds__(256) void k_word_dist(const uint8_t* __restrict__ dist, int bgy, int bgz,
+                                                   uint32_t sh1, uint32_t sh2, uint64_t pwords, uint64_t nw,
+                                                   uint8_t* __restrict__ pdist) {
+  const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= nw) return;
+  int best = 255;
+  if (w < pwords) {
+    const uint64_t r0 = (w * 32u) >> sh1, r1 = (w * 32u + 31u) >> sh1;
+    const uint64_t ym = (1ull << (sh2 - sh1)) - 1ull;
+    for (uint64_t r = r0; r <= r1; ++r) {
+      const uint64_t y = r & ym, z = r >> (sh2 - sh1);
+      if (y < (uint64_t)bgy && z < (uint64_t)bgz) best = min(best, (int)dist[y + z * (uint64_t)bgy]);
+    }
+  }
+  pdist[w] = (uint8_t)best;
+}
+
 // One wave per brick of the slab; occupied bricks copy mask + voxels.
 __global__ __launch_bounds__(256) void k_pack_pool(const uint32_t* __restrict__ vox, int nx, int ny,
                                                    int bgx, int bgy, int bz0, int nbz,
@@ -162,6 +219,7 @@ __global__ __launch_bounds__(256) void k_pack_pool(const uint32_t* __restrict__ 
 //   TV_FAST    single-loop DDA with both levels in LDS (vrt_dda_fast.h):
 //              bitmap + prefix counts + per-lane mask slot
 //   TV_FAST1   TV_FAST traversal; the path tracer uses shade_pixel_1
+//   TV_FASTSKIP TV_FAST with the bulk skip of empty brick rows
 // What a lane needs to trace in a given variant.
 struct LaneMem {
   LdsMem m;    // TV_LDS
@@ -174,6 +232,8 @@ VRT_D void trace_sel(const SceneDev& s, const LaneMem& L, f3 ro, f3 rd, float tm
     trace_flat(s, ro, rd, tmax, h);
   else if constexpr (TV == TV_LDS)
     trace_m(s, L.m, ro, rd, tmax, h);
+  else if constexpr (TV == TV_FASTSKIP)
+    trace_fast<false, true>(s, L.f, ro, rd, tmax, h);
   else if constexpr (tv_fast(TV))
     trace_fast(s, L.f, ro, rd, tmax, h);
   else
@@ -186,7 +246,7 @@ VRT_D void trace_sel(const SceneDev& s, const LaneMem& L, f3 ro, f3 rd, float tm
 // block must call this.
 template <int TV>
 VRT_D LaneMem lds_setup(const SceneDev& s, uint32_t* lds) {
-  LaneMem L{{nullptr, nullptr, 0u}, {nullptr, 0u, 0u, 0u}};
+  LaneMem L{{nullptr, nullptr, 0u}, {nullptr, 0u, 0u, 0u, 0u}};
   if constexpr (TV == TV_LDS) {
     uint32_t* bits = lds + 16u * 256u;
     for (uint32_t i = threadIdx.x; i < s.bit_words; i += blockDim.x) bits[i] = s.bits[i];
@@ -201,11 +261,14 @@ VRT_D LaneMem lds_setup(const SceneDev& s, uint32_t* lds) {
       bits[i] = s.pbits[i];
       bits[s.pwords + i] = s.pprefix[i];
     }
+    const uint32_t dw = (s.pwords + 3u) / 4u;
+    for (uint32_t i = threadIdx.x; i < dw; i += blockDim.x) bits[2u * s.pwords + i] = ((const uint32_t*)s.pdist)[i];
     __syncthreads();
     L.f.lds = lds;
     L.f.lane = threadIdx.x;
     L.f.bits_off = 16u * 256u;
     L.f.prefix_off = 16u * 256u + s.pwords;
+    L.f.dist_off = 16u * 256u + 2u * s.pwords;
   }
   return L;
 }
@@ -229,7 +292,7 @@ __global__ __launch_bounds__(256) void k_trace_rays(SceneDev s, const float* __r
     const float* r = rays + i * 6;
     TraceResult h;
     if constexpr (tv_fast(TV))  // wave-batched brick entries (SceneDev::entry_batch)
-      trace_fast<true>(s, L.f, mk3(r[0], r[1], r[2]), mk3(r[3], r[4], r[5]), tmax, h);
+      trace_fast<true, TV == TV_FASTSKIP>(s, L.f, mk3(r[0], r[1], r[2]), mk3(r[3], r[4], r[5]), tmax, h);
     else
       trace_sel<TV>(s, L, mk3(r[0], r[1], r[2]), mk3(r[3], r[4], r[5]), tmax, h);
     uint4 rec;
@@ -281,6 +344,7 @@ struct FlatRunner {
 // A lane that reaches an occupied brick waits (pend) until entry_batch lanes
 // of its wave wait, or every lane walking a ray does; settle(), called by the
 // whole wave after each step, lets them enter together.
+template <bool Skip>
 struct FastRunner {
   DdaFast st;
   bool pend = false;
@@ -289,7 +353,7 @@ struct FastRunner {
   }
   VRT_D bool step(const SceneDev& s, const LaneMem& L, TraceResult& o) {
     if (pend) return false;
-    const uint32_t r = st.step(s, L.f);
+    const uint32_t r = st.step<Skip>(s, L.f);
     if (r == DDA_RUN) return false;
     if (r == DDA_ENTER) {
       pend = true;
@@ -900,7 +964,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WV))) void 
   __shared__ uint32_t s_tile;
   const uint32_t tiles = head->tiles, nviews = head->nviews;
   const void* loaded = nullptr;  // scene whose occupancy is in LDS
-  LaneMem L{{nullptr, nullptr, 0u}, {nullptr, 0u, 0u, 0u}};
+  LaneMem L{{nullptr, nullptr, 0u}, {nullptr, 0u, 0u, 0u, 0u}};
   for (;;) {
     __syncthreads();  // every lane is done with the previous tile (s_tile, LDS)
     if (threadIdx.x == 0u) s_tile = atomicAdd(&head->queue, 1u);
@@ -1059,9 +1123,9 @@ static inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b
 // TV_LDS launches: 16 mask words per lane (256-thread blocks) + the bitmap,
 // kept within 64 KiB of LDS per block.
 // LDS words a variant stages beyond the mask slots: TV_LDS the bitmap,
-// TV_FAST the padded bitmap and its prefix counts.
+// TV_FAST the padded bitmap, its prefix counts and the row distances (bytes).
 static inline uint32_t lds_words(int tv, uint32_t bit_words, uint32_t pwords) {
-  return tv_fast(tv) ? 2u * pwords : (tv == TV_LDS ? bit_words : 0u);
+  return tv_fast(tv) ? 2u * pwords + (pwords + 3u) / 4u : (tv == TV_LDS ? bit_words : 0u);
 }
 static inline uint32_t lds_bytes(int tv, uint32_t bit_words, uint32_t pwords) {
   return tv_uses_lds(tv) ? (16u * 256u + lds_words(tv, bit_words, pwords)) * 4u : 0u;
@@ -1099,6 +1163,18 @@ hipError_t launch_brick_pbits(hipStream_t st, const uint32_t* bricks, int bgx, i
   return hipGetLastError();
 }
 
+hipError_t launch_row_dist(hipStream_t st, const uint32_t* bricks, int bgx, int bgy, int bgz, uint32_t sh1,
+                           uint32_t sh2, uint64_t pwords, uint8_t* tmp, uint8_t* pdist) {
+  const uint64_t rows = (uint64_t)bgy * (uint64_t)bgz, nw = (pwords + 3u) & ~3ull;
+  uint8_t *occ = tmp, *dy = tmp + rows, *dist = tmp + 2 * rows;
+  hipLaunchKernelGGL(k_row_occ, dim3(div_up(rows, 256)), dim3(256), 0, st, bricks, bgx, bgy, bgz, occ);
+  hipLaunchKernelGGL(k_row_dy, dim3(div_up(rows, 256)), dim3(256), 0, st, occ, bgy, bgz, dy);
+  hipLaunchKernelGGL(k_row_dist, dim3(div_up(rows, 256)), dim3(256), 0, st, dy, bgy, bgz, dist);
+  hipLaunchKernelGGL(k_word_dist, dim3(div_up(nw, 256)), dim3(256), 0, st, dist, bgy, bgz, sh1, sh2, pwords, nw,
+                     pdist);
+  return hipGetLastError();
+}
+
 hipError_t launch_popc(hipStream_t st, const uint32_t* words, uint64_t n, uint32_t* counts) {
   hipLaunchKernelGGL(k_popc, dim3(div_up(n, 256)), dim3(256), 0, st, words, n, counts);
   return hipGetLastError();
@@ -1123,6 +1199,9 @@ hipError_t launch_trace_rays(hipStream_t st, const SceneDev& s, const float* ray
   const uint32_t shmem = lds_bytes(tv, s.bit_words, s.pwords);
   if (tv == TV_FAST)
     hipLaunchKernelGGL(k_trace_rays<TV_FAST>, g, dim3(256), shmem, st, s, rays, n, tmax, (uint4*)hits, counters);
+  else if (tv == TV_FASTSKIP)
+    hipLaunchKernelGGL(k_trace_rays<TV_FASTSKIP>, g, dim3(256), shmem, st, s, rays, n, tmax, (uint4*)hits,
+                       counters);
   else if (tv == TV_LDS)
     hipLaunchKernelGGL(k_trace_rays<TV_LDS>, g, dim3(256), shmem, st, s, rays, n, tmax, (uint4*)hits, counters);
   else if (tv == TV_FLAT)
@@ -1134,14 +1213,19 @@ hipError_t launch_trace_rays(hipStream_t st, const SceneDev& s, const float* ray
 
 hipError_t launch_trace_persistent(hipStream_t st, const SceneDev& s, const float* rays, uint32_t n,
                                   float tmax, void* hits, unsigned long long* counters,
-                                  uint32_t* queue, int blocks, int steps, int min_refill, bool fast) {
+                                  uint32_t* queue, int blocks, int steps, int min_refill, int tv) {
   hipError_t e = hipMemsetAsync(queue, 0, sizeof(uint32_t), st);
   if (e != hipSuccess) return e;
   const dim3 g(blocks > 0 ? blocks : 1);
   const int sp = steps < 1 ? 1 : steps, mr = min_refill < 1 ? 1 : min_refill;
-  if (fast && lds_variant(TV_FAST, s.bit_words, s.pwords) == TV_FAST)
-    hipLaunchKernelGGL((k_trace_persistent<FastRunner, TV_FAST>), g, dim3(256), lds_bytes(TV_FAST, s.bit_words, s.pwords),
-                       st, s, rays, n, tmax, (uint4*)hits, counters, queue, sp, mr);
+  tv = tv_fast(tv) ? lds_variant(tv, s.bit_words, s.pwords) : TV_NESTED;
+  const uint32_t shmem = lds_bytes(tv, s.bit_words, s.pwords);
+  if (tv == TV_FASTSKIP)
+    hipLaunchKernelGGL((k_trace_persistent<FastRunner<true>, TV_FASTSKIP>), g, dim3(256), shmem, st, s, rays, n,
+                       tmax, (uint4*)hits, counters, queue, sp, mr);
+  else if (tv_fast(tv))
+    hipLaunchKernelGGL((k_trace_persistent<FastRunner<false>, TV_FAST>), g, dim3(256), shmem, st, s, rays, n, tmax,
+                       (uint4*)hits, counters, queue, sp, mr);
   else
     hipLaunchKernelGGL((k_trace_persistent<FlatRunner, TV_NESTED>), g, dim3(256), 0, st, s, rays, n, tmax,
                        (uint4*)hits, counters, queue, sp, mr);
@@ -1163,6 +1247,9 @@ hipError_t launch_pathtrace(hipStream_t st, const SceneDev& s, const Uniforms& u
     case TV_FAST1 * 8 + 4: go(k_pathtrace<TV_FAST1, 4>); break;
     case TV_FAST1 * 8 + 5: go(k_pathtrace<TV_FAST1, 5>); break;
     case TV_FAST1 * 8 + 1: go(k_pathtrace<TV_FAST1, 1>); break;
+    case TV_FASTSKIP * 8 + 4: go(k_pathtrace<TV_FASTSKIP, 4>); break;
+    case TV_FASTSKIP * 8 + 5: go(k_pathtrace<TV_FASTSKIP, 5>); break;
+    case TV_FASTSKIP * 8 + 1: go(k_pathtrace<TV_FASTSKIP, 1>); break;
     case TV_FAST * 8 + 4: go(k_pathtrace<TV_FAST, 4>); break;
     case TV_FAST * 8 + 5: go(k_pathtrace<TV_FAST, 5>); break;
     case TV_FAST * 8 + 1: go(k_pathtrace<TV_FAST, 1>); break;
@@ -1204,6 +1291,9 @@ hipError_t launch_pathtrace_batch(hipStream_t st, const ViewDesc* views, uint32_
     case TV_FAST1 * 8 + 4: go(k_pathtrace_batch<TV_FAST1, 4>); break;
     case TV_FAST1 * 8 + 5: go(k_pathtrace_batch<TV_FAST1, 5>); break;
     case TV_FAST1 * 8 + 1: go(k_pathtrace_batch<TV_FAST1, 1>); break;
+    case TV_FASTSKIP * 8 + 4: go(k_pathtrace_batch<TV_FASTSKIP, 4>); break;
+    case TV_FASTSKIP * 8 + 5: go(k_pathtrace_batch<TV_FASTSKIP, 5>); break;
+    case TV_FASTSKIP * 8 + 1: go(k_pathtrace_batch<TV_FASTSKIP, 1>); break;
     case TV_FAST * 8 + 4: go(k_pathtrace_batch<TV_FAST, 4>); break;
     case TV_FAST * 8 + 5: go(k_pathtrace_batch<TV_FAST, 5>); break;
     case TV_FAST * 8 + 1: go(k_pathtrace_batch<TV_FAST, 1>); break;
@@ -1232,6 +1322,9 @@ hipError_t launch_batch_persistent(hipStream_t st, BatchHead* head, const ViewDe
     case TV_FAST1 * 8 + 4: go(k_pathtrace_batch_q<TV_FAST1, 4>); break;
     case TV_FAST1 * 8 + 5: go(k_pathtrace_batch_q<TV_FAST1, 5>); break;
     case TV_FAST1 * 8 + 1: go(k_pathtrace_batch_q<TV_FAST1, 1>); break;
+    case TV_FASTSKIP * 8 + 4: go(k_pathtrace_batch_q<TV_FASTSKIP, 4>); break;
+    case TV_FASTSKIP * 8 + 5: go(k_pathtrace_batch_q<TV_FASTSKIP, 5>); break;
+    case TV_FASTSKIP * 8 + 1: go(k_pathtrace_batch_q<TV_FASTSKIP, 1>); break;
     case TV_FAST * 8 + 4: go(k_pathtrace_batch_q<TV_FAST, 4>); break;
     case TV_FAST * 8 + 5: go(k_pathtrace_batch_q<TV_FAST, 5>); break;
     case TV_FAST * 8 + 1: go(k_pathtrace_batch_q<TV_FAST, 1>); break;

@@ -117,6 +117,7 @@ vrt::SceneDev vrt_scene::dev() const {
   s.prefix = (const uint32_t*)d_prefix.ptr;
   s.pbits = (const uint32_t*)d_pbits.ptr;
   s.pprefix = (const uint32_t*)d_pprefix.ptr;
+  s.pdist = (const uint8_t*)d_pdist.ptr;
   s.pwords = psh3 >= 5 ? (1u << (psh3 - 5)) : 1u;
   s.psh1 = psh1;
   s.psh2 = psh2;
@@ -333,8 +334,9 @@ static int scene_build(vrt_ctx* ctx, vrt_scene* sc, const uint32_t* hvox, const 
   // editable scenes keep 25 % headroom in the pool so that edits rarely reallocate
   const uint64_t pool_bytes = sc->d_pool.bytes >= pool_need ? sc->d_pool.bytes
                               : (sc->d_dense.ptr ? pool_need + pool_need / 4 : pool_need);
+  const uint64_t pdist_bytes = (((pbits_n + 31) / 32) + 3) & ~3ull;
   const uint64_t bytes = sc->d_dense.bytes + nb * 4 + pool_bytes + (nb + 63) / 64 * 8 + (nb + 31) / 32 * 4 +
-                         (pbits_n + 63) / 64 * 8 + (pbits_n + 31) / 32 * 4;
+                         (pbits_n + 63) / 64 * 8 + (pbits_n + 31) / 32 * 4 + pdist_bytes;
   if (bytes > sc->bytes) {
     int rc = ctx->reserve(bytes - sc->bytes);
     if (rc) return rc;
@@ -347,6 +349,7 @@ static int scene_build(vrt_ctx* ctx, vrt_scene* sc, const uint32_t* hvox, const 
   if (e == hipSuccess && !sc->d_prefix.ptr) e = sc->d_prefix.alloc((nb + 31) / 32 * 4);
   if (e == hipSuccess && !sc->d_pbits.ptr) e = sc->d_pbits.alloc(((pbits_n + 63) / 64) * 8);
   if (e == hipSuccess && !sc->d_pprefix.ptr) e = sc->d_pprefix.alloc((pbits_n + 31) / 32 * 4);
+  if (e == hipSuccess && !sc->d_pdist.ptr) e = sc->d_pdist.alloc(pdist_bytes);
   if (e != hipSuccess)
     return ctx->fail(e == hipErrorOutOfMemory ? VRT_ERR_NO_MEMORY : VRT_ERR_DEVICE, "scene allocation failed: %s",
                      hipGetErrorString(e));
@@ -384,6 +387,14 @@ static int scene_build(vrt_ctx* ctx, vrt_scene* sc, const uint32_t* hvox, const 
       e = vrt::exclusive_scan_u32(ctx->stream, tmp.ptr, &need, (const uint32_t*)flags.ptr,
                                   (uint32_t*)sc->d_pprefix.ptr, pw);
     if (e != hipSuccess) return err("padded prefix", e);
+    // row distances for the empty-row skip of the single-loop DDA
+    vrt::DeviceBuffer rows;
+    e = rows.alloc(3ull * (uint64_t)bgy * (uint64_t)bgz);
+    if (e == hipSuccess)
+      e = vrt::launch_row_dist(ctx->stream, (const uint32_t*)sc->d_bricks.ptr, bgx, bgy, bgz, sc->psh1, sc->psh2, pw,
+                               (uint8_t*)rows.ptr, (uint8_t*)sc->d_pdist.ptr);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);  // before rows is freed
+    if (e != hipSuccess) return err("row distances", e);
   }
   // pass 2: pack occupied bricks into the pool
   for (int bz0 = 0; bz0 < bgz; bz0 += slab) {
@@ -537,12 +548,13 @@ int vrt_scene_bricks(vrt_ctx* ctx, const vrt_scene* scene, uint32_t* bricks, uin
 // ---------------------------------------------------------------------------
 // vrt_tuning kernel id -> trace variant of the one-per-lane kernels:
 // 0 nested, 2 single-stream (flat), 3 LDS-staged occupancy, 4 single loop
-// with both levels in LDS, 5 as 4 with one trace call site in the path tracer.
-static int lane_variant(uint32_t kernel) {
+// with both levels in LDS (with the empty-row skip when skip is set), 5 as 4
+// with one trace call site in the path tracer.
+static int lane_variant(uint32_t kernel, uint32_t skip) {
   switch (kernel) {
     case 2: return vrt::TV_FLAT;
     case 3: return vrt::TV_LDS;
-    case 4: return vrt::TV_FAST;
+    case 4: return skip ? vrt::TV_FASTSKIP : vrt::TV_FAST;
     case 5: return vrt::TV_FAST1;
     default: return vrt::TV_NESTED;
   }
@@ -557,7 +569,7 @@ hipError_t vrt_ctx::launch_pathtrace(const vrt::SceneDev& s0, const vrt_uniforms
                                             (int)tuning.pt_blocks, (int)tuning.pt_steps,
                                             (int)tuning.pt_min_refill, (int)tuning.pt_chunk);
   return vrt::launch_pathtrace(stream, s, u, fclamp, spp, radiance, albedo, gbuf,
-                               (unsigned long long*)d_counters.ptr, lane_variant(tuning.pt_kernel),
+                               (unsigned long long*)d_counters.ptr, lane_variant(tuning.pt_kernel, tuning.empty_skip),
                                (int)tuning.pt_waves);
 }
 
@@ -568,9 +580,11 @@ hipError_t vrt_ctx::launch_trace(const vrt::SceneDev& s0, const float* rays, uin
     return vrt::launch_trace_persistent(stream, s, rays, (uint32_t)n, tmax, hits,
                                         (unsigned long long*)d_counters.ptr, (uint32_t*)d_queue.ptr,
                                         (int)tuning.trace_blocks, (int)tuning.trace_steps,
-                                        (int)tuning.trace_min_refill, tuning.trace_kernel == 5);
+                                        (int)tuning.trace_min_refill,
+                                        tuning.trace_kernel != 5 ? vrt::TV_NESTED
+                                        : (tuning.empty_skip ? vrt::TV_FASTSKIP : vrt::TV_FAST));
   return vrt::launch_trace_rays(stream, s, rays, n, tmax, hits, (unsigned long long*)d_counters.ptr,
-                                trace_blocks, lane_variant(tuning.trace_kernel));
+                                trace_blocks, lane_variant(tuning.trace_kernel, tuning.empty_skip));
 }
 
 int vrt_trace_rays_device(vrt_ctx* ctx, const vrt_scene* scene, const float* d_rays, uint64_t n,
@@ -852,7 +866,8 @@ int vrt_tuning_set(vrt_ctx* ctx, const vrt_tuning* in) {
       in->pt_blocks > (1u << 20) || in->trace_steps > 1024 || in->pt_steps > 1024 ||
       in->trace_min_refill > 64 || in->pt_min_refill > 64 || in->pt_chunk > (1u << 16) ||
       !(in->pt_waves == 0 || in->pt_waves == 4 || in->pt_waves == 5) || in->batch_mode > 2 ||
-      in->batch_blocks > (1u << 16) || in->dda_batch > 64)
+      in->batch_blocks > (1u << 16) || in->dda_batch > 64 ||
+      in->empty_skip > 1)
     return ctx->fail(VRT_ERR_INVALID, "tuning value out of range");
   ctx->tuning = *in;
   return VRT_OK;
@@ -1178,7 +1193,7 @@ int vrt_ctx::render_batch(vrt::BatchLane& lane, const vrt_view* views, uint32_t 
   HIP_TRY(this, hipEventRecord(lane.desc_done[stage], st));
   HIP_TRY(this, hipEventRecord(ev[0], st));
   if (tuning.batch_mode == 1 || tuning.batch_mode == 2) {
-    const int tv = lane_variant(tuning.pt_kernel), wv = (int)tuning.pt_waves;
+    const int tv = lane_variant(tuning.pt_kernel, tuning.empty_skip), wv = (int)tuning.pt_waves;
     const uint32_t ptb = tuning.batch_blocks ? tuning.batch_blocks : (uint32_t)cus * 8u;
     const uint32_t gsb = (uint32_t)cus * 8u;
     auto enqueue = [&] {
@@ -1216,7 +1231,7 @@ int vrt_ctx::render_batch(vrt::BatchLane& lane, const vrt_view* views, uint32_t 
     return VRT_OK;
   }
   HIP_TRY(this, vrt::launch_pathtrace_batch(st, dd, nviews, tiles, (unsigned long long*)d_counters.ptr,
-                                            lane_variant(tuning.pt_kernel), (int)tuning.pt_waves,
+                                            lane_variant(tuning.pt_kernel, tuning.empty_skip), (int)tuning.pt_waves,
                                             max_bit_words, max_pwords));
   HIP_TRY(this, hipEventRecord(ev[1], st));
   HIP_TRY(this, vrt::launch_temporal_batch(st, dd, nviews, tiles));

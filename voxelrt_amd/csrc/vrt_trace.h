@@ -30,6 +30,12 @@ struct SceneDev {
   // its prefix counts: the layout of the single-loop DDA (vrt_dda_fast.h).
   const uint32_t* pbits;
   const uint32_t* pprefix;
+  // Per padded bitmap word, the Chebyshev distance (in bricks, over y and z)
+  // from the brick rows the word covers to the nearest row holding an
+  // occupied brick; 0 when one of its rows is occupied, at most 255.  Every
+  // brick of a row within distance d - 1 of such a row is empty, whatever its
+  // x.  pwords bytes, padded to whole words.
+  const uint8_t* pdist;
   uint32_t pwords;
   uint32_t psh1, psh2;
   int nx, ny, nz;
