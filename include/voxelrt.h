@@ -245,7 +245,9 @@ typedef struct vrt_tuning {
                                  3 = one pixel per lane, occupancy staged in LDS,
                                  4 = one pixel per lane, single loop, both levels in LDS,
                                  5 = as 4 with one trace call site per lane (shadow and
-                                     bounce rays of different lanes share the DDA loop) */
+                                     bounce rays of different lanes share the DDA loop),
+                                 6 = as 1 with the single-loop DDA in LDS (brick
+                                     entries batched per dda_batch) */
   uint32_t pt_blocks;
   uint32_t pt_steps;
   uint32_t pt_min_refill;

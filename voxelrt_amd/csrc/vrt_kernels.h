@@ -49,7 +49,7 @@ hipError_t launch_pathtrace(hipStream_t st, const SceneDev& s, const Uniforms& u
 hipError_t launch_pathtrace_persistent(hipStream_t st, const SceneDev& s, const Uniforms& u, float fclamp,
                                        uint32_t spp, void* radiance, void* albedo, void* gbuf,
                                        unsigned long long* counters, uint32_t* queue, int blocks,
-                                       int steps, int min_refill, int chunk);
+                                       int steps, int min_refill, int chunk, int tv, int waves);
 struct ViewDesc;
 hipError_t launch_pathtrace_batch(hipStream_t st, const ViewDesc* views, uint32_t nviews, uint32_t tiles,
                                   unsigned long long* counters, int tv, int waves, uint32_t max_bit_words,

@@ -340,6 +340,7 @@ struct FlatRunner {
   }
   VRT_D bool step(const SceneDev& s, const LaneMem&, TraceResult& o) { return st.step(s, o); }
   VRT_D void settle(const SceneDev&, const LaneMem&, bool) {}
+  VRT_D f3 dir() const { return mk3(st.rd[0], st.rd[1], st.rd[2]); }
 };
 // A lane that reaches an occupied brick waits (pend) until entry_batch lanes
 // of its wave wait, or every lane walking a ray does; settle(), called by the
@@ -371,6 +372,7 @@ struct FastRunner {
       pend = false;
     }
   }
+  VRT_D f3 dir() const { return mk3(st.rdv[0], st.rdv[1], st.rdv[2]); }
 };
 
 template <class Runner, int TV>
@@ -1269,13 +1271,23 @@ hipError_t launch_pathtrace(hipStream_t st, const SceneDev& s, const Uniforms& u
 hipError_t launch_pathtrace_persistent(hipStream_t st, const SceneDev& s, const Uniforms& u, float fclamp,
                                        uint32_t spp, void* radiance, void* albedo, void* gbuf,
                                        unsigned long long* counters, uint32_t* queue, int blocks,
-                                       int steps, int min_refill, int chunk) {
+                                       int steps, int min_refill, int chunk, int tv, int waves) {
   hipError_t e = hipMemsetAsync(queue, 0, sizeof(uint32_t), st);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_pathtrace_persistent, dim3(blocks > 0 ? blocks : 1), dim3(256), 0, st, s, u, fclamp,
-                     spp, (ushort4*)radiance, (uchar4*)albedo, (float4*)gbuf, counters, queue,
-                     steps < 1 ? 1 : steps, min_refill < 1 ? 1 : min_refill,
-                     (uint32_t)(chunk < 64 ? 64 : chunk));
+  tv = tv == TV_FAST ? lds_variant(TV_FAST, s.bit_words, s.pwords) : TV_NESTED;
+  const uint32_t shmem = lds_bytes(tv, s.bit_words, s.pwords);
+  auto go = [&](auto kernel) {
+    hipLaunchKernelGGL(kernel, dim3(blocks > 0 ? blocks : 1), dim3(256), shmem, st, s, u, fclamp, spp,
+                       (ushort4*)radiance, (uchar4*)albedo, (float4*)gbuf, counters, queue, steps < 1 ? 1 : steps,
+                       min_refill < 1 ? 1 : min_refill, (uint32_t)(chunk < 64 ? 64 : chunk));
+  };
+  if (tv == TV_FAST) {
+    if (waves == 4) go(k_pathtrace_persistent<FastRunner<false>, TV_FAST, 4>);
+    else if (waves == 5) go(k_pathtrace_persistent<FastRunner<false>, TV_FAST, 5>);
+    else go(k_pathtrace_persistent<FastRunner<false>, TV_FAST, 1>);
+  } else {
+    go(k_pathtrace_persistent<FlatRunner, TV_NESTED, 1>);
+  }
   return hipGetLastError();
 }
 

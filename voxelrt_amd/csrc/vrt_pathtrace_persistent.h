@@ -14,9 +14,11 @@
 //      is issued for many lanes at once;
 //   3. lanes with a ray in flight advance their DDA `steps` iterations.
 // Every path consumes its PCG stream in exactly the order of the reference
-// loop, so results equal the one-pixel-per-lane kernel bit for bit.
+// loop, so results equal the one-pixel-per-lane kernel bit for bit.  The DDA
+// is a Runner of vrt_kernels.hip: FlatRunner (single-stream, occupancy read
+// from HBM/L2) or FastRunner (single loop on the LDS-staged occupancy; its
+// brick entries wait for SceneDev::entry_batch lanes of the wave).
 #pragma once
-#include "vrt_dda_flat.h"
 
 namespace vrt {
 
@@ -50,8 +52,9 @@ VRT_D f3 primary_dir(const Uniforms& u, uint32_t spp, PathLane& L, uint32_t w, u
 // Run the path from a finished ray (or a freshly assigned pixel) to the next
 // ray begin, or to the end of the pixel (phase -> PH_NONE).  Written with one
 // ray-begin site and one sky evaluation so each is emitted once.
-VRT_D void path_transition(const SceneDev& s, const Uniforms& u, float fclamp, uint32_t spp, PathLane& L,
-                           DdaFlat& st, TraceResult& o, bool have_result, ushort4* __restrict__ radiance_out,
+template <class Runner>
+VRT_D void path_transition(const SceneDev& s, const LaneMem& M, const Uniforms& u, float fclamp, uint32_t spp,
+                           PathLane& L, Runner& st, TraceResult& o, bool have_result, ushort4* __restrict__ radiance_out,
                            uchar4* __restrict__ albedo_out, float4* __restrict__ gbuf_out, RayCount& rc,
                            uint32_t w, uint32_t h) {
   const uint32_t nb = u.params0[1];
@@ -64,7 +67,7 @@ VRT_D void path_transition(const SceneDev& s, const Uniforms& u, float fclamp, u
   // of at most spp samples.
   for (;;) {
     if (next == S_BEGIN) {
-      if (!st.begin(s, ray_o, ray_d, 1e4f, o)) return;
+      if (!st.begin(s, M, ray_o, ray_d, 1e4f, o)) return;
       next = S_RESULT;  // missed the grid at setup
     }
     if (next == S_START) {
@@ -81,7 +84,7 @@ VRT_D void path_transition(const SceneDev& s, const Uniforms& u, float fclamp, u
     }
     if (next == S_RESULT) {
       rc.add(o);
-      const f3 rd = mk3(st.rd[0], st.rd[1], st.rd[2]);
+      const f3 rd = st.dir();
       if (o.t < 0.0f && L.phase != PH_SHADOW) {
         // primary miss: sky with the sun disc; bounce miss: sky without
         const f3 sky = sky_color(u, rd, L.phase == PH_PRIMARY);
@@ -186,13 +189,15 @@ VRT_D void path_transition(const SceneDev& s, const Uniforms& u, float fclamp, u
   }
 }
 
-__global__ __launch_bounds__(256) void k_pathtrace_persistent(SceneDev s, Uniforms u, float fclamp, uint32_t spp,
-                                                              ushort4* __restrict__ radiance,
-                                                              uchar4* __restrict__ albedo,
-                                                              float4* __restrict__ gbuf,
-                                                              unsigned long long* __restrict__ counters,
-                                                              uint32_t* __restrict__ queue, int steps,
-                                                              int min_refill, uint32_t chunk) {
+// WV: minimum waves per SIMD the register allocation must allow (1 = the
+// compiler's choice), as for k_pathtrace.
+template <class Runner, int TV, int WV>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WV))) void k_pathtrace_persistent(
+    SceneDev s, Uniforms u, float fclamp, uint32_t spp, ushort4* __restrict__ radiance, uchar4* __restrict__ albedo,
+    float4* __restrict__ gbuf, unsigned long long* __restrict__ counters, uint32_t* __restrict__ queue, int steps,
+    int min_refill, uint32_t chunk) {
+  extern __shared__ uint32_t lds_dyn[];
+  const LaneMem M = lds_setup<TV>(s, lds_dyn);
   const uint32_t w = (uint32_t)u.params1[0], h = (uint32_t)u.params1[1];
   // queue ids walk 8x8 pixel tiles row-major, 64 ids per tile, so a wave's
   // chunk covers compact tiles (coherent primary rays)
@@ -203,7 +208,7 @@ __global__ __launch_bounds__(256) void k_pathtrace_persistent(SceneDev s, Unifor
   PathLane L;
   L.phase = PH_NONE;
   L.fin = false;
-  DdaFlat st;
+  Runner st;
   TraceResult o;
   RayCount rc{0u, 0u, 0u, 0u};
   uint32_t cbase = 0u, cleft = 0u;
@@ -244,15 +249,17 @@ __global__ __launch_bounds__(256) void k_pathtrace_persistent(SceneDev s, Unifor
         }
       }
       if (L.fin || fresh) {
-        path_transition(s, u, fclamp, spp, L, st, o, L.fin, radiance, albedo, gbuf, rc, w, h);
+        path_transition(s, M, u, fclamp, spp, L, st, o, L.fin, radiance, albedo, gbuf, rc, w, h);
         L.fin = false;
       }
     }
     if (__ballot(L.phase != PH_DONE) == 0ull) break;
     for (int k = 0; k < steps; ++k) {
-      if (!L.fin && (L.phase == PH_PRIMARY || L.phase == PH_SHADOW || L.phase == PH_BOUNCE)) {
-        if (st.step(s, o)) L.fin = true;
+      const bool walking = !L.fin && (L.phase == PH_PRIMARY || L.phase == PH_SHADOW || L.phase == PH_BOUNCE);
+      if (walking) {
+        if (st.step(s, M, o)) L.fin = true;
       }
+      st.settle(s, M, walking);
     }
   }
   count4(counters, rc.rays, rc.bsteps, rc.vsteps, rc.hits);

@@ -556,6 +556,7 @@ static int lane_variant(uint32_t kernel, uint32_t skip) {
     case 3: return vrt::TV_LDS;
     case 4: return skip ? vrt::TV_FASTSKIP : vrt::TV_FAST;
     case 5: return vrt::TV_FAST1;
+    case 6: return vrt::TV_FAST;  // (batched form of the persistent kernel 6)
     default: return vrt::TV_NESTED;
   }
 }
@@ -563,11 +564,13 @@ static int lane_variant(uint32_t kernel, uint32_t skip) {
 hipError_t vrt_ctx::launch_pathtrace(const vrt::SceneDev& s0, const vrt_uniforms& u, float fclamp,
                                      uint32_t spp, void* radiance, void* albedo, void* gbuf) {
   const vrt::SceneDev s = tuned(s0);
-  if (tuning.pt_kernel == 1)
+  if (tuning.pt_kernel == 1 || tuning.pt_kernel == 6)
     return vrt::launch_pathtrace_persistent(stream, s, u, fclamp, spp, radiance, albedo, gbuf,
                                             (unsigned long long*)d_counters.ptr, (uint32_t*)d_queue.ptr + 1,
                                             (int)tuning.pt_blocks, (int)tuning.pt_steps,
-                                            (int)tuning.pt_min_refill, (int)tuning.pt_chunk);
+                                            (int)tuning.pt_min_refill, (int)tuning.pt_chunk,
+                                            tuning.pt_kernel == 6 ? vrt::TV_FAST : vrt::TV_NESTED,
+                                            (int)tuning.pt_waves);
   return vrt::launch_pathtrace(stream, s, u, fclamp, spp, radiance, albedo, gbuf,
                                (unsigned long long*)d_counters.ptr, lane_variant(tuning.pt_kernel, tuning.empty_skip),
                                (int)tuning.pt_waves);
@@ -862,7 +865,7 @@ int vrt_tuning_get(vrt_ctx* ctx, vrt_tuning* out) {
 int vrt_tuning_set(vrt_ctx* ctx, const vrt_tuning* in) {
   if (!ctx || !in) return VRT_ERR_INVALID;
   std::lock_guard<std::mutex> lock_(ctx->mu);
-  if (in->trace_kernel > 5 || in->pt_kernel > 5 || in->trace_blocks > (1u << 20) ||
+  if (in->trace_kernel > 5 || in->pt_kernel > 6 || in->trace_blocks > (1u << 20) ||
       in->pt_blocks > (1u << 20) || in->trace_steps > 1024 || in->pt_steps > 1024 ||
       in->trace_min_refill > 64 || in->pt_min_refill > 64 || in->pt_chunk > (1u << 16) ||
       !(in->pt_waves == 0 || in->pt_waves == 4 || in->pt_waves == 5) || in->batch_mode > 2 ||
@@ -1030,6 +1033,33 @@ int vrt_autotune(vrt_ctx* ctx, const vrt_scene* scene, uint64_t nrays, const vrt
               best.pt_chunk = ch;
             }
           }
+    // persistent with the single-loop DDA: register budget x launch shape x
+    // brick-entry batch
+    for (uint32_t wv : {0u, 4u})
+      for (uint32_t b : {1024u, 2048u})
+        for (uint32_t st : {4u, 8u, 16u})
+          for (uint32_t rf : {16u, 32u})
+            for (uint32_t db : {1u, 8u}) {
+              ctx->tuning.pt_kernel = 6;
+              ctx->tuning.pt_waves = wv;
+              ctx->tuning.pt_blocks = b;
+              ctx->tuning.pt_steps = st;
+              ctx->tuning.pt_min_refill = rf;
+              ctx->tuning.pt_chunk = 64;
+              ctx->tuning.dda_batch = db;
+              rc = time_it(launch, &ms);
+              if (rc) return rc;
+              if (ms < best_ms) {
+                best_ms = ms;
+                best.pt_kernel = 6;
+                best.pt_waves = wv;
+                best.pt_blocks = b;
+                best.pt_steps = st;
+                best.pt_min_refill = rf;
+                best.pt_chunk = 64;
+                best.dda_batch = db;
+              }
+            }
   }
   ctx->tuning = best;
   // the sweep's rays do not count in the statistics
