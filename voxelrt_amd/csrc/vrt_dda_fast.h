@@ -47,8 +47,9 @@
 // fine tMax recomputed from the origin on entry, ties x, then y, then z.  The
 // reference's integer bounds tests become counts of cells left along each
 // axis (an axis with zero direction never leaves, as its position never
-// changes) and its iteration caps are kept.  Step counts are the reference's
-// loop iterations.
+// changes) and its iteration caps are kept (they bind for degenerate
+// directions, e.g. a zero vector, whose steps never leave the cell).  Step
+// counts are the reference's loop iterations.
 #pragma once
 #include "vrt_trace.h"
 
@@ -97,6 +98,15 @@ VRT_D int isel(bool c, int a, int b) {
   return (int)((m & (uint32_t)a) | (~m & (uint32_t)b));
 }
 
+// Bit (i & 31) of w.
+VRT_D bool bit_at(uint32_t w, uint32_t i) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_ubfe(w, i, 1u) != 0u;
+#else
+  return ((w >> (i & 31u)) & 1u) != 0u;
+#endif
+}
+
 struct DdaFast {
   float ro[3], rdv[3], inv[3];
   int sd[3];
@@ -105,9 +115,9 @@ struct DdaFast {
   float tm[3], d[3];
   int di[3];
   uint32_t idx, cnt, iter;
-  uint32_t base, wshift, inc;  // LDS word base / stride shift, step-counter increment
+  uint32_t base4, wsh, inc;  // LDS byte base, word stride shift (2: words, 10: slot rows), step increment
   float tBound, t;             // coarse: leave at t > tExit; fine: never
-  uint32_t ax;
+  uint32_t axs;  // axis of the last step x 8 (its field offset in cnt)
   // coarse state saved while walking a brick
   float tmB[3];
   uint32_t idxB, cntB, iterB, blk;
@@ -145,7 +155,7 @@ struct DdaFast {
     const float tEnter = vmax(vmax(tmin[0], tmin[1]), vmax(tmin[2], 0.0f));
     tExit = vmin(vmin(tmax[0], tmax[1]), vmin(tmax[2], maxT));
     if (tEnter > tExit) return true;
-    ax = (tmin[0] >= tmin[1] && tmin[0] >= tmin[2]) ? 0u : (tmin[1] >= tmin[2] ? 1u : 2u);
+    axs = (tmin[0] >= tmin[1] && tmin[0] >= tmin[2]) ? 0u : (tmin[1] >= tmin[2] ? 8u : 16u);
     t = tEnter;
     const float ts = t + 1e-4f;
     int bp[3];
@@ -163,8 +173,8 @@ struct DdaFast {
     idx = (uint32_t)bp[0] | ((uint32_t)bp[1] << s.psh1) | ((uint32_t)bp[2] << s.psh2);
     cnt = pack_left(bp, BG, sd);
     iter = (uint32_t)(s.bgx + s.bgy + s.bgz + 2);
-    base = L.bits_off;
-    wshift = 0u;
+    base4 = 4u * L.bits_off;
+    wsh = 2u;
     inc = 1u;
     tBound = tExit;
     tmB[0] = tmB[1] = tmB[2] = 0.0f;
@@ -181,7 +191,6 @@ struct DdaFast {
     const bool c0 = m0 <= m1 && m0 <= m2;
     const bool c1 = !c0 && m1 <= m2;
     t = vsel(c0, m0, vsel(c1, m1, m2));
-    ax = c0 ? 0u : (c1 ? 1u : 2u);
     const float n0 = m0 + d[0], n1 = m1 + d[1], n2 = m2 + d[2];
     tm[0] = vsel(c0, n0, m0);
     tm[1] = vsel(c1, n1, m1);
@@ -189,6 +198,7 @@ struct DdaFast {
     const int dsel = isel(c0, i0, isel(c1, i1, i2));  // 0 exactly when that axis does not move
     idx += (uint32_t)dsel;
     const uint32_t sh = c0 ? 0u : (c1 ? 8u : 16u);
+    axs = sh;
     const bool out = ((cnt >> sh) & 255u) == 0u || iter == 1u;
     cnt -= (dsel != 0 ? 1u : 0u) << sh;
     iter -= 1u;
@@ -238,8 +248,8 @@ struct DdaFast {
     const int n8[3] = {8, 8, 8};
     cnt = pack_left(lp, n8, sd);
     iter = 24u;
-    base = L.lane;
-    wshift = 8u;
+    base4 = 4u * L.lane;
+    wsh = 10u;
     inc = 0x10000u;
     tBound = __builtin_inff();
   }
@@ -257,8 +267,8 @@ struct DdaFast {
     idx = idxB;
     cnt = cntB;
     iter = iterB;
-    base = L.bits_off;
-    wshift = 0u;
+    base4 = 4u * L.bits_off;
+    wsh = 2u;
     inc = 1u;
     tBound = tExit;
   }
@@ -269,7 +279,7 @@ struct DdaFast {
   VRT_D bool skip_rows(uint32_t D) {
     const uint32_t c0 = cnt & 255u, c1 = (cnt >> 8) & 255u, c2 = (cnt >> 16) & 255u;
     const uint32_t r1 = D - 1u < c1 ? D - 1u : c1, r2 = D - 1u < c2 ? D - 1u : c2;
-    if (iter <= c0 + r1 + r2) return false;  // iteration budget (never binds inside the grid)
+    if (iter <= c0 + r1 + r2) return false;  // keep the iteration budget out of the run
     // T: below the t of step r+1 along each moving axis (half a tDelta of
     // margin over the rounding of r additions), and not beyond tExit
     float T = tBound;
@@ -303,10 +313,10 @@ struct DdaFast {
         al = (uint32_t)c;
       }
     t = tl;
-    ax = al;
+    axs = 8u * al;
+    iter -= K;
     idx += (uint32_t)((int)k[0] * di[0] + (int)k[1] * di[1] + (int)k[2] * di[2]);
     cnt -= (di[0] != 0 ? k[0] : 0u) | ((di[1] != 0 ? k[1] : 0u) << 8) | ((di[2] != 0 ? k[2] : 0u) << 16);
-    iter -= K;
     steps += K - 1u;  // the cells passed over are tested (empty); the last one is tested next
     return true;
   }
@@ -317,9 +327,9 @@ struct DdaFast {
   template <bool Skip = false>
   VRT_D uint32_t step(const SceneDev& s, const LdsScene& L) {
     if (t > tBound) return DDA_MISS;
-    const uint32_t word = L.lds[base + ((idx >> 5) << wshift)];
+    const uint32_t word = *(const uint32_t*)((const char*)L.lds + (((idx >> 5) << wsh) + base4));
     steps += inc;
-    if ((word >> (idx & 31u)) & 1u) return inc != 1u ? DDA_HIT : DDA_ENTER;  // voxel.wgsl:122 / 111
+    if (bit_at(word, idx)) return inc != 1u ? DDA_HIT : DDA_ENTER;  // voxel.wgsl:122 / 111
     if constexpr (Skip) {
       if (inc == 1u) {
         const uint32_t D = (L.lds[L.dist_off + (idx >> 7)] >> (8u * ((idx >> 5) & 3u))) & 255u;
@@ -375,6 +385,7 @@ struct DdaFast {
     o.bsteps = steps & 0xffffu;
     o.vsteps = steps >> 16;
     if (status != DDA_HIT || t > maxT) return;  // a hit beyond maxT reports a miss (voxel.wgsl:124)
+    const uint32_t ax = axs >> 3;
     o.t = t * s.voxel_size;  // voxel.wgsl:122-128
     float nn[3];
 #pragma unroll
