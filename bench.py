@@ -92,8 +92,7 @@ def traverse(args):
     variants = {"lane": dict(trace_kernel=0), "lane_flat": dict(trace_kernel=2), "lane_lds": dict(trace_kernel=3),
                 "lane_fast": dict(trace_kernel=4, dda_batch=1),
                 "lane_fast_b16": dict(trace_kernel=4, dda_batch=16),
-                "persistent_fast_b16": dict(trace_kernel=5, trace_blocks=1024, trace_steps=8, trace_min_refill=16,
-                                            dda_batch=16),
+                "persistent_fast_b16": dict(PROFILED_TRAVERSE),
                 "persistent": dict(trace_kernel=1, trace_blocks=2048, trace_steps=8, trace_min_refill=16)}
     for p in (0.05, 0.25, 0.6):
         vox = cf.traverse_scene(p)
@@ -145,6 +144,10 @@ def _traverse_cpu(vox, rays, p, sample=1 << 20):
     return out
 
 
+# the traversal variant whose HBM traffic profiles/traverse_pmc.txt holds
+PROFILED_TRAVERSE = dict(trace_kernel=5, trace_blocks=1024, trace_steps=8, trace_min_refill=16, dda_batch=16)
+
+
 def _traverse_one(args, ctx, sc, p, name, d_rays, d_hits, n_rays, tuning):
     import numpy as np
     for _ in range(args.warmup):
@@ -172,6 +175,8 @@ def _traverse_one(args, ctx, sc, p, name, d_rays, d_hits, n_rays, tuning):
                       "roofline": {"bound": "hbm", "kernel": "k_trace_rays", "achieved": round(gbs, 1),
                                    "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                    "frac": round(gbs / HBM_PEAK_GBS, 4),
+                                   "traffic": _pmc_traffic("traverse_pmc.txt", f"trace_kernel=5,p={p}")
+                                   if tuning == PROFILED_TRAVERSE else None,
                                    "alg_bytes_per_launch": int(alg)}}), flush=True)
 
 

@@ -1,12 +1,14 @@
 """Profiling target for rocprofv3 runs (kernel traces and --pmc passes).
 
-python voxelrt_amd/profile_target.py [--pt-kernel K] [--pt-waves W] [--trace-kernel T]
+python voxelrt_amd/profile_target.py [--pt-kernel K] [--pt-waves W]
+        [--trace-kernel T] [--dda-batch B] [--traverse-p P ...]
   1. renders 3 frames of frame_1080 (1920x1080, 256^3, 2 bounces, temporal +
      a-trous) with the given path-tracer variant (vrt_tuning.pt_kernel /
      pt_waves; default the one-pixel-per-lane kernel),
-  2. traces 2^24 rays twice on the p=0.25 traversal grid with the given
-     traversal variant (default the persistent kernel, 1024 blocks, 4 steps
-     per refill check, refill at 1 idle lane),
+  2. for each traversal grid occupancy P (default 0.25), traces 2^24 rays
+     twice with the given traversal variant (default the persistent kernel,
+     1024 blocks, 8 steps per refill check, refill at 16 idle lanes; brick
+     entries batched per --dda-batch for the single-loop kernels 4 and 5),
   3. copies 256 MiB device to device twice (the calibration copy for the
      FETCH_SIZE / WRITE_SIZE counters).
 """
@@ -27,6 +29,8 @@ def main():
     ap.add_argument("--pt-kernel", type=int, default=0)
     ap.add_argument("--pt-waves", type=int, default=0)
     ap.add_argument("--trace-kernel", type=int, default=1)
+    ap.add_argument("--dda-batch", type=int, default=1)
+    ap.add_argument("--traverse-p", type=float, nargs="+", default=[0.25])
     args = ap.parse_args()
     print("profile target start", flush=True)
     ctx = v.Context(0)
@@ -39,15 +43,18 @@ def main():
         r.frame()
     ctx.stats()
     print("frames done", flush=True)
-    sc2 = ctx.upload(cf.traverse_scene(0.25))
     n = 1 << 24
     d_rays = torch.from_numpy(cf.traverse_rays(n, seed=77)).to("cuda:0")
     d_hits = torch.empty(n * 4, dtype=torch.int32, device="cuda:0")
     torch.cuda.synchronize()
-    ctx.set_tuning(trace_kernel=args.trace_kernel, trace_blocks=1024, trace_steps=4, trace_min_refill=1)
-    for _ in range(2):
-        ctx.trace_rays_device(sc2, d_rays.data_ptr(), n, d_hits.data_ptr())
-    print("traverse done", flush=True)
+    ctx.set_tuning(trace_kernel=args.trace_kernel, trace_blocks=1024, trace_steps=8, trace_min_refill=16,
+                   dda_batch=args.dda_batch)
+    for p in args.traverse_p:
+        sc2 = ctx.upload(cf.traverse_scene(p))
+        for _ in range(2):
+            ctx.trace_rays_device(sc2, d_rays.data_ptr(), n, d_hits.data_ptr())
+        sc2.release()
+        print(f"traverse p={p} done", flush=True)
     a = torch.empty(256 << 20, dtype=torch.uint8, device="cuda:0")
     b = torch.empty_like(a)
     a.fill_(1)
