@@ -49,6 +49,7 @@ def dda(tmp_path_factory):
         if iters is not None:
             iters.append(int(it[0]))
         return hits
+    run.lib = L
     return run
 
 
@@ -126,3 +127,13 @@ def test_empty_row_skip_takes_runs(dda):
             assert b[0] == a[0], (name, a, b)
         if name in most:
             assert b[0] < most[name] * a[0], (name, a, b)
+
+
+def test_div_rcp_is_ieee_division(dda):
+    """The a-trous taps divide by a per-pixel divisor through its rounded
+    reciprocal and one fused-multiply-add correction (vrt_device.h
+    div_rcp); every quotient must equal IEEE a / b bit for bit."""
+    L = dda.lib
+    L.div_rcp_check.argtypes = [C.c_uint64, C.c_uint64]
+    L.div_rcp_check.restype = C.c_uint64
+    assert L.div_rcp_check(20_000_000, 7) == 0

@@ -510,8 +510,22 @@ VRT_D f3 cosine_hemisphere(f3 n, uint32_t& rng) {
 // ---------------------------------------------------------------------------
 struct PixelOut {
   f3 radiance;
-  f3 albedo_sqrt;
-  float4 gbuf;
+};
+
+// Where the first sample of a pixel stores its albedo (rgba8unorm of
+// sqrt(albedo)) and G-buffer texel, written as soon as the primary ray is
+// resolved so that neither stays live through the bounce loop.
+struct GbufOut {
+  uchar4* albedo;
+  float4* gbuf;
+  uint64_t i;
+  bool write;
+  VRT_D void store(f3 alb_sqrt, float4 g) const {
+    if (!write) return;
+    albedo[i] = make_uchar4((uint8_t)f32_to_unorm8(alb_sqrt.x), (uint8_t)f32_to_unorm8(alb_sqrt.y),
+                            (uint8_t)f32_to_unorm8(alb_sqrt.z), 255);
+    gbuf[i] = g;
+  }
 };
 
 struct RayCount {
@@ -526,7 +540,7 @@ struct RayCount {
 
 template <int TV>
 VRT_D PixelOut shade_pixel(const SceneDev& s, const LaneMem& L, const Uniforms& u, float fclamp, uint32_t px,
-                           uint32_t py, uint32_t frame, uint32_t w, uint32_t h, RayCount& rc) {
+                           uint32_t py, uint32_t frame, uint32_t w, uint32_t h, RayCount& rc, const GbufOut& go) {
   PixelOut res;
   uint32_t rng = init_rng(px, py, frame);
   const float jx = rand01(rng) - 0.5f;
@@ -542,15 +556,14 @@ VRT_D PixelOut shade_pixel(const SceneDev& s, const LaneMem& L, const Uniforms& 
   const bool indirect = (u.params0[2] & VRT_FLAG_INDIRECT) != 0u;
   if (h0.t < 0.0f) {
     res.radiance = indirect ? mk3(0.0f, 0.0f, 0.0f) : sky_color(u, rd, true);
-    res.albedo_sqrt = mk3(1.0f, 1.0f, 1.0f);
-    res.gbuf = make_float4(-rd.x, -rd.y, -rd.z, -1.0f);
+    go.store(mk3(1.0f, 1.0f, 1.0f), make_float4(-rd.x, -rd.y, -rd.z, -1.0f));
     return res;
   }
   const Material mat1 = unpack_material(h0.mat);
   const f3 x1 = ro + rd * h0.t;
   const f3 n1 = h0.n;
-  res.albedo_sqrt = mk3(vsqrt(mat1.albedo.x), vsqrt(mat1.albedo.y), vsqrt(mat1.albedo.z));
-  res.gbuf = make_float4(n1.x, n1.y, n1.z, h0.t);
+  go.store(mk3(vsqrt(mat1.albedo.x), vsqrt(mat1.albedo.y), vsqrt(mat1.albedo.z)),
+           make_float4(n1.x, n1.y, n1.z, h0.t));
 
   f3 radiance = mk3(0.0f, 0.0f, 0.0f);
   f3 tp = mk3(1.0f, 1.0f, 1.0f);
@@ -605,7 +618,8 @@ VRT_D PixelOut shade_pixel(const SceneDev& s, const LaneMem& L, const Uniforms& 
 // the same order (same RNG draws, same FP32 sequence): results are identical.
 template <int TV>
 VRT_D PixelOut shade_pixel_1(const SceneDev& s, const LaneMem& L, const Uniforms& u, float fclamp, uint32_t px,
-                             uint32_t py, uint32_t frame, uint32_t w, uint32_t h, RayCount& rc) {
+                             uint32_t py, uint32_t frame, uint32_t w, uint32_t h, RayCount& rc,
+                             const GbufOut& go) {
   enum : uint32_t { PRIMARY, SHADOW, BOUNCE, DONE, SKY };
   PixelOut res;
   uint32_t rng = init_rng(px, py, frame);
@@ -623,8 +637,6 @@ VRT_D PixelOut shade_pixel_1(const SceneDev& s, const LaneMem& L, const Uniforms
   Material mat{};
   float ndl = 0.0f;
   uint32_t bounce = 0u, phase = PRIMARY;
-  res.albedo_sqrt = mk3(1.0f, 1.0f, 1.0f);
-  res.gbuf = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   while (phase < DONE) {
     TraceResult hh;
     trace_sel<TV>(s, L, ro, phase == SHADOW ? l : rd, 1e4f, hh);
@@ -633,7 +645,7 @@ VRT_D PixelOut shade_pixel_1(const SceneDev& s, const LaneMem& L, const Uniforms
     if (phase == PRIMARY) {
       if (hh.t < 0.0f) {
         res.radiance = indirect ? mk3(0.0f, 0.0f, 0.0f) : sky_color(u, rd, true);
-        res.gbuf = make_float4(-rd.x, -rd.y, -rd.z, -1.0f);
+        go.store(mk3(1.0f, 1.0f, 1.0f), make_float4(-rd.x, -rd.y, -rd.z, -1.0f));
         phase = SKY;
         break;
       }
@@ -641,8 +653,7 @@ VRT_D PixelOut shade_pixel_1(const SceneDev& s, const LaneMem& L, const Uniforms
       alb1 = mat.albedo;
       pos = ro + rd * hh.t;
       n = hh.n;
-      res.albedo_sqrt = mk3(vsqrt(alb1.x), vsqrt(alb1.y), vsqrt(alb1.z));
-      res.gbuf = make_float4(n.x, n.y, n.z, hh.t);
+      go.store(mk3(vsqrt(alb1.x), vsqrt(alb1.y), vsqrt(alb1.z)), make_float4(n.x, n.y, n.z, hh.t));
       vertex = true;
     } else if (phase == SHADOW) {
       if (!(hh.t >= 0.0f)) radiance = radiance + (((tp * mat.albedo) * INV_PI_F) * ndl) * sunRad;
@@ -716,22 +727,17 @@ VRT_D void pathtrace_tile(const SceneDev& s, const LaneMem& L, const Uniforms& u
   if (tile_pixel(w, h, tile, px, py)) {
     const uint64_t i = (uint64_t)py * w + px;
     f3 acc = mk3(0.0f, 0.0f, 0.0f);
-    PixelOut first;
     for (uint32_t smp = 0; smp < spp; ++smp) {
+      const GbufOut go{albedo, gbuf, i, smp == 0u};  // the first sample's albedo / G-buffer
       PixelOut r;
       if constexpr (TV == TV_FAST1)
-        r = shade_pixel_1<TV>(s, L, u, fclamp, px, py, u.params0[0] * spp + smp, w, h, rc);
+        r = shade_pixel_1<TV>(s, L, u, fclamp, px, py, u.params0[0] * spp + smp, w, h, rc, go);
       else
-        r = shade_pixel<TV>(s, L, u, fclamp, px, py, u.params0[0] * spp + smp, w, h, rc);
-      if (smp == 0) first = r;
+        r = shade_pixel<TV>(s, L, u, fclamp, px, py, u.params0[0] * spp + smp, w, h, rc, go);
       acc = acc + r.radiance;
     }
     const f3 o = acc / (float)spp;
     radiance[i] = make_ushort4(f32_to_f16(o.x), f32_to_f16(o.y), f32_to_f16(o.z), f32_to_f16(1.0f));
-    albedo[i] = make_uchar4((uint8_t)f32_to_unorm8(first.albedo_sqrt.x),
-                            (uint8_t)f32_to_unorm8(first.albedo_sqrt.y),
-                            (uint8_t)f32_to_unorm8(first.albedo_sqrt.z), 255);
-    gbuf[i] = first.gbuf;
   }
 }
 
@@ -905,6 +911,7 @@ VRT_D void atrous_tile(const Uniforms& u, uint32_t tile, const void* __restrict_
   float wSum = K[0] * K[0];
   const f3 gn = mk3(g.x, g.y, g.z);
   const float dscale = 0.05f * g.w + 0.01f;
+  const float rdscale = 1.0f / dscale, rsigmaL = 1.0f / sigmaL;  // div_rcp: the taps' quotients
 #pragma unroll
   for (int dy = -2; dy <= 2; ++dy) {
 #pragma unroll
@@ -918,8 +925,8 @@ VRT_D void atrous_tile(const Uniforms& u, uint32_t tile, const void* __restrict_
       const float4 cq = sc[l];
       const float wKernel = K[dx < 0 ? -dx : dx] * K[dy < 0 ? -dy : dy];
       const float wNormal = pow32(vmax(dot(gn, mk3(gq.x, gq.y, gq.z)), 0.0f));
-      const float wDepth = exp_r(-vabs(g.w - gq.w) / dscale);
-      const float wLum = exp_r(-vabs(lumC - dot(mk3(cq.x, cq.y, cq.z), lw)) / sigmaL);
+      const float wDepth = exp_r(-div_rcp(vabs(g.w - gq.w), dscale, rdscale));
+      const float wLum = exp_r(-div_rcp(vabs(lumC - dot(mk3(cq.x, cq.y, cq.z), lw)), sigmaL, rsigmaL));
       const float wt = ((wKernel * wNormal) * wDepth) * wLum;
       sum = sum + mk3(cq.x, cq.y, cq.z) * wt;
       wSum += wt;
