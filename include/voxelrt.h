@@ -269,7 +269,12 @@ typedef struct vrt_tuning {
                                  coarse steps through runs of empty brick rows in
                                  bulk (same results, fewer instructions on sparse
                                  scenes), 0 = step by step */
-  uint32_t reserved[2];
+  uint32_t pt_order;          /* one-pixel-per-lane path tracer: order in which blocks
+                                 take tiles: 0 row-major, 1 one band of tiles per
+                                 XCD, 2 row-major from the last tile (default: the
+                                 bottom rows, usually the costliest, start first),
+                                 3 bands from their last tile */
+  uint32_t reserved[1];
 } vrt_tuning;
 int vrt_tuning_get(vrt_ctx* ctx, vrt_tuning* out);
 int vrt_tuning_set(vrt_ctx* ctx, const vrt_tuning* in);

@@ -48,6 +48,7 @@ int main(int argc, char** argv) {
     std::fprintf(stderr, "usage: %s <scene> <frames> <event_frame> <out_dir>\n", argv[0]);
     return 2;
   }
+  setvbuf(stdout, nullptr, _IOLBF, 0);  // progress lines survive an abnormal end
   auto s = make_scene(argv[1]);
   const int frames = std::atoi(argv[2]), event_frame = std::atoi(argv[3]);
   const std::string out = argv[4];
@@ -56,8 +57,10 @@ int main(int argc, char** argv) {
   collectEmitters();
   s->camera();
   buildGBuffer();
+  std::printf("scene built\n");
   MethodHIP m;
   m.init();
+  std::printf("technique initialised\n");
   if (!m.ctx || !m.scene) return 3;
   if (!write_grid(out + "/grid0.bin", m)) return 4;
   for (int f = 0; f < frames; ++f) {
@@ -107,5 +110,6 @@ int main(int argc, char** argv) {
   std::fprintf(o, "pixels %d agree %d both_hit %d median_rel_depth %.6f\n", W * H, agree, both,
                rel.empty() ? -1.0 : (double)rel[rel.size() / 2]);
   std::fclose(o);
+  std::printf("done\n");
   return 0;
 }

@@ -213,6 +213,8 @@ PT_VARIANTS = [dict(pt_kernel=0), dict(pt_kernel=0, pt_waves=5), dict(pt_kernel=
                dict(pt_kernel=3), dict(pt_kernel=3, pt_waves=4), dict(pt_kernel=3, pt_waves=5),
                dict(pt_kernel=4), dict(pt_kernel=4, pt_waves=5), dict(pt_kernel=5), dict(pt_kernel=5, pt_waves=5),
                dict(pt_kernel=4, pt_waves=5, empty_skip=1), dict(pt_kernel=4, pt_waves=0, empty_skip=1),
+               dict(pt_kernel=4, pt_waves=5, pt_order=1), dict(pt_kernel=5, pt_waves=5, pt_order=2),
+               dict(pt_kernel=3, pt_order=3),
                dict(pt_kernel=1, pt_blocks=1024, pt_steps=8, pt_min_refill=32, pt_chunk=64),
                dict(pt_kernel=1, pt_blocks=97, pt_steps=1, pt_min_refill=1, pt_chunk=64),
                dict(pt_kernel=1, pt_blocks=2048, pt_steps=16, pt_min_refill=64, pt_chunk=256),
@@ -225,6 +227,7 @@ PT_VARIANTS = [dict(pt_kernel=0), dict(pt_kernel=0, pt_waves=5), dict(pt_kernel=
 @pytest.fixture(params=range(len(PT_VARIANTS)),
                 ids=["lane", "lane-w5", "lane-flat-w4", "lane-lds", "lane-lds-w4", "lane-lds-w5", "lane-fast",
                      "lane-fast-w5", "lane-fast1", "lane-fast1-w5", "lane-fast-w5-skip", "lane-fast-skip",
+                     "lane-fast-w5-xcd", "lane-fast1-w5-rev", "lane-lds-xcdrev",
                      "persist",
                      "persist-s1", "persist-s16", "persist-fast", "persist-fast-w4-b8", "persist-fast-w5-b64"])
 def pt_variant(request, ctx):

@@ -157,6 +157,7 @@ struct vrt_ctx {
   // A scene's device view with this context's launch-time tuning applied.
   vrt::SceneDev tuned(vrt::SceneDev s) const {
     s.entry_batch = tuning.dda_batch ? tuning.dda_batch : 1u;
+    s.tile_order = tuning.pt_order;
     return s;
   }
   hipError_t launch_trace(const vrt::SceneDev& s, const float* rays, uint64_t n, float tmax,

@@ -745,6 +745,16 @@ VRT_D void pathtrace_tile(const SceneDev& s, const LaneMem& L, const Uniforms& u
 // constraint).  The path tracer's natural allocation (~142 VGPRs) leaves 3
 // waves per SIMD to hide the DDA's dependent loads; 4 or 5 trade a few
 // spills for more waves in flight.
+// Tile of block b of an n-block launch in the given order (SceneDev::tile_order).
+VRT_D uint32_t map_tile(uint32_t b, uint32_t n, uint32_t order) {
+  if (order & 2u) b = n - 1u - b;
+  if (order & 1u) {
+    const uint32_t q = n / 8u, r = n % 8u, x = b % 8u, j = b / 8u;
+    b = (x < r ? x * (q + 1u) : r * (q + 1u) + (x - r) * q) + j;
+  }
+  return b;
+}
+
 template <int TV, int WV>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WV))) void k_pathtrace(SceneDev s, Uniforms u, float fclamp,
                                                    uint32_t spp, ushort4* __restrict__ radiance,
@@ -754,7 +764,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WV))) void 
   extern __shared__ uint32_t lds_dyn[];
   const LaneMem L = lds_setup<TV>(s, lds_dyn);
   RayCount rc{0u, 0u, 0u, 0u};
-  pathtrace_tile<TV>(s, L, u, fclamp, spp, blockIdx.x, radiance, albedo, gbuf, rc);
+  pathtrace_tile<TV>(s, L, u, fclamp, spp, map_tile(blockIdx.x, gridDim.x, s.tile_order), radiance, albedo, gbuf,
+                     rc);
   count4(counters, rc.rays, rc.bsteps, rc.vsteps, rc.hits);
 }
 

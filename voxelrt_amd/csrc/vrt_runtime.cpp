@@ -118,6 +118,8 @@ vrt::SceneDev vrt_scene::dev() const {
   s.pbits = (const uint32_t*)d_pbits.ptr;
   s.pprefix = (const uint32_t*)d_pprefix.ptr;
   s.pdist = (const uint8_t*)d_pdist.ptr;
+  s.entry_batch = 1u;
+  s.tile_order = 0u;
   s.pwords = psh3 >= 5 ? (1u << (psh3 - 5)) : 1u;
   s.psh1 = psh1;
   s.psh2 = psh2;
@@ -242,6 +244,7 @@ int vrt_create(const vrt_create_info* info, vrt_ctx** out) {
   ctx->tuning.trace_steps = 8;
   ctx->tuning.trace_min_refill = 16;
   ctx->tuning.pt_kernel = 0;
+  ctx->tuning.pt_order = 2;  // last tile first: the bottom rows (near ground) usually cost most
   ctx->tuning.pt_blocks = 1024;
   ctx->tuning.pt_steps = 8;
   ctx->tuning.pt_min_refill = 32;
@@ -870,7 +873,7 @@ int vrt_tuning_set(vrt_ctx* ctx, const vrt_tuning* in) {
       in->trace_min_refill > 64 || in->pt_min_refill > 64 || in->pt_chunk > (1u << 16) ||
       !(in->pt_waves == 0 || in->pt_waves == 4 || in->pt_waves == 5) || in->batch_mode > 2 ||
       in->batch_blocks > (1u << 16) || in->dda_batch > 64 ||
-      in->empty_skip > 1)
+      in->empty_skip > 1 || in->pt_order > 3)
     return ctx->fail(VRT_ERR_INVALID, "tuning value out of range");
   ctx->tuning = *in;
   return VRT_OK;
@@ -1008,7 +1011,21 @@ int vrt_autotune(vrt_ctx* ctx, const vrt_scene* scene, uint64_t nrays, const vrt
           best.pt_waves = wv;
         }
       }
+    // tile order of the best one-pixel-per-lane variant (costly tiles last
+    // leave a tail of under-filled CUs)
+    ctx->tuning.pt_kernel = best.pt_kernel;
     ctx->tuning.pt_waves = best.pt_waves;
+    for (uint32_t o : {0u, 1u, 2u, 3u}) {
+      if (o == best.pt_order) continue;  // measured above
+      ctx->tuning.pt_order = o;
+      rc = time_it(launch, &ms);
+      if (rc) return rc;
+      if (ms < best_ms) {
+        best_ms = ms;
+        best.pt_order = o;
+      }
+    }
+    ctx->tuning.pt_order = best.pt_order;
     const uint32_t blocks[] = {512, 1024, 2048};
     const uint32_t steps[] = {4, 8, 16};
     const uint32_t refill[] = {16, 32, 48};

@@ -46,6 +46,12 @@ struct SceneDev {
   // walking) wait, then enter their bricks together (vrt_dda_fast.h).
   // 1 = enter at once.
   uint32_t entry_batch;
+  // Order in which the blocks of a one-pixel-per-lane path-trace launch take
+  // its tiles (map_tile in vrt_kernels.hip): 0 row-major, 1 one contiguous
+  // band of tiles per XCD (blocks are dealt to the 8 XCDs round-robin, so
+  // each XCD's L2 sees the bricks of one band), 2 row-major from the last
+  // tile, 3 bands from their last tile.
+  uint32_t tile_order;
   // Always 0.0f.  Passed at run time so that (zero - x) stays a real
   // subtraction: the gfx950 instruction selector folds a literal (0.0f - x)
   // feeding a multiply into a negated operand, which turns +0 into -0.
