@@ -379,9 +379,12 @@ def test_frame_sequence_bitexact(ctx, pt_variant):
                                     dict(pt_kernel=0, pt_waves=0, batch_mode=1, batch_blocks=7),
                                     dict(pt_kernel=4, pt_waves=5, batch_mode=2, empty_skip=1),
                                     dict(pt_kernel=5, pt_waves=4, batch_mode=1),
+                                    dict(pt_kernel=4, pt_waves=5, batch_mode=0, pt_order=0),
+                                    dict(pt_kernel=4, pt_waves=5, batch_mode=1, pt_order=0),
                                     dict(pt_kernel=3, pt_waves=4, batch_mode=2)],
                          ids=["lane", "lds-w5", "flat-w4", "fast-w5", "queue-fast", "queue-7blocks",
-                              "graph-fast", "queue-fast1-w4", "graph-lds"])
+                              "graph-fast", "queue-fast1-w4", "grid-fast-rowmajor", "queue-fast-rowmajor",
+                              "graph-lds"])
 def test_batched_views_match_single_views(ctx, tuning):
     descs = [cf.world_desc(64, 1), cf.world_desc(128, 2), dict(cf.world_desc(64, 3), ny=32)]
     scenes = [ctx.upload(v.generate_scene(**d)) for d in descs]

@@ -958,15 +958,25 @@ __global__ __launch_bounds__(256) void k_atrous(Uniforms u, const void* __restri
 // ---------------------------------------------------------------------------
 // batched form (vrt_batch.h): one launch per pass over every view's tiles
 // ---------------------------------------------------------------------------
+// The view's own tile for batch tile b: the views keep their order in the
+// batch; with tile_order 2/3 each view's tiles run from its last one.
+VRT_D uint32_t view_tile(const ViewDesc& d, uint32_t b) {
+  const uint32_t t = b - d.tile0;
+  if (!(d.s.tile_order & 2u)) return t;
+  const uint32_t n = d.tiles_x * ((d.h + 15u) >> 4);
+  return n - 1u - t;
+}
+
 template <int TV, int WV>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WV))) void k_pathtrace_batch(
     const ViewDesc* __restrict__ views, uint32_t nviews, unsigned long long* __restrict__ counters) {
   extern __shared__ uint32_t lds_dyn[];
-  const uint32_t vi = find_view(views, nviews, blockIdx.x);
+  const uint32_t b = blockIdx.x;
+  const uint32_t vi = find_view(views, nviews, b);
   const ViewDesc& d = views[vi];
   const LaneMem L = lds_setup<TV>(d.s, lds_dyn);
   RayCount rc{0u, 0u, 0u, 0u};
-  pathtrace_tile<TV>(d.s, L, d.u, d.fclamp, d.spp, blockIdx.x - d.tile0, (ushort4*)d.radiance, (uchar4*)d.albedo,
+  pathtrace_tile<TV>(d.s, L, d.u, d.fclamp, d.spp, view_tile(d, b), (ushort4*)d.radiance, (uchar4*)d.albedo,
                      (float4*)d.gbuf_cur, rc);
   count4(counters, rc.rays, rc.bsteps, rc.vsteps, rc.hits);
   count4(d.vcount, rc.rays, rc.bsteps, rc.vsteps, rc.hits);
@@ -1000,7 +1010,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WV))) void 
       }
     }
     RayCount rc{0u, 0u, 0u, 0u};
-    pathtrace_tile<TV>(d.s, L, d.u, d.fclamp, d.spp, b - d.tile0, (ushort4*)d.radiance, (uchar4*)d.albedo,
+    pathtrace_tile<TV>(d.s, L, d.u, d.fclamp, d.spp, view_tile(d, b), (ushort4*)d.radiance, (uchar4*)d.albedo,
                        (float4*)d.gbuf_cur, rc);
     count4(counters, rc.rays, rc.bsteps, rc.vsteps, rc.hits);
     count4(d.vcount, rc.rays, rc.bsteps, rc.vsteps, rc.hits);

@@ -891,7 +891,7 @@ int vrt_autotune(vrt_ctx* ctx, const vrt_scene* scene, uint64_t nrays, const vrt
   auto time_it = [&](auto&& launch, float* out_ms) -> int {
     HIP_TRY(ctx, launch());  // warm-up
     HIP_TRY(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
-    for (int r = 0; r < 3; ++r) HIP_TRY(ctx, launch());
+    for (int r = 0; r < 5; ++r) HIP_TRY(ctx, launch());
     HIP_TRY(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
     HIP_TRY(ctx, hipEventSynchronize(ctx->ev[5]));
     HIP_TRY(ctx, hipEventElapsedTime(out_ms, ctx->ev[4], ctx->ev[5]));
