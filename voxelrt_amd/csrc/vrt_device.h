@@ -99,24 +99,29 @@ VRT_D void sincos_r(float x, float* s, float* c) {
 }
 
 // 2^x = 2^k * e^(f ln2), k = round(x), Taylor series to degree 7.
+// Branch-free: the special cases (NaN -> NaN, x <= -126 -> 0, x >= 128 ->
+// inf) are selected over the polynomial of an in-range stand-in, so lanes of
+// a wave never diverge here; in-range arguments take exactly the operations
+// of the scalar definition (the oracle's exp2_own).
 VRT_D float exp2_r(float x) {
-  if (x != x) return x;
-  if (!(x > -126.0f)) return 0.0f;
-  if (x >= 128.0f) return __builtin_inff();
+  const bool nan = x != x, lo = !(x > -126.0f), hi = x >= 128.0f;
+  const float xs = (nan || lo || hi) ? 0.0f : x;
   const float LN2 = 0x1.62e43p-1f;
-  float k = vfloor(x + 0.5f);
-  float f = x - k;
-  float y = f * LN2;
+  const float k = vfloor(xs + 0.5f);
+  const float f = xs - k;
+  const float y = f * LN2;
   float p = ((((((float)(1.0 / 5040.0) * y + (float)(1.0 / 720.0)) * y +
                 (float)(1.0 / 120.0)) * y + (float)(1.0 / 24.0)) * y +
               (float)(1.0 / 6.0)) * y + 0.5f) * y;
   p = (p + 1.0f) * y + 1.0f;
   int ki = (int)k;
-  if (ki > 127) {
-    p = p * 2.0f;
-    ki -= 1;
-  }
-  return p * __builtin_bit_cast(float, (uint32_t)(ki + 127) << 23);
+  const bool top = ki > 127;  // 2^128 = 2 * 2^127
+  p = top ? p * 2.0f : p;
+  ki = top ? ki - 1 : ki;
+  float r = p * __builtin_bit_cast(float, (uint32_t)(ki + 127) << 23);
+  r = hi ? __builtin_inff() : r;
+  r = lo ? 0.0f : r;
+  return nan ? x : r;
 }
 
 // log2 x = e + ln(m)/ln2, ln m = 2 atanh(s), s = (m-1)/(m+1), m in

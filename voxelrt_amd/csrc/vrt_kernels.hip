@@ -928,19 +928,22 @@ VRT_D void atrous_tile(const Uniforms& u, uint32_t tile, const void* __restrict_
 #pragma unroll
     for (int dx = -2; dx <= 2; ++dx) {
       if (dx == 0 && dy == 0) continue;
+      // taps outside the image or on sky add nothing (atrous.wgsl:50-53); the
+      // weight is computed for every lane and the sums select, so the wave
+      // does not branch per tap (the halo holds every tap, in or out)
       const int qx = x + dx * STEP, qy = y + dy * STEP;
-      if (qx < 0 || qy < 0 || qx >= w || qy >= h) continue;
       const int l = (ty + HALO + dy * STEP) * S + (tx + HALO + dx * STEP);
       const float4 gq = sg[l];
-      if (gq.w < 0.0f) continue;
       const float4 cq = sc[l];
+      const bool use = !(qx < 0 || qy < 0 || qx >= w || qy >= h) && !(gq.w < 0.0f);
       const float wKernel = K[dx < 0 ? -dx : dx] * K[dy < 0 ? -dy : dy];
       const float wNormal = pow32(vmax(dot(gn, mk3(gq.x, gq.y, gq.z)), 0.0f));
       const float wDepth = exp_r(-div_rcp(vabs(g.w - gq.w), dscale, rdscale));
       const float wLum = exp_r(-div_rcp(vabs(lumC - dot(mk3(cq.x, cq.y, cq.z), lw)), sigmaL, rsigmaL));
       const float wt = ((wKernel * wNormal) * wDepth) * wLum;
-      sum = sum + mk3(cq.x, cq.y, cq.z) * wt;
-      wSum += wt;
+      const f3 s2 = sum + mk3(cq.x, cq.y, cq.z) * wt;
+      sum = mk3(use ? s2.x : sum.x, use ? s2.y : sum.y, use ? s2.z : sum.z);
+      wSum = use ? wSum + wt : wSum;
     }
   }
   const f3 o = sum / vmax(wSum, 1e-6f);
