@@ -400,7 +400,7 @@ def main():
     ap.add_argument("--serve-views", type=int, default=64)
     ap.add_argument("--serve-pixels", type=int, default=16 << 20)
     ap.add_argument("--batch-mode", type=int, default=2, choices=[0, 1, 2])
-    ap.add_argument("--pt-kernel", default="auto", choices=["auto", "0", "1", "2", "3", "4", "5", "6"])
+    ap.add_argument("--pt-kernel", default="auto", choices=["auto", "0", "1", "2", "3", "4", "5", "6", "7"])
     ap.add_argument("--pt-waves", type=int, default=0, choices=[0, 4, 5])
     args = ap.parse_args()
     if args.gpus != 1:

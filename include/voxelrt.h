@@ -236,7 +236,10 @@ typedef struct vrt_tuning {
                                  2 = one ray per lane, single-stream DDA,
                                  3 = one ray per lane, occupancy staged in LDS,
                                  4 = one ray per lane, single loop, both levels in LDS,
-                                 5 = persistent ray queue, single loop in LDS */
+                                 5 = persistent ray queue, single loop in LDS,
+                                 6 = one ray per lane, single loop on global memory
+                                     (the form 4 falls back to when its LDS does not
+                                     fit, 512^3 and up) */
   uint32_t trace_blocks;      /* persistent grid (256-thread blocks), 0 = default */
   uint32_t trace_steps;       /* DDA iterations between queue refill checks */
   uint32_t trace_min_refill;  /* idle lanes (of 64) that trigger a refill */
@@ -247,7 +250,8 @@ typedef struct vrt_tuning {
                                  5 = as 4 with one trace call site per lane (shadow and
                                      bounce rays of different lanes share the DDA loop),
                                  6 = as 1 with the single-loop DDA in LDS (brick
-                                     entries batched per dda_batch) */
+                                     entries batched per dda_batch),
+                                 7 = as 4 with the single loop on global memory */
   uint32_t pt_blocks;
   uint32_t pt_steps;
   uint32_t pt_min_refill;
@@ -261,7 +265,7 @@ typedef struct vrt_tuning {
                                      queue + fixed-grid filter passes,
                                  2 = as 1, replayed from a captured hipGraph */
   uint32_t batch_blocks;      /* persistent path-tracer grid, 0 = 8 per CU */
-  uint32_t dda_batch;         /* traversal kernels 4 and 5: lanes of a wave that wait
+  uint32_t dda_batch;         /* traversal kernels 4, 5 and 6: lanes of a wave that wait
                                  at occupied bricks before they enter them together
                                  (1..64; 0 = 1, enter at once).  The path tracer
                                  always enters at once. */

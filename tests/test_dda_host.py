@@ -21,7 +21,8 @@ from voxelrt_amd import configs as cf
 from tests.sparse import sparse_rays as _sparse_rays, sparse_scenes as _sparse_scenes
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-VARIANTS = {"nested": 0, "flat": 1, "lds": 2, "fast": 3, "fast_skip": 4, "fast_batched_skip": 5}
+VARIANTS = {"nested": 0, "flat": 1, "lds": 2, "fast": 3, "fast_skip": 4, "fast_batched_skip": 5, "fast_global": 6,
+            "fast_global_batched": 7}
 
 
 @pytest.fixture(scope="module")
@@ -102,7 +103,7 @@ def test_variants_on_world_scene_with_short_tmax(dda, variant):
                f"{variant} tmax={tmax}")
 
 
-@pytest.mark.parametrize("variant", ["fast", "fast_skip", "fast_batched_skip"])
+@pytest.mark.parametrize("variant", ["fast", "fast_skip", "fast_batched_skip", "fast_global"])
 @pytest.mark.parametrize("name", [s[0] for s in _sparse_scenes()])
 def test_empty_row_skip_matches_oracle(dda, variant, name):
     vox = dict(_sparse_scenes())[name]

@@ -78,6 +78,7 @@ def traverse(request, ctx):
 TRACE_VARIANTS = [dict(trace_kernel=0), dict(trace_kernel=2), dict(trace_kernel=3), dict(trace_kernel=4),
                   dict(trace_kernel=4, dda_batch=8), dict(trace_kernel=4, dda_batch=64),
                   dict(trace_kernel=4, empty_skip=1), dict(trace_kernel=4, dda_batch=16, empty_skip=1),
+                  dict(trace_kernel=6), dict(trace_kernel=6, dda_batch=16),
                   dict(trace_kernel=1, trace_blocks=2048, trace_steps=8, trace_min_refill=16),
                   dict(trace_kernel=1, trace_blocks=512, trace_steps=1, trace_min_refill=1),
                   dict(trace_kernel=1, trace_blocks=4096, trace_steps=16, trace_min_refill=64),
@@ -90,7 +91,7 @@ TRACE_VARIANTS = [dict(trace_kernel=0), dict(trace_kernel=2), dict(trace_kernel=
 
 @pytest.fixture(params=range(len(TRACE_VARIANTS)),
                 ids=["lane", "lane-flat", "lane-lds", "lane-fast", "lane-fast-b8", "lane-fast-b64", "lane-fast-skip",
-                     "lane-fast-b16-skip", "persist", "persist-s1", "persist-s16", "persist-fast", "persist-fast-333",
+                     "lane-fast-b16-skip", "lane-fast-global", "lane-fast-global-b16", "persist", "persist-s1", "persist-s16", "persist-fast", "persist-fast-333",
                      "persist-fast-b16", "persist-fast-skip"])
 def trace_variant(request, ctx):
     old = ctx.tuning().as_dict()
@@ -214,7 +215,7 @@ PT_VARIANTS = [dict(pt_kernel=0), dict(pt_kernel=0, pt_waves=5), dict(pt_kernel=
                dict(pt_kernel=4), dict(pt_kernel=4, pt_waves=5), dict(pt_kernel=5), dict(pt_kernel=5, pt_waves=5),
                dict(pt_kernel=4, pt_waves=5, empty_skip=1), dict(pt_kernel=4, pt_waves=0, empty_skip=1),
                dict(pt_kernel=4, pt_waves=5, pt_order=1), dict(pt_kernel=5, pt_waves=5, pt_order=2),
-               dict(pt_kernel=3, pt_order=3),
+               dict(pt_kernel=3, pt_order=3), dict(pt_kernel=7), dict(pt_kernel=7, pt_waves=5),
                dict(pt_kernel=1, pt_blocks=1024, pt_steps=8, pt_min_refill=32, pt_chunk=64),
                dict(pt_kernel=1, pt_blocks=97, pt_steps=1, pt_min_refill=1, pt_chunk=64),
                dict(pt_kernel=1, pt_blocks=2048, pt_steps=16, pt_min_refill=64, pt_chunk=256),
@@ -227,7 +228,8 @@ PT_VARIANTS = [dict(pt_kernel=0), dict(pt_kernel=0, pt_waves=5), dict(pt_kernel=
 @pytest.fixture(params=range(len(PT_VARIANTS)),
                 ids=["lane", "lane-w5", "lane-flat-w4", "lane-lds", "lane-lds-w4", "lane-lds-w5", "lane-fast",
                      "lane-fast-w5", "lane-fast1", "lane-fast1-w5", "lane-fast-w5-skip", "lane-fast-skip",
-                     "lane-fast-w5-xcd", "lane-fast1-w5-rev", "lane-lds-xcdrev",
+                     "lane-fast-w5-xcd", "lane-fast1-w5-rev", "lane-lds-xcdrev", "lane-fast-global",
+                     "lane-fast-global-w5",
                      "persist",
                      "persist-s1", "persist-s16", "persist-fast", "persist-fast-w4-b8", "persist-fast-w5-b64"])
 def pt_variant(request, ctx):
@@ -381,9 +383,11 @@ def test_frame_sequence_bitexact(ctx, pt_variant):
                                     dict(pt_kernel=5, pt_waves=4, batch_mode=1),
                                     dict(pt_kernel=4, pt_waves=5, batch_mode=0, pt_order=0),
                                     dict(pt_kernel=4, pt_waves=5, batch_mode=1, pt_order=0),
+                                    dict(pt_kernel=7, pt_waves=4, batch_mode=1),
                                     dict(pt_kernel=3, pt_waves=4, batch_mode=2)],
                          ids=["lane", "lds-w5", "flat-w4", "fast-w5", "queue-fast", "queue-7blocks",
                               "graph-fast", "queue-fast1-w4", "grid-fast-rowmajor", "queue-fast-rowmajor",
+                              "queue-fast-global-w4",
                               "graph-lds"])
 def test_batched_views_match_single_views(ctx, tuning):
     descs = [cf.world_desc(64, 1), cf.world_desc(128, 2), dict(cf.world_desc(64, 3), ny=32)]
@@ -473,3 +477,37 @@ def test_scene_edits_rebuild_on_device(ctx):
     with pytest.raises(v.VrtError):
         plain.edit(np.zeros((1, 1, 1), np.uint32), (0, 0, 0))  # not editable
     plain.release()
+
+
+def test_large_scene_runs_the_global_dda(ctx):
+    """A 512^3 grid: what the LDS form of the single-loop DDA stages (padded
+    bitmap, prefix counts, row distances: 72 KiB) exceeds the LDS budget, so
+    trace_kernel 4 and pt_kernel 4 run its global-memory form; hits and a
+    small frame stay bit-exact, as for the explicit global variants."""
+    rng = np.random.default_rng(21)
+    vox = np.zeros((512, 512, 512), np.uint32)
+    for _ in range(40):
+        x, y, z = (int(c) for c in rng.integers(0, 480, 3))
+        s = int(rng.integers(4, 32))
+        vox[z:z + s, y:y + s, x:x + s] = int(rng.integers(1, 1 << 24)) | 1
+    vox[:, :16, :] = 0x00406080 | 1
+    print("large scene: 512^3, 40 blocks + floor", flush=True)
+    sc = ctx.upload(vox)
+    og = oracle.OracleGrid(vox)
+    rays = cf.traverse_rays(1 << 16, n=512, seed=4)
+    want = oracle.trace_rays(og, rays)
+    old = ctx.tuning().as_dict()
+    try:
+        for tuning in (dict(trace_kernel=4), dict(trace_kernel=6, dda_batch=16), dict(trace_kernel=0)):
+            ctx.set_tuning(**tuning)
+            _hits_equal(ctx.trace_rays(sc, rays), want, f"512^3 {tuning}")
+        u, _ = v.make_uniforms((16.0, 6.0, 3.0), 0.3, -0.1, 96, 64, frame=3, bounces=2, flags=0)
+        rad, alb, gb, _ = oracle.pathtrace(og, to_oracle_uniforms(u), 48.0, 1)
+        for tuning in (dict(pt_kernel=4, pt_waves=5), dict(pt_kernel=7)):
+            ctx.set_tuning(**tuning)
+            got, _ = _render_one(ctx, sc, 990, u, 48.0, 1, 96, 64)
+            assert_bits_equal(f"512^3 radiance {tuning}", got["radiance"], rad)
+            assert_bits_equal(f"512^3 gbuf {tuning}", got["gbuf"], gb)
+    finally:
+        ctx.set_tuning(**old)
+        sc.release()

@@ -107,7 +107,15 @@ VRT_D bool bit_at(uint32_t w, uint32_t i) {
 #endif
 }
 
-struct DdaFast {
+// GM = false: occupancy in LDS (LdsScene), the entered brick's mask copied into
+// the lane's LDS slot.  GM = true: the same loop on global memory -- the padded
+// bitmap, its prefix counts and the row distances read from the scene's
+// buffers (L1/L2-resident), the entered brick's mask read in place from its
+// pool entry (no copy) -- for scenes whose bitmaps do not fit in LDS.  Both
+// levels then read words of one array at a base pointer (bitmap, or the pool
+// entry) with a 4-byte stride, so the level still only switches registers.
+template <bool GM>
+struct DdaFastT {
   float ro[3], rdv[3], inv[3];
   int sd[3];
   float tExit, maxT;
@@ -116,6 +124,7 @@ struct DdaFast {
   int di[3];
   uint32_t idx, cnt, iter;
   uint32_t base4, wsh, inc;  // LDS byte base, word stride shift (2: words, 10: slot rows), step increment
+  const char* gbase;         // GM: the current level's word array
   float tBound, t;             // coarse: leave at t > tExit; fine: never
   uint32_t axs;  // axis of the last step x 8 (its field offset in cnt)
   // coarse state saved while walking a brick
@@ -173,7 +182,13 @@ struct DdaFast {
     idx = (uint32_t)bp[0] | ((uint32_t)bp[1] << s.psh1) | ((uint32_t)bp[2] << s.psh2);
     cnt = pack_left(bp, BG, sd);
     iter = (uint32_t)(s.bgx + s.bgy + s.bgz + 2);
-    base4 = 4u * L.bits_off;
+    if constexpr (GM) {
+      gbase = (const char*)s.pbits;
+      base4 = 0u;
+    } else {
+      gbase = nullptr;
+      base4 = 4u * L.bits_off;
+    }
     wsh = 2u;
     inc = 1u;
     tBound = tExit;
@@ -211,17 +226,24 @@ struct DdaFast {
   // ro + rd * (t + 1e-4), clamped into the brick, with tMax from the origin.
   VRT_D void enter(const SceneDev& s, const LdsScene& L) {
     const uint32_t w = idx >> 5;
-    const uint32_t word = L.lds[L.bits_off + w];
-    const uint32_t slot = L.lds[L.prefix_off + w] + (uint32_t)__builtin_popcount(word & ((1u << (idx & 31u)) - 1u));
-    blk = slot * (uint32_t)POOL_WORDS;
-    const uint4* q = (const uint4*)(s.pool + blk);
+    if constexpr (GM) {
+      const uint32_t word = s.pbits[w];
+      const uint32_t slot = s.pprefix[w] + (uint32_t)__builtin_popcount(word & ((1u << (idx & 31u)) - 1u));
+      blk = slot * (uint32_t)POOL_WORDS;
+      gbase = (const char*)(s.pool + blk);  // mask words 0..15 of the entry
+    } else {
+      const uint32_t word = L.lds[L.bits_off + w];
+      const uint32_t slot = L.lds[L.prefix_off + w] + (uint32_t)__builtin_popcount(word & ((1u << (idx & 31u)) - 1u));
+      blk = slot * (uint32_t)POOL_WORDS;
+      const uint4* q = (const uint4*)(s.pool + blk);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const uint4 v = q[k];
-      L.lds[((4 * k + 0) << 8) + L.lane] = v.x;
-      L.lds[((4 * k + 1) << 8) + L.lane] = v.y;
-      L.lds[((4 * k + 2) << 8) + L.lane] = v.z;
-      L.lds[((4 * k + 3) << 8) + L.lane] = v.w;
+      for (int k = 0; k < 4; ++k) {
+        const uint4 v = q[k];
+        L.lds[((4 * k + 0) << 8) + L.lane] = v.x;
+        L.lds[((4 * k + 1) << 8) + L.lane] = v.y;
+        L.lds[((4 * k + 2) << 8) + L.lane] = v.z;
+        L.lds[((4 * k + 3) << 8) + L.lane] = v.w;
+      }
     }
     const uint32_t mx = (1u << s.psh1) - 1u, my = (1u << (s.psh2 - s.psh1)) - 1u;
     const uint32_t bx = idx & mx, by = (idx >> s.psh1) & my, bz = idx >> s.psh2;
@@ -248,8 +270,10 @@ struct DdaFast {
     const int n8[3] = {8, 8, 8};
     cnt = pack_left(lp, n8, sd);
     iter = 24u;
-    base4 = 4u * L.lane;
-    wsh = 10u;
+    if constexpr (!GM) {
+      base4 = 4u * L.lane;
+      wsh = 10u;
+    }
     inc = 0x10000u;
     tBound = __builtin_inff();
   }
@@ -267,8 +291,12 @@ struct DdaFast {
     idx = idxB;
     cnt = cntB;
     iter = iterB;
-    base4 = 4u * L.bits_off;
-    wsh = 2u;
+    if constexpr (GM) {
+      gbase = (const char*)s.pbits;
+    } else {
+      base4 = 4u * L.bits_off;
+      wsh = 2u;
+    }
     inc = 1u;
     tBound = tExit;
   }
@@ -327,12 +355,17 @@ struct DdaFast {
   template <bool Skip = false>
   VRT_D uint32_t step(const SceneDev& s, const LdsScene& L) {
     if (t > tBound) return DDA_MISS;
-    const uint32_t word = *(const uint32_t*)((const char*)L.lds + (((idx >> 5) << wsh) + base4));
+    uint32_t word;
+    if constexpr (GM)
+      word = *(const uint32_t*)(gbase + ((idx >> 5) << 2));
+    else
+      word = *(const uint32_t*)((const char*)L.lds + (((idx >> 5) << wsh) + base4));
     steps += inc;
     if (bit_at(word, idx)) return inc != 1u ? DDA_HIT : DDA_ENTER;  // voxel.wgsl:122 / 111
     if constexpr (Skip) {
       if (inc == 1u) {
-        const uint32_t D = (L.lds[L.dist_off + (idx >> 7)] >> (8u * ((idx >> 5) & 3u))) & 255u;
+        const uint32_t D = GM ? (uint32_t)s.pdist[idx >> 5]
+                             : (L.lds[L.dist_off + (idx >> 7)] >> (8u * ((idx >> 5) & 3u))) & 255u;
         if (D >= 2u && skip_rows(D)) return DDA_RUN;  // runs that can also move in y and z
       }
     }
@@ -401,11 +434,13 @@ struct DdaFast {
   }
 };
 
-template <bool Batched = false, bool Skip = false>
+using DdaFast = DdaFastT<false>;
+
+template <bool Batched = false, bool Skip = false, bool GM = false>
 VRT_D void trace_fast(const SceneDev& s, const LdsScene& L, f3 roW, f3 rd, float maxTWorld, TraceResult& o) {
-  DdaFast st;
+  DdaFastT<GM> st;
   if (st.begin(s, L, roW, rd, maxTWorld, o)) return;
-  st.finish(s, st.run<Batched, Skip>(s, L), o);
+  st.finish(s, st.template run<Batched, Skip>(s, L), o);
 }
 
 }  // namespace vrt

@@ -12,10 +12,12 @@ using Uniforms = vrt_uniforms;
 struct SceneDev;
 
 // Trace variant ids for the one-ray/pixel-per-lane kernels (launch_* `tv`).
-constexpr int TV_NESTED = 0, TV_FLAT = 1, TV_LDS = 2, TV_FAST = 3, TV_FAST1 = 4, TV_FASTSKIP = 5;
+constexpr int TV_NESTED = 0, TV_FLAT = 1, TV_LDS = 2, TV_FAST = 3, TV_FAST1 = 4, TV_FASTSKIP = 5, TV_FASTG = 6;
 // Variants that trace with the single-loop DDA on LDS (TV_FAST1: the same
 // DDA, path tracer with one trace call site per lane).
 // TV_FASTSKIP: the same DDA taking runs of empty brick rows in bulk.
+// TV_FASTG: the single-loop DDA on global memory (no LDS), the form for
+// scenes whose bitmaps do not fit in LDS.
 constexpr bool tv_fast(int tv) { return tv == TV_FAST || tv == TV_FAST1 || tv == TV_FASTSKIP; }
 constexpr bool tv_uses_lds(int tv) { return tv == TV_LDS || tv_fast(tv); }
 // Largest occupancy bitmap staged in LDS (words): 48 KiB, 393216 bricks

@@ -220,6 +220,7 @@ __global__ __launch_bounds__(256) void k_pack_pool(const uint32_t* __restrict__ 
 //              bitmap + prefix counts + per-lane mask slot
 //   TV_FAST1   TV_FAST traversal; the path tracer uses shade_pixel_1
 //   TV_FASTSKIP TV_FAST with the bulk skip of empty brick rows
+//   TV_FASTG   the single-loop DDA on global memory (DdaFastT<true>)
 // What a lane needs to trace in a given variant.
 struct LaneMem {
   LdsMem m;    // TV_LDS
@@ -234,6 +235,8 @@ VRT_D void trace_sel(const SceneDev& s, const LaneMem& L, f3 ro, f3 rd, float tm
     trace_m(s, L.m, ro, rd, tmax, h);
   else if constexpr (TV == TV_FASTSKIP)
     trace_fast<false, true>(s, L.f, ro, rd, tmax, h);
+  else if constexpr (TV == TV_FASTG)
+    trace_fast<false, false, true>(s, L.f, ro, rd, tmax, h);
   else if constexpr (tv_fast(TV))
     trace_fast(s, L.f, ro, rd, tmax, h);
   else
@@ -291,8 +294,9 @@ __global__ __launch_bounds__(256) void k_trace_rays(SceneDev s, const float* __r
   for (uint64_t i = base0; i < n; i += stride) {
     const float* r = rays + i * 6;
     TraceResult h;
-    if constexpr (tv_fast(TV))  // wave-batched brick entries (SceneDev::entry_batch)
-      trace_fast<true, TV == TV_FASTSKIP>(s, L.f, mk3(r[0], r[1], r[2]), mk3(r[3], r[4], r[5]), tmax, h);
+    if constexpr (tv_fast(TV) || TV == TV_FASTG)  // wave-batched brick entries (SceneDev::entry_batch)
+      trace_fast<true, TV == TV_FASTSKIP, TV == TV_FASTG>(s, L.f, mk3(r[0], r[1], r[2]), mk3(r[3], r[4], r[5]),
+                                                           tmax, h);
     else
       trace_sel<TV>(s, L, mk3(r[0], r[1], r[2]), mk3(r[3], r[4], r[5]), tmax, h);
     uint4 rec;
@@ -1163,10 +1167,12 @@ static inline uint32_t lds_words(int tv, uint32_t bit_words, uint32_t pwords) {
 static inline uint32_t lds_bytes(int tv, uint32_t bit_words, uint32_t pwords) {
   return tv_uses_lds(tv) ? (16u * 256u + lds_words(tv, bit_words, pwords)) * 4u : 0u;
 }
-// Variant actually launched: LDS variants fall back to nested loops when the
-// bitmap does not fit.
+// Variant actually launched: LDS variants fall back when what they stage
+// does not fit -- the single-loop ones to their global-memory form, the
+// nested one to nested loops on HBM.
 static inline int lds_variant(int tv, uint32_t bit_words, uint32_t pwords) {
-  return tv_uses_lds(tv) && lds_words(tv, bit_words, pwords) > LDS_BIT_WORDS_MAX ? TV_NESTED : tv;
+  if (!tv_uses_lds(tv) || lds_words(tv, bit_words, pwords) <= LDS_BIT_WORDS_MAX) return tv;
+  return tv_fast(tv) ? TV_FASTG : TV_NESTED;
 }
 
 hipError_t launch_brick_masks(hipStream_t st, const uint32_t* vox, int nx, int ny, int bgx,
@@ -1232,6 +1238,8 @@ hipError_t launch_trace_rays(hipStream_t st, const SceneDev& s, const float* ray
   const uint32_t shmem = lds_bytes(tv, s.bit_words, s.pwords);
   if (tv == TV_FAST)
     hipLaunchKernelGGL(k_trace_rays<TV_FAST>, g, dim3(256), shmem, st, s, rays, n, tmax, (uint4*)hits, counters);
+  else if (tv == TV_FASTG)
+    hipLaunchKernelGGL(k_trace_rays<TV_FASTG>, g, dim3(256), 0, st, s, rays, n, tmax, (uint4*)hits, counters);
   else if (tv == TV_FASTSKIP)
     hipLaunchKernelGGL(k_trace_rays<TV_FASTSKIP>, g, dim3(256), shmem, st, s, rays, n, tmax, (uint4*)hits,
                        counters);
@@ -1283,6 +1291,9 @@ hipError_t launch_pathtrace(hipStream_t st, const SceneDev& s, const Uniforms& u
     case TV_FASTSKIP * 8 + 4: go(k_pathtrace<TV_FASTSKIP, 4>); break;
     case TV_FASTSKIP * 8 + 5: go(k_pathtrace<TV_FASTSKIP, 5>); break;
     case TV_FASTSKIP * 8 + 1: go(k_pathtrace<TV_FASTSKIP, 1>); break;
+    case TV_FASTG * 8 + 4: go(k_pathtrace<TV_FASTG, 4>); break;
+    case TV_FASTG * 8 + 5: go(k_pathtrace<TV_FASTG, 5>); break;
+    case TV_FASTG * 8 + 1: go(k_pathtrace<TV_FASTG, 1>); break;
     case TV_FAST * 8 + 4: go(k_pathtrace<TV_FAST, 4>); break;
     case TV_FAST * 8 + 5: go(k_pathtrace<TV_FAST, 5>); break;
     case TV_FAST * 8 + 1: go(k_pathtrace<TV_FAST, 1>); break;
@@ -1337,6 +1348,9 @@ hipError_t launch_pathtrace_batch(hipStream_t st, const ViewDesc* views, uint32_
     case TV_FASTSKIP * 8 + 4: go(k_pathtrace_batch<TV_FASTSKIP, 4>); break;
     case TV_FASTSKIP * 8 + 5: go(k_pathtrace_batch<TV_FASTSKIP, 5>); break;
     case TV_FASTSKIP * 8 + 1: go(k_pathtrace_batch<TV_FASTSKIP, 1>); break;
+    case TV_FASTG * 8 + 4: go(k_pathtrace_batch<TV_FASTG, 4>); break;
+    case TV_FASTG * 8 + 5: go(k_pathtrace_batch<TV_FASTG, 5>); break;
+    case TV_FASTG * 8 + 1: go(k_pathtrace_batch<TV_FASTG, 1>); break;
     case TV_FAST * 8 + 4: go(k_pathtrace_batch<TV_FAST, 4>); break;
     case TV_FAST * 8 + 5: go(k_pathtrace_batch<TV_FAST, 5>); break;
     case TV_FAST * 8 + 1: go(k_pathtrace_batch<TV_FAST, 1>); break;
@@ -1368,6 +1382,9 @@ hipError_t launch_batch_persistent(hipStream_t st, BatchHead* head, const ViewDe
     case TV_FASTSKIP * 8 + 4: go(k_pathtrace_batch_q<TV_FASTSKIP, 4>); break;
     case TV_FASTSKIP * 8 + 5: go(k_pathtrace_batch_q<TV_FASTSKIP, 5>); break;
     case TV_FASTSKIP * 8 + 1: go(k_pathtrace_batch_q<TV_FASTSKIP, 1>); break;
+    case TV_FASTG * 8 + 4: go(k_pathtrace_batch_q<TV_FASTG, 4>); break;
+    case TV_FASTG * 8 + 5: go(k_pathtrace_batch_q<TV_FASTG, 5>); break;
+    case TV_FASTG * 8 + 1: go(k_pathtrace_batch_q<TV_FASTG, 1>); break;
     case TV_FAST * 8 + 4: go(k_pathtrace_batch_q<TV_FAST, 4>); break;
     case TV_FAST * 8 + 5: go(k_pathtrace_batch_q<TV_FAST, 5>); break;
     case TV_FAST * 8 + 1: go(k_pathtrace_batch_q<TV_FAST, 1>); break;

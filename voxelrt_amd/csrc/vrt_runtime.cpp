@@ -549,17 +549,33 @@ int vrt_scene_bricks(vrt_ctx* ctx, const vrt_scene* scene, uint32_t* bricks, uin
 // ---------------------------------------------------------------------------
 // traversal
 // ---------------------------------------------------------------------------
-// vrt_tuning kernel id -> trace variant of the one-per-lane kernels:
+// vrt_tuning.trace_kernel -> trace variant of the one-ray-per-lane kernel:
 // 0 nested, 2 single-stream (flat), 3 LDS-staged occupancy, 4 single loop
-// with both levels in LDS (with the empty-row skip when skip is set), 5 as 4
-// with one trace call site in the path tracer.
-static int lane_variant(uint32_t kernel, uint32_t skip) {
+// with both levels in LDS (with the empty-row skip when skip is set), 6 the
+// single loop on global memory.
+static int trace_variant(uint32_t kernel, uint32_t skip) {
+  switch (kernel) {
+    case 2: return vrt::TV_FLAT;
+    case 3: return vrt::TV_LDS;
+    case 4: return skip ? vrt::TV_FASTSKIP : vrt::TV_FAST;
+    case 6: return vrt::TV_FASTG;
+    default: return vrt::TV_NESTED;
+  }
+}
+
+// vrt_tuning.pt_kernel -> trace variant of the one-pixel-per-lane path
+// tracers (the single-view kernel and the batched forms): 0 nested, 2 flat,
+// 3 LDS-staged occupancy, 4 single loop in LDS (empty-row skip when skip is
+// set), 5 as 4 with one trace call site, 6 (persistent in the single-view
+// form) as 4, 7 the single loop on global memory.
+static int pt_variant(uint32_t kernel, uint32_t skip) {
   switch (kernel) {
     case 2: return vrt::TV_FLAT;
     case 3: return vrt::TV_LDS;
     case 4: return skip ? vrt::TV_FASTSKIP : vrt::TV_FAST;
     case 5: return vrt::TV_FAST1;
-    case 6: return vrt::TV_FAST;  // (batched form of the persistent kernel 6)
+    case 6: return vrt::TV_FAST;
+    case 7: return vrt::TV_FASTG;
     default: return vrt::TV_NESTED;
   }
 }
@@ -575,7 +591,7 @@ hipError_t vrt_ctx::launch_pathtrace(const vrt::SceneDev& s0, const vrt_uniforms
                                             tuning.pt_kernel == 6 ? vrt::TV_FAST : vrt::TV_NESTED,
                                             (int)tuning.pt_waves);
   return vrt::launch_pathtrace(stream, s, u, fclamp, spp, radiance, albedo, gbuf,
-                               (unsigned long long*)d_counters.ptr, lane_variant(tuning.pt_kernel, tuning.empty_skip),
+                               (unsigned long long*)d_counters.ptr, pt_variant(tuning.pt_kernel, tuning.empty_skip),
                                (int)tuning.pt_waves);
 }
 
@@ -590,7 +606,7 @@ hipError_t vrt_ctx::launch_trace(const vrt::SceneDev& s0, const float* rays, uin
                                         tuning.trace_kernel != 5 ? vrt::TV_NESTED
                                         : (tuning.empty_skip ? vrt::TV_FASTSKIP : vrt::TV_FAST));
   return vrt::launch_trace_rays(stream, s, rays, n, tmax, hits, (unsigned long long*)d_counters.ptr,
-                                trace_blocks, lane_variant(tuning.trace_kernel, tuning.empty_skip));
+                                trace_blocks, trace_variant(tuning.trace_kernel, tuning.empty_skip));
 }
 
 int vrt_trace_rays_device(vrt_ctx* ctx, const vrt_scene* scene, const float* d_rays, uint64_t n,
@@ -868,7 +884,7 @@ int vrt_tuning_get(vrt_ctx* ctx, vrt_tuning* out) {
 int vrt_tuning_set(vrt_ctx* ctx, const vrt_tuning* in) {
   if (!ctx || !in) return VRT_ERR_INVALID;
   std::lock_guard<std::mutex> lock_(ctx->mu);
-  if (in->trace_kernel > 5 || in->pt_kernel > 6 || in->trace_blocks > (1u << 20) ||
+  if (in->trace_kernel > 6 || in->pt_kernel > 7 || in->trace_blocks > (1u << 20) ||
       in->pt_blocks > (1u << 20) || in->trace_steps > 1024 || in->pt_steps > 1024 ||
       in->trace_min_refill > 64 || in->pt_min_refill > 64 || in->pt_chunk > (1u << 16) ||
       !(in->pt_waves == 0 || in->pt_waves == 4 || in->pt_waves == 5) || in->batch_mode > 2 ||
@@ -940,9 +956,9 @@ int vrt_autotune(vrt_ctx* ctx, const vrt_scene* scene, uint64_t nrays, const vrt
     best.dda_batch = 1;
     // one-ray-per-lane variants; the single-loop one also over its brick
     // entry batch
-    for (uint32_t k : {2u, 3u, 4u})
+    for (uint32_t k : {2u, 3u, 4u, 6u})
       for (uint32_t db : {1u, 8u, 16u, 32u}) {
-        if (k != 4u && db != 1u) continue;
+        if (k < 4u && db != 1u) continue;
         ctx->tuning.trace_kernel = k;
         ctx->tuning.dda_batch = db;
         rc = time_it([&] { return ctx->launch_trace(s, (const float*)ctx->scratch_rays.ptr, n, 1e4f,
@@ -998,7 +1014,7 @@ int vrt_autotune(vrt_ctx* ctx, const vrt_scene* scene, uint64_t nrays, const vrt
     best.pt_kernel = 0;
     best.pt_waves = 0;
     // one-pixel-per-lane variants x register budgets (waves per SIMD)
-    for (uint32_t k : {0u, 2u, 3u, 4u, 5u})
+    for (uint32_t k : {0u, 2u, 3u, 4u, 5u, 7u})
       for (uint32_t wv : {0u, 4u, 5u}) {
         if (k == 0 && wv == 0) continue;  // measured above
         ctx->tuning.pt_kernel = k;
@@ -1240,7 +1256,7 @@ int vrt_ctx::render_batch(vrt::BatchLane& lane, const vrt_view* views, uint32_t 
   HIP_TRY(this, hipEventRecord(lane.desc_done[stage], st));
   HIP_TRY(this, hipEventRecord(ev[0], st));
   if (tuning.batch_mode == 1 || tuning.batch_mode == 2) {
-    const int tv = lane_variant(tuning.pt_kernel, tuning.empty_skip), wv = (int)tuning.pt_waves;
+    const int tv = pt_variant(tuning.pt_kernel, tuning.empty_skip), wv = (int)tuning.pt_waves;
     const uint32_t ptb = tuning.batch_blocks ? tuning.batch_blocks : (uint32_t)cus * 8u;
     const uint32_t gsb = (uint32_t)cus * 8u;
     auto enqueue = [&] {
@@ -1278,7 +1294,7 @@ int vrt_ctx::render_batch(vrt::BatchLane& lane, const vrt_view* views, uint32_t 
     return VRT_OK;
   }
   HIP_TRY(this, vrt::launch_pathtrace_batch(st, dd, nviews, tiles, (unsigned long long*)d_counters.ptr,
-                                            lane_variant(tuning.pt_kernel, tuning.empty_skip), (int)tuning.pt_waves,
+                                            pt_variant(tuning.pt_kernel, tuning.empty_skip), (int)tuning.pt_waves,
                                             max_bit_words, max_pwords));
   HIP_TRY(this, hipEventRecord(ev[1], st));
   HIP_TRY(this, vrt::launch_temporal_batch(st, dd, nviews, tiles));
