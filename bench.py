@@ -477,7 +477,7 @@ def main():
                      "traffic": _pmc_traffic("pathtrace_frame1080_pmc.txt",
                                              f"pt_kernel={tuned['pt_kernel']},pt_waves={tuned['pt_waves']}"),
                      "alg_bytes_per_launch": int(alg_bytes), "avg_launch_ms": round(pt_avg_ms, 4)},
-        "tuning": {k: tuned[k] for k in ("pt_kernel", "pt_waves", "pt_order", "pt_blocks", "pt_steps",
+        "tuning": {k: tuned[k] for k in ("pt_kernel", "pt_waves", "pt_order", "pt_tile", "pt_blocks", "pt_steps",
                                           "pt_min_refill", "pt_chunk", "dda_batch")},
         "breakdown": {"gpu_ms_pathtrace": round(pt_avg_ms, 4), "gpu_ms_temporal": round(tp_ms / K, 4),
                       "gpu_ms_atrous3": round(at_ms / K, 4), "gpu_ms_frame": round(fr_ms / K, 4),

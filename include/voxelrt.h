@@ -278,7 +278,9 @@ typedef struct vrt_tuning {
                                  XCD, 2 row-major from the last tile (default: the
                                  bottom rows, usually the costliest, start first),
                                  3 bands from their last tile */
-  uint32_t reserved[1];
+  uint32_t pt_tile;           /* path tracer 7 (global memory): 0 = 16x16 tiles of 256
+                                 threads, 1 = 8x8 tiles of one wave */
+  uint32_t reserved[4];
 } vrt_tuning;
 int vrt_tuning_get(vrt_ctx* ctx, vrt_tuning* out);
 int vrt_tuning_set(vrt_ctx* ctx, const vrt_tuning* in);

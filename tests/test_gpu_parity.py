@@ -216,6 +216,7 @@ PT_VARIANTS = [dict(pt_kernel=0), dict(pt_kernel=0, pt_waves=5), dict(pt_kernel=
                dict(pt_kernel=4, pt_waves=5, empty_skip=1), dict(pt_kernel=4, pt_waves=0, empty_skip=1),
                dict(pt_kernel=4, pt_waves=5, pt_order=1), dict(pt_kernel=5, pt_waves=5, pt_order=2),
                dict(pt_kernel=3, pt_order=3), dict(pt_kernel=7), dict(pt_kernel=7, pt_waves=5),
+               dict(pt_kernel=7, pt_tile=1), dict(pt_kernel=7, pt_waves=4, pt_tile=1, pt_order=0),
                dict(pt_kernel=1, pt_blocks=1024, pt_steps=8, pt_min_refill=32, pt_chunk=64),
                dict(pt_kernel=1, pt_blocks=97, pt_steps=1, pt_min_refill=1, pt_chunk=64),
                dict(pt_kernel=1, pt_blocks=2048, pt_steps=16, pt_min_refill=64, pt_chunk=256),
@@ -229,7 +230,7 @@ PT_VARIANTS = [dict(pt_kernel=0), dict(pt_kernel=0, pt_waves=5), dict(pt_kernel=
                 ids=["lane", "lane-w5", "lane-flat-w4", "lane-lds", "lane-lds-w4", "lane-lds-w5", "lane-fast",
                      "lane-fast-w5", "lane-fast1", "lane-fast1-w5", "lane-fast-w5-skip", "lane-fast-skip",
                      "lane-fast-w5-xcd", "lane-fast1-w5-rev", "lane-lds-xcdrev", "lane-fast-global",
-                     "lane-fast-global-w5",
+                     "lane-fast-global-w5", "lane-fast-global-8x8", "lane-fast-global-8x8-w4-rowmajor",
                      "persist",
                      "persist-s1", "persist-s16", "persist-fast", "persist-fast-w4-b8", "persist-fast-w5-b64"])
 def pt_variant(request, ctx):

@@ -714,21 +714,23 @@ VRT_D PixelOut shade_pixel_1(const SceneDev& s, const LaneMem& L, const Uniforms
 
 // Pixel p of a launch covering TILE x TILE tiles, row-major tiles: keeps the
 // 64 lanes of a wave on a compact 16x4 patch (coherent primary rays).
+// T = 8: 8x8 tiles of one wave each.
+template <int T = 16>
 VRT_D bool tile_pixel(uint32_t w, uint32_t h, uint32_t tile, uint32_t& px, uint32_t& py) {
-  const uint32_t tiles_x = (w + 15u) >> 4;
-  px = (tile % tiles_x) * 16u + (threadIdx.x & 15u);
-  py = (tile / tiles_x) * 16u + (threadIdx.x >> 4);
+  const uint32_t tiles_x = (w + (uint32_t)T - 1u) / (uint32_t)T;
+  px = (tile % tiles_x) * (uint32_t)T + (threadIdx.x % (uint32_t)T);
+  py = (tile / tiles_x) * (uint32_t)T + (threadIdx.x / (uint32_t)T);
   return px < w && py < h;
 }
 
-// One 16x16 tile of pathtrace.wgsl:248-330 (one pixel per lane).
-template <int TV>
+// One T x T tile of pathtrace.wgsl:248-330 (one pixel per lane).
+template <int TV, int T = 16>
 VRT_D void pathtrace_tile(const SceneDev& s, const LaneMem& L, const Uniforms& u, float fclamp, uint32_t spp, uint32_t tile,
                           ushort4* __restrict__ radiance, uchar4* __restrict__ albedo,
                           float4* __restrict__ gbuf, RayCount& rc) {
   const uint32_t w = (uint32_t)u.params1[0], h = (uint32_t)u.params1[1];
   uint32_t px, py;
-  if (tile_pixel(w, h, tile, px, py)) {
+  if (tile_pixel<T>(w, h, tile, px, py)) {
     const uint64_t i = (uint64_t)py * w + px;
     f3 acc = mk3(0.0f, 0.0f, 0.0f);
     for (uint32_t smp = 0; smp < spp; ++smp) {
@@ -745,10 +747,6 @@ VRT_D void pathtrace_tile(const SceneDev& s, const LaneMem& L, const Uniforms& u
   }
 }
 
-// WV: minimum waves per SIMD the register allocation must allow (1 = no
-// constraint).  The path tracer's natural allocation (~142 VGPRs) leaves 3
-// waves per SIMD to hide the DDA's dependent loads; 4 or 5 trade a few
-// spills for more waves in flight.
 // Tile of block b of an n-block launch in the given order (SceneDev::tile_order).
 VRT_D uint32_t map_tile(uint32_t b, uint32_t n, uint32_t order) {
   if (order & 2u) b = n - 1u - b;
@@ -759,6 +757,9 @@ VRT_D uint32_t map_tile(uint32_t b, uint32_t n, uint32_t order) {
   return b;
 }
 
+// WV: minimum waves per SIMD the register allocation must allow (1 = no
+// constraint).  The path tracer's natural allocation (~150-165 VGPRs) leaves
+// 3 waves per SIMD; 4 or 5 trade a few spills for more waves in flight.
 template <int TV, int WV>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WV))) void k_pathtrace(SceneDev s, Uniforms u, float fclamp,
                                                    uint32_t spp, ushort4* __restrict__ radiance,
@@ -770,6 +771,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WV))) void 
   RayCount rc{0u, 0u, 0u, 0u};
   pathtrace_tile<TV>(s, L, u, fclamp, spp, map_tile(blockIdx.x, gridDim.x, s.tile_order), radiance, albedo, gbuf,
                      rc);
+  count4(counters, rc.rays, rc.bsteps, rc.vsteps, rc.hits);
+}
+
+// The global-memory variant (no LDS to share) on 8x8 tiles of one wave per
+// block: four times the blocks of finer cost, so the launch's tail is
+// shorter, and square 8x8 patches of coherent primary rays.
+template <int WV>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WV))) void k_pathtrace8(
+    SceneDev s, Uniforms u, float fclamp, uint32_t spp, ushort4* __restrict__ radiance, uchar4* __restrict__ albedo,
+    float4* __restrict__ gbuf, unsigned long long* __restrict__ counters) {
+  const LaneMem L{{nullptr, nullptr, 0u}, {nullptr, 0u, 0u, 0u, 0u}};
+  RayCount rc{0u, 0u, 0u, 0u};
+  pathtrace_tile<TV_FASTG, 8>(s, L, u, fclamp, spp, map_tile(blockIdx.x, gridDim.x, s.tile_order), radiance, albedo,
+                              gbuf, rc);
   count4(counters, rc.rays, rc.bsteps, rc.vsteps, rc.hits);
 }
 
@@ -1275,10 +1290,20 @@ hipError_t launch_trace_persistent(hipStream_t st, const SceneDev& s, const floa
 
 hipError_t launch_pathtrace(hipStream_t st, const SceneDev& s, const Uniforms& u, float fclamp,
                             uint32_t spp, void* radiance, void* albedo, void* gbuf,
-                            unsigned long long* counters, int tv, int waves) {
+                            unsigned long long* counters, int tv, int waves, bool tile8) {
   const uint32_t w = (uint32_t)u.params1[0], h = (uint32_t)u.params1[1];
   const uint32_t tiles = div_up(w, 16) * div_up(h, 16);
   tv = lds_variant(tv, s.bit_words, s.pwords);
+  if (tile8 && tv == TV_FASTG) {
+    auto go8 = [&](auto kernel) {
+      hipLaunchKernelGGL(kernel, dim3(div_up(w, 8) * div_up(h, 8)), dim3(64), 0, st, s, u, fclamp, spp,
+                         (ushort4*)radiance, (uchar4*)albedo, (float4*)gbuf, counters);
+    };
+    if (waves == 4) go8(k_pathtrace8<4>);
+    else if (waves == 5) go8(k_pathtrace8<5>);
+    else go8(k_pathtrace8<1>);
+    return hipGetLastError();
+  }
   const uint32_t shmem = lds_bytes(tv, s.bit_words, s.pwords);
   auto go = [&](auto kernel) {
     hipLaunchKernelGGL(kernel, dim3(tiles), dim3(256), shmem, st, s, u, fclamp, spp, (ushort4*)radiance,
