@@ -278,8 +278,11 @@ typedef struct vrt_tuning {
                                  XCD, 2 row-major from the last tile (default: the
                                  bottom rows, usually the costliest, start first),
                                  3 bands from their last tile */
-  uint32_t pt_tile;           /* path tracer 7 (global memory): 0 = 16x16 tiles of 256
-                                 threads, 1 = 8x8 tiles of one wave */
+  uint32_t pt_tile;           /* one-pixel-per-lane path tracer: 0 = a 16x16 tile per
+                                 256-thread block, 1 = 8x8 tiles of one wave (pt 7),
+                                 2 / 3 = 32x32 / 32x16 tiles whose lanes take the next
+                                 pixel when theirs is done, one ray per lane per
+                                 round (pt 4 and 7) */
   uint32_t reserved[4];
 } vrt_tuning;
 int vrt_tuning_get(vrt_ctx* ctx, vrt_tuning* out);

@@ -47,7 +47,7 @@ hipError_t launch_trace_persistent(hipStream_t st, const SceneDev& s, const floa
                                   uint32_t* queue, int blocks, int steps, int min_refill, int tv);
 hipError_t launch_pathtrace(hipStream_t st, const SceneDev& s, const Uniforms& u, float fclamp,
                             uint32_t spp, void* radiance, void* albedo, void* gbuf,
-                            unsigned long long* counters, int tv, int waves, bool tile8);
+                            unsigned long long* counters, int tv, int waves, int form);
 hipError_t launch_pathtrace_persistent(hipStream_t st, const SceneDev& s, const Uniforms& u, float fclamp,
                                        uint32_t spp, void* radiance, void* albedo, void* gbuf,
                                        unsigned long long* counters, uint32_t* queue, int blocks,

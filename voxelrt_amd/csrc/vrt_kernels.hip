@@ -712,6 +712,109 @@ VRT_D PixelOut shade_pixel_1(const SceneDev& s, const LaneMem& L, const Uniforms
   return res;
 }
 
+// pathtrace.wgsl:248-330 as a resumable per-lane sample path: start() sets
+// up a sample's primary ray, after() consumes the result of the ray just
+// traced and either sets up the next ray (shadow or bounce) or ends the
+// sample.  The operations and RNG draws are those of shade_pixel in the same
+// order, so a lane that runs start(), then after() once per traced ray,
+// produces the same bits.
+struct SamplePath {
+  enum : uint32_t { PRIMARY, SHADOW, BOUNCE };
+  uint32_t rng, bounce, phase;
+  f3 ro, rd, l, radiance, tp, n, surf, alb1;
+  Material mat;
+  float ndl;
+  VRT_D f3 dir() const { return phase == SHADOW ? l : rd; }
+  VRT_D void start(const Uniforms& u, uint32_t px, uint32_t py, uint32_t frame, uint32_t w, uint32_t h) {
+    rng = init_rng(px, py, frame);
+    const float jx = rand01(rng) - 0.5f;
+    const float jy = rand01(rng) - 0.5f;
+    const float uvx = (((float)px + 0.5f) + jx) / (float)w;
+    const float uvy = (((float)py + 0.5f) + jy) / (float)h;
+    ro = mk3(u.cam_pos[0], u.cam_pos[1], u.cam_pos[2]);
+    rd = camera_ray(u, uvx, uvy);
+    radiance = mk3(0.0f, 0.0f, 0.0f);
+    tp = mk3(1.0f, 1.0f, 1.0f);
+    bounce = 0u;
+    phase = PRIMARY;
+  }
+  // True when the sample is finished; *out is then its (clamped,
+  // demodulated) radiance, or the sky for a primary miss.
+  VRT_D bool after(const Uniforms& u, float fclamp, const TraceResult& hh, const GbufOut& go, f3* out) {
+    const bool indirect = (u.params0[2] & VRT_FLAG_INDIRECT) != 0u;
+    const uint32_t nb = u.params0[1];
+    bool vertex = false, after_nee = false, done = false;
+    f3 pos = ro;
+    if (phase == PRIMARY) {
+      if (hh.t < 0.0f) {
+        *out = indirect ? mk3(0.0f, 0.0f, 0.0f) : sky_color(u, rd, true);
+        go.store(mk3(1.0f, 1.0f, 1.0f), make_float4(-rd.x, -rd.y, -rd.z, -1.0f));
+        return true;
+      }
+      mat = unpack_material(hh.mat);
+      alb1 = mat.albedo;
+      pos = ro + rd * hh.t;
+      n = hh.n;
+      go.store(mk3(vsqrt(alb1.x), vsqrt(alb1.y), vsqrt(alb1.z)), make_float4(n.x, n.y, n.z, hh.t));
+      vertex = true;
+    } else if (phase == SHADOW) {
+      const f3 sunRad = mk3(u.sun_radiance[0], u.sun_radiance[1], u.sun_radiance[2]);
+      if (!(hh.t >= 0.0f)) radiance = radiance + (((tp * mat.albedo) * INV_PI_F) * ndl) * sunRad;
+      after_nee = true;
+    } else {  // BOUNCE
+      if (hh.t < 0.0f) {
+        radiance = radiance + tp * sky_color(u, rd, false);
+        done = true;
+      } else {
+        pos = surf + rd * hh.t;
+        n = hh.n;
+        mat = unpack_material(hh.mat);
+        ++bounce;
+        vertex = true;
+      }
+    }
+    if (vertex) {  // emission + sun NEE at the path vertex
+      const bool direct = !(indirect && bounce == 0u);
+      if (mat.emissive > 0.0f && direct) radiance = radiance + (tp * mat.albedo) * (mat.emissive * EMISSIVE_SCALE);
+      surf = pos + n * 1e-3f;
+      l = sample_sun_dir(u, rng);
+      ndl = dot(n, l);
+      if (ndl > 0.0f && l.y > 0.0f && direct) {
+        phase = SHADOW;
+        ro = surf;
+      } else {
+        after_nee = true;
+      }
+    }
+    if (after_nee) {  // cosine bounce + Russian roulette
+      if (bounce == nb) {
+        done = true;
+      } else {
+        rd = cosine_hemisphere(n, rng);
+        tp = tp * mat.albedo;
+        bool alive = true;
+        if (bounce >= 2u) {
+          const float p = vclamp(vmax(tp.x, vmax(tp.y, tp.z)), 0.05f, 0.95f);
+          if (rand01(rng) > p) alive = false;
+          else tp = tp / p;
+        }
+        if (alive) {
+          phase = BOUNCE;
+          ro = surf;
+        } else {
+          done = true;
+        }
+      }
+    }
+    if (!done) return false;
+    f3 r = radiance;
+    if (!(fclamp <= 0.0f)) r = mk3(vmin(r.x, fclamp), vmin(r.y, fclamp), vmin(r.z, fclamp));
+    const f3 den = mk3(vmax(alb1.x, 1e-3f), vmax(alb1.y, 1e-3f), vmax(alb1.z, 1e-3f));
+    *out = r / den;
+    return true;
+  }
+};
+
 // Pixel p of a launch covering TILE x TILE tiles, row-major tiles: keeps the
 // 64 lanes of a wave on a compact 16x4 patch (coherent primary rays).
 // T = 8: 8x8 tiles of one wave each.
@@ -785,6 +888,80 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WV))) void k
   RayCount rc{0u, 0u, 0u, 0u};
   pathtrace_tile<TV_FASTG, 8>(s, L, u, fclamp, spp, map_tile(blockIdx.x, gridDim.x, s.tile_order), radiance, albedo,
                               gbuf, rc);
+  count4(counters, rc.rays, rc.bsteps, rc.vsteps, rc.hits);
+}
+
+// One block renders a TW x TH tile, its lanes taking pixels from the tile's
+// LDS counter (a wave takes as many consecutive pixels as it has idle lanes,
+// in 8x8 patches): each loop round traces one ray per busy lane -- primary,
+// shadow or bounce, one call site -- and a lane whose pixel is finished
+// takes the next one, so lanes do not idle while other lanes of the wave
+// finish longer paths.
+template <int TV, int WV, int TW, int TH>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WV))) void k_pathtrace_refill(
+    SceneDev s, Uniforms u, float fclamp, uint32_t spp, ushort4* __restrict__ radiance, uchar4* __restrict__ albedo,
+    float4* __restrict__ gbuf, unsigned long long* __restrict__ counters) {
+  static_assert(TW % 8 == 0 && TH % 8 == 0, "tiles of 8x8 patches");
+  constexpr uint32_t NP = (uint32_t)(TW * TH), PW = (uint32_t)(TW / 8);
+  extern __shared__ uint32_t lds_dyn[];
+  __shared__ uint32_t s_next;
+  if (threadIdx.x == 0u) s_next = 0u;
+  const LaneMem L = lds_setup<TV>(s, lds_dyn);
+  __syncthreads();
+  const uint32_t w = (uint32_t)u.params1[0], h = (uint32_t)u.params1[1];
+  const uint32_t tiles_x = (w + (uint32_t)TW - 1u) / (uint32_t)TW;
+  const uint32_t tile = map_tile(blockIdx.x, gridDim.x, s.tile_order);
+  const uint32_t x0 = (tile % tiles_x) * (uint32_t)TW, y0 = (tile / tiles_x) * (uint32_t)TH;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint64_t lt_mask = lane == 0u ? 0ull : (~0ull >> (64u - lane));
+  RayCount rc{0u, 0u, 0u, 0u};
+  SamplePath P;
+  bool have = false, exhausted = false;
+  uint32_t px = 0u, py = 0u, smp = 0u;
+  f3 acc = mk3(0.0f, 0.0f, 0.0f);
+  for (;;) {
+    const uint64_t need = __ballot(!have && !exhausted);
+    if (need != 0ull) {
+      const uint32_t first = (uint32_t)__builtin_ctzll(need);
+      uint32_t base = 0u;
+      if (lane == first) base = atomicAdd(&s_next, (uint32_t)__popcll(need));
+      base = __shfl(base, (int)first, 64);
+      if (!have && !exhausted) {
+        const uint32_t p = base + (uint32_t)__popcll(need & lt_mask);
+        if (p >= NP) {
+          exhausted = true;
+        } else {
+          const uint32_t patch = p >> 6, q = p & 63u;
+          px = x0 + (patch % PW) * 8u + (q & 7u);
+          py = y0 + (patch / PW) * 8u + (q >> 3);
+          if (px < w && py < h) {
+            have = true;
+            smp = 0u;
+            acc = mk3(0.0f, 0.0f, 0.0f);
+            P.start(u, px, py, u.params0[0] * spp, w, h);
+          }
+        }
+      }
+    }
+    if (exhausted) break;
+    if (!have) continue;  // a pixel outside the image: take another
+    TraceResult hh;
+    trace_sel<TV>(s, L, P.ro, P.dir(), 1e4f, hh);
+    rc.add(hh);
+    const uint64_t i = (uint64_t)py * w + px;
+    f3 val;
+    if (P.after(u, fclamp, hh, GbufOut{albedo, gbuf, i, smp == 0u}, &val)) {
+      acc = acc + val;
+      ++smp;
+      if (smp < spp) {
+        P.start(u, px, py, u.params0[0] * spp + smp, w, h);
+      } else {
+        const f3 o = acc / (float)spp;
+        radiance[i] = make_ushort4(f32_to_f16(o.x), f32_to_f16(o.y), f32_to_f16(o.z), f32_to_f16(1.0f));
+        have = false;
+      }
+    }
+  }
   count4(counters, rc.rays, rc.bsteps, rc.vsteps, rc.hits);
 }
 
@@ -1290,11 +1467,38 @@ hipError_t launch_trace_persistent(hipStream_t st, const SceneDev& s, const floa
 
 hipError_t launch_pathtrace(hipStream_t st, const SceneDev& s, const Uniforms& u, float fclamp,
                             uint32_t spp, void* radiance, void* albedo, void* gbuf,
-                            unsigned long long* counters, int tv, int waves, bool tile8) {
+                            unsigned long long* counters, int tv, int waves, int form) {
   const uint32_t w = (uint32_t)u.params1[0], h = (uint32_t)u.params1[1];
   const uint32_t tiles = div_up(w, 16) * div_up(h, 16);
   tv = lds_variant(tv, s.bit_words, s.pwords);
-  if (tile8 && tv == TV_FASTG) {
+  if ((form == 2 || form == 3) && (tv == TV_FAST || tv == TV_FASTG)) {
+    // refill: 32x32 (form 2) or 32x16 (form 3) tiles per block
+    const uint32_t th = form == 2 ? 32u : 16u;
+    const dim3 grid(div_up(w, 32) * div_up(h, th));
+    const uint32_t shmem = lds_bytes(tv, s.bit_words, s.pwords);
+    auto go = [&](auto kernel) {
+      hipLaunchKernelGGL(kernel, grid, dim3(256), shmem, st, s, u, fclamp, spp, (ushort4*)radiance,
+                         (uchar4*)albedo, (float4*)gbuf, counters);
+    };
+    const int wv = waves == 4 || waves == 5 ? waves : 1;
+    const int key = (tv == TV_FASTG ? 100 : 0) + (form == 2 ? 10 : 0) + wv;
+    switch (key) {
+      case 111: go(k_pathtrace_refill<TV_FASTG, 1, 32, 32>); break;
+      case 114: go(k_pathtrace_refill<TV_FASTG, 4, 32, 32>); break;
+      case 115: go(k_pathtrace_refill<TV_FASTG, 5, 32, 32>); break;
+      case 101: go(k_pathtrace_refill<TV_FASTG, 1, 32, 16>); break;
+      case 104: go(k_pathtrace_refill<TV_FASTG, 4, 32, 16>); break;
+      case 105: go(k_pathtrace_refill<TV_FASTG, 5, 32, 16>); break;
+      case 11: go(k_pathtrace_refill<TV_FAST, 1, 32, 32>); break;
+      case 14: go(k_pathtrace_refill<TV_FAST, 4, 32, 32>); break;
+      case 15: go(k_pathtrace_refill<TV_FAST, 5, 32, 32>); break;
+      case 1: go(k_pathtrace_refill<TV_FAST, 1, 32, 16>); break;
+      case 4: go(k_pathtrace_refill<TV_FAST, 4, 32, 16>); break;
+      default: go(k_pathtrace_refill<TV_FAST, 5, 32, 16>); break;
+    }
+    return hipGetLastError();
+  }
+  if (form == 1 && tv == TV_FASTG) {
     auto go8 = [&](auto kernel) {
       hipLaunchKernelGGL(kernel, dim3(div_up(w, 8) * div_up(h, 8)), dim3(64), 0, st, s, u, fclamp, spp,
                          (ushort4*)radiance, (uchar4*)albedo, (float4*)gbuf, counters);

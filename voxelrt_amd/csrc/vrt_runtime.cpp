@@ -592,7 +592,7 @@ hipError_t vrt_ctx::launch_pathtrace(const vrt::SceneDev& s0, const vrt_uniforms
                                             (int)tuning.pt_waves);
   return vrt::launch_pathtrace(stream, s, u, fclamp, spp, radiance, albedo, gbuf,
                                (unsigned long long*)d_counters.ptr, pt_variant(tuning.pt_kernel, tuning.empty_skip),
-                               (int)tuning.pt_waves, tuning.pt_tile != 0u);
+                               (int)tuning.pt_waves, (int)tuning.pt_tile);
 }
 
 hipError_t vrt_ctx::launch_trace(const vrt::SceneDev& s0, const float* rays, uint64_t n, float tmax,
@@ -889,7 +889,7 @@ int vrt_tuning_set(vrt_ctx* ctx, const vrt_tuning* in) {
       in->trace_min_refill > 64 || in->pt_min_refill > 64 || in->pt_chunk > (1u << 16) ||
       !(in->pt_waves == 0 || in->pt_waves == 4 || in->pt_waves == 5) || in->batch_mode > 2 ||
       in->batch_blocks > (1u << 16) || in->dda_batch > 64 ||
-      in->empty_skip > 1 || in->pt_order > 3 || in->pt_tile > 1)
+      in->empty_skip > 1 || in->pt_order > 3 || in->pt_tile > 3)
     return ctx->fail(VRT_ERR_INVALID, "tuning value out of range");
   ctx->tuning = *in;
   return VRT_OK;
@@ -1042,20 +1042,24 @@ int vrt_autotune(vrt_ctx* ctx, const vrt_scene* scene, uint64_t nrays, const vrt
       }
     }
     ctx->tuning.pt_order = best.pt_order;
-    // the global-memory variant on 8x8 tiles of one wave
-    for (uint32_t wv : {0u, 4u, 5u}) {
-      ctx->tuning.pt_kernel = 7;
-      ctx->tuning.pt_waves = wv;
-      ctx->tuning.pt_tile = 1;
-      rc = time_it(launch, &ms);
-      if (rc) return rc;
-      if (ms < best_ms) {
-        best_ms = ms;
-        best.pt_kernel = 7;
-        best.pt_waves = wv;
-        best.pt_tile = 1;
-      }
-    }
+    // the global-memory variant on 8x8 tiles of one wave; refill tiles
+    // (32x32, 32x16) of the single-loop variants
+    for (uint32_t k : {4u, 7u})
+      for (uint32_t form : {1u, 2u, 3u})
+        for (uint32_t wv : {0u, 4u, 5u}) {
+          if (form == 1u && k != 7u) continue;
+          ctx->tuning.pt_kernel = k;
+          ctx->tuning.pt_waves = wv;
+          ctx->tuning.pt_tile = form;
+          rc = time_it(launch, &ms);
+          if (rc) return rc;
+          if (ms < best_ms) {
+            best_ms = ms;
+            best.pt_kernel = k;
+            best.pt_waves = wv;
+            best.pt_tile = form;
+          }
+        }
     ctx->tuning.pt_tile = best.pt_tile;
     const uint32_t blocks[] = {512, 1024, 2048};
     const uint32_t steps[] = {4, 8, 16};
