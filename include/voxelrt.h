@@ -282,7 +282,8 @@ typedef struct vrt_tuning {
                                  256-thread block, 1 = 8x8 tiles of one wave (pt 7),
                                  2 / 3 = 32x32 / 32x16 tiles whose lanes take the next
                                  pixel when theirs is done, one ray per lane per
-                                 round (pt 4 and 7) */
+                                 round (pt 4 and 7), 4 / 5 = the same on one-wave
+                                 blocks of 16x16 / 16x8 tiles (pt 7) */
   uint32_t reserved[4];
 } vrt_tuning;
 int vrt_tuning_get(vrt_ctx* ctx, vrt_tuning* out);

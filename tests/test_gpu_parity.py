@@ -218,7 +218,8 @@ PT_VARIANTS = [dict(pt_kernel=0), dict(pt_kernel=0, pt_waves=5), dict(pt_kernel=
                dict(pt_kernel=3, pt_order=3), dict(pt_kernel=7), dict(pt_kernel=7, pt_waves=5),
                dict(pt_kernel=7, pt_tile=1), dict(pt_kernel=7, pt_waves=4, pt_tile=1, pt_order=0),
                dict(pt_kernel=4, pt_waves=4, pt_tile=2), dict(pt_kernel=7, pt_waves=5, pt_tile=3),
-               dict(pt_kernel=7, pt_tile=2, pt_order=0),
+               dict(pt_kernel=7, pt_tile=2, pt_order=0), dict(pt_kernel=7, pt_waves=4, pt_tile=4),
+               dict(pt_kernel=7, pt_tile=5, pt_order=0),
                dict(pt_kernel=1, pt_blocks=1024, pt_steps=8, pt_min_refill=32, pt_chunk=64),
                dict(pt_kernel=1, pt_blocks=97, pt_steps=1, pt_min_refill=1, pt_chunk=64),
                dict(pt_kernel=1, pt_blocks=2048, pt_steps=16, pt_min_refill=64, pt_chunk=256),
@@ -234,6 +235,7 @@ PT_VARIANTS = [dict(pt_kernel=0), dict(pt_kernel=0, pt_waves=5), dict(pt_kernel=
                      "lane-fast-w5-xcd", "lane-fast1-w5-rev", "lane-lds-xcdrev", "lane-fast-global",
                      "lane-fast-global-w5", "lane-fast-global-8x8", "lane-fast-global-8x8-w4-rowmajor",
                      "refill-fast-32x32-w4", "refill-global-32x16-w5", "refill-global-32x32-rowmajor",
+                     "refill-wave-16x16-w4", "refill-wave-16x8-rowmajor",
                      "persist",
                      "persist-s1", "persist-s16", "persist-fast", "persist-fast-w4-b8", "persist-fast-w5-b64"])
 def pt_variant(request, ctx):

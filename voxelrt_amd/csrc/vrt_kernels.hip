@@ -897,8 +897,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WV))) void k
 // shadow or bounce, one call site -- and a lane whose pixel is finished
 // takes the next one, so lanes do not idle while other lanes of the wave
 // finish longer paths.
-template <int TV, int WV, int TW, int TH>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WV))) void k_pathtrace_refill(
+template <int TV, int WV, int TW, int TH, int NT = 256>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WV))) void k_pathtrace_refill(
     SceneDev s, Uniforms u, float fclamp, uint32_t spp, ushort4* __restrict__ radiance, uchar4* __restrict__ albedo,
     float4* __restrict__ gbuf, unsigned long long* __restrict__ counters) {
   static_assert(TW % 8 == 0 && TH % 8 == 0, "tiles of 8x8 patches");
@@ -1471,6 +1471,25 @@ hipError_t launch_pathtrace(hipStream_t st, const SceneDev& s, const Uniforms& u
   const uint32_t w = (uint32_t)u.params1[0], h = (uint32_t)u.params1[1];
   const uint32_t tiles = div_up(w, 16) * div_up(h, 16);
   tv = lds_variant(tv, s.bit_words, s.pwords);
+  if ((form == 4 || form == 5) && tv == TV_FASTG) {
+    // refill on one-wave blocks: 16x16 (form 4) or 16x8 (form 5) tiles
+    const uint32_t th = form == 4 ? 16u : 8u;
+    const dim3 grid(div_up(w, 16) * div_up(h, th));
+    auto go = [&](auto kernel) {
+      hipLaunchKernelGGL(kernel, grid, dim3(64), 0, st, s, u, fclamp, spp, (ushort4*)radiance, (uchar4*)albedo,
+                         (float4*)gbuf, counters);
+    };
+    const int wv = waves == 4 || waves == 5 ? waves : 1;
+    switch ((form == 4 ? 10 : 0) + wv) {
+      case 11: go(k_pathtrace_refill<TV_FASTG, 1, 16, 16, 64>); break;
+      case 14: go(k_pathtrace_refill<TV_FASTG, 4, 16, 16, 64>); break;
+      case 15: go(k_pathtrace_refill<TV_FASTG, 5, 16, 16, 64>); break;
+      case 1: go(k_pathtrace_refill<TV_FASTG, 1, 16, 8, 64>); break;
+      case 4: go(k_pathtrace_refill<TV_FASTG, 4, 16, 8, 64>); break;
+      default: go(k_pathtrace_refill<TV_FASTG, 5, 16, 8, 64>); break;
+    }
+    return hipGetLastError();
+  }
   if ((form == 2 || form == 3) && (tv == TV_FAST || tv == TV_FASTG)) {
     // refill: 32x32 (form 2) or 32x16 (form 3) tiles per block
     const uint32_t th = form == 2 ? 32u : 16u;

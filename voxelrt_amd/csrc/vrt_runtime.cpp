@@ -889,7 +889,7 @@ int vrt_tuning_set(vrt_ctx* ctx, const vrt_tuning* in) {
       in->trace_min_refill > 64 || in->pt_min_refill > 64 || in->pt_chunk > (1u << 16) ||
       !(in->pt_waves == 0 || in->pt_waves == 4 || in->pt_waves == 5) || in->batch_mode > 2 ||
       in->batch_blocks > (1u << 16) || in->dda_batch > 64 ||
-      in->empty_skip > 1 || in->pt_order > 3 || in->pt_tile > 3)
+      in->empty_skip > 1 || in->pt_order > 3 || in->pt_tile > 5)
     return ctx->fail(VRT_ERR_INVALID, "tuning value out of range");
   ctx->tuning = *in;
   return VRT_OK;
@@ -1045,9 +1045,9 @@ int vrt_autotune(vrt_ctx* ctx, const vrt_scene* scene, uint64_t nrays, const vrt
     // the global-memory variant on 8x8 tiles of one wave; refill tiles
     // (32x32, 32x16) of the single-loop variants
     for (uint32_t k : {4u, 7u})
-      for (uint32_t form : {1u, 2u, 3u})
+      for (uint32_t form : {1u, 2u, 3u, 4u, 5u})
         for (uint32_t wv : {0u, 4u, 5u}) {
-          if (form == 1u && k != 7u) continue;
+          if ((form == 1u || form >= 4u) && k != 7u) continue;
           ctx->tuning.pt_kernel = k;
           ctx->tuning.pt_waves = wv;
           ctx->tuning.pt_tile = form;
