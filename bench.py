@@ -245,7 +245,7 @@ def batch(args):
     out = {"metric": "batched aggregate Mrays/s vs plain per-view (64 views, 8 scenes)", "unit": "Mrays/s",
            "value": results[best]["mrays_s"], "best_mode": best, **results,
            "speedup": round(results[best]["mrays_s"] / max(results["plain"]["mrays_s"], 1e-9), 2),
-           "tuning": {k: tuned[k] for k in ("pt_kernel", "pt_waves")},
+           "tuning": {k: tuned[k] for k in ("pt_kernel", "pt_waves", "pt_tile")},
            "views": len(specs), "pixels_per_batch": pixels, "scene_upload_s": round(upload_s, 2),
            "steps": args.steps, "warmup": args.warmup}
     print(json.dumps(out), flush=True)
@@ -381,7 +381,7 @@ def serve(args):
            "camera_streams": len(cams), "scene_upload_s": round(upload_s, 2),
            "serve_config": {"streams": args.serve_streams, "max_batch_views": args.serve_views,
                             "max_batch_pixels": args.serve_pixels, "batch_mode": args.batch_mode},
-           "tuning": {k: tuned[k] for k in ("pt_kernel", "pt_waves")}}
+           "tuning": {k: tuned[k] for k in ("pt_kernel", "pt_waves", "pt_tile")}}
     print(json.dumps(out), flush=True)
     ctx.close()
 

@@ -390,11 +390,23 @@ def test_frame_sequence_bitexact(ctx, pt_variant):
                                     dict(pt_kernel=4, pt_waves=5, batch_mode=0, pt_order=0),
                                     dict(pt_kernel=4, pt_waves=5, batch_mode=1, pt_order=0),
                                     dict(pt_kernel=7, pt_waves=4, batch_mode=1),
-                                    dict(pt_kernel=3, pt_waves=4, batch_mode=2)],
+                                    dict(pt_kernel=3, pt_waves=4, batch_mode=2),
+                                    dict(pt_kernel=7, pt_waves=4, pt_tile=5, batch_mode=0),
+                                    dict(pt_kernel=7, pt_waves=5, pt_tile=4, batch_mode=0, pt_order=0),
+                                    dict(pt_kernel=7, pt_waves=4, pt_tile=5, batch_mode=1),
+                                    dict(pt_kernel=7, pt_waves=0, pt_tile=4, batch_mode=1, batch_blocks=5),
+                                    dict(pt_kernel=7, pt_waves=4, pt_tile=5, batch_mode=2, pt_order=1),
+                                    dict(pt_kernel=7, pt_waves=4, pt_tile=3, batch_mode=0),
+                                    dict(pt_kernel=4, pt_waves=5, pt_tile=2, batch_mode=1),
+                                    dict(pt_kernel=4, pt_waves=4, pt_tile=3, batch_mode=2, pt_order=0),
+                                    dict(pt_kernel=7, pt_waves=5, pt_tile=2, batch_mode=1, batch_blocks=3)],
                          ids=["lane", "lds-w5", "flat-w4", "fast-w5", "queue-fast", "queue-7blocks",
                               "graph-fast", "queue-fast1-w4", "grid-fast-rowmajor", "queue-fast-rowmajor",
                               "queue-fast-global-w4",
-                              "graph-lds"])
+                              "graph-lds", "grid-refill16x8", "grid-refill16x16-rowmajor", "queue-refill16x8",
+                              "queue-refill16x16-5blocks", "graph-refill16x8", "grid-refill32x16-global",
+                              "queue-refill32x32-lds", "graph-refill32x16-lds-rowmajor",
+                              "queue-refill32x32-global-3blocks"])
 def test_batched_views_match_single_views(ctx, tuning):
     descs = [cf.world_desc(64, 1), cf.world_desc(128, 2), dict(cf.world_desc(64, 3), ny=32)]
     scenes = [ctx.upload(v.generate_scene(**d)) for d in descs]

@@ -19,6 +19,8 @@ struct ViewDesc {
   float fclamp;
   uint32_t spp, fmt, tile0;
   uint32_t w, h, tiles_x, denoise;
+  uint32_t rtile0;  // first of the view's path-trace tiles of the refill forms
+  uint32_t rtx;     // their count across the view
   void* radiance;    // rgba16f
   void* albedo;      // rgba8unorm
   void* gbuf_cur;    // rgba32f
@@ -38,7 +40,8 @@ struct BatchHead {
   uint32_t nviews;
   uint32_t tiles;
   uint32_t queue;
-  uint32_t pad[13];
+  uint32_t rtiles;  // path-trace tiles of the refill forms (32x32, 32x16, 16x16 or 16x8)
+  uint32_t pad[12];
 };
 constexpr uint32_t BATCH_HEAD_BYTES = 64;
 static_assert(sizeof(BatchHead) == BATCH_HEAD_BYTES, "BatchHead layout");
@@ -49,6 +52,17 @@ VRT_D uint32_t find_view(const ViewDesc* __restrict__ views, uint32_t nviews, ui
   while (hi - lo > 1u) {
     const uint32_t mid = (lo + hi) >> 1;
     if (views[mid].tile0 <= b) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Index of the view owning refill tile `b` (views sorted by rtile0).
+VRT_D uint32_t find_view_r(const ViewDesc* __restrict__ views, uint32_t nviews, uint32_t b) {
+  uint32_t lo = 0, hi = nviews;
+  while (hi - lo > 1u) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (views[mid].rtile0 <= b) lo = mid;
     else hi = mid;
   }
   return lo;

@@ -4,6 +4,7 @@
 
 #include <chrono>
 #include <map>
+#include <utility>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -56,7 +57,7 @@ struct BatchLane {
   uint32_t desc_next = 0;
   DeviceBuffer desc_dev;  // BatchHead, then the ViewDesc array (vrt_batch.h)
   // instantiated graphs of the persistent batched passes, by launch shape
-  std::map<uint64_t, hipGraphExec_t> graphs;
+  std::map<std::pair<uint64_t, uint64_t>, hipGraphExec_t> graphs;
   BatchLane() = default;
   BatchLane(const BatchLane&) = delete;
   BatchLane& operator=(const BatchLane&) = delete;

@@ -53,20 +53,30 @@ hipError_t launch_pathtrace_persistent(hipStream_t st, const SceneDev& s, const 
                                        unsigned long long* counters, uint32_t* queue, int blocks,
                                        int steps, int min_refill, int chunk, int tv, int waves);
 struct ViewDesc;
+// The refill tile form (2: 32x32, 3: 32x16 on 256-thread blocks; 4: 16x16,
+// 5: 16x8 on one-wave blocks) the path tracer runs for variant tv and tuning
+// form `form` over scenes of up to the given bitmap sizes, or 0 for the
+// 16x16 one-pixel-per-lane tiles.  refill_tile_dims gives a form's tile size.
+int refill_form(int tv, int form, uint32_t max_bit_words, uint32_t max_pwords);
+void refill_tile_dims(int rform, uint32_t* tw, uint32_t* th);
+// With rform != 0 (from refill_form) the grid is `rtiles` blocks over the
+// views' refill tiles (ViewDesc::rtile0 / rtx), else `tiles` blocks over
+// their 16x16 tiles.
 hipError_t launch_pathtrace_batch(hipStream_t st, const ViewDesc* views, uint32_t nviews, uint32_t tiles,
                                   unsigned long long* counters, int tv, int waves, uint32_t max_bit_words,
-                                  uint32_t max_pwords);
+                                  uint32_t max_pwords, int rform, uint32_t rtiles);
 hipError_t launch_temporal_batch(hipStream_t st, const ViewDesc* views, uint32_t nviews, uint32_t tiles);
 hipError_t launch_hdr_batch(hipStream_t st, const ViewDesc* views, uint32_t nviews, uint32_t tiles);
 struct BatchHead;
 // Persistent batched form: the path tracer on a fixed grid of pt_blocks pulling
-// tiles from head->queue, then temporal / a-trous x3 (denoise) / hdr on a
+// tiles from head->queue (with rform != 0 the refill tiles of that form,
+// head->rtiles of them), then temporal / a-trous x3 (denoise) / hdr on a
 // fixed grid-stride grid of gs_blocks.  No argument depends on the batch, so
 // the sequence can be captured once into a graph and replayed.
 hipError_t launch_batch_persistent(hipStream_t st, BatchHead* head, const ViewDesc* views,
                                    unsigned long long* counters, int tv, int waves, uint32_t max_bit_words,
                                    uint32_t max_pwords, uint32_t pt_blocks, uint32_t gs_blocks, bool denoise,
-                                   bool hdr);
+                                   bool hdr, int rform);
 hipError_t launch_atrous_batch(hipStream_t st, const ViewDesc* views, uint32_t nviews, uint32_t tiles, int pass);
 hipError_t launch_temporal(hipStream_t st, const Uniforms& u, const void* radiance,
                            const void* gcur, const void* aprev, const void* gprev, void* aout,

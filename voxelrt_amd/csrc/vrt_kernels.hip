@@ -891,41 +891,40 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WV))) void k
   count4(counters, rc.rays, rc.bsteps, rc.vsteps, rc.hits);
 }
 
-// One block renders a TW x TH tile, its lanes taking pixels from the tile's
-// LDS counter (a wave takes as many consecutive pixels as it has idle lanes,
-// in 8x8 patches): each loop round traces one ray per busy lane -- primary,
-// shadow or bounce, one call site -- and a lane whose pixel is finished
-// takes the next one, so lanes do not idle while other lanes of the wave
-// finish longer paths.
-template <int TV, int WV, int TW, int TH, int NT = 256>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WV))) void k_pathtrace_refill(
-    SceneDev s, Uniforms u, float fclamp, uint32_t spp, ushort4* __restrict__ radiance, uchar4* __restrict__ albedo,
-    float4* __restrict__ gbuf, unsigned long long* __restrict__ counters) {
+// One block (NT threads) renders the TW x TH tile at (x0, y0), its lanes
+// taking pixels from the tile's counter (a wave takes as many consecutive
+// pixels as it has idle lanes, in 8x8 patches): each loop round traces one
+// ray per busy lane -- primary, shadow or bounce, one call site -- and a lane
+// whose pixel is finished takes the next one, so lanes do not idle while
+// other lanes of the wave finish longer paths.  The counter is a wave-uniform
+// register for one-wave blocks, else *s_next in LDS (set to 0 and published
+// by the caller).
+template <int TV, int TW, int TH, int NT>
+VRT_D void refill_tile(const SceneDev& s, const LaneMem& L, const Uniforms& u, float fclamp, uint32_t spp,
+                       uint32_t x0, uint32_t y0, ushort4* __restrict__ radiance, uchar4* __restrict__ albedo,
+                       float4* __restrict__ gbuf, RayCount& rc, uint32_t* s_next) {
   static_assert(TW % 8 == 0 && TH % 8 == 0, "tiles of 8x8 patches");
   constexpr uint32_t NP = (uint32_t)(TW * TH), PW = (uint32_t)(TW / 8);
-  extern __shared__ uint32_t lds_dyn[];
-  __shared__ uint32_t s_next;
-  if (threadIdx.x == 0u) s_next = 0u;
-  const LaneMem L = lds_setup<TV>(s, lds_dyn);
-  __syncthreads();
   const uint32_t w = (uint32_t)u.params1[0], h = (uint32_t)u.params1[1];
-  const uint32_t tiles_x = (w + (uint32_t)TW - 1u) / (uint32_t)TW;
-  const uint32_t tile = map_tile(blockIdx.x, gridDim.x, s.tile_order);
-  const uint32_t x0 = (tile % tiles_x) * (uint32_t)TW, y0 = (tile / tiles_x) * (uint32_t)TH;
   const uint32_t lane = threadIdx.x & 63u;
   const uint64_t lt_mask = lane == 0u ? 0ull : (~0ull >> (64u - lane));
-  RayCount rc{0u, 0u, 0u, 0u};
   SamplePath P;
   bool have = false, exhausted = false;
-  uint32_t px = 0u, py = 0u, smp = 0u;
+  uint32_t px = 0u, py = 0u, smp = 0u, next = 0u;
   f3 acc = mk3(0.0f, 0.0f, 0.0f);
   for (;;) {
     const uint64_t need = __ballot(!have && !exhausted);
     if (need != 0ull) {
-      const uint32_t first = (uint32_t)__builtin_ctzll(need);
-      uint32_t base = 0u;
-      if (lane == first) base = atomicAdd(&s_next, (uint32_t)__popcll(need));
-      base = __shfl(base, (int)first, 64);
+      uint32_t base;
+      if constexpr (NT == 64) {
+        base = next;
+        next += (uint32_t)__popcll(need);
+      } else {
+        const uint32_t first = (uint32_t)__builtin_ctzll(need);
+        base = 0u;
+        if (lane == first) base = atomicAdd(s_next, (uint32_t)__popcll(need));
+        base = __shfl(base, (int)first, 64);
+      }
       if (!have && !exhausted) {
         const uint32_t p = base + (uint32_t)__popcll(need & lt_mask);
         if (p >= NP) {
@@ -962,6 +961,23 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WV))) void k
       }
     }
   }
+}
+
+template <int TV, int WV, int TW, int TH, int NT = 256>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WV))) void k_pathtrace_refill(
+    SceneDev s, Uniforms u, float fclamp, uint32_t spp, ushort4* __restrict__ radiance, uchar4* __restrict__ albedo,
+    float4* __restrict__ gbuf, unsigned long long* __restrict__ counters) {
+  extern __shared__ uint32_t lds_dyn[];
+  __shared__ uint32_t s_next;
+  if (threadIdx.x == 0u) s_next = 0u;
+  const LaneMem L = lds_setup<TV>(s, lds_dyn);
+  __syncthreads();
+  const uint32_t w = (uint32_t)u.params1[0];
+  const uint32_t tiles_x = (w + (uint32_t)TW - 1u) / (uint32_t)TW;
+  const uint32_t tile = map_tile(blockIdx.x, gridDim.x, s.tile_order);
+  RayCount rc{0u, 0u, 0u, 0u};
+  refill_tile<TV, TW, TH, NT>(s, L, u, fclamp, spp, (tile % tiles_x) * (uint32_t)TW, (tile / tiles_x) * (uint32_t)TH,
+                              radiance, albedo, gbuf, rc, &s_next);
   count4(counters, rc.rays, rc.bsteps, rc.vsteps, rc.hits);
 }
 
@@ -1213,6 +1229,80 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WV))) void 
                        (float4*)d.gbuf_cur, rc);
     count4(counters, rc.rays, rc.bsteps, rc.vsteps, rc.hits);
     count4(d.vcount, rc.rays, rc.bsteps, rc.vsteps, rc.hits);
+  }
+}
+
+// The view's own refill tile for batch refill tile b, ordered as view_tile
+// orders the 16x16 ones.
+template <int TH>
+VRT_D uint32_t view_rtile(const ViewDesc& d, uint32_t b) {
+  const uint32_t t = b - d.rtile0;
+  if (!(d.s.tile_order & 2u)) return t;
+  const uint32_t n = d.rtx * ((d.h + (uint32_t)TH - 1u) / (uint32_t)TH);
+  return n - 1u - t;
+}
+
+template <int TV, int TW, int TH, int NT>
+VRT_D void batch_refill_tile(const ViewDesc& d, const LaneMem& L, uint32_t b, unsigned long long* counters,
+                             uint32_t* s_next) {
+  const uint32_t t = view_rtile<TH>(d, b);
+  RayCount rc{0u, 0u, 0u, 0u};
+  refill_tile<TV, TW, TH, NT>(d.s, L, d.u, d.fclamp, d.spp, (t % d.rtx) * (uint32_t)TW, (t / d.rtx) * (uint32_t)TH,
+                              (ushort4*)d.radiance, (uchar4*)d.albedo, (float4*)d.gbuf_cur, rc, s_next);
+  count4(counters, rc.rays, rc.bsteps, rc.vsteps, rc.hits);
+  count4(d.vcount, rc.rays, rc.bsteps, rc.vsteps, rc.hits);
+}
+
+// Refill path tracing over every view's TW x TH tiles, one block per tile.
+template <int TV, int WV, int TW, int TH, int NT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WV))) void k_pathtrace_batch_r(
+    const ViewDesc* __restrict__ views, uint32_t nviews, unsigned long long* __restrict__ counters) {
+  extern __shared__ uint32_t lds_dyn[];
+  __shared__ uint32_t s_next;
+  const uint32_t b = blockIdx.x;
+  const ViewDesc& d = views[find_view_r(views, nviews, b)];
+  if (threadIdx.x == 0u) s_next = 0u;
+  const LaneMem L = lds_setup<TV>(d.s, lds_dyn);
+  if constexpr (NT > 64) __syncthreads();
+  batch_refill_tile<TV, TW, TH, NT>(d, L, b, counters, &s_next);
+}
+
+// Persistent form of k_pathtrace_batch_r: each block pulls the next refill
+// tile of any view from the table's queue (one atomic per tile); LDS-staged
+// occupancy is reloaded only when consecutive tiles belong to different
+// scenes.
+template <int TV, int WV, int TW, int TH, int NT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WV))) void k_pathtrace_batch_qr(
+    BatchHead* __restrict__ head, const ViewDesc* __restrict__ views, unsigned long long* __restrict__ counters) {
+  extern __shared__ uint32_t lds_dyn[];
+  __shared__ uint32_t s_tile, s_next;
+  const uint32_t tiles = head->rtiles, nviews = head->nviews;
+  const void* loaded = nullptr;  // scene whose occupancy is in LDS
+  LaneMem L{{nullptr, nullptr, 0u}, {nullptr, 0u, 0u, 0u, 0u}};
+  for (;;) {
+    uint32_t b;
+    if constexpr (NT == 64) {
+      b = 0u;
+      if (threadIdx.x == 0u) b = atomicAdd(&head->queue, 1u);
+      b = __shfl(b, 0, 64);
+    } else {
+      __syncthreads();  // every lane is done with the previous tile (s_tile, s_next, LDS)
+      if (threadIdx.x == 0u) {
+        s_tile = atomicAdd(&head->queue, 1u);
+        s_next = 0u;
+      }
+      __syncthreads();
+      b = s_tile;
+    }
+    if (b >= tiles) break;
+    const ViewDesc& d = views[find_view_r(views, nviews, b)];
+    if (tv_uses_lds(TV)) {
+      if (d.s.pool != loaded) {
+        L = lds_setup<TV>(d.s, lds_dyn);
+        loaded = d.s.pool;
+      }
+    }
+    batch_refill_tile<TV, TW, TH, NT>(d, L, b, counters, &s_next);
   }
 }
 
@@ -1581,9 +1671,68 @@ hipError_t launch_pathtrace_persistent(hipStream_t st, const SceneDev& s, const 
   return hipGetLastError();
 }
 
+int refill_form(int tv, int form, uint32_t max_bit_words, uint32_t max_pwords) {
+  tv = lds_variant(tv, max_bit_words, max_pwords);
+  if ((form == 4 || form == 5) && tv == TV_FASTG) return form;
+  if ((form == 2 || form == 3) && (tv == TV_FAST || tv == TV_FASTG)) return form;
+  return 0;
+}
+
+void refill_tile_dims(int rform, uint32_t* tw, uint32_t* th) {
+  *tw = rform == 2 || rform == 3 ? 32u : 16u;
+  *th = rform == 2 ? 32u : rform == 5 ? 8u : 16u;
+}
+
+// The batched refill kernel K<TV, WV, TW, TH, NT> for (tv, rform, waves).
+template <template <int, int, int, int, int> class K, typename Go>
+static void refill_batch_switch(int tv, int rform, int waves, Go go) {
+  const int wv = waves == 4 || waves == 5 ? waves : 1;
+  const int key = (tv == TV_FASTG ? 100 : 0) + rform * 10 + wv;
+  switch (key) {
+    case 141: go(K<TV_FASTG, 1, 16, 16, 64>{}); break;
+    case 144: go(K<TV_FASTG, 4, 16, 16, 64>{}); break;
+    case 145: go(K<TV_FASTG, 5, 16, 16, 64>{}); break;
+    case 151: go(K<TV_FASTG, 1, 16, 8, 64>{}); break;
+    case 154: go(K<TV_FASTG, 4, 16, 8, 64>{}); break;
+    case 155: go(K<TV_FASTG, 5, 16, 8, 64>{}); break;
+    case 121: go(K<TV_FASTG, 1, 32, 32, 256>{}); break;
+    case 124: go(K<TV_FASTG, 4, 32, 32, 256>{}); break;
+    case 125: go(K<TV_FASTG, 5, 32, 32, 256>{}); break;
+    case 131: go(K<TV_FASTG, 1, 32, 16, 256>{}); break;
+    case 134: go(K<TV_FASTG, 4, 32, 16, 256>{}); break;
+    case 135: go(K<TV_FASTG, 5, 32, 16, 256>{}); break;
+    case 21: go(K<TV_FAST, 1, 32, 32, 256>{}); break;
+    case 24: go(K<TV_FAST, 4, 32, 32, 256>{}); break;
+    case 25: go(K<TV_FAST, 5, 32, 32, 256>{}); break;
+    case 31: go(K<TV_FAST, 1, 32, 16, 256>{}); break;
+    case 34: go(K<TV_FAST, 4, 32, 16, 256>{}); break;
+    default: go(K<TV_FAST, 5, 32, 16, 256>{}); break;
+  }
+}
+
+template <int TV, int WV, int TW, int TH, int NT>
+struct BatchR {
+  static constexpr int nt = NT;
+  static constexpr auto fn = k_pathtrace_batch_r<TV, WV, TW, TH, NT>;
+};
+template <int TV, int WV, int TW, int TH, int NT>
+struct BatchQR {
+  static constexpr int nt = NT;
+  static constexpr auto fn = k_pathtrace_batch_qr<TV, WV, TW, TH, NT>;
+};
+
 hipError_t launch_pathtrace_batch(hipStream_t st, const ViewDesc* views, uint32_t nviews, uint32_t tiles,
                                   unsigned long long* counters, int tv, int waves, uint32_t max_bit_words,
-                                  uint32_t max_pwords) {
+                                  uint32_t max_pwords, int rform, uint32_t rtiles) {
+  if (rform != 0) {
+    tv = lds_variant(tv, max_bit_words, max_pwords);
+    const uint32_t shmem = tv == TV_FASTG ? 0u : lds_bytes(tv, max_bit_words, max_pwords);
+    refill_batch_switch<BatchR>(tv, rform, waves, [&](auto k) {
+      using K = decltype(k);
+      hipLaunchKernelGGL(K::fn, dim3(rtiles > 0 ? rtiles : 1), dim3(K::nt), shmem, st, views, nviews, counters);
+    });
+    return hipGetLastError();
+  }
   tv = lds_variant(tv, max_bit_words, max_pwords);
   const uint32_t shmem = lds_bytes(tv, max_bit_words, max_pwords);
   auto go = [&](auto kernel) {
@@ -1618,12 +1767,18 @@ hipError_t launch_pathtrace_batch(hipStream_t st, const ViewDesc* views, uint32_
 hipError_t launch_batch_persistent(hipStream_t st, BatchHead* head, const ViewDesc* views,
                                    unsigned long long* counters, int tv, int waves, uint32_t max_bit_words,
                                    uint32_t max_pwords, uint32_t pt_blocks, uint32_t gs_blocks, bool denoise,
-                                   bool hdr) {
+                                   bool hdr, int rform) {
   tv = lds_variant(tv, max_bit_words, max_pwords);
   const uint32_t shmem = lds_bytes(tv, max_bit_words, max_pwords);
   const dim3 gp(pt_blocks > 0 ? pt_blocks : 1), gg(gs_blocks > 0 ? gs_blocks : 1);
   auto go = [&](auto kernel) { hipLaunchKernelGGL(kernel, gp, dim3(256), shmem, st, head, views, counters); };
-  switch (tv * 8 + (waves == 4 || waves == 5 ? waves : 1)) {
+  const int wvr = waves == 4 || waves == 5 ? waves : 1;
+  if (rform != 0)
+    refill_batch_switch<BatchQR>(tv, rform, waves, [&](auto k) {
+      using K = decltype(k);
+      hipLaunchKernelGGL(K::fn, gp, dim3(K::nt), tv == TV_FASTG ? 0u : shmem, st, head, views, counters);
+    });
+  else switch (tv * 8 + wvr) {
     case TV_FAST1 * 8 + 4: go(k_pathtrace_batch_q<TV_FAST1, 4>); break;
     case TV_FAST1 * 8 + 5: go(k_pathtrace_batch_q<TV_FAST1, 5>); break;
     case TV_FAST1 * 8 + 1: go(k_pathtrace_batch_q<TV_FAST1, 1>); break;

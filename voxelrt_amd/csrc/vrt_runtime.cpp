@@ -1264,10 +1264,24 @@ int vrt_ctx::render_batch(vrt::BatchLane& lane, const vrt_view* views, uint32_t 
     any_hdr |= d[k].hdr != nullptr;
     tiles += d[k].tiles_x * ((vs.h + 15u) / 16u);
   }
+  // refill forms: the path tracer's own 16 x rth tiles, numbered per view
+  const int ptv = pt_variant(tuning.pt_kernel, tuning.empty_skip);
+  const int rform = vrt::refill_form(ptv, (int)tuning.pt_tile, max_bit_words, max_pwords);
+  uint32_t rtiles = 0;
+  if (rform != 0) {
+    uint32_t tw = 0, th = 0;
+    vrt::refill_tile_dims(rform, &tw, &th);
+    for (uint32_t k = 0; k < nviews; ++k) {
+      d[k].rtile0 = rtiles;
+      d[k].rtx = (d[k].w + tw - 1u) / tw;
+      rtiles += d[k].rtx * ((d[k].h + th - 1u) / th);
+    }
+  }
   std::memset(hh, 0, sizeof *hh);
   hh->nviews = nviews;
   hh->tiles = tiles;
   hh->queue = 0;
+  hh->rtiles = rtiles;
   vrt::BatchHead* dh = (vrt::BatchHead*)lane.desc_dev.ptr;
   const vrt::ViewDesc* dd = (const vrt::ViewDesc*)((char*)lane.desc_dev.ptr + vrt::BATCH_HEAD_BYTES);
   HIP_TRY(this, hipMemcpyAsync(lane.desc_dev.ptr, lane.desc_host[stage],
@@ -1275,19 +1289,22 @@ int vrt_ctx::render_batch(vrt::BatchLane& lane, const vrt_view* views, uint32_t 
   HIP_TRY(this, hipEventRecord(lane.desc_done[stage], st));
   HIP_TRY(this, hipEventRecord(ev[0], st));
   if (tuning.batch_mode == 1 || tuning.batch_mode == 2) {
-    const int tv = pt_variant(tuning.pt_kernel, tuning.empty_skip), wv = (int)tuning.pt_waves;
-    const uint32_t ptb = tuning.batch_blocks ? tuning.batch_blocks : (uint32_t)cus * 8u;
+    const int tv = ptv, wv = (int)tuning.pt_waves;
+    // one-wave blocks for the 16-wide refill tiles: four times as many for the same waves
+    const uint32_t ptb = tuning.batch_blocks ? tuning.batch_blocks : (uint32_t)cus * (rform >= 4 ? 32u : 8u);
     const uint32_t gsb = (uint32_t)cus * 8u;
     auto enqueue = [&] {
       return vrt::launch_batch_persistent(st, dh, dd, (unsigned long long*)d_counters.ptr, tv, wv, max_bit_words,
-                                          max_pwords, ptb, gsb, any_denoise, any_hdr);
+                                          max_pwords, ptb, gsb, any_denoise, any_hdr, rform);
     };
     if (tuning.batch_mode == 1) {
       HIP_TRY(this, enqueue());
     } else {
       // one graph per launch shape; the table it reads is rewritten per batch
-      const uint64_t key = (uint64_t)tv | (uint64_t)wv << 4 | (uint64_t)any_denoise << 8 | (uint64_t)any_hdr << 9 |
-                           (uint64_t)max_bit_words << 10 | (uint64_t)max_pwords << 34 | (uint64_t)(ptb & 0xffff) << 48;
+      const std::pair<uint64_t, uint64_t> key{
+          (uint64_t)tv | (uint64_t)wv << 4 | (uint64_t)rform << 8 | (uint64_t)any_denoise << 12 |
+              (uint64_t)any_hdr << 13 | (uint64_t)ptb << 16,
+          (uint64_t)max_bit_words | (uint64_t)max_pwords << 32};
       auto it = lane.graphs.find(key);
       if (it == lane.graphs.end()) {
         if (lane.graphs.size() >= 16) lane.drop_graphs();
@@ -1313,8 +1330,7 @@ int vrt_ctx::render_batch(vrt::BatchLane& lane, const vrt_view* views, uint32_t 
     return VRT_OK;
   }
   HIP_TRY(this, vrt::launch_pathtrace_batch(st, dd, nviews, tiles, (unsigned long long*)d_counters.ptr,
-                                            pt_variant(tuning.pt_kernel, tuning.empty_skip), (int)tuning.pt_waves,
-                                            max_bit_words, max_pwords));
+                                            ptv, (int)tuning.pt_waves, max_bit_words, max_pwords, rform, rtiles));
   HIP_TRY(this, hipEventRecord(ev[1], st));
   HIP_TRY(this, vrt::launch_temporal_batch(st, dd, nviews, tiles));
   HIP_TRY(this, hipEventRecord(ev[2], st));
