@@ -402,6 +402,8 @@ def main():
     ap.add_argument("--batch-mode", type=int, default=2, choices=[0, 1, 2])
     ap.add_argument("--pt-kernel", default="auto", choices=["auto", "0", "1", "2", "3", "4", "5", "6", "7"])
     ap.add_argument("--pt-waves", type=int, default=0, choices=[0, 4, 5])
+    ap.add_argument("--pt-tile", type=int, default=0, choices=[0, 1, 2, 3, 4, 5],
+                    help="path-tracer tile form with a fixed --pt-kernel")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (--gpus {args.gpus} "
@@ -437,7 +439,8 @@ def main():
         r.set_pose(*cf.world_orbit(desc, 0))
         tuned = ctx.autotune(sc, 0, r.view().u).as_dict()
     else:
-        tuned = ctx.set_tuning(pt_kernel=int(args.pt_kernel), pt_waves=args.pt_waves).as_dict()
+        tuned = ctx.set_tuning(pt_kernel=int(args.pt_kernel), pt_waves=args.pt_waves,
+                               pt_tile=args.pt_tile).as_dict()
 
     for i in range(args.warmup):
         step(i)
@@ -475,7 +478,8 @@ def main():
         "roofline": {"bound": "hbm", "kernel": "k_pathtrace", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": _pmc_traffic("pathtrace_frame1080_pmc.txt",
-                                             f"pt_kernel={tuned['pt_kernel']},pt_waves={tuned['pt_waves']}"),
+                                             f"pt_kernel={tuned['pt_kernel']},pt_waves={tuned['pt_waves']},"
+                                             f"pt_tile={tuned['pt_tile']}"),
                      "alg_bytes_per_launch": int(alg_bytes), "avg_launch_ms": round(pt_avg_ms, 4)},
         "tuning": {k: tuned[k] for k in ("pt_kernel", "pt_waves", "pt_order", "pt_tile", "pt_blocks", "pt_steps",
                                           "pt_min_refill", "pt_chunk", "dda_batch")},

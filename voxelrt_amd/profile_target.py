@@ -1,11 +1,11 @@
 """Profiling target for rocprofv3 runs (kernel traces and --pmc passes).
 
 python voxelrt_amd/profile_target.py [--pt-kernel K] [--pt-waves W] [--pt-order O]
-        [--trace-kernel T] [--dda-batch B] [--traverse-p P ...]
+        [--pt-tile F] [--trace-kernel T] [--dda-batch B] [--traverse-p P ...]
   1. renders 3 frames of frame_1080 (1920x1080, 256^3, 2 bounces, temporal +
      a-trous) with the given path-tracer variant (vrt_tuning.pt_kernel /
-     pt_waves / pt_order; default the one-pixel-per-lane kernel, tiles from
-     the last one),
+     pt_waves / pt_order / pt_tile; default the one-pixel-per-lane kernel on
+     16x16 tiles, tiles from the last one),
   2. for each traversal grid occupancy P (default 0.25), traces 2^24 rays
      twice with the given traversal variant (default the persistent kernel,
      1024 blocks, 8 steps per refill check, refill at 16 idle lanes; brick
@@ -30,6 +30,7 @@ def main():
     ap.add_argument("--pt-kernel", type=int, default=0)
     ap.add_argument("--pt-waves", type=int, default=0)
     ap.add_argument("--pt-order", type=int, default=2)
+    ap.add_argument("--pt-tile", type=int, default=0)
     ap.add_argument("--trace-kernel", type=int, default=1)
     ap.add_argument("--dda-batch", type=int, default=1)
     ap.add_argument("--traverse-p", type=float, nargs="*", default=[0.25])
@@ -39,7 +40,7 @@ def main():
     desc = dict(cf.FRAME_1080_SCENE)
     sc = ctx.upload(cf.frame_1080_scene())
     r = v.Renderer(ctx, sc, 1920, 1080, bounces=2, temporal=True, denoise=True, fclamp=48.0, view_id=1)
-    ctx.set_tuning(pt_kernel=args.pt_kernel, pt_waves=args.pt_waves, pt_order=args.pt_order)
+    ctx.set_tuning(pt_kernel=args.pt_kernel, pt_waves=args.pt_waves, pt_order=args.pt_order, pt_tile=args.pt_tile)
     for i in range(3):
         r.set_pose(*cf.world_orbit(desc, i))
         r.frame()
