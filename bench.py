@@ -44,6 +44,19 @@ def _pmc_traffic(name: str, key: str):
     return None
 
 
+def _pt_kernel_name(t):
+    """The path-tracer kernel a tuning launches at frame_1080 (vrt_tuning's
+    pt_kernel / pt_tile; the single-loop DDA fits LDS at 256^3)."""
+    k, f = t["pt_kernel"], t["pt_tile"]
+    if k in (1, 6):
+        return "k_pathtrace_persistent"
+    if (f in (2, 3) and k in (4, 7)) or (f in (4, 5) and k == 7):
+        return "k_pathtrace_refill"
+    if f == 1 and k == 7:
+        return "k_pathtrace8"
+    return "k_pathtrace"
+
+
 def cpu_baseline(vox, frames, w, h, fclamp, desc):
     """The CPU oracle on `frames` frames of the workload (all host cores)."""
     import numpy as np  # noqa: F401
@@ -475,7 +488,7 @@ def main():
         "config": {"workload": "frame_1080: 1920x1080, 256^3 grid, 1 spp, 2 bounces, temporal "
                                "(rgba16f, cap 64) + a-trous 1/2/4, 64-frame orbit",
                    "width": w, "height": h, "grid": 256, "spp": 1, "bounces": 2},
-        "roofline": {"bound": "hbm", "kernel": "k_pathtrace", "achieved": round(achieved, 1),
+        "roofline": {"bound": "hbm", "kernel": _pt_kernel_name(tuned), "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": _pmc_traffic("pathtrace_frame1080_pmc.txt",
                                              f"pt_kernel={tuned['pt_kernel']},pt_waves={tuned['pt_waves']},"
