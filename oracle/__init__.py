@@ -14,6 +14,7 @@ into ``oracle/_ref`` where the reference tree exists.
 from __future__ import annotations
 
 import ctypes as C
+import hashlib
 import os
 import subprocess
 
@@ -333,9 +334,11 @@ def vb_trace(occ: np.ndarray, rays_vox: np.ndarray, naive: bool, tmax: float = 1
     R = ref_lib()
     if R is None:
         raise RuntimeError("oracle/_ref/libvoxelbench_core.so not built")
-    key = (id(occ), occ.shape, occ.__array_interface__["data"][0])
-    if key != _vb_grid_key:  # the VoxelBench grid + brickmap is rebuilt only for a new grid
-        occ8 = np.ascontiguousarray(occ != 0, np.uint8)
+    # the VoxelBench grid + brickmap is rebuilt only for a new occupancy (keyed
+    # by content: a freed array's id and address are reused by the next one)
+    occ8 = np.ascontiguousarray(occ != 0, np.uint8)
+    key = (occ8.shape, hashlib.blake2b(occ8.tobytes(), digest_size=16).digest())
+    if key != _vb_grid_key:
         nz, ny, nx = occ8.shape
         if R.vb_set_grid(_ptr(occ8), nx, ny, nz) != 0:
             raise ValueError("VoxelBench dims must be multiples of 4")
