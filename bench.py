@@ -228,9 +228,29 @@ def batch(args):
     d0 = descs[specs[0]["scene"]]
     u0, _ = v.make_uniforms(*cf.world_orbit(d0, specs[0]["orbit"]), 1920, 1080, bounces=2, flags=3)
     tuned = ctx.autotune(scenes[specs[0]["scene"]], 0, u0).as_dict()
+    # batched form: launch mode, tile form and persistent grid timed on the
+    # batch itself (its own view states; outside the timed region)
+    tuned_b = ctx.autotune_batch(views_for(0, 90000)).as_dict()
+    # plain per-view baseline: the tile form that renders the 64 views one by
+    # one fastest (the single-view autotuner saw one view only)
+    plain_ms = {}
+    for form in (0, 2, 3, 4, 5):
+        ctx.set_tuning(**dict(tuned, pt_tile=form))
+        vs = views_for(0, 80000 + 100 * form)
+        for view in vs:
+            ctx.render([view])
+        ctx.stats()
+        t0 = time.perf_counter()
+        for view in vs:
+            ctx.render([view])
+        ctx.stats()
+        plain_ms[form] = time.perf_counter() - t0
+        for view in vs:
+            ctx.release_view(view.view_id)
+    tuned_p = dict(tuned, pt_tile=min(plain_ms, key=plain_ms.get))
     modes = {"plain": 0, "batched": 0, "batched_queue": 1, "batched_graph": 2}
     for i, (mode, bm) in enumerate(modes.items()):
-        ctx.set_tuning(batch_mode=bm)
+        ctx.set_tuning(**(tuned_p if mode == "plain" else dict(tuned_b, batch_mode=bm)))
         base = 10000 * (i + 1)
         for f in range(args.warmup):
             vs = views_for(f, base)
@@ -259,6 +279,8 @@ def batch(args):
            "value": results[best]["mrays_s"], "best_mode": best, **results,
            "speedup": round(results[best]["mrays_s"] / max(results["plain"]["mrays_s"], 1e-9), 2),
            "tuning": {k: tuned[k] for k in ("pt_kernel", "pt_waves", "pt_tile")},
+           "tuning_plain": {k: tuned_p[k] for k in ("pt_kernel", "pt_waves", "pt_tile")},
+           "tuning_batched": {k: tuned_b[k] for k in ("batch_mode", "pt_tile", "batch_blocks")},
            "views": len(specs), "pixels_per_batch": pixels, "scene_upload_s": round(upload_s, 2),
            "steps": args.steps, "warmup": args.warmup}
     print(json.dumps(out), flush=True)
@@ -300,12 +322,14 @@ def serve(args):
     scenes = [ctx.upload(v.generate_scene(**d)) for d in descs]
     upload_s = time.perf_counter() - t0
     # kernel variant: the project's autotuner on a 256^3 scene at 1080p (both
-    # plain and served use it); the served form's launch mode from --batch-mode
+    # plain and served use it)
     s256 = next(i for i, d in enumerate(descs) if d["n"] == 256)
     u0, _ = v.make_uniforms(*cf.world_orbit(descs[s256], 0), 1920, 1080, bounces=2, flags=3)
     tuned = ctx.autotune(scenes[s256], 0, u0).as_dict()
-    ctx.set_tuning(batch_mode=args.batch_mode)
-    hdr = [np.zeros((c["height"], c["width"], 3), np.float32) for c in cams]
+    # read-back destinations in page-locked memory (DMA at link speed; a
+    # pageable destination's staged copies made the timing depend on the
+    # driver's staging state)
+    hdr = [ctx.host_array((c["height"], c["width"], 3), np.float32) for c in cams]
 
     def views_for(seq, base):
         prev, frame_of, out = {}, {}, []
@@ -322,8 +346,13 @@ def serve(args):
         return out
 
     warm = list(range(len(cams)))  # one frame per stream: allocates every view's state
+    # the served form's batched launch shape: the batch autotuner on 8 of the
+    # streams' views (about a saturated batch), in --batch-mode (-1: any mode)
+    modes = (0, 1, 2) if args.batch_mode < 0 else (args.batch_mode,)
+    served_tuning = ctx.autotune_batch(views_for(warm[:8], 90000), modes).as_dict()
 
     def run_plain(base, arrivals=None):
+        ctx.set_tuning(**tuned)
         views = views_for(warm + order, base)
         for k in range(len(warm)):
             ctx.render([views[k]], [{"hdr": hdr[warm[k]]}])
@@ -346,6 +375,7 @@ def serve(args):
                     p50_ms=_pct(lat, 50), p99_ms=_pct(lat, 99), mean_service_ms=round(float(np.mean(service)), 3))
 
     def run_served(base, arrivals=None):
+        ctx.set_tuning(**served_tuning)
         views = views_for(warm + order, base)
         for k in range(len(warm)):
             ctx.wait(ctx.submit(views[k]), hdr[warm[k]])
@@ -393,7 +423,8 @@ def serve(args):
            "runs": res, "serve_stats": st, "requests": len(order), "scenes": len(descs),
            "camera_streams": len(cams), "scene_upload_s": round(upload_s, 2),
            "serve_config": {"streams": args.serve_streams, "max_batch_views": args.serve_views,
-                            "max_batch_pixels": args.serve_pixels, "batch_mode": args.batch_mode},
+                            "max_batch_pixels": args.serve_pixels},
+           "tuning_served": {k: served_tuning[k] for k in ("batch_mode", "pt_tile", "batch_blocks")},
            "tuning": {k: tuned[k] for k in ("pt_kernel", "pt_waves", "pt_tile")}}
     print(json.dumps(out), flush=True)
     ctx.close()
@@ -412,7 +443,8 @@ def main():
     ap.add_argument("--serve-streams", type=int, default=2)
     ap.add_argument("--serve-views", type=int, default=64)
     ap.add_argument("--serve-pixels", type=int, default=16 << 20)
-    ap.add_argument("--batch-mode", type=int, default=2, choices=[0, 1, 2])
+    ap.add_argument("--batch-mode", type=int, default=2, choices=[-1, 0, 1, 2],
+                    help="served form's batch mode (-1: the batch autotuner's choice among all)")
     ap.add_argument("--pt-kernel", default="auto", choices=["auto", "0", "1", "2", "3", "4", "5", "6", "7"])
     ap.add_argument("--pt-waves", type=int, default=0, choices=[0, 4, 5])
     ap.add_argument("--pt-tile", type=int, default=0, choices=[0, 1, 2, 3, 4, 5],

@@ -215,6 +215,14 @@ int vrt_render(vrt_ctx* ctx, const vrt_view* views, uint32_t nviews,
                vrt_frame* outs);
 /* Drop the resident state of one view (history restarts). */
 int vrt_view_release(vrt_ctx* ctx, uint32_t view_id);
+/* Page-locked host memory for frame outputs (vrt_frame destinations,
+ * vrt_wait's hdr): read-backs into it run as direct DMA at full link speed,
+ * where a pageable destination goes through driver staging buffers.  Plays
+ * the part of the MAP_READ staging buffer the WebGPU host maps after
+ * copyTextureToBuffer (main.js:571-626).  Free with vrt_host_free (ctx may
+ * then be NULL, e.g. after vrt_destroy). */
+int vrt_host_alloc(vrt_ctx* ctx, uint64_t bytes, void** ptr);
+int vrt_host_free(vrt_ctx* ctx, void* ptr);
 
 /* Single passes on host arrays, for the WebGPU pass interface
  * (temporal.wgsl:7-12, atrous.wgsl:8-18). */
@@ -293,6 +301,16 @@ int vrt_tuning_set(vrt_ctx* ctx, const vrt_tuning* in);
  * given by u) and keep the fastest.  Either part may be skipped with 0. */
 int vrt_autotune(vrt_ctx* ctx, const vrt_scene* scene, uint64_t nrays,
                  const vrt_uniforms* u, vrt_tuning* chosen);
+/* Time the batched form's launch shapes on `views` (2 or more: their scenes,
+ * sizes, poses, spp, bounces and flags, rendered as one vrt_render call
+ * would) and keep the fastest: batch_mode (among those whose bit is set in
+ * `modes`, bit m = batch_mode m; 0 = all), pt_tile (the path tracer's tile
+ * form) and batch_blocks (the persistent grid), with the path-tracer variant
+ * and waves of the current tuning (e.g. from vrt_autotune).  The sweep
+ * renders into view states of its own: the views' histories are untouched,
+ * and its rays do not count in the statistics.  No reference counterpart. */
+int vrt_autotune_batch(vrt_ctx* ctx, const vrt_view* views, uint32_t nviews,
+                       uint32_t modes, vrt_tuning* chosen);
 
 /* ---- stats ------------------------------------------------------------------ */
 int vrt_stats_get(vrt_ctx* ctx, vrt_stats* out);

@@ -451,6 +451,53 @@ def test_batched_views_match_single_views(ctx, tuning):
         sc.release()
 
 
+def test_autotune_batch_keeps_histories(ctx):
+    """vrt_autotune_batch picks the batched form's launch shape on the caller's
+    views without touching their histories or the ray statistics: the frame
+    after the sweep equals the same views' frame rendered without it."""
+    descs = [cf.world_desc(64, 4), cf.world_desc(128, 5)]
+    scenes = [ctx.upload(v.generate_scene(**d)) for d in descs]
+    specs = [(0, 80, 48, 2, 3), (1, 120, 64, 3, 3), (0, 40, 40, 1, 1)]  # scene, w, h, bounces, flags
+
+    def views(frame, base):
+        out = []
+        for k, (si, w, h, nb, flags) in enumerate(specs):
+            pos, yaw, pitch = cf.world_orbit(descs[si], frame * 3 + k, 64)
+            u, _ = v.make_uniforms(pos, yaw, pitch, w, h, frame=frame + 10 * k, bounces=nb, flags=flags)
+            out.append(v.View(scenes[si].handle, u, 48.0, base + k, 1, 0))
+        return out
+
+    def outs():
+        return [{"radiance": np.zeros((h, w, 4), np.uint16), "accum": np.zeros((h, w, 4), np.uint16),
+                 "denoised": np.zeros((h, w, 4), np.uint16)} for _, w, h, _, _ in specs]
+
+    old = ctx.tuning().as_dict()
+    ctx.set_tuning(pt_kernel=7, pt_waves=4)
+    ctx.render(views(0, 4000))
+    ctx.render(views(0, 5000))
+    before = ctx.stats()["rays"]
+    t = ctx.autotune_batch(views(1, 4000)).as_dict()
+    assert t["batch_mode"] in (0, 1, 2) and t["pt_tile"] in (0, 2, 3, 4, 5)
+    t = ctx.autotune_batch(views(1, 4000), modes=(1, 2)).as_dict()  # the persistent modes only
+    assert t["batch_mode"] in (1, 2) and t["pt_tile"] in (0, 2, 3, 4, 5)
+    assert ctx.stats()["rays"] == before
+    assert ctx.tuning().as_dict() == t
+    a, b = outs(), outs()
+    ctx.render(views(1, 4000), a)
+    ctx.render(views(1, 5000), b)
+    for k in range(len(specs)):
+        for n in ("radiance", "accum", "denoised"):
+            if n == "denoised" and not (specs[k][4] & 2):
+                continue
+            assert_bits_equal(f"view {k} {n}", a[k][n], b[k][n])
+    ctx.set_tuning(**old)
+    for k in range(len(specs)):
+        ctx.release_view(4000 + k)
+        ctx.release_view(5000 + k)
+    for sc in scenes:
+        sc.release()
+
+
 # ---------------------------------------------------------------------------
 # editable scenes: device rebuild after a box edit
 # ---------------------------------------------------------------------------

@@ -44,7 +44,11 @@ def _view(scenes, descs, s, vid, prev=None):
     return v.View(scenes[s["si"]].handle, u, 48.0, vid, s["spp"], s["fmt"]), (vp, pos)
 
 
-def _hdr(s):
+def _hdr(s, ctx=None):
+    """HDR destination: pageable numpy, or page-locked (vrt_host_alloc) when
+    a context is given."""
+    if ctx is not None:
+        return ctx.host_array((s["h"], s["w"], 3), np.float32)
     return np.zeros((s["h"], s["w"], 3), np.float32)
 
 
@@ -78,7 +82,7 @@ def test_served_equals_alone(world, mode):
                 time.sleep(0.001)  # arrivals while earlier batches run
         batch_sizes = []
         for k, (t, s) in enumerate(zip(tickets, specs)):
-            hdr = _hdr(s)
+            hdr = _hdr(s, ctx if mode == 2 else None)  # graph form: page-locked destinations
             res = ctx.wait(t, hdr, timeout_ms=120000)
             assert res is not None, f"request {k} not finished"
             assert_bits_equal(f"request {k} hdr", hdr, alone[k][0])

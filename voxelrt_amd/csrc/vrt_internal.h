@@ -153,6 +153,9 @@ struct vrt_ctx {
   double now_ms() const {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   }
+  // device buffers of a view state (budget reserved; history zeroed), not
+  // entered in `views`
+  int alloc_view_state(ViewState& v, uint32_t w, uint32_t h, uint32_t fmt, hipStream_t st);
   int view_state(uint32_t id, uint32_t w, uint32_t h, uint32_t fmt, ViewState** out,
                  hipStream_t st = nullptr);  // st: stream for the initial clears (default: the context's)
   // A scene's device view with this context's launch-time tuning applied.

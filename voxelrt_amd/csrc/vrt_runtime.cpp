@@ -152,10 +152,20 @@ int vrt_ctx::view_state(uint32_t id, uint32_t w, uint32_t h, uint32_t fmt, ViewS
     unreserve(v.bytes);
     views.erase(it);
   }
+  ViewState& v = views[id];
+  int rc = alloc_view_state(v, w, h, fmt, st);
+  if (rc) {
+    views.erase(id);
+    return rc;
+  }
+  *out = &v;
+  return VRT_OK;
+}
+
+int vrt_ctx::alloc_view_state(ViewState& v, uint32_t w, uint32_t h, uint32_t fmt, hipStream_t st) {
   const uint64_t bytes = view_bytes(w, h, fmt);
   int rc = reserve(bytes);
   if (rc) return rc;
-  ViewState& v = views[id];
   v.w = w;
   v.h = h;
   v.fmt = fmt;
@@ -175,11 +185,9 @@ int vrt_ctx::view_state(uint32_t id, uint32_t w, uint32_t h, uint32_t fmt, ViewS
   for (int k = 0; k < 2 && e == hipSuccess; ++k) e = hipMemsetAsync(v.accum[k].ptr, 0, abytes, st);
   if (e != hipSuccess) {
     unreserve(bytes);
-    views.erase(id);
     return fail(e == hipErrorOutOfMemory ? VRT_ERR_NO_MEMORY : VRT_ERR_DEVICE,
                 "view state allocation failed: %s", hipGetErrorString(e));
   }
-  *out = &v;
   return VRT_OK;
 }
 
@@ -825,6 +833,32 @@ int vrt_view_release(vrt_ctx* ctx, uint32_t view_id) {
   GUARD_END(ctx)
 }
 
+int vrt_host_alloc(vrt_ctx* ctx, uint64_t bytes, void** ptr) {
+  if (!ctx || !ptr || bytes == 0) return ctx ? ctx->fail(VRT_ERR_INVALID, "host_alloc arguments") : VRT_ERR_INVALID;
+  GUARD_BEGIN
+  std::lock_guard<std::mutex> lock_(ctx->mu);
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  *ptr = nullptr;
+  const hipError_t e = hipHostMalloc(ptr, bytes, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    *ptr = nullptr;
+    return ctx->fail(VRT_ERR_NO_MEMORY, "host_alloc %llu B: %s", (unsigned long long)bytes, hipGetErrorString(e));
+  }
+  return VRT_OK;
+  GUARD_END(ctx)
+}
+
+int vrt_host_free(vrt_ctx* ctx, void* ptr) {
+  if (!ptr) return VRT_OK;
+  if (!ctx) return hipHostFree(ptr) == hipSuccess ? VRT_OK : VRT_ERR_DEVICE;
+  GUARD_BEGIN
+  std::lock_guard<std::mutex> lock_(ctx->mu);
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  HIP_TRY(ctx, hipHostFree(ptr));
+  return VRT_OK;
+  GUARD_END(ctx)
+}
+
 int vrt_temporal_pass(vrt_ctx* ctx, const vrt_uniforms* u, const uint16_t* radiance,
                       const float* gbuf_cur, const void* accum_prev, const float* gbuf_prev,
                       uint32_t accum_format, void* accum_out) {
@@ -1117,6 +1151,98 @@ int vrt_autotune(vrt_ctx* ctx, const vrt_scene* scene, uint64_t nrays, const vrt
   // the sweep's rays do not count in the statistics
   HIP_TRY(ctx, hipMemcpyAsync(ctx->d_counters.ptr, before, sizeof before, hipMemcpyHostToDevice, ctx->stream));
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  if (chosen) *chosen = best;
+  return VRT_OK;
+  GUARD_END(ctx)
+}
+
+int vrt_autotune_batch(vrt_ctx* ctx, const vrt_view* views, uint32_t nviews, uint32_t modes,
+                       vrt_tuning* chosen) {
+  if (!ctx || !views || nviews < 2)
+    return ctx ? ctx->fail(VRT_ERR_INVALID, "autotune_batch needs 2 or more views") : VRT_ERR_INVALID;
+  if (modes == 0u) modes = 7u;
+  if ((modes & 7u) == 0u || modes > 7u)
+    return ctx->fail(VRT_ERR_INVALID, "autotune_batch modes: bits 0..2 (batch_mode 0..2)");
+  GUARD_BEGIN
+  std::lock_guard<std::mutex> lock_(ctx->mu);
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  for (uint32_t k = 0; k < nviews; ++k) {
+    int rc = ctx->validate_view(views[k]);
+    if (rc) return rc;
+  }
+  {
+    int rc = ctx->order_after_serve();
+    if (rc) return rc;
+  }
+  // the sweep renders into view states of its own: the callers' histories
+  // are not touched
+  std::vector<ViewState> tmp(nviews);
+  std::vector<ViewState*> states(nviews);
+  uint64_t reserved = 0;
+  struct Release {  // runs before tmp's buffers are freed
+    vrt_ctx* c;
+    uint64_t* b;
+    ~Release() {
+      (void)hipStreamSynchronize(c->stream);
+      c->unreserve(*b);
+    }
+  } release_{ctx, &reserved};
+  for (uint32_t k = 0; k < nviews; ++k) {
+    const vrt_view& v = views[k];
+    int rc = ctx->alloc_view_state(tmp[k], (uint32_t)v.u.params1[0], (uint32_t)v.u.params1[1], v.accum_format,
+                                   ctx->stream);
+    if (rc) return rc;
+    reserved += tmp[k].bytes;
+    states[k] = &tmp[k];
+  }
+  uint64_t before[4];
+  int rc = ctx->counters_to_host(before);
+  if (rc) return rc;
+  const vrt_tuning start = ctx->tuning;
+  vrt_tuning best = start;
+  float best_ms = 1e30f;
+  auto time_batch = [&](float* out_ms) -> int {
+    int r = ctx->render_batch(ctx->lane0, views, nviews, states.data(), ctx->ev, nullptr, nullptr);  // warm-up
+    if (r) return r;
+    HIP_TRY(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
+    for (int i = 0; i < 3; ++i) {
+      r = ctx->render_batch(ctx->lane0, views, nviews, states.data(), ctx->ev, nullptr, nullptr);
+      if (r) return r;
+    }
+    HIP_TRY(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
+    HIP_TRY(ctx, hipEventSynchronize(ctx->ev[5]));
+    HIP_TRY(ctx, hipEventElapsedTime(out_ms, ctx->ev[4], ctx->ev[5]));
+    return VRT_OK;
+  };
+  // launch mode x path-trace tile form x persistent grid (0 = the default:
+  // 8 blocks of 256 threads per CU, or 32 one-wave blocks for the 16-wide
+  // refill tiles)
+  const uint32_t cus = (uint32_t)ctx->cus;
+  for (uint32_t mode : {0u, 1u, 2u})
+    for (uint32_t form : {0u, 2u, 3u, 4u, 5u})
+      for (uint32_t b : {0u, 1u, 2u}) {
+        if (!(modes >> mode & 1u) || (mode == 0u && b != 0u)) continue;
+        ctx->tuning = start;
+        ctx->tuning.batch_mode = mode;
+        ctx->tuning.pt_tile = form;
+        // half or twice the default grid
+        ctx->tuning.batch_blocks = b == 0u ? 0u : cus * (form >= 4u ? 32u : 8u) * (b == 1u ? 1u : 4u) / 2u;
+        float ms = 0;
+        rc = time_batch(&ms);
+        if (rc) {
+          ctx->tuning = start;
+          return rc;
+        }
+        if (ms < best_ms) {
+          best_ms = ms;
+          best = ctx->tuning;
+        }
+      }
+  ctx->tuning = best;
+  // the sweep's rays do not count in the statistics
+  HIP_TRY(ctx, hipMemcpyAsync(ctx->d_counters.ptr, before, sizeof before, hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->lane0.drop_graphs();  // captured by the sweep; recaptured on first use
   if (chosen) *chosen = best;
   return VRT_OK;
   GUARD_END(ctx)

@@ -4,6 +4,7 @@ from __future__ import annotations
 import ctypes as C
 import os
 import subprocess
+import weakref
 
 import numpy as np
 
@@ -169,11 +170,14 @@ def _declare(L):
     L.vrt_trace_rays_device.argtypes = [P, P, P, C.c_uint64, C.c_float, P, C.POINTER(C.c_float)]
     L.vrt_render.argtypes = [P, C.POINTER(View), C.c_uint32, C.POINTER(Frame)]
     L.vrt_view_release.argtypes = [P, C.c_uint32]
+    L.vrt_host_alloc.argtypes = [P, C.c_uint64, C.POINTER(P)]
+    L.vrt_host_free.argtypes = [P, P]
     L.vrt_temporal_pass.argtypes = [P, C.POINTER(Uniforms), P, P, P, P, C.c_uint32, P]
     L.vrt_atrous_pass.argtypes = [P, C.POINTER(Uniforms), P, C.c_uint32, P, C.c_int32, P]
     L.vrt_tuning_get.argtypes = [P, C.POINTER(Tuning)]
     L.vrt_tuning_set.argtypes = [P, C.POINTER(Tuning)]
     L.vrt_autotune.argtypes = [P, P, C.c_uint64, C.POINTER(Uniforms), C.POINTER(Tuning)]
+    L.vrt_autotune_batch.argtypes = [P, C.POINTER(View), C.c_uint32, C.c_uint32, C.POINTER(Tuning)]
     L.vrt_stats_get.argtypes = [P, C.POINTER(Stats)]
     L.vrt_stats_reset.argtypes = [P]
     L.vrt_camera_matrices.argtypes = [P, C.c_double, C.c_double, C.c_double, C.c_double,
@@ -197,9 +201,10 @@ def _declare(L):
         getattr(L, name).restype = I
     for name in ("vrt_create", "vrt_device_count", "vrt_scene_upload", "vrt_scene_release",
                  "vrt_scene_bricks", "vrt_scene_dims", "vrt_trace_rays", "vrt_trace_rays_device",
-                 "vrt_render", "vrt_view_release", "vrt_temporal_pass", "vrt_atrous_pass",
+                 "vrt_render", "vrt_view_release", "vrt_host_alloc", "vrt_host_free",
+                 "vrt_temporal_pass", "vrt_atrous_pass",
                  "vrt_stats_get", "vrt_stats_reset", "vrt_generate_scene", "vrt_scene_spawn",
-                 "vrt_tuning_get", "vrt_tuning_set", "vrt_autotune"):
+                 "vrt_tuning_get", "vrt_tuning_set", "vrt_autotune", "vrt_autotune_batch"):
         getattr(L, name).restype = I
 
 
@@ -385,6 +390,20 @@ class Context:
     def release_view(self, view_id: int):
         self._check(self.L.vrt_view_release(self.h, view_id))
 
+    def host_array(self, shape, dtype=np.float32) -> np.ndarray:
+        """A zeroed numpy array in page-locked host memory (vrt_host_alloc), for
+        frame read-backs at full link speed; freed when the array and its views
+        are collected."""
+        dt = np.dtype(dtype)
+        n = int(np.prod(shape)) * dt.itemsize
+        p = C.c_void_p()
+        self._check(self.L.vrt_host_alloc(self.h, max(n, 1), C.byref(p)))
+        buf = (C.c_uint8 * max(n, 1)).from_address(p.value)
+        a = np.frombuffer(buf, np.uint8, n).view(dt).reshape(shape)
+        a[...] = 0
+        weakref.finalize(buf, self.L.vrt_host_free, None, p.value)
+        return a
+
     def temporal_pass(self, u: Uniforms, radiance, gbuf_cur, accum_prev, gbuf_prev, fmt=0):
         w, h = int(u.params1[0]), int(u.params1[1])
         out = np.zeros((h, w, 4), np.uint16 if fmt == 0 else np.float32)
@@ -415,6 +434,16 @@ class Context:
         t = Tuning()
         self._check(self.L.vrt_autotune(self.h, scene.handle, nrays,
                                         None if u is None else C.byref(u), C.byref(t)))
+        return t
+
+    def autotune_batch(self, views, modes=(0, 1, 2)) -> Tuning:
+        """Batched-form launch shapes (batch_mode among `modes`, pt_tile,
+        batch_blocks) timed on `views` rendered as one call; the views'
+        histories are untouched."""
+        nv = len(views)
+        t = Tuning()
+        mask = sum(1 << m for m in modes)
+        self._check(self.L.vrt_autotune_batch(self.h, (View * nv)(*views), nv, mask, C.byref(t)))
         return t
 
     def stats(self) -> dict:
