@@ -1214,7 +1214,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WV))) void 
     __syncthreads();  // every lane is done with the previous tile (s_tile, LDS)
     if (threadIdx.x == 0u) s_tile = atomicAdd(&head->queue, 1u);
     __syncthreads();
-    const uint32_t b = s_tile;
+    // block-uniform: read into a scalar register, so the view's descriptor
+    // (scene, uniforms) loads as scalars instead of into every lane's VGPRs
+    const uint32_t b = __builtin_amdgcn_readfirstlane(s_tile);
     if (b >= tiles) break;
     const uint32_t vi = find_view(views, nviews, b);
     const ViewDesc& d = views[vi];
@@ -1280,11 +1282,14 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WV))) void k
   const void* loaded = nullptr;  // scene whose occupancy is in LDS
   LaneMem L{{nullptr, nullptr, 0u}, {nullptr, 0u, 0u, 0u, 0u}};
   for (;;) {
+    // b is block-uniform; readfirstlane puts it in a scalar register, so the
+    // view's descriptor (scene, uniforms) loads as scalars instead of into
+    // every lane's VGPRs
     uint32_t b;
     if constexpr (NT == 64) {
       b = 0u;
       if (threadIdx.x == 0u) b = atomicAdd(&head->queue, 1u);
-      b = __shfl(b, 0, 64);
+      b = __builtin_amdgcn_readfirstlane(__shfl(b, 0, 64));
     } else {
       __syncthreads();  // every lane is done with the previous tile (s_tile, s_next, LDS)
       if (threadIdx.x == 0u) {
@@ -1292,7 +1297,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WV))) void k
         s_next = 0u;
       }
       __syncthreads();
-      b = s_tile;
+      b = __builtin_amdgcn_readfirstlane(s_tile);
     }
     if (b >= tiles) break;
     const ViewDesc& d = views[find_view_r(views, nviews, b)];
