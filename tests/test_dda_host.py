@@ -130,6 +130,16 @@ def test_empty_row_skip_takes_runs(dda):
             assert b[0] < most[name] * a[0], (name, a, b)
 
 
+def test_exp_nonpos_matches_exp(dda):
+    """The a-trous weights call exp_r_nonpos, exp_r without its overflow
+    branches: it must equal exp_r bit for bit on every float <= 0 (all 2^31
+    negative bit patterns, +0) and on NaNs."""
+    L = dda.lib
+    L.exp_nonpos_check.argtypes = []
+    L.exp_nonpos_check.restype = C.c_uint64
+    assert L.exp_nonpos_check() == 0
+
+
 def test_div_rcp_is_ieee_division(dda):
     """The a-trous taps divide by a per-pixel divisor through its rounded
     reciprocal and one fused-multiply-add correction (vrt_device.h

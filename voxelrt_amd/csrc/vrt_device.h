@@ -156,6 +156,26 @@ VRT_D float pow_r(float x, float y) {
   return exp2_r(y * log2_r(x));
 }
 VRT_D float exp_r(float x) { return exp2_r(x * 0x1.715476p+0f); }
+
+// exp2_r for x <= 0 (and NaN), bit for bit: without the x >= 128 and
+// 2^128 cases, which such arguments never reach.
+VRT_D float exp2_r_nonpos(float x) {
+  const bool nan = x != x, lo = !(x > -126.0f);
+  const float xs = (nan || lo) ? 0.0f : x;
+  const float LN2 = 0x1.62e43p-1f;
+  const float k = vfloor(xs + 0.5f);  // in [-126, 0]
+  const float f = xs - k;
+  const float y = f * LN2;
+  float p = ((((((float)(1.0 / 5040.0) * y + (float)(1.0 / 720.0)) * y +
+                (float)(1.0 / 120.0)) * y + (float)(1.0 / 24.0)) * y +
+              (float)(1.0 / 6.0)) * y + 0.5f) * y;
+  p = (p + 1.0f) * y + 1.0f;
+  float r = p * __builtin_bit_cast(float, (uint32_t)((int)k + 127) << 23);
+  r = lo ? 0.0f : r;
+  return nan ? x : r;
+}
+// exp_r for x <= 0 (and NaN), bit for bit (the a-trous weights: e^-(|d|/s)).
+VRT_D float exp_r_nonpos(float x) { return exp2_r_nonpos(x * 0x1.715476p+0f); }
 VRT_D float pow8(float x) {
   float x2 = x * x;
   float x4 = x2 * x2;
