@@ -81,17 +81,24 @@ void sincos_own(float x, float* s, float* c) {
   }
 }
 
-float exp2_own(float x) {
+// 2^x = 2^k * e^(f ln2), k = round(x): Taylor series to degree 7, Horner
+// steps as correctly rounded fused multiply-adds (the device's exp2_r).
+// target("fma"): the fmaf calls as FMA3 instructions rather than libm calls
+// (the same correctly rounded results).
+__attribute__((target("fma"))) float exp2_own(float x) {
   if (x != x) return x;
   if (!(x > -126.0f)) return 0.0f;
   if (x >= 128.0f) return INFINITY;
   float k = std::floor(x + 0.5f);
   float f = x - k;
   float y = f * LN2_F;
-  float p = ((((((float)(1.0 / 5040.0) * y + (float)(1.0 / 720.0)) * y +
-                (float)(1.0 / 120.0)) * y + (float)(1.0 / 24.0)) * y +
-              (float)(1.0 / 6.0)) * y + 0.5f) * y;
-  p = (p + 1.0f) * y + 1.0f;
+  float p = std::fmaf((float)(1.0 / 5040.0), y, (float)(1.0 / 720.0));
+  p = std::fmaf(p, y, (float)(1.0 / 120.0));
+  p = std::fmaf(p, y, (float)(1.0 / 24.0));
+  p = std::fmaf(p, y, (float)(1.0 / 6.0));
+  p = std::fmaf(p, y, 0.5f);
+  p = std::fmaf(p, y, 1.0f);
+  p = std::fmaf(p, y, 1.0f);
   int ki = (int)k;
   if (ki > 127) {
     p = p * 2.0f;

@@ -28,8 +28,10 @@ VARIANTS = {"nested": 0, "flat": 1, "lds": 2, "fast": 3, "fast_skip": 4, "fast_b
 @pytest.fixture(scope="module")
 def dda(tmp_path_factory):
     out = str(tmp_path_factory.mktemp("dda") / "dda_host.so")
+    # -mfma: the explicit fused multiply-adds (div_rcp, exp2_r) as one
+    # instruction each instead of libm calls (same correctly rounded result)
     cmd = ["/opt/rocm/bin/hipcc", "-x", "hip", "--cuda-host-only", "-O2", "-std=c++17", "-ffp-contract=off",
-           "-fPIC", "-shared", os.path.join(ROOT, "tests", "dda_host.cpp"), "-o", out]
+           "-mfma", "-fPIC", "-shared", os.path.join(ROOT, "tests", "dda_host.cpp"), "-o", out]
     if not os.path.exists(cmd[0]):
         pytest.skip("hipcc not available")
     subprocess.run(cmd, check=True, capture_output=True)

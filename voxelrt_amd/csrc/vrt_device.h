@@ -98,7 +98,8 @@ VRT_D void sincos_r(float x, float* s, float* c) {
   *c = ((q + 1) & 2) ? -cc : cc;
 }
 
-// 2^x = 2^k * e^(f ln2), k = round(x), Taylor series to degree 7.
+// 2^x = 2^k * e^(f ln2), k = round(x), Taylor series to degree 7 in
+// fused multiply-add Horner form (each step one correctly rounded fma).
 // Branch-free: the special cases (NaN -> NaN, x <= -126 -> 0, x >= 128 ->
 // inf) are selected over the polynomial of an in-range stand-in, so lanes of
 // a wave never diverge here; in-range arguments take exactly the operations
@@ -110,10 +111,13 @@ VRT_D float exp2_r(float x) {
   const float k = vfloor(xs + 0.5f);
   const float f = xs - k;
   const float y = f * LN2;
-  float p = ((((((float)(1.0 / 5040.0) * y + (float)(1.0 / 720.0)) * y +
-                (float)(1.0 / 120.0)) * y + (float)(1.0 / 24.0)) * y +
-              (float)(1.0 / 6.0)) * y + 0.5f) * y;
-  p = (p + 1.0f) * y + 1.0f;
+  float p = __builtin_fmaf((float)(1.0 / 5040.0), y, (float)(1.0 / 720.0));
+  p = __builtin_fmaf(p, y, (float)(1.0 / 120.0));
+  p = __builtin_fmaf(p, y, (float)(1.0 / 24.0));
+  p = __builtin_fmaf(p, y, (float)(1.0 / 6.0));
+  p = __builtin_fmaf(p, y, 0.5f);
+  p = __builtin_fmaf(p, y, 1.0f);
+  p = __builtin_fmaf(p, y, 1.0f);
   int ki = (int)k;
   const bool top = ki > 127;  // 2^128 = 2 * 2^127
   p = top ? p * 2.0f : p;
@@ -166,10 +170,13 @@ VRT_D float exp2_r_nonpos(float x) {
   const float k = vfloor(xs + 0.5f);  // in [-126, 0]
   const float f = xs - k;
   const float y = f * LN2;
-  float p = ((((((float)(1.0 / 5040.0) * y + (float)(1.0 / 720.0)) * y +
-                (float)(1.0 / 120.0)) * y + (float)(1.0 / 24.0)) * y +
-              (float)(1.0 / 6.0)) * y + 0.5f) * y;
-  p = (p + 1.0f) * y + 1.0f;
+  float p = __builtin_fmaf((float)(1.0 / 5040.0), y, (float)(1.0 / 720.0));
+  p = __builtin_fmaf(p, y, (float)(1.0 / 120.0));
+  p = __builtin_fmaf(p, y, (float)(1.0 / 24.0));
+  p = __builtin_fmaf(p, y, (float)(1.0 / 6.0));
+  p = __builtin_fmaf(p, y, 0.5f);
+  p = __builtin_fmaf(p, y, 1.0f);
+  p = __builtin_fmaf(p, y, 1.0f);
   float r = p * __builtin_bit_cast(float, (uint32_t)((int)k + 127) << 23);
   r = lo ? 0.0f : r;
   return nan ? x : r;
