@@ -105,18 +105,24 @@ def traverse(args):
     variants = {"lane": dict(trace_kernel=0), "lane_flat": dict(trace_kernel=2), "lane_lds": dict(trace_kernel=3),
                 "lane_fast": dict(trace_kernel=4, dda_batch=1),
                 "lane_fast_b16": dict(trace_kernel=4, dda_batch=16),
+                "lane_fast_b16_w6": dict(trace_kernel=4, dda_batch=16, trace_waves=6),
                 "persistent_fast_b16": dict(PROFILED_TRAVERSE),
+                "persistent_fast_b16_w6": dict(PROFILED_TRAVERSE, trace_waves=6),
                 "persistent": dict(trace_kernel=1, trace_blocks=2048, trace_steps=8, trace_min_refill=16)}
+    # every variant from the same base (an autotuned tuning of the previous
+    # scene must not leak into the next scene's fixed variants)
+    base = dict(trace_waves=0, empty_skip=0, dda_batch=1)
     for p in (0.05, 0.25, 0.6):
         vox = cf.traverse_scene(p)
         sc = ctx.upload(vox)
         if not args.no_cpu:
             print(json.dumps(_traverse_cpu(vox, rays, p)), flush=True)
-        for name in ("lane", "lane_fast", "lane_fast_b16", "persistent", "persistent_fast_b16", "autotuned"):
+        for name in ("lane", "lane_fast", "lane_fast_b16", "lane_fast_b16_w6", "persistent", "persistent_fast_b16",
+                     "persistent_fast_b16_w6", "autotuned"):
             if name == "autotuned":
                 tuned = ctx.autotune(sc, 1 << 22).as_dict()
             else:
-                ctx.set_tuning(**variants[name])
+                ctx.set_tuning(**dict(base, **variants[name]))
             _traverse_one(args, ctx, sc, p, name, d_rays, d_hits, n_rays,
                           tuned if name == "autotuned" else variants[name])
         sc.release()

@@ -292,7 +292,10 @@ typedef struct vrt_tuning {
                                  pixel when theirs is done, one ray per lane per
                                  round (pt 4 and 7), 4 / 5 = the same on one-wave
                                  blocks of 16x16 / 16x8 tiles (pt 7) */
-  uint32_t reserved[4];
+  uint32_t trace_waves;       /* traversal kernels 4, 5 and 6: minimum waves per SIMD
+                                 the register allocation must allow: 0 = compiler's
+                                 choice, 6 (pays on sparse scenes) */
+  uint32_t reserved[3];
 } vrt_tuning;
 int vrt_tuning_get(vrt_ctx* ctx, vrt_tuning* out);
 int vrt_tuning_set(vrt_ctx* ctx, const vrt_tuning* in);

@@ -86,13 +86,17 @@ TRACE_VARIANTS = [dict(trace_kernel=0), dict(trace_kernel=2), dict(trace_kernel=
                   dict(trace_kernel=5, trace_blocks=333, trace_steps=1, trace_min_refill=64),
                   dict(trace_kernel=5, trace_blocks=1024, trace_steps=8, trace_min_refill=16, dda_batch=16),
                   dict(trace_kernel=5, trace_blocks=700, trace_steps=3, trace_min_refill=8, dda_batch=16,
-                       empty_skip=1)]
+                       empty_skip=1),
+                  dict(trace_kernel=4, dda_batch=16, trace_waves=6), dict(trace_kernel=6, trace_waves=6),
+                  dict(trace_kernel=5, trace_blocks=1024, trace_steps=8, trace_min_refill=16, dda_batch=16,
+                       trace_waves=6)]
 
 
 @pytest.fixture(params=range(len(TRACE_VARIANTS)),
                 ids=["lane", "lane-flat", "lane-lds", "lane-fast", "lane-fast-b8", "lane-fast-b64", "lane-fast-skip",
                      "lane-fast-b16-skip", "lane-fast-global", "lane-fast-global-b16", "persist", "persist-s1", "persist-s16", "persist-fast", "persist-fast-333",
-                     "persist-fast-b16", "persist-fast-skip"])
+                     "persist-fast-b16", "persist-fast-skip", "lane-fast-b16-w6", "lane-fast-global-w6",
+                     "persist-fast-b16-w6"])
 def trace_variant(request, ctx):
     old = ctx.tuning().as_dict()
     ctx.set_tuning(**TRACE_VARIANTS[request.param])

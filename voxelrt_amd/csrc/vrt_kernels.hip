@@ -279,8 +279,11 @@ VRT_D LaneMem lds_setup(const SceneDev& s, uint32_t* lds) {
 // ---------------------------------------------------------------------------
 // traversal microbench: one ray per lane, grid-stride
 // ---------------------------------------------------------------------------
-template <int TV>
-__global__ __launch_bounds__(256) void k_trace_rays(SceneDev s, const float* __restrict__ rays,
+// WV: minimum waves per SIMD the register allocation must allow (1 = the
+// compiler's choice; 6 trades a few registers for more waves in flight,
+// which pays on sparse scenes, where rays take long empty walks).
+template <int TV, int WV = 1>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WV))) void k_trace_rays(SceneDev s, const float* __restrict__ rays,
                                                     uint64_t n, float tmax,
                                                     uint4* __restrict__ hits,
                                                     unsigned long long* __restrict__ counters) {
@@ -379,8 +382,8 @@ struct FastRunner {
   VRT_D f3 dir() const { return mk3(st.rdv[0], st.rdv[1], st.rdv[2]); }
 };
 
-template <class Runner, int TV>
-__global__ __launch_bounds__(256) void k_trace_persistent(SceneDev s, const float* __restrict__ rays,
+template <class Runner, int TV, int WV = 1>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WV))) void k_trace_persistent(SceneDev s, const float* __restrict__ rays,
                                                           uint32_t n, float tmax,
                                                           uint4* __restrict__ hits,
                                                           unsigned long long* __restrict__ counters,
@@ -1518,15 +1521,21 @@ hipError_t launch_pack_pool(hipStream_t st, const uint32_t* vox, int nx, int ny,
 }
 
 hipError_t launch_trace_rays(hipStream_t st, const SceneDev& s, const float* rays, uint64_t n,
-                             float tmax, void* hits, unsigned long long* counters, int blocks, int tv) {
+                             float tmax, void* hits, unsigned long long* counters, int blocks, int tv,
+                             int waves) {
   if (blocks <= 0) blocks = 256 * 16;
   const uint32_t need = div_up(n, 256);
   const uint32_t nb = need < (uint32_t)blocks ? need : (uint32_t)blocks;
   const dim3 g(nb > 0 ? nb : 1);
   tv = lds_variant(tv, s.bit_words, s.pwords);
   const uint32_t shmem = lds_bytes(tv, s.bit_words, s.pwords);
-  if (tv == TV_FAST)
+  const bool w6 = waves == 6;
+  if (tv == TV_FAST && w6)
+    hipLaunchKernelGGL((k_trace_rays<TV_FAST, 6>), g, dim3(256), shmem, st, s, rays, n, tmax, (uint4*)hits, counters);
+  else if (tv == TV_FAST)
     hipLaunchKernelGGL(k_trace_rays<TV_FAST>, g, dim3(256), shmem, st, s, rays, n, tmax, (uint4*)hits, counters);
+  else if (tv == TV_FASTG && w6)
+    hipLaunchKernelGGL((k_trace_rays<TV_FASTG, 6>), g, dim3(256), 0, st, s, rays, n, tmax, (uint4*)hits, counters);
   else if (tv == TV_FASTG)
     hipLaunchKernelGGL(k_trace_rays<TV_FASTG>, g, dim3(256), 0, st, s, rays, n, tmax, (uint4*)hits, counters);
   else if (tv == TV_FASTSKIP)
@@ -1543,7 +1552,7 @@ hipError_t launch_trace_rays(hipStream_t st, const SceneDev& s, const float* ray
 
 hipError_t launch_trace_persistent(hipStream_t st, const SceneDev& s, const float* rays, uint32_t n,
                                   float tmax, void* hits, unsigned long long* counters,
-                                  uint32_t* queue, int blocks, int steps, int min_refill, int tv) {
+                                  uint32_t* queue, int blocks, int steps, int min_refill, int tv, int waves) {
   hipError_t e = hipMemsetAsync(queue, 0, sizeof(uint32_t), st);
   if (e != hipSuccess) return e;
   const dim3 g(blocks > 0 ? blocks : 1);
@@ -1552,6 +1561,9 @@ hipError_t launch_trace_persistent(hipStream_t st, const SceneDev& s, const floa
   const uint32_t shmem = lds_bytes(tv, s.bit_words, s.pwords);
   if (tv == TV_FASTSKIP)
     hipLaunchKernelGGL((k_trace_persistent<FastRunner<true>, TV_FASTSKIP>), g, dim3(256), shmem, st, s, rays, n,
+                       tmax, (uint4*)hits, counters, queue, sp, mr);
+  else if (tv_fast(tv) && waves == 6)
+    hipLaunchKernelGGL((k_trace_persistent<FastRunner<false>, TV_FAST, 6>), g, dim3(256), shmem, st, s, rays, n,
                        tmax, (uint4*)hits, counters, queue, sp, mr);
   else if (tv_fast(tv))
     hipLaunchKernelGGL((k_trace_persistent<FastRunner<false>, TV_FAST>), g, dim3(256), shmem, st, s, rays, n, tmax,

@@ -612,9 +612,11 @@ hipError_t vrt_ctx::launch_trace(const vrt::SceneDev& s0, const float* rays, uin
                                         (int)tuning.trace_blocks, (int)tuning.trace_steps,
                                         (int)tuning.trace_min_refill,
                                         tuning.trace_kernel != 5 ? vrt::TV_NESTED
-                                        : (tuning.empty_skip ? vrt::TV_FASTSKIP : vrt::TV_FAST));
+                                        : (tuning.empty_skip ? vrt::TV_FASTSKIP : vrt::TV_FAST),
+                                        (int)tuning.trace_waves);
   return vrt::launch_trace_rays(stream, s, rays, n, tmax, hits, (unsigned long long*)d_counters.ptr,
-                                trace_blocks, trace_variant(tuning.trace_kernel, tuning.empty_skip));
+                                trace_blocks, trace_variant(tuning.trace_kernel, tuning.empty_skip),
+                                (int)tuning.trace_waves);
 }
 
 int vrt_trace_rays_device(vrt_ctx* ctx, const vrt_scene* scene, const float* d_rays, uint64_t n,
@@ -923,7 +925,7 @@ int vrt_tuning_set(vrt_ctx* ctx, const vrt_tuning* in) {
       in->trace_min_refill > 64 || in->pt_min_refill > 64 || in->pt_chunk > (1u << 16) ||
       !(in->pt_waves == 0 || in->pt_waves == 4 || in->pt_waves == 5) || in->batch_mode > 2 ||
       in->batch_blocks > (1u << 16) || in->dda_batch > 64 ||
-      in->empty_skip > 1 || in->pt_order > 3 || in->pt_tile > 5)
+      in->empty_skip > 1 || in->pt_order > 3 || in->pt_tile > 5 || !(in->trace_waves == 0 || in->trace_waves == 6))
     return ctx->fail(VRT_ERR_INVALID, "tuning value out of range");
   ctx->tuning = *in;
   return VRT_OK;
@@ -991,43 +993,50 @@ int vrt_autotune(vrt_ctx* ctx, const vrt_scene* scene, uint64_t nrays, const vrt
     // one-ray-per-lane variants; the single-loop one also over its brick
     // entry batch
     for (uint32_t k : {2u, 3u, 4u, 6u})
-      for (uint32_t db : {1u, 8u, 16u, 32u}) {
-        if (k < 4u && db != 1u) continue;
-        ctx->tuning.trace_kernel = k;
-        ctx->tuning.dda_batch = db;
-        rc = time_it([&] { return ctx->launch_trace(s, (const float*)ctx->scratch_rays.ptr, n, 1e4f,
-                                                    ctx->scratch_hits.ptr); }, &ms);
-        if (rc) return rc;
-        if (ms < best_ms) {
-          best_ms = ms;
-          best.trace_kernel = k;
-          best.dda_batch = db;
+      for (uint32_t db : {1u, 8u, 16u, 32u})
+        for (uint32_t wv : {0u, 6u}) {
+          if (k < 4u && (db != 1u || wv != 0u)) continue;
+          ctx->tuning.trace_kernel = k;
+          ctx->tuning.dda_batch = db;
+          ctx->tuning.trace_waves = wv;
+          rc = time_it([&] { return ctx->launch_trace(s, (const float*)ctx->scratch_rays.ptr, n, 1e4f,
+                                                      ctx->scratch_hits.ptr); }, &ms);
+          if (rc) return rc;
+          if (ms < best_ms) {
+            best_ms = ms;
+            best.trace_kernel = k;
+            best.dda_batch = db;
+            best.trace_waves = wv;
+          }
         }
-      }
+    ctx->tuning.trace_waves = 0;
     for (uint32_t pk : {1u, 5u})
-      for (uint32_t db : {1u, 16u}) {
-        if (pk == 1u && db != 1u) continue;
-        for (uint32_t b : blocks)
-          for (uint32_t st : steps)
-            for (uint32_t rf : refill) {
-              ctx->tuning.trace_kernel = pk;
-              ctx->tuning.trace_blocks = b;
-              ctx->tuning.trace_steps = st;
-              ctx->tuning.trace_min_refill = rf;
-              ctx->tuning.dda_batch = db;
-              rc = time_it([&] { return ctx->launch_trace(s, (const float*)ctx->scratch_rays.ptr, n, 1e4f,
-                                                          ctx->scratch_hits.ptr); }, &ms);
-              if (rc) return rc;
-              if (ms < best_ms) {
-                best_ms = ms;
-                best.trace_kernel = pk;
-                best.trace_blocks = b;
-                best.trace_steps = st;
-                best.trace_min_refill = rf;
-                best.dda_batch = db;
+      for (uint32_t db : {1u, 16u})
+        for (uint32_t wv : {0u, 6u}) {
+          if (pk == 1u && (db != 1u || wv != 0u)) continue;
+          for (uint32_t b : blocks)
+            for (uint32_t st : steps)
+              for (uint32_t rf : refill) {
+                ctx->tuning.trace_kernel = pk;
+                ctx->tuning.trace_blocks = b;
+                ctx->tuning.trace_steps = st;
+                ctx->tuning.trace_min_refill = rf;
+                ctx->tuning.dda_batch = db;
+                ctx->tuning.trace_waves = wv;
+                rc = time_it([&] { return ctx->launch_trace(s, (const float*)ctx->scratch_rays.ptr, n, 1e4f,
+                                                            ctx->scratch_hits.ptr); }, &ms);
+                if (rc) return rc;
+                if (ms < best_ms) {
+                  best_ms = ms;
+                  best.trace_kernel = pk;
+                  best.trace_blocks = b;
+                  best.trace_steps = st;
+                  best.trace_min_refill = rf;
+                  best.dda_batch = db;
+                  best.trace_waves = wv;
+                }
               }
-            }
-      }
+        }
   }
   if (u != nullptr) {
     // path tracer on the given view (its size, pose and bounce count)

@@ -86,7 +86,7 @@ class Tuning(C.Structure):
                 ("pt_min_refill", C.c_uint32), ("pt_chunk", C.c_uint32), ("pt_waves", C.c_uint32),
                 ("batch_mode", C.c_uint32), ("batch_blocks", C.c_uint32), ("dda_batch", C.c_uint32),
                 ("empty_skip", C.c_uint32), ("pt_order", C.c_uint32), ("pt_tile", C.c_uint32),
-                ("reserved", C.c_uint32 * 4)]
+                ("trace_waves", C.c_uint32), ("reserved", C.c_uint32 * 3)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
