@@ -721,13 +721,19 @@ VRT_D PixelOut shade_pixel_1(const SceneDev& s, const LaneMem& L, const Uniforms
 // sample.  The operations and RNG draws are those of shade_pixel in the same
 // order, so a lane that runs start(), then after() once per traced ray,
 // produces the same bits.
+// Register reuse (the path's values are those of shade_pixel): ro is the
+// origin of the next ray -- the camera, then the offset surface point of the
+// last vertex, where both its shadow and its bounce ray start -- and rd its
+// direction: the sun sample during SHADOW (the previous direction is dead
+// once the vertex is known, and the bounce direction is drawn after the
+// shadow ray), else the primary or bounce direction.
 struct SamplePath {
   enum : uint32_t { PRIMARY, SHADOW, BOUNCE };
   uint32_t rng, bounce, phase;
-  f3 ro, rd, l, radiance, tp, n, surf, alb1;
+  f3 ro, rd, radiance, tp, n, alb1;
   Material mat;
   float ndl;
-  VRT_D f3 dir() const { return phase == SHADOW ? l : rd; }
+  VRT_D f3 dir() const { return rd; }
   VRT_D void start(const Uniforms& u, uint32_t px, uint32_t py, uint32_t frame, uint32_t w, uint32_t h) {
     rng = init_rng(px, py, frame);
     const float jx = rand01(rng) - 0.5f;
@@ -769,7 +775,7 @@ struct SamplePath {
         radiance = radiance + tp * sky_color(u, rd, false);
         done = true;
       } else {
-        pos = surf + rd * hh.t;
+        pos = ro + rd * hh.t;
         n = hh.n;
         mat = unpack_material(hh.mat);
         ++bounce;
@@ -779,12 +785,11 @@ struct SamplePath {
     if (vertex) {  // emission + sun NEE at the path vertex
       const bool direct = !(indirect && bounce == 0u);
       if (mat.emissive > 0.0f && direct) radiance = radiance + (tp * mat.albedo) * (mat.emissive * EMISSIVE_SCALE);
-      surf = pos + n * 1e-3f;
-      l = sample_sun_dir(u, rng);
-      ndl = dot(n, l);
-      if (ndl > 0.0f && l.y > 0.0f && direct) {
+      ro = pos + n * 1e-3f;
+      rd = sample_sun_dir(u, rng);
+      ndl = dot(n, rd);
+      if (ndl > 0.0f && rd.y > 0.0f && direct) {
         phase = SHADOW;
-        ro = surf;
       } else {
         after_nee = true;
       }
@@ -803,7 +808,6 @@ struct SamplePath {
         }
         if (alive) {
           phase = BOUNCE;
-          ro = surf;
         } else {
           done = true;
         }
