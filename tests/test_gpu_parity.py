@@ -288,6 +288,21 @@ def test_pathtrace_variants_bitexact(ctx, frame128, bounces, spp, fclamp, frame,
     assert fr.rays == cnt.rays
 
 
+def test_pathtrace_ragged_size_bitexact(ctx, frame128, pt_variant):
+    """An image no tile form divides (77 x 45: partial 8x8, 16x8, 16x16,
+    32x16 and 32x32 tiles at the right and bottom edges)."""
+    vox, sc, og = frame128
+    pos, yaw, pitch = cf.frame_128_pose()
+    w, h = 77, 45
+    u, _ = v.make_uniforms(pos, yaw - 0.2, pitch, w, h, frame=11, bounces=3)
+    outs, fr = _render_one(ctx, sc, 903, u, 48.0, 2, w, h)
+    rad, alb, gb, cnt = oracle.pathtrace(og, to_oracle_uniforms(u), 48.0, 2)
+    assert_bits_equal("gbuf", outs["gbuf"], gb)
+    assert_bits_equal("albedo", outs["albedo"], alb)
+    assert_bits_equal("radiance", outs["radiance"], rad)
+    assert fr.rays == cnt.rays
+
+
 def test_pathtrace_converged_relmse(ctx, frame128):
     """4096 spp over a crop: the GPU estimate against the oracle's own
     4096-spp estimate of the same pixels, relMSE < 1e-3."""
