@@ -98,6 +98,17 @@ VRT_D int isel(bool c, int a, int b) {
   return (int)((m & (uint32_t)a) | (~m & (uint32_t)b));
 }
 
+// floor(x) clamped to [lo, lo + 7] (lo an integer-valued float), as a float:
+// (float)floor_clamp(x, lo, lo + 7) for every finite x.  One median-of-three
+// on the device instead of two compare-and-selects.
+VRT_D float floor_clamp7f(float x, float lo) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_fmed3f(vfloor(x), lo, lo + 7.0f);
+#else
+  return vclamp(vfloor(x), lo, lo + 7.0f);
+#endif
+}
+
 // Bit (i & 31) of w.
 VRT_D bool bit_at(uint32_t w, uint32_t i) {
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -253,11 +264,11 @@ struct DdaFastT {
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       tmB[c] = tm[c];
-      const float p = ro[c] + rdv[c] * ts;
-      const int vp = floor_clamp(p, bmin[c], bmin[c] + 7);
+      const float p = ro[c] + rdv[c] * ts;  // finite: ro, rd finite, ts <= tExit
+      const float vp = floor_clamp7f(p, (float)bmin[c]);  // integer-valued
       const float ge = rdv[c] >= 0.0f ? 1.0f : 0.0f;
-      tm[c] = (((float)vp + ge) - ro[c]) * inv[c];
-      lp[c] = vp - bmin[c];
+      tm[c] = ((vp + ge) - ro[c]) * inv[c];
+      lp[c] = (int)vp - bmin[c];
       d[c] = vabs(inv[c]);
     }
     di[0] = sd[0];
