@@ -543,7 +543,7 @@ def test_scene_edits_rebuild_on_device(ctx):
         assert np.array_equal(words != 0, ob != 0), k
         assert np.array_equal(masks, om), k
         og = oracle.OracleGrid(vox)
-        for tk, skip in ((0, 0), (4, 0), (4, 1)):
+        for tk, skip in ((0, 0), (4, 0), (4, 1), (6, 0)):  # 6: global-memory bitmap/prefix pairs
             old = ctx.tuning().as_dict()
             ctx.set_tuning(trace_kernel=tk, empty_skip=skip)
             _hits_equal(ctx.trace_rays(sc, rays), oracle.trace_rays(og, rays), f"edit {k} trace_kernel {tk}")
