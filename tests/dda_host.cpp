@@ -130,7 +130,9 @@ void build(const uint32_t* vox, int nx, int ny, int nz, HostScene& h) {
 // variant: 0 nested (HBM), 1 flat, 2 nested with LDS occupancy, 3 single loop
 // with both levels in LDS, 4 the same with the empty-row skip, 5 the batched
 // form of 4 (trace_fast<true>; one lane here), 6 the single loop on global
-// memory (DdaFastT<true>), 7 its batched form.  hits: n records of 4 u32
+// memory (DdaFastT<true>), 7 its batched form, 8 and 9 the path tracer's
+// forms of 6 and 3 (brick-entry cell counts from kfine / mfine, KM = true).
+// hits: n records of 4 u32
 // (t bits, voxel, brick_steps | voxel_steps << 16, face), as vrt_trace_rays
 // writes them.  *iters: loop iterations (step() calls) of variants 3 and 4.
 extern "C" int dda_host_trace(const uint32_t* vox, int nx, int ny, int nz, const float* rays, uint64_t n,
@@ -179,6 +181,8 @@ extern "C" int dda_host_trace(const uint32_t* vox, int nx, int ny, int nz, const
       case 5: vrt::trace_fast<true, true>(h.dev, ls, ro, rd, tmax, o); break;
       case 6: vrt::trace_fast<false, false, true>(h.dev, ls, ro, rd, tmax, o); break;
       case 7: vrt::trace_fast<true, false, true>(h.dev, ls, ro, rd, tmax, o); break;
+      case 8: vrt::trace_fast<false, false, true, true>(h.dev, ls, ro, rd, tmax, o); break;
+      case 9: vrt::trace_fast<false, false, false, true>(h.dev, ls, ro, rd, tmax, o); break;
       default: return -1;
     }
     uint32_t tb;

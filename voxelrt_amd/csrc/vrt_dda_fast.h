@@ -125,7 +125,11 @@ VRT_D bool bit_at(uint32_t w, uint32_t i) {
 // pool entry (no copy) -- for scenes whose bitmaps do not fit in LDS.  Both
 // levels then read words of one array at a base pointer (bitmap, or the pool
 // entry) with a 4-byte stride, so the level still only switches registers.
-template <bool GM>
+// KM: the brick-entry cell counts through two per-ray constants (kfine,
+// mfine) instead of per-axis selects: fewer instructions on the entry path,
+// two more live registers -- the path tracer's choice; the traversal kernels,
+// tighter on registers, keep pack_left.
+template <bool GM, bool KM = false>
 struct DdaFastT {
   float ro[3], rdv[3], inv[3];
   int sd[3];
@@ -138,6 +142,9 @@ struct DdaFastT {
   const char* gbase;         // GM: the current level's word array
   float tBound, t;             // coarse: leave at t > tExit; fine: never
   uint32_t axs;  // axis of the last step x 8 (its field offset in cnt)
+  // KM: pack_left over a brick's 8 cells as kfine ^ (cells & mfine): per
+  // axis 7 - p = 7 ^ p (sd > 0), p = 0 ^ p (sd < 0), 255 = 255 ^ 0 (sd == 0)
+  uint32_t kfine, mfine;
   // coarse state saved while walking a brick
   float tmB[3];
   uint32_t idxB, cntB, iterB, blk;
@@ -192,6 +199,14 @@ struct DdaFastT {
     di[2] = sd[2] * (int)(1u << s.psh2);
     idx = (uint32_t)bp[0] | ((uint32_t)bp[1] << s.psh1) | ((uint32_t)bp[2] << s.psh2);
     cnt = pack_left(bp, BG, sd);
+    kfine = mfine = 0u;
+    if constexpr (KM) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        kfine |= (sd[c] > 0 ? 7u : (sd[c] < 0 ? 0u : 255u)) << (8 * c);
+        mfine |= (sd[c] != 0 ? 7u : 0u) << (8 * c);
+      }
+    }
     iter = (uint32_t)(s.bgx + s.bgy + s.bgz + 2);
     if constexpr (GM) {
       gbase = (const char*)s.pbits;
@@ -278,8 +293,12 @@ struct DdaFastT {
     cntB = cnt;
     iterB = iter;
     idx = (uint32_t)lp[0] + 8u * (uint32_t)lp[1] + 64u * (uint32_t)lp[2];
-    const int n8[3] = {8, 8, 8};
-    cnt = pack_left(lp, n8, sd);
+    if constexpr (KM) {
+      cnt = kfine ^ (((uint32_t)lp[0] | ((uint32_t)lp[1] << 8) | ((uint32_t)lp[2] << 16)) & mfine);
+    } else {
+      const int n8[3] = {8, 8, 8};
+      cnt = pack_left(lp, n8, sd);
+    }
     iter = 24u;
     if constexpr (!GM) {
       base4 = 4u * L.lane;
@@ -447,9 +466,9 @@ struct DdaFastT {
 
 using DdaFast = DdaFastT<false>;
 
-template <bool Batched = false, bool Skip = false, bool GM = false>
+template <bool Batched = false, bool Skip = false, bool GM = false, bool KM = false>
 VRT_D void trace_fast(const SceneDev& s, const LdsScene& L, f3 roW, f3 rd, float maxTWorld, TraceResult& o) {
-  DdaFastT<GM> st;
+  DdaFastT<GM, KM> st;
   if (st.begin(s, L, roW, rd, maxTWorld, o)) return;
   st.finish(s, st.template run<Batched, Skip>(s, L), o);
 }

@@ -236,9 +236,9 @@ VRT_D void trace_sel(const SceneDev& s, const LaneMem& L, f3 ro, f3 rd, float tm
   else if constexpr (TV == TV_FASTSKIP)
     trace_fast<false, true>(s, L.f, ro, rd, tmax, h);
   else if constexpr (TV == TV_FASTG)
-    trace_fast<false, false, true>(s, L.f, ro, rd, tmax, h);
+    trace_fast<false, false, true, true>(s, L.f, ro, rd, tmax, h);
   else if constexpr (tv_fast(TV))
-    trace_fast(s, L.f, ro, rd, tmax, h);
+    trace_fast<false, false, false, true>(s, L.f, ro, rd, tmax, h);
   else
     trace(s, ro, rd, tmax, h);
 }
