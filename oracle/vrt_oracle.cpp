@@ -888,9 +888,12 @@ void orc_atrous(const vrt_uniforms* u, const void* src, uint32_t src_format, con
                      load_tex(src, src_format, j * 4 + 2));
           float wKernel = K[dx < 0 ? -dx : dx] * K[dy < 0 ? -dy : dy];
           float wNormal = pow32(fmax_(dot3(gn, mk(gq[0], gq[1], gq[2])), 0.0f));
-          float wDepth = exp_own(-std::fabs(g[3] - gq[3]) / (0.05f * g[3] + 0.01f));
-          float wLum = exp_own(-std::fabs(lumC - dot3(cq, lw)) / sigmaL);
-          float wt = ((wKernel * wNormal) * wDepth) * wLum;
+          // wDepth * wLum = exp(-|dz|/s) * exp(-|dl|/sigmaL) (atrous.wgsl:73-76)
+          // as one exponential of the summed exponents, e^a e^b = e^(a+b)
+          // (within the WGSL exp's own error bound)
+          float qd = std::fabs(g[3] - gq[3]) / (0.05f * g[3] + 0.01f);
+          float ql = std::fabs(lumC - dot3(cq, lw)) / sigmaL;
+          float wt = (wKernel * wNormal) * exp_own(-(qd + ql));
           sum = add(sum, muls(cq, wt));
           wSum += wt;
         }

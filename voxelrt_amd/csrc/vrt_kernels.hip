@@ -1157,11 +1157,13 @@ VRT_D void atrous_tile(const Uniforms& u, uint32_t tile, const void* __restrict_
       const bool use = !(qx < 0 || qy < 0 || qx >= w || qy >= h) && !(gq.w < 0.0f);
       const float wKernel = K[dx < 0 ? -dx : dx] * K[dy < 0 ? -dy : dy];
       const float wNormal = pow32(vmax(dot(gn, mk3(gq.x, gq.y, gq.z)), 0.0f));
-      // arguments <= 0: the quotients of non-negative numbers by positive
-      // divisors (or NaN), so the non-positive form of exp_r applies
-      const float wDepth = exp_r_nonpos(-div_rcp(vabs(g.w - gq.w), dscale, rdscale));
-      const float wLum = exp_r_nonpos(-div_rcp(vabs(lumC - dot(mk3(cq.x, cq.y, cq.z), lw)), sigmaL, rsigmaL));
-      const float wt = ((wKernel * wNormal) * wDepth) * wLum;
+      // wDepth * wLum as one exponential of the summed exponents (e^a e^b =
+      // e^(a+b)); the argument is <= 0 -- minus a sum of quotients of
+      // non-negative numbers by positive divisors (or NaN) -- so the
+      // non-positive form of exp_r applies
+      const float qd = div_rcp(vabs(g.w - gq.w), dscale, rdscale);
+      const float ql = div_rcp(vabs(lumC - dot(mk3(cq.x, cq.y, cq.z), lw)), sigmaL, rsigmaL);
+      const float wt = (wKernel * wNormal) * exp_r_nonpos(-(qd + ql));
       const f3 s2 = sum + mk3(cq.x, cq.y, cq.z) * wt;
       sum = mk3(use ? s2.x : sum.x, use ? s2.y : sum.y, use ? s2.z : sum.z);
       wSum = use ? wSum + wt : wSum;
