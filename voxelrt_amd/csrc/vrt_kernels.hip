@@ -1084,7 +1084,10 @@ __global__ __launch_bounds__(256) void k_temporal(Uniforms u, const ushort4* __r
 
 // ---------------------------------------------------------------------------
 // atrous.wgsl:26-83.  A 16x16 tile stages its (16+4*step)^2 neighbourhood of
-// colour (as f32x4) and G-buffer in LDS; every tap then reads LDS.
+// colour and G-buffer in LDS; every tap then reads LDS.  The staged colour's
+// fourth component holds the texel's luminance (computed once per texel
+// instead of once per tap that reads it; the history count it replaces is
+// only read at the centre, which loads its own texel).
 // ---------------------------------------------------------------------------
 template <int STEP>
 struct AtrousLds {
@@ -1105,6 +1108,7 @@ VRT_D void atrous_tile(const Uniforms& u, uint32_t tile, const void* __restrict_
   const int tiles_x = (w + T - 1) / T;
   const int ox = (int)(tile % (uint32_t)tiles_x) * T - HALO;
   const int oy = (int)(tile / (uint32_t)tiles_x) * T - HALO;
+  const f3 lw = mk3(0.2126f, 0.7152f, 0.0722f);
   for (int k = (int)threadIdx.x; k < S * S; k += 256) {
     const int lx = k % S, ly = k / S;
     const int gx = ox + lx, gy = oy + ly;
@@ -1115,6 +1119,7 @@ VRT_D void atrous_tile(const Uniforms& u, uint32_t tile, const void* __restrict_
       c = load_accum<FMT>(src, j);
       g = gbuf[j];
     }
+    c.w = dot(mk3(c.x, c.y, c.z), lw);  // luminance (atrous.wgsl luminance())
     sc[k] = c;
     sg[k] = g;
   }
@@ -1123,18 +1128,17 @@ VRT_D void atrous_tile(const Uniforms& u, uint32_t tile, const void* __restrict_
   const int x = ox + HALO + tx, y = oy + HALO + ty;
   if (x >= w || y >= h) return;
   const int cidx = (ty + HALO) * S + (tx + HALO);
-  const float4 center = sc[cidx];
-  const float4 g = sg[cidx];
   const uint64_t i = (uint64_t)y * w + x;
+  const float4 center = load_accum<FMT>(src, i);  // with its history count
+  const float4 g = sg[cidx];
   if (g.w < 0.0f) {
     dst[i] = make_ushort4(f32_to_f16(center.x), f32_to_f16(center.y), f32_to_f16(center.z),
                           f32_to_f16(center.w));
     return;
   }
   const float K[3] = {0.375f, 0.25f, 0.0625f};
-  const f3 lw = mk3(0.2126f, 0.7152f, 0.0722f);
   const float history = vmax(center.w, 1.0f);
-  const float lumC = dot(mk3(center.x, center.y, center.z), lw);
+  const float lumC = sc[cidx].w;
   const float relax = 2.0f / vsqrt(history);
   const float sigmaL = vmax(0.08f, relax);
   f3 sum = mk3(center.x, center.y, center.z) * (K[0] * K[0]);
@@ -1162,7 +1166,7 @@ VRT_D void atrous_tile(const Uniforms& u, uint32_t tile, const void* __restrict_
       // non-negative numbers by positive divisors (or NaN) -- so the
       // non-positive form of exp_r applies
       const float qd = div_rcp(vabs(g.w - gq.w), dscale, rdscale);
-      const float ql = div_rcp(vabs(lumC - dot(mk3(cq.x, cq.y, cq.z), lw)), sigmaL, rsigmaL);
+      const float ql = div_rcp(vabs(lumC - cq.w), sigmaL, rsigmaL);
       const float wt = (wKernel * wNormal) * exp_r_nonpos(-(qd + ql));
       const f3 s2 = sum + mk3(cq.x, cq.y, cq.z) * wt;
       sum = mk3(use ? s2.x : sum.x, use ? s2.y : sum.y, use ? s2.z : sum.z);
