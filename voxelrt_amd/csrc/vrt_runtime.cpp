@@ -1104,6 +1104,21 @@ int vrt_autotune(vrt_ctx* ctx, const vrt_scene* scene, uint64_t nrays, const vrt
           }
         }
     ctx->tuning.pt_tile = best.pt_tile;
+    // tile order again for the chosen tile form (refill tiles finish in a
+    // different order than one-pixel-per-lane tiles)
+    ctx->tuning.pt_kernel = best.pt_kernel;
+    ctx->tuning.pt_waves = best.pt_waves;
+    for (uint32_t o : {0u, 1u, 2u, 3u}) {
+      if (o == best.pt_order) continue;  // measured above
+      ctx->tuning.pt_order = o;
+      rc = time_it(launch, &ms);
+      if (rc) return rc;
+      if (ms < best_ms) {
+        best_ms = ms;
+        best.pt_order = o;
+      }
+    }
+    ctx->tuning.pt_order = best.pt_order;
     const uint32_t blocks[] = {512, 1024, 2048};
     const uint32_t steps[] = {4, 8, 16};
     const uint32_t refill[] = {16, 32, 48};
