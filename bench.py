@@ -11,9 +11,11 @@ Prints one JSON line:
   roofline     the path-trace kernel (dominant): algorithmic bytes per launch
                (4 B per brick step + 4 B per voxel step + 4 B per hit voxel +
                28 B of stores per pixel) / its HIP-event duration, vs 8 TB/s
-  cpu_baseline the CPU oracle on 3 frames of the same workload (host cores)
+  cpu_baseline the CPU oracle on --cpu-frames (8) frames of the same orbit
+               (all host cores, OpenMP over rows)
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+                [--workload frame_1080|traverse|batch|serve]
 """
 from __future__ import annotations
 
