@@ -1151,14 +1151,15 @@ VRT_D void atrous_tile(const Uniforms& u, uint32_t tile, const void* __restrict_
 #pragma unroll
     for (int dx = -2; dx <= 2; ++dx) {
       if (dx == 0 && dy == 0) continue;
-      // taps outside the image or on sky add nothing (atrous.wgsl:50-53); the
+      // taps outside the image or on sky add nothing (atrous.wgsl:64-69); the
       // weight is computed for every lane and the sums select, so the wave
-      // does not branch per tap (the halo holds every tap, in or out)
-      const int qx = x + dx * STEP, qy = y + dy * STEP;
+      // does not branch per tap (the halo holds every tap, in or out).  One
+      // test covers both: texels outside the image are staged with the sky
+      // marker depth -1.
       const int l = (ty + HALO + dy * STEP) * S + (tx + HALO + dx * STEP);
       const float4 gq = sg[l];
       const float4 cq = sc[l];
-      const bool use = !(qx < 0 || qy < 0 || qx >= w || qy >= h) && !(gq.w < 0.0f);
+      const bool use = !(gq.w < 0.0f);
       const float wKernel = K[dx < 0 ? -dx : dx] * K[dy < 0 ? -dy : dy];
       const float wNormal = pow32(vmax(dot(gn, mk3(gq.x, gq.y, gq.z)), 0.0f));
       // wDepth * wLum as one exponential of the summed exponents (e^a e^b =

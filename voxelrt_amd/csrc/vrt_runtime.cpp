@@ -940,13 +940,22 @@ int vrt_autotune(vrt_ctx* ctx, const vrt_scene* scene, uint64_t nrays, const vrt
   const vrt::SceneDev s = scene->dev();
   const vrt_tuning start = ctx->tuning;
   vrt_tuning best = start;
+  // a warm-up launch, then the fastest of three rounds of three launches:
+  // a transient slowdown of one round (clocks, a neighbour on the device)
+  // does not decide the choice
   auto time_it = [&](auto&& launch, float* out_ms) -> int {
     HIP_TRY(ctx, launch());  // warm-up
-    HIP_TRY(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
-    for (int r = 0; r < 5; ++r) HIP_TRY(ctx, launch());
-    HIP_TRY(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
-    HIP_TRY(ctx, hipEventSynchronize(ctx->ev[5]));
-    HIP_TRY(ctx, hipEventElapsedTime(out_ms, ctx->ev[4], ctx->ev[5]));
+    float best_round = 1e30f;
+    for (int round = 0; round < 3; ++round) {
+      HIP_TRY(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
+      for (int r = 0; r < 3; ++r) HIP_TRY(ctx, launch());
+      HIP_TRY(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
+      HIP_TRY(ctx, hipEventSynchronize(ctx->ev[5]));
+      float ms = 0.0f;
+      HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev[4], ctx->ev[5]));
+      best_round = ms < best_round ? ms : best_round;
+    }
+    *out_ms = best_round;
     return VRT_OK;
   };
   uint64_t before[4];
@@ -1079,7 +1088,10 @@ int vrt_autotune(vrt_ctx* ctx, const vrt_scene* scene, uint64_t nrays, const vrt
       ctx->tuning.pt_order = o;
       rc = time_it(launch, &ms);
       if (rc) return rc;
-      if (ms < best_ms) {
+      // another order must win clearly: the tuning view is one pose, and the
+      // default (costly bottom rows first) is the better one over a camera
+      // path unless the pose says otherwise by more than the noise
+      if (ms < best_ms * 0.98f) {
         best_ms = ms;
         best.pt_order = o;
       }
@@ -1113,7 +1125,10 @@ int vrt_autotune(vrt_ctx* ctx, const vrt_scene* scene, uint64_t nrays, const vrt
       ctx->tuning.pt_order = o;
       rc = time_it(launch, &ms);
       if (rc) return rc;
-      if (ms < best_ms) {
+      // another order must win clearly: the tuning view is one pose, and the
+      // default (costly bottom rows first) is the better one over a camera
+      // path unless the pose says otherwise by more than the noise
+      if (ms < best_ms * 0.98f) {
         best_ms = ms;
         best.pt_order = o;
       }
