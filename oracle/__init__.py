@@ -102,9 +102,11 @@ def _declare(L):
     L.orc_pcg_hash.argtypes = [C.c_uint32]
     L.orc_pcg_hash.restype = C.c_uint32
     L.orc_rand_stream.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, P, P]
-    for f in ("orc_sin", "orc_cos", "orc_exp2", "orc_log2", "orc_exp"):
+    for f in ("orc_sin", "orc_cos", "orc_exp2", "orc_log2", "orc_exp", "orc_exp_w"):
         getattr(L, f).argtypes = [C.c_float]
         getattr(L, f).restype = C.c_float
+    L.orc_sub_texel.argtypes = [C.c_float, P, P]
+    L.orc_exp_w_table.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, P]
     L.orc_pow.argtypes = [C.c_float, C.c_float]
     L.orc_pow.restype = C.c_float
     L.orc_f32_to_f16.argtypes = [C.c_float]
@@ -126,6 +128,25 @@ def _declare(L):
     L.orc_atrous.argtypes = [C.POINTER(Uniforms), P, C.c_uint32, P, C.c_int, P, C.c_int]
     L.orc_hdr.argtypes = [C.c_int, C.c_int, P, C.c_uint32, P, P]
     L.orc_present.argtypes = [C.c_int, C.c_int, P, C.c_uint32, P, C.c_float, P]
+
+
+def exp_w(x: float) -> float:
+    """e^x as the a-trous weights compute it (atrous.wgsl:74-75)."""
+    return lib().orc_exp_w(x)
+
+
+def exp_w_table(start: int, stride: int, n: int) -> np.ndarray:
+    """exp_w over the float bit patterns start + i*stride (mod 2^32)."""
+    out = np.empty(n, np.float32)
+    lib().orc_exp_w_table(start, stride, n, _ptr(out))
+    return out
+
+
+def sub_texel(tc: float):
+    """(floor, fraction) of a bilinear texel coordinate at 8 fractional bits."""
+    fl, fr = C.c_float(), C.c_float()
+    lib().orc_sub_texel(tc, C.byref(fl), C.byref(fr))
+    return fl.value, fr.value
 
 
 def _ptr(a: np.ndarray):

@@ -47,6 +47,22 @@ const float TWO_PI_F = 2.0f * PI_F;  // const-folded (2*PI) as in WGSL
 const float INV_PI_F = 0.31830988618f;
 const float EMISSIVE_SCALE = 24.0f;
 
+// Bilinear filter position of a texel coordinate tc = uv * size - 0.5
+// (temporal.wgsl:63, textureSampleLevel with a linear sampler).  WGSL leaves
+// the filter's precision to the implementation; texture units filter in
+// fixed point with 8 fractional bits of sub-texel position, so tc is rounded
+// to a multiple of 1/256 (nearest, ties to even) before it is split into the
+// floor and the fraction.  (An exact-FP32 split instead blends ~1e-4 of a
+// neighbour into the history at every static-camera frame, where the
+// reprojected position misses the texel centre by rounding alone; over
+// hundreds of accumulated frames that leaks several percent of the
+// neighbouring pixels -- e.g. sky -- into the accumulation.)
+void sub_texel(float tc, float* fl, float* fr) {
+  const float q = std::rint(tc * 256.0f);
+  *fl = std::floor(q * (1.0f / 256.0f));
+  *fr = (q - *fl * 256.0f) * (1.0f / 256.0f);
+}
+
 // ---------------------------------------------------------------------------
 // Transcendentals.  The reference calls WGSL cos/sin/pow/exp, whose results
 // are implementation-defined (WGSL allows 2^-11 absolute error for cos/sin).
@@ -144,6 +160,27 @@ inline float pow_own(float x, float y) {
   return exp2_own(y * log2_own(x));
 }
 inline float exp_own(float x) { return exp2_own(x * LOG2E_F); }
+
+// e^x for the a-trous weights (atrous.wgsl:74-75), whose arguments are <= 0
+// or NaN: z = x log2(e), floored at -160 (a NaN argument becomes -160, so
+// the weight is 0), k = rint(z) (nearest, ties to even), then
+// 2^k * e^((z - k) ln 2) with the degree-7 Taylor polynomial of exp2_own in
+// fused multiply-add steps, scaled by ldexp (correctly rounded, subnormal
+// results included).  Every step is exact or correctly rounded, so the
+// device's exp_w (vrt_device.h) gives the same bits.
+__attribute__((target("fma"))) float exp_w(float x) {
+  const float z = std::fmax(x * LOG2E_F, -160.0f);
+  const float k = std::rint(z);
+  const float y = (z - k) * LN2_F;
+  float p = std::fmaf((float)(1.0 / 5040.0), y, (float)(1.0 / 720.0));
+  p = std::fmaf(p, y, (float)(1.0 / 120.0));
+  p = std::fmaf(p, y, (float)(1.0 / 24.0));
+  p = std::fmaf(p, y, (float)(1.0 / 6.0));
+  p = std::fmaf(p, y, 0.5f);
+  p = std::fmaf(p, y, 1.0f);
+  p = std::fmaf(p, y, 1.0f);
+  return std::ldexp(p, (int)k);
+}
 inline float pow8(float x) {
   float x2 = x * x;
   float x4 = x2 * x2;
@@ -722,6 +759,17 @@ void orc_camera_matrices(const double pos[3], double yaw, double pitch, double f
   mul4(c2w, iproj, inv_view_proj);
 }
 
+float orc_exp_w(float x) { return exp_w(x); }
+void orc_exp_w_table(uint64_t start, uint64_t stride, uint64_t n, float* out) {
+  for (uint64_t i = 0; i < n; ++i) {
+    const uint32_t b = (uint32_t)(start + i * stride);
+    float x;
+    std::memcpy(&x, &b, 4);
+    out[i] = exp_w(x);
+  }
+}
+void orc_sub_texel(float tc, float* fl, float* fr) { sub_texel(tc, fl, fr); }
+
 void orc_camera_ray(const vrt_uniforms* u, float uvx, float uvy, float out[3]) {
   v3 r = camera_ray(u, uvx, uvy);
   out[0] = r.x; out[1] = r.y; out[2] = r.z;
@@ -819,11 +867,12 @@ void orc_temporal(const vrt_uniforms* u, const uint16_t* radiance, const float* 
           bool depthOk = pg[3] > 0.0f && std::fabs(pg[3] - expectedT) < 0.08f * expectedT + 0.02f;
           bool normalOk = dot3(mk(pg[0], pg[1], pg[2]), mk(g[0], g[1], g[2])) > 0.85f;
           if (depthOk && normalOk) {
-            // textureSampleLevel with a linear, clamp-to-edge sampler, in FP32
-            float tcx = pux * (float)w - 0.5f;
-            float tcy = puy * (float)h - 0.5f;
-            float flx = std::floor(tcx), fly = std::floor(tcy);
-            float fx = tcx - flx, fy = tcy - fly;
+            // textureSampleLevel with a linear, clamp-to-edge sampler, in FP32;
+            // the texel coordinate carries 8 fractional bits, as in a texture
+            // unit's fixed-point filter (sub_texel)
+            float flx, fly, fx, fy;
+            sub_texel(pux * (float)w - 0.5f, &flx, &fx);
+            sub_texel(puy * (float)h - 0.5f, &fly, &fy);
             int x0 = (int)flx, y0 = (int)fly;
             int x1 = x0 + 1, y1 = y0 + 1;
             x0 = x0 < 0 ? 0 : (x0 > w - 1 ? w - 1 : x0);
@@ -888,12 +937,9 @@ void orc_atrous(const vrt_uniforms* u, const void* src, uint32_t src_format, con
                      load_tex(src, src_format, j * 4 + 2));
           float wKernel = K[dx < 0 ? -dx : dx] * K[dy < 0 ? -dy : dy];
           float wNormal = pow32(fmax_(dot3(gn, mk(gq[0], gq[1], gq[2])), 0.0f));
-          // wDepth * wLum = exp(-|dz|/s) * exp(-|dl|/sigmaL) (atrous.wgsl:73-76)
-          // as one exponential of the summed exponents, e^a e^b = e^(a+b)
-          // (within the WGSL exp's own error bound)
-          float qd = std::fabs(g[3] - gq[3]) / (0.05f * g[3] + 0.01f);
-          float ql = std::fabs(lumC - dot3(cq, lw)) / sigmaL;
-          float wt = (wKernel * wNormal) * exp_own(-(qd + ql));
+          float wDepth = exp_w(-std::fabs(g[3] - gq[3]) / (0.05f * g[3] + 0.01f));
+          float wLum = exp_w(-std::fabs(lumC - dot3(cq, lw)) / sigmaL);
+          float wt = ((wKernel * wNormal) * wDepth) * wLum;
           sum = add(sum, muls(cq, wt));
           wSum += wt;
         }

@@ -17,7 +17,9 @@
  *   - dot(a,b) = (ax*bx + ay*by) + az*bz; normalize(v) = v / sqrt(dot(v,v));
  *   - mix(a,b,t) = a*(1-t) + b*t (WGSL definition);
  *   - sin/cos/exp2/log2 are this project's own (Cody-Waite reduction + Taylor
- *     series, see vrt_oracle.cpp); pow(x,8) and pow(x,32) by squaring.
+ *     series, see vrt_oracle.cpp); pow(x,8) and pow(x,32) by squaring; the
+ *     a-trous weights' e^x is exp_w (rint split, Taylor, ldexp);
+ *   - bilinear history sampling with 8 fractional bits of texel position.
  */
 #ifndef VRT_ORACLE_H
 #define VRT_ORACLE_H
@@ -69,6 +71,13 @@ float orc_exp2(float x);
 float orc_log2(float x);
 float orc_exp(float x);
 float orc_pow(float x, float y);
+/* e^x as the a-trous weights compute it (x <= 0; NaN and x <= -160 ln 2
+ * give 0) */
+float orc_exp_w(float x);
+/* orc_exp_w of the float bit patterns start + i*stride (mod 2^32), i < n */
+void orc_exp_w_table(uint64_t start, uint64_t stride, uint64_t n, float* out);
+/* bilinear filter position of texel coordinate tc, 8 fractional bits */
+void orc_sub_texel(float tc, float* fl, float* fr);
 
 /* f32 -> f16 bits, round to nearest even (denormals, overflow to inf) */
 uint16_t orc_f32_to_f16(float f);

@@ -196,33 +196,20 @@ extern "C" int dda_host_trace(const uint32_t* vox, int nx, int ny, int nz, const
   return 0;
 }
 
+// vrt::exp_w (the a-trous weights' exponential, built from the device
+// header) on the float bit patterns start, start + stride, ... (mod 2^32).
+extern "C" void exp_w_table(uint64_t start, uint64_t stride, uint64_t n, float* out) {
+  for (uint64_t i = 0; i < n; ++i) {
+    const uint32_t b = (uint32_t)(start + i * stride);
+    float x;
+    std::memcpy(&x, &b, 4);
+    out[i] = vrt::exp_w(x);
+  }
+}
+
 // vrt::div_rcp against IEEE division on n random pairs (divisors in
 // [2^-8, 2^10), dividends in [2^-37, 2^20) and zero): the number of results
 // whose bits differ.
-// exp_r_nonpos against exp_r over every float bit pattern whose value is
-// <= 0 or NaN (sign bit set, plus +0 and the positive NaNs): mismatches.
-extern "C" uint64_t exp_nonpos_check() {
-  uint64_t bad = 0;
-  auto same = [](float a, float b) {
-    uint32_t ua, ub;
-    std::memcpy(&ua, &a, 4);
-    std::memcpy(&ub, &b, 4);
-    return ua == ub;
-  };
-  for (uint64_t u = 0x80000000ull; u <= 0xffffffffull; ++u) {
-    float x;
-    const uint32_t b = (uint32_t)u;
-    std::memcpy(&x, &b, 4);
-    bad += !same(vrt::exp_r_nonpos(x), vrt::exp_r(x));
-  }
-  for (uint32_t b : {0x00000000u, 0x7fc00000u, 0x7f800001u, 0x7fffffffu}) {
-    float x;
-    std::memcpy(&x, &b, 4);
-    bad += !same(vrt::exp_r_nonpos(x), vrt::exp_r(x));
-  }
-  return bad;
-}
-
 extern "C" uint64_t div_rcp_check(uint64_t n, uint64_t seed) {
   uint64_t z = seed, bad = 0;
   auto next = [&z]() {

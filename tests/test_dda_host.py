@@ -132,14 +132,21 @@ def test_empty_row_skip_takes_runs(dda):
             assert b[0] < most[name] * a[0], (name, a, b)
 
 
-def test_exp_nonpos_matches_exp(dda):
-    """The a-trous weights call exp_r_nonpos, exp_r without its overflow
-    branches: it must equal exp_r bit for bit on every float <= 0 (all 2^31
-    negative bit patterns, +0) and on NaNs."""
+def test_exp_w_device_equals_oracle(dda):
+    """The a-trous weights' exponential (vrt_device.h exp_w, built here from
+    the device header) equals the oracle's exp_w bit for bit on every 61st
+    float bit pattern with the sign bit set (35M values: all negative
+    magnitudes, -0, the negative NaNs) and on +0 and the positive NaNs."""
+    import oracle
     L = dda.lib
-    L.exp_nonpos_check.argtypes = []
-    L.exp_nonpos_check.restype = C.c_uint64
-    assert L.exp_nonpos_check() == 0
+    L.exp_w_table.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, C.c_void_p]
+    n = (1 << 31) // 61
+    for start, stride, cnt in ((0x80000000, 61, n), (0x7fc00000, 1, 1 << 16), (0, 1, 1)):
+        got = np.empty(cnt, np.float32)
+        L.exp_w_table(start, stride, cnt, got.ctypes.data_as(C.c_void_p))
+        want = oracle.exp_w_table(start, stride, cnt)
+        bad = np.flatnonzero(got.view(np.uint32) != want.view(np.uint32))
+        assert len(bad) == 0, (hex(start), bad[:5], got[bad[:5]], want[bad[:5]])
 
 
 def test_div_rcp_is_ieee_division(dda):

@@ -224,6 +224,7 @@ PT_VARIANTS = [dict(pt_kernel=0), dict(pt_kernel=0, pt_waves=5), dict(pt_kernel=
                dict(pt_kernel=4, pt_waves=4, pt_tile=2), dict(pt_kernel=7, pt_waves=5, pt_tile=3),
                dict(pt_kernel=7, pt_tile=2, pt_order=0), dict(pt_kernel=7, pt_waves=4, pt_tile=4),
                dict(pt_kernel=7, pt_tile=5, pt_order=0),
+               dict(pt_kernel=7, pt_waves=4, pt_tile=3), dict(pt_kernel=7, pt_waves=4, pt_tile=5),
                dict(pt_kernel=1, pt_blocks=1024, pt_steps=8, pt_min_refill=32, pt_chunk=64),
                dict(pt_kernel=1, pt_blocks=97, pt_steps=1, pt_min_refill=1, pt_chunk=64),
                dict(pt_kernel=1, pt_blocks=2048, pt_steps=16, pt_min_refill=64, pt_chunk=256),
@@ -240,6 +241,7 @@ PT_VARIANTS = [dict(pt_kernel=0), dict(pt_kernel=0, pt_waves=5), dict(pt_kernel=
                      "lane-fast-global-w5", "lane-fast-global-8x8", "lane-fast-global-8x8-w4-rowmajor",
                      "refill-fast-32x32-w4", "refill-global-32x16-w5", "refill-global-32x32-rowmajor",
                      "refill-wave-16x16-w4", "refill-wave-16x8-rowmajor",
+                     "refill-global-32x16-w4", "refill-wave-16x8-w4",
                      "persist",
                      "persist-s1", "persist-s16", "persist-fast", "persist-fast-w4-b8", "persist-fast-w5-b64"])
 def pt_variant(request, ctx):
@@ -304,18 +306,34 @@ def test_pathtrace_ragged_size_bitexact(ctx, frame128, pt_variant):
 
 
 def test_pathtrace_converged_relmse(ctx, frame128):
-    """4096 spp over a crop: the GPU estimate against the oracle's own
-    4096-spp estimate of the same pixels, relMSE < 1e-3."""
+    """4096 spp over a crop.  With the same PCG per-pixel stream the GPU's
+    converged image reaches relMSE < 1e-3 against the oracle's (it is 0: the
+    two are equal bit for bit).  Against the oracle's estimate from an
+    independent stream (frame seed + 1) the two differ by noise only: their
+    per-channel image means agree within 4 standard errors of the
+    difference, and the relMSE between them is the noise level (< 1e-2)."""
     vox, sc, og = frame128
     pos, yaw, pitch = cf.frame_128_pose()
     w, h = 64, 36
-    u, _ = v.make_uniforms(pos, yaw, pitch, w, h, frame=0, bounces=2, sun_elevation=cf.SUN_40DEG["sun_elevation"],
-                           sun_cos_radius=cf.SUN_40DEG["sun_cos_radius"])
+    kw = dict(bounces=2, sun_elevation=cf.SUN_40DEG["sun_elevation"], sun_cos_radius=cf.SUN_40DEG["sun_cos_radius"])
+    u, _ = v.make_uniforms(pos, yaw, pitch, w, h, frame=0, **kw)
     outs, _ = _render_one(ctx, sc, 902, u, 10.0, 4096, w, h)
-    rad, _, _, _ = oracle.pathtrace(og, to_oracle_uniforms(u), 10.0, 4096)
-    a, b = f16(outs["radiance"])[..., :3], f16(rad)[..., :3]
-    rel = np.mean((a - b) ** 2 / (b * b + 1e-2))
-    assert rel < 1e-3, rel
+    a = f16(outs["radiance"])[..., :3].astype(np.float64)
+
+    def relmse(x, y):
+        return float(np.mean((x - y) ** 2 / (y * y + 1e-2)))
+
+    same, _, _, _ = oracle.pathtrace(og, to_oracle_uniforms(u), 10.0, 4096)
+    assert relmse(a, f16(same)[..., :3].astype(np.float64)) < 1e-3
+    u2, _ = v.make_uniforms(pos, yaw, pitch, w, h, frame=1, **kw)
+    rad, _, _, _ = oracle.pathtrace(og, to_oracle_uniforms(u2), 10.0, 4096)
+    b = f16(rad)[..., :3].astype(np.float64)
+    d = (a - b).reshape(-1, 3)
+    se = d.std(0) / np.sqrt(len(d))
+    rel = relmse(a, b)
+    print(f"independent 4096-spp estimates: mean diff {d.mean(0)} (se {se}), relMSE {rel:.2e}", flush=True)
+    assert np.all(np.abs(d.mean(0)) < 4 * se), (d.mean(0), se)
+    assert 0.0 < rel < 1e-2, rel
 
 
 # ---------------------------------------------------------------------------

@@ -161,15 +161,15 @@ VRT_D float pow_r(float x, float y) {
 }
 VRT_D float exp_r(float x) { return exp2_r(x * 0x1.715476p+0f); }
 
-// exp2_r for x <= 0 (and NaN), bit for bit: without the x >= 128 and
-// 2^128 cases, which such arguments never reach.
-VRT_D float exp2_r_nonpos(float x) {
-  const bool nan = x != x, lo = !(x > -126.0f);
-  const float xs = (nan || lo) ? 0.0f : x;
-  const float LN2 = 0x1.62e43p-1f;
-  const float k = vfloor(xs + 0.5f);  // in [-126, 0]
-  const float f = xs - k;
-  const float y = f * LN2;
+// e^x for the a-trous weights, whose arguments are <= 0 or NaN:
+// z = max(x log2(e), -160) (IEEE maximum: a NaN argument becomes -160 and
+// the weight 0), k = rint(z) in [-160, 0], 2^k * e^((z - k) ln 2) with the
+// degree-7 Taylor polynomial of exp2_r in fused multiply-add steps, scaled
+// by ldexp.  Each step is exact or correctly rounded (14 instructions).
+VRT_D float exp_w(float x) {
+  const float z = __builtin_fmaxf(x * 0x1.715476p+0f, -160.0f);
+  const float k = __builtin_rintf(z);
+  const float y = (z - k) * 0x1.62e43p-1f;
   float p = __builtin_fmaf((float)(1.0 / 5040.0), y, (float)(1.0 / 720.0));
   p = __builtin_fmaf(p, y, (float)(1.0 / 120.0));
   p = __builtin_fmaf(p, y, (float)(1.0 / 24.0));
@@ -177,12 +177,8 @@ VRT_D float exp2_r_nonpos(float x) {
   p = __builtin_fmaf(p, y, 0.5f);
   p = __builtin_fmaf(p, y, 1.0f);
   p = __builtin_fmaf(p, y, 1.0f);
-  float r = p * __builtin_bit_cast(float, (uint32_t)((int)k + 127) << 23);
-  r = lo ? 0.0f : r;
-  return nan ? x : r;
+  return __builtin_ldexpf(p, (int)k);
 }
-// exp_r for x <= 0 (and NaN), bit for bit (the a-trous weights: e^-(|d|/s)).
-VRT_D float exp_r_nonpos(float x) { return exp2_r_nonpos(x * 0x1.715476p+0f); }
 VRT_D float pow8(float x) {
   float x2 = x * x;
   float x4 = x2 * x2;
@@ -194,6 +190,17 @@ VRT_D float pow32(float x) {
   a = a * a;
   a = a * a;
   return a * a;
+}
+
+// Bilinear filter position of texel coordinate tc (= uv * size - 0.5):
+// floor and fraction after rounding tc to 8 fractional bits (nearest, ties
+// to even), the sub-texel precision of a hardware texture filter.  Every
+// step is exact: scaling by 256 and 1/256, rounding to an integer, and the
+// difference of two integers below 2^24.
+VRT_D void sub_texel(float tc, float& fl, float& fr) {
+  const float q = __builtin_rintf(tc * 256.0f);
+  fl = vfloor(q * (1.0f / 256.0f));
+  fr = (q - fl * 256.0f) * (1.0f / 256.0f);
 }
 
 // ---- storage formats ----------------------------------------------------------

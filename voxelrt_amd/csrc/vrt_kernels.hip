@@ -1044,11 +1044,12 @@ VRT_D void temporal_tile(const Uniforms& u, uint32_t tile, const ushort4* __rest
       const bool depthOk = pg.w > 0.0f && vabs(pg.w - expectedT) < 0.08f * expectedT + 0.02f;
       const bool normalOk = dot(mk3(pg.x, pg.y, pg.z), mk3(g.x, g.y, g.z)) > 0.85f;
       if (depthOk && normalOk) {
-        // textureSampleLevel(linear, clamp-to-edge) restated in FP32
-        const float tcx = pux * (float)w - 0.5f;
-        const float tcy = puy * (float)h - 0.5f;
-        const float flx = vfloor(tcx), fly = vfloor(tcy);
-        const float fx = tcx - flx, fy = tcy - fly;
+        // textureSampleLevel(linear, clamp-to-edge) restated in FP32, with
+        // the texel coordinate rounded to 8 fractional bits as a texture
+        // unit's fixed-point filter does (sub_texel)
+        float flx, fly, fx, fy;
+        sub_texel(pux * (float)w - 0.5f, flx, fx);
+        sub_texel(puy * (float)h - 0.5f, fly, fy);
         int x0 = (int)flx, y0 = (int)fly;
         int x1 = x0 + 1, y1 = y0 + 1;
         x0 = x0 < 0 ? 0 : (x0 > (int)w - 1 ? (int)w - 1 : x0);
@@ -1083,17 +1084,42 @@ __global__ __launch_bounds__(256) void k_temporal(Uniforms u, const ushort4* __r
 }
 
 // ---------------------------------------------------------------------------
-// atrous.wgsl:26-83.  A 16x16 tile stages its (16+4*step)^2 neighbourhood of
-// colour and G-buffer in LDS; every tap then reads LDS.  The staged colour's
-// fourth component holds the texel's luminance (computed once per texel
-// instead of once per tap that reads it; the history count it replaces is
-// only read at the centre, which loads its own texel).
+// atrous.wgsl:26-83.  A 16x16 tile stages its (16+4*step)^2 neighbourhood in
+// LDS: per texel the colour with its luminance in the fourth component
+// (computed once per texel instead of once per tap), its t, and a code for
+// its normal.  Sky texels and texels outside the image are staged with colour
+// 0, t = -1 and code NCODE_SKY.
+//
+// Normal weight.  pow(max(dot(n, nq), 0), 32) (atrous.wgsl:73) is exactly 1
+// or +0 when both normals are axis vectors or zero -- what the path tracer
+// writes (-vmask * stepDir, voxel.wgsl:126): 1 iff they are the same non-zero
+// vector.  Each texel's normal is coded 1..6 (+x,-x,+y,-y,+z,-z), 0 (zero
+// vector) or NCODE_GENERAL (anything else); the centre's zero code is mapped
+// to a value no tap carries, and the weight becomes a code compare.  A tile
+// with any general normal in its neighbourhood (host-supplied G-buffers) takes
+// the pow32 form instead, reading the normals from the G-buffer.
+//
+// Weights: ((wKernel * wNormal) * wDepth) * wLum as atrous.wgsl:76 multiplies
+// them, wDepth and wLum each through exp_w; the divisions by the per-pixel
+// depth scale and sigmaL as div_rcp (bit-identical to IEEE division).
 // ---------------------------------------------------------------------------
+constexpr uint32_t NCODE_GENERAL = 7u, NCODE_SKY = 15u, NCODE_NONE = 31u;
+
+VRT_D uint32_t normal_code(float x, float y, float z) {
+  const float ax = vabs(x), ay = vabs(y), az = vabs(z);
+  const uint32_t nz = (ax != 0.0f ? 1u : 0u) + (ay != 0.0f ? 1u : 0u) + (az != 0.0f ? 1u : 0u);
+  if (nz == 0u) return 0u;
+  if (nz != 1u || !(ax == 1.0f || ay == 1.0f || az == 1.0f)) return NCODE_GENERAL;
+  const uint32_t axis = ax != 0.0f ? 0u : (ay != 0.0f ? 1u : 2u);
+  const float v = axis == 0u ? x : (axis == 1u ? y : z);
+  return 1u + 2u * axis + (v < 0.0f ? 1u : 0u);
+}
+
 template <int STEP>
 struct AtrousLds {
   static constexpr int T = 16, HALO = 2 * STEP, S = T + 2 * HALO;
-  float4 c[S * S];
-  float4 g[S * S];
+  float4 c[S * S];  // rgb, luminance
+  float2 g[S * S];  // t, normal code (as float)
 };
 
 template <int FMT, int STEP>
@@ -1103,35 +1129,41 @@ VRT_D void atrous_tile(const Uniforms& u, uint32_t tile, const void* __restrict_
   constexpr int HALO = 2 * STEP;
   constexpr int S = T + 2 * HALO;
   float4* sc = lds.c;
-  float4* sg = lds.g;
+  float2* sg = lds.g;
   const int w = (int)u.params1[0], h = (int)u.params1[1];
   const int tiles_x = (w + T - 1) / T;
   const int ox = (int)(tile % (uint32_t)tiles_x) * T - HALO;
   const int oy = (int)(tile / (uint32_t)tiles_x) * T - HALO;
   const f3 lw = mk3(0.2126f, 0.7152f, 0.0722f);
+  int general = 0;
   for (int k = (int)threadIdx.x; k < S * S; k += 256) {
     const int lx = k % S, ly = k / S;
     const int gx = ox + lx, gy = oy + ly;
     float4 c = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    float4 g = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
+    float2 tg = make_float2(-1.0f, __builtin_bit_cast(float, NCODE_SKY));
     if (gx >= 0 && gy >= 0 && gx < w && gy < h) {
       const uint64_t j = (uint64_t)gy * w + gx;
-      c = load_accum<FMT>(src, j);
-      g = gbuf[j];
+      const float4 g = gbuf[j];
+      if (!(g.w < 0.0f)) {  // sky taps add nothing (atrous.wgsl:69)
+        c = load_accum<FMT>(src, j);
+        c.w = dot(mk3(c.x, c.y, c.z), lw);  // luminance (atrous.wgsl:22-24)
+        const uint32_t code = normal_code(g.x, g.y, g.z);
+        general |= code == NCODE_GENERAL ? 1 : 0;
+        tg = make_float2(g.w, __builtin_bit_cast(float, code));
+      }
     }
-    c.w = dot(mk3(c.x, c.y, c.z), lw);  // luminance (atrous.wgsl luminance())
     sc[k] = c;
-    sg[k] = g;
+    sg[k] = tg;
   }
-  __syncthreads();
+  general = __syncthreads_or(general);
   const int tx = (int)(threadIdx.x & 15u), ty = (int)(threadIdx.x >> 4);
   const int x = ox + HALO + tx, y = oy + HALO + ty;
   if (x >= w || y >= h) return;
   const int cidx = (ty + HALO) * S + (tx + HALO);
   const uint64_t i = (uint64_t)y * w + x;
   const float4 center = load_accum<FMT>(src, i);  // with its history count
-  const float4 g = sg[cidx];
-  if (g.w < 0.0f) {
+  const float2 g = sg[cidx];
+  if (g.x < 0.0f) {  // sky: copy (atrous.wgsl:38-41)
     dst[i] = make_ushort4(f32_to_f16(center.x), f32_to_f16(center.y), f32_to_f16(center.z),
                           f32_to_f16(center.w));
     return;
@@ -1143,35 +1175,48 @@ VRT_D void atrous_tile(const Uniforms& u, uint32_t tile, const void* __restrict_
   const float sigmaL = vmax(0.08f, relax);
   f3 sum = mk3(center.x, center.y, center.z) * (K[0] * K[0]);
   float wSum = K[0] * K[0];
-  const f3 gn = mk3(g.x, g.y, g.z);
-  const float dscale = 0.05f * g.w + 0.01f;
-  const float rdscale = 1.0f / dscale, rsigmaL = 1.0f / sigmaL;  // div_rcp: the taps' quotients
+  const float dscale = 0.05f * g.x + 0.01f;
+  const float rdscale = 1.0f / dscale, rsigmaL = 1.0f / sigmaL;
+  if (!general) {
+    const uint32_t cc0 = __builtin_bit_cast(uint32_t, g.y);
+    const uint32_t cc = cc0 == 0u ? NCODE_NONE : cc0;  // a zero normal weighs every tap 0
 #pragma unroll
-  for (int dy = -2; dy <= 2; ++dy) {
+    for (int dy = -2; dy <= 2; ++dy) {
 #pragma unroll
-    for (int dx = -2; dx <= 2; ++dx) {
-      if (dx == 0 && dy == 0) continue;
-      // taps outside the image or on sky add nothing (atrous.wgsl:64-69); the
-      // weight is computed for every lane and the sums select, so the wave
-      // does not branch per tap (the halo holds every tap, in or out).  One
-      // test covers both: texels outside the image are staged with the sky
-      // marker depth -1.
-      const int l = (ty + HALO + dy * STEP) * S + (tx + HALO + dx * STEP);
-      const float4 gq = sg[l];
-      const float4 cq = sc[l];
-      const bool use = !(gq.w < 0.0f);
-      const float wKernel = K[dx < 0 ? -dx : dx] * K[dy < 0 ? -dy : dy];
-      const float wNormal = pow32(vmax(dot(gn, mk3(gq.x, gq.y, gq.z)), 0.0f));
-      // wDepth * wLum as one exponential of the summed exponents (e^a e^b =
-      // e^(a+b)); the argument is <= 0 -- minus a sum of quotients of
-      // non-negative numbers by positive divisors (or NaN) -- so the
-      // non-positive form of exp_r applies
-      const float qd = div_rcp(vabs(g.w - gq.w), dscale, rdscale);
-      const float ql = div_rcp(vabs(lumC - cq.w), sigmaL, rsigmaL);
-      const float wt = (wKernel * wNormal) * exp_r_nonpos(-(qd + ql));
-      const f3 s2 = sum + mk3(cq.x, cq.y, cq.z) * wt;
-      sum = mk3(use ? s2.x : sum.x, use ? s2.y : sum.y, use ? s2.z : sum.z);
-      wSum = use ? wSum + wt : wSum;
+      for (int dx = -2; dx <= 2; ++dx) {
+        if (dx == 0 && dy == 0) continue;
+        // sky and outside taps have code NCODE_SKY and colour 0: weight +0,
+        // contribution 0, as if skipped (atrous.wgsl:64-69)
+        const int l = (ty + HALO + dy * STEP) * S + (tx + HALO + dx * STEP);
+        const float2 gq = sg[l];
+        const float4 cq = sc[l];
+        const float wKN = __builtin_bit_cast(uint32_t, gq.y) == cc ? K[dx < 0 ? -dx : dx] * K[dy < 0 ? -dy : dy] : 0.0f;
+        const float wDepth = exp_w(-div_rcp(vabs(g.x - gq.x), dscale, rdscale));
+        const float wLum = exp_w(-div_rcp(vabs(lumC - cq.w), sigmaL, rsigmaL));
+        const float wt = (wKN * wDepth) * wLum;
+        sum = sum + mk3(cq.x, cq.y, cq.z) * wt;
+        wSum = wSum + wt;
+      }
+    }
+  } else {
+    const float4 gc = gbuf[i];
+    const f3 gn = mk3(gc.x, gc.y, gc.z);
+    for (int dy = -2; dy <= 2; ++dy) {
+      for (int dx = -2; dx <= 2; ++dx) {
+        if (dx == 0 && dy == 0) continue;
+        const int l = (ty + HALO + dy * STEP) * S + (tx + HALO + dx * STEP);
+        const float2 gq = sg[l];
+        if (gq.x < 0.0f) continue;  // sky or outside the image
+        const float4 cq = sc[l];
+        const float4 nq = gbuf[(uint64_t)(y + dy * STEP) * w + (x + dx * STEP)];
+        const float wKernel = K[dx < 0 ? -dx : dx] * K[dy < 0 ? -dy : dy];
+        const float wNormal = pow32(vmax(dot(gn, mk3(nq.x, nq.y, nq.z)), 0.0f));
+        const float wDepth = exp_w(-div_rcp(vabs(g.x - gq.x), dscale, rdscale));
+        const float wLum = exp_w(-div_rcp(vabs(lumC - cq.w), sigmaL, rsigmaL));
+        const float wt = ((wKernel * wNormal) * wDepth) * wLum;
+        sum = sum + mk3(cq.x, cq.y, cq.z) * wt;
+        wSum = wSum + wt;
+      }
     }
   }
   const f3 o = sum / vmax(wSum, 1e-6f);
