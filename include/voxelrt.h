@@ -207,6 +207,19 @@ int vrt_trace_rays_device(vrt_ctx* ctx, const vrt_scene* scene,
                           const float* d_rays, uint64_t n, float tmax,
                           vrt_hit* d_hits, float* ms);
 
+/* Device buffers for the device-resident entry points (rays, hits), owned by
+ * the context and counted in its HBM budget; freed by vrt_device_free or
+ * vrt_destroy.  vrt_copy moves bytes between host and device (or device and
+ * device) on the context's stream and waits for the copy.  No reference
+ * counterpart (the WebGPU host creates its buffers with
+ * device.createBuffer, main.js:187-203). */
+#define VRT_COPY_TO_DEVICE 1u
+#define VRT_COPY_TO_HOST   2u
+#define VRT_COPY_DEVICE    3u
+int vrt_device_alloc(vrt_ctx* ctx, uint64_t bytes, void** dptr);
+int vrt_device_free(vrt_ctx* ctx, void* dptr);
+int vrt_copy(vrt_ctx* ctx, void* dst, const void* src, uint64_t bytes, uint32_t kind);
+
 /* ---- frames ---------------------------------------------------------------- *
  * Render nviews views: pathtrace -> temporal -> (if VRT_FLAG_DENOISE)
  * a-trous steps 1/2/4, per view.  nviews > 1 runs the batched form: all

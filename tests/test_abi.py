@@ -53,6 +53,14 @@ def test_serving_entry_points_reject_null_context():
     assert L.vrt_serve_configure(None, None) == -1
 
 
+def test_device_entry_points_reject_null_context():
+    L = v.load()
+    p = C.c_void_p()
+    assert L.vrt_device_alloc(None, 16, C.byref(p)) == -1
+    assert L.vrt_device_free(None, None) == -1
+    assert L.vrt_copy(None, None, None, 0, 1) == -1
+
+
 def test_error_paths_without_device():
     L = v.load()
     n = C.c_int32(-1)

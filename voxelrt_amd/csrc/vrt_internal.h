@@ -164,8 +164,14 @@ struct vrt_ctx {
     s.tile_order = tuning.pt_order;
     return s;
   }
+  // counters / queue: where the kernels count and keep their ray-queue head
+  // (null = the context's own); clear_queue: reset the head with a memset
+  // (the allocation-free entry point clears its scratch with a kernel)
   hipError_t launch_trace(const vrt::SceneDev& s, const float* rays, uint64_t n, float tmax,
-                          void* hits);
+                          void* hits, unsigned long long* counters = nullptr, uint32_t* queue = nullptr,
+                          bool clear_queue = true);
+  // device buffers handed out by vrt_device_alloc (bytes, counted in the budget)
+  std::unordered_map<void*, uint64_t> user_buffers;
   hipError_t launch_pathtrace(const vrt::SceneDev& s, const vrt_uniforms& u, float fclamp,
                               uint32_t spp, void* radiance, void* albedo, void* gbuf);
 };

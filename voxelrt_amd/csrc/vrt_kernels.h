@@ -42,12 +42,15 @@ hipError_t launch_pack_pool(hipStream_t st, const uint32_t* vox, int nx, int ny,
                             uint32_t* pool);
 // waves: 6 = the single-loop kernels built for >= 6 waves per SIMD, else the
 // compiler's register allocation
+// Zero n 32-bit words at p on st (a kernel, not a memset node).
+hipError_t launch_zero_words(hipStream_t st, void* p, uint32_t n);
 hipError_t launch_trace_rays(hipStream_t st, const SceneDev& s, const float* rays, uint64_t n,
                              float tmax, void* hits, unsigned long long* counters, int blocks, int tv,
                              int waves);
 hipError_t launch_trace_persistent(hipStream_t st, const SceneDev& s, const float* rays, uint32_t n,
                                   float tmax, void* hits, unsigned long long* counters,
-                                  uint32_t* queue, int blocks, int steps, int min_refill, int tv, int waves);
+                                  uint32_t* queue, int blocks, int steps, int min_refill, int tv, int waves,
+                                  bool clear_queue = true);
 hipError_t launch_pathtrace(hipStream_t st, const SceneDev& s, const Uniforms& u, float fclamp,
                             uint32_t spp, void* radiance, void* albedo, void* gbuf,
                             unsigned long long* counters, int tv, int waves, int form);

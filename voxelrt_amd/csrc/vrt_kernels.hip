@@ -1606,11 +1606,25 @@ hipError_t launch_trace_rays(hipStream_t st, const SceneDev& s, const float* ray
   return hipGetLastError();
 }
 
+// Zero n 32-bit words with a kernel: a launch a caller may capture into a
+// graph (no memset node).
+__global__ __launch_bounds__(64) void k_zero_words(uint32_t* __restrict__ p, uint32_t n) {
+  for (uint32_t i = threadIdx.x; i < n; i += 64u) p[i] = 0u;
+}
+
+hipError_t launch_zero_words(hipStream_t st, void* p, uint32_t n) {
+  hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, st, (uint32_t*)p, n);
+  return hipGetLastError();
+}
+
 hipError_t launch_trace_persistent(hipStream_t st, const SceneDev& s, const float* rays, uint32_t n,
                                   float tmax, void* hits, unsigned long long* counters,
-                                  uint32_t* queue, int blocks, int steps, int min_refill, int tv, int waves) {
-  hipError_t e = hipMemsetAsync(queue, 0, sizeof(uint32_t), st);
-  if (e != hipSuccess) return e;
+                                  uint32_t* queue, int blocks, int steps, int min_refill, int tv, int waves,
+                                  bool clear_queue) {
+  if (clear_queue) {
+    const hipError_t e = hipMemsetAsync(queue, 0, sizeof(uint32_t), st);
+    if (e != hipSuccess) return e;
+  }
   const dim3 g(blocks > 0 ? blocks : 1);
   const int sp = steps < 1 ? 1 : steps, mr = min_refill < 1 ? 1 : min_refill;
   tv = tv_fast(tv) ? lds_variant(tv, s.bit_words, s.pwords) : TV_NESTED;

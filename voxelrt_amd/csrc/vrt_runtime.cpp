@@ -604,19 +604,20 @@ hipError_t vrt_ctx::launch_pathtrace(const vrt::SceneDev& s0, const vrt_uniforms
 }
 
 hipError_t vrt_ctx::launch_trace(const vrt::SceneDev& s0, const float* rays, uint64_t n, float tmax,
-                                 void* hits) {
+                                 void* hits, unsigned long long* counters, uint32_t* queue,
+                                 bool clear_queue) {
   const vrt::SceneDev s = tuned(s0);
+  if (!counters) counters = (unsigned long long*)d_counters.ptr;
+  if (!queue) queue = (uint32_t*)d_queue.ptr;
   if ((tuning.trace_kernel == 1 || tuning.trace_kernel == 5) && n < 0x80000000ull)
-    return vrt::launch_trace_persistent(stream, s, rays, (uint32_t)n, tmax, hits,
-                                        (unsigned long long*)d_counters.ptr, (uint32_t*)d_queue.ptr,
+    return vrt::launch_trace_persistent(stream, s, rays, (uint32_t)n, tmax, hits, counters, queue,
                                         (int)tuning.trace_blocks, (int)tuning.trace_steps,
                                         (int)tuning.trace_min_refill,
                                         tuning.trace_kernel != 5 ? vrt::TV_NESTED
                                         : (tuning.empty_skip ? vrt::TV_FASTSKIP : vrt::TV_FAST),
-                                        (int)tuning.trace_waves);
-  return vrt::launch_trace_rays(stream, s, rays, n, tmax, hits, (unsigned long long*)d_counters.ptr,
-                                trace_blocks, trace_variant(tuning.trace_kernel, tuning.empty_skip),
-                                (int)tuning.trace_waves);
+                                        (int)tuning.trace_waves, clear_queue);
+  return vrt::launch_trace_rays(stream, s, rays, n, tmax, hits, counters, trace_blocks,
+                                trace_variant(tuning.trace_kernel, tuning.empty_skip), (int)tuning.trace_waves);
 }
 
 int vrt_trace_rays_device(vrt_ctx* ctx, const vrt_scene* scene, const float* d_rays, uint64_t n,
@@ -635,6 +636,55 @@ int vrt_trace_rays_device(vrt_ctx* ctx, const vrt_scene* scene, const float* d_r
     HIP_TRY(ctx, hipEventSynchronize(ctx->ev[1]));
     HIP_TRY(ctx, hipEventElapsedTime(ms, ctx->ev[0], ctx->ev[1]));
   }
+  return VRT_OK;
+  GUARD_END(ctx)
+}
+
+int vrt_device_alloc(vrt_ctx* ctx, uint64_t bytes, void** dptr) {
+  if (!ctx || !dptr || bytes == 0) return ctx ? ctx->fail(VRT_ERR_INVALID, "bad argument") : VRT_ERR_INVALID;
+  GUARD_BEGIN
+  std::lock_guard<std::mutex> lock_(ctx->mu);
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  const int rc = ctx->reserve(bytes);
+  if (rc) return rc;
+  void* p = nullptr;
+  if (hipMalloc(&p, bytes) != hipSuccess || !p) {
+    ctx->unreserve(bytes);
+    return ctx->fail(VRT_ERR_NO_MEMORY, "hipMalloc of %llu bytes failed", (unsigned long long)bytes);
+  }
+  ctx->user_buffers[p] = bytes;
+  *dptr = p;
+  return VRT_OK;
+  GUARD_END(ctx)
+}
+
+int vrt_device_free(vrt_ctx* ctx, void* dptr) {
+  if (!ctx) return VRT_ERR_INVALID;
+  if (!dptr) return VRT_OK;
+  GUARD_BEGIN
+  std::lock_guard<std::mutex> lock_(ctx->mu);
+  auto it = ctx->user_buffers.find(dptr);
+  if (it == ctx->user_buffers.end()) return ctx->fail(VRT_ERR_INVALID, "not a vrt_device_alloc buffer");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  HIP_TRY(ctx, hipFree(dptr));
+  ctx->unreserve(it->second);
+  ctx->user_buffers.erase(it);
+  return VRT_OK;
+  GUARD_END(ctx)
+}
+
+int vrt_copy(vrt_ctx* ctx, void* dst, const void* src, uint64_t bytes, uint32_t kind) {
+  if (!ctx || (bytes && (!dst || !src)) || kind < VRT_COPY_TO_DEVICE || kind > VRT_COPY_DEVICE)
+    return ctx ? ctx->fail(VRT_ERR_INVALID, "bad argument") : VRT_ERR_INVALID;
+  GUARD_BEGIN
+  std::lock_guard<std::mutex> lock_(ctx->mu);
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  if (bytes == 0) return VRT_OK;
+  const hipMemcpyKind k = kind == VRT_COPY_TO_DEVICE ? hipMemcpyHostToDevice
+                          : kind == VRT_COPY_TO_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+  HIP_TRY(ctx, hipMemcpyAsync(dst, src, bytes, k, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   return VRT_OK;
   GUARD_END(ctx)
 }
@@ -1534,6 +1584,8 @@ vrt_ctx::~vrt_ctx() {
   for (vrt_scene* s : scenes) delete s;
   scenes.clear();
   views.clear();
+  for (auto& kv : user_buffers) (void)hipFree(kv.first);
+  user_buffers.clear();
   for (auto& e : ev)
     if (e) (void)hipEventDestroy(e);
   for (auto& slot : ring)

@@ -168,6 +168,9 @@ def _declare(L):
     L.vrt_scene_dims.argtypes = [P, P, C.POINTER(C.c_uint64)]
     L.vrt_trace_rays.argtypes = [P, P, P, C.c_uint64, C.c_float, P]
     L.vrt_trace_rays_device.argtypes = [P, P, P, C.c_uint64, C.c_float, P, C.POINTER(C.c_float)]
+    L.vrt_device_alloc.argtypes = [P, C.c_uint64, C.POINTER(P)]
+    L.vrt_device_free.argtypes = [P, P]
+    L.vrt_copy.argtypes = [P, P, P, C.c_uint64, C.c_uint32]
     L.vrt_render.argtypes = [P, C.POINTER(View), C.c_uint32, C.POINTER(Frame)]
     L.vrt_view_release.argtypes = [P, C.c_uint32]
     L.vrt_host_alloc.argtypes = [P, C.c_uint64, C.POINTER(P)]
@@ -201,6 +204,7 @@ def _declare(L):
         getattr(L, name).restype = I
     for name in ("vrt_create", "vrt_device_count", "vrt_scene_upload", "vrt_scene_release",
                  "vrt_scene_bricks", "vrt_scene_dims", "vrt_trace_rays", "vrt_trace_rays_device",
+                 "vrt_device_alloc", "vrt_device_free", "vrt_copy",
                  "vrt_render", "vrt_view_release", "vrt_host_alloc", "vrt_host_free",
                  "vrt_temporal_pass", "vrt_atrous_pass",
                  "vrt_stats_get", "vrt_stats_reset", "vrt_generate_scene", "vrt_scene_spawn",
@@ -370,6 +374,12 @@ class Context:
                                                  tmax, C.c_void_p(d_hits), C.byref(ms)))
         return ms.value
 
+    def device_array(self, nbytes: int) -> "DeviceArray":
+        """A device buffer owned by this context (vrt_device_alloc)."""
+        p = C.c_void_p()
+        self._check(self.L.vrt_device_alloc(self.h, max(int(nbytes), 1), C.byref(p)))
+        return DeviceArray(self, p.value, int(nbytes))
+
     def render(self, views, outs=None):
         """views: list of View; outs: list of dicts of numpy destinations."""
         nv = len(views)
@@ -494,3 +504,29 @@ class Context:
         s = ServeStats()
         self._check(self.L.vrt_serve_stats_get(self.h, C.byref(s)))
         return {k: getattr(s, k) for k, _ in ServeStats._fields_}
+
+
+class DeviceArray:
+    """Device memory from vrt_device_alloc: ``ptr`` for the device-resident
+    entry points, ``upload``/``download`` through vrt_copy (synchronous)."""
+
+    def __init__(self, ctx: Context, ptr: int, nbytes: int):
+        self.ctx, self.ptr, self.nbytes = ctx, ptr, nbytes
+
+    def upload(self, a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        if a.nbytes > self.nbytes:
+            raise ValueError("array larger than the device buffer")
+        self.ctx._check(self.ctx.L.vrt_copy(self.ctx.h, C.c_void_p(self.ptr), _ptr(a), a.nbytes, 1))
+        return self
+
+    def download(self, a: np.ndarray) -> np.ndarray:
+        if not a.flags["C_CONTIGUOUS"] or a.nbytes > self.nbytes:
+            raise ValueError("destination must be C-contiguous and fit the device buffer")
+        self.ctx._check(self.ctx.L.vrt_copy(self.ctx.h, _ptr(a), C.c_void_p(self.ptr), a.nbytes, 2))
+        return a
+
+    def free(self):
+        if self.ptr and self.ctx.h:
+            self.ctx._check(self.ctx.L.vrt_device_free(self.ctx.h, C.c_void_p(self.ptr)))
+        self.ptr = 0
