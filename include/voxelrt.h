@@ -207,6 +207,25 @@ int vrt_trace_rays_device(vrt_ctx* ctx, const vrt_scene* scene,
                           const float* d_rays, uint64_t n, float tmax,
                           vrt_hit* d_hits, float* ms);
 
+/* Allocation-free form of vrt_trace_rays_device.  No reference counterpart:
+ * for callers that cannot allocate device memory -- code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  The same
+ * kernels and the same hits as vrt_trace_rays_device with the context's
+ * tuning; their scratch (the ray / step / hit counters and the persistent
+ * kernels' ray-queue head) is a fixed 256-byte device array inside the
+ * library, one per GPU, held until the process ends and cleared by a kernel
+ * at the start of each call (no hipMalloc, no hipFree, no memset node).
+ * n is at most VRT_NOALLOC_MAX_RAYS (the queue head is 32-bit); larger n is
+ * rejected with VRT_ERR_INVALID.  Every call on one GPU shares that array:
+ * calls that overlap on the device (other contexts or streams) race and
+ * must be serialised by the caller.  The launch is enqueued on the
+ * context's stream and the call returns without waiting; its rays are not
+ * added to vrt_stats. */
+#define VRT_NOALLOC_MAX_RAYS 0x7fffffffull
+int vrt_trace_rays_device_noalloc(vrt_ctx* ctx, const vrt_scene* scene,
+                                  const float* d_rays, uint64_t n, float tmax,
+                                  vrt_hit* d_hits);
+
 /* Device buffers for the device-resident entry points (rays, hits), owned by
  * the context and counted in its HBM budget; freed by vrt_device_free or
  * vrt_destroy.  vrt_copy moves bytes between host and device (or device and

@@ -56,6 +56,7 @@ def test_serving_entry_points_reject_null_context():
 def test_device_entry_points_reject_null_context():
     L = v.load()
     p = C.c_void_p()
+    assert L.vrt_trace_rays_device_noalloc(None, None, None, 0, 1.0, None) == -1
     assert L.vrt_device_alloc(None, 16, C.byref(p)) == -1
     assert L.vrt_device_free(None, None) == -1
     assert L.vrt_copy(None, None, None, 0, 1) == -1

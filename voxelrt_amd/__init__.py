@@ -12,7 +12,7 @@ from __future__ import annotations
 
 from .lib import (  # noqa: F401
     ACCUM_RGBA16F, ACCUM_RGBA32F, FACE_MISS, FACE_NONE, FLAG_DENOISE, FLAG_TEMPORAL,
-    HIT_DTYPE, SCENE_RANDOM_BRICKS, SCENE_VOXELRT, SCENE_WORLD, VOXEL_MISS, VOXEL_SIZE,
+    HIT_DTYPE, NOALLOC_MAX_RAYS, SCENE_RANDOM_BRICKS, SCENE_VOXELRT, SCENE_WORLD, VOXEL_MISS, VOXEL_SIZE,
     Context, DeviceArray, Frame, Scene, SceneDesc, Tuning, Uniforms, VrtError, View, build, camera_matrices,
     device_count, generate_scene, lib_path, load, make_uniforms, scene_spawn,
 )

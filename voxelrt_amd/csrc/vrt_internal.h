@@ -170,6 +170,9 @@ struct vrt_ctx {
   hipError_t launch_trace(const vrt::SceneDev& s, const float* rays, uint64_t n, float tmax,
                           void* hits, unsigned long long* counters = nullptr, uint32_t* queue = nullptr,
                           bool clear_queue = true);
+  // device address of the allocation-free entry point's scratch on this
+  // context's device (set by vrt_create)
+  void* noalloc_scratch = nullptr;
   // device buffers handed out by vrt_device_alloc (bytes, counted in the budget)
   std::unordered_map<void*, uint64_t> user_buffers;
   hipError_t launch_pathtrace(const vrt::SceneDev& s, const vrt_uniforms& u, float fclamp,

@@ -203,6 +203,39 @@ int vrt_device_count(int32_t* count) {
   return e == hipSuccess ? VRT_OK : VRT_ERR_NO_DEVICE;
 }
 
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free traversal entry point
+// (vrt_trace_rays_device_noalloc): the four 64-bit counters its kernels add
+// rays, brick steps, voxel steps and hits into (bytes 0..31) and the
+// persistent kernels' 32-bit ray-queue head (bytes 32..35); 256 bytes, one
+// copy per device, held until the process ends.  Host code takes its device
+// address per device once (scratch_address, from vrt_create, with the device
+// current) and uses only that address afterwards.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned long long scratch[32];
+constexpr uint32_t NOALLOC_SCRATCH_WORDS = 9;  // u32 words the entry point clears per call
+
+static std::mutex g_scratch_mu;
+static std::unordered_map<int, void*> g_scratch_addr;  // device ordinal -> address of scratch
+
+static hipError_t scratch_address(int device, void** out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch_addr.find(device);
+  if (it == g_scratch_addr.end()) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (!p) return hipErrorInvalidSymbol;
+    it = g_scratch_addr.emplace(device, p).first;
+  }
+  *out = it->second;
+  return hipSuccess;
+}
+
+extern "C" {
+
 int vrt_create(const vrt_create_info* info, vrt_ctx** out) {
   if (!out) return VRT_ERR_INVALID;
   *out = nullptr;
@@ -228,6 +261,12 @@ int vrt_create(const vrt_create_info* info, vrt_ctx** out) {
     return VRT_ERR_DEVICE;
   }
   ctx->lane0.stream = ctx->stream;
+  // the allocation-free entry point's scratch on this device; without it
+  // that entry point reports an error and everything else works
+  if (scratch_address(ctx->device, &ctx->noalloc_scratch) != hipSuccess) {
+    ctx->noalloc_scratch = nullptr;
+    (void)hipGetLastError();
+  }
   {
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) == hipSuccess && cus > 0)
@@ -636,6 +675,31 @@ int vrt_trace_rays_device(vrt_ctx* ctx, const vrt_scene* scene, const float* d_r
     HIP_TRY(ctx, hipEventSynchronize(ctx->ev[1]));
     HIP_TRY(ctx, hipEventElapsedTime(ms, ctx->ev[0], ctx->ev[1]));
   }
+  return VRT_OK;
+  GUARD_END(ctx)
+}
+
+int vrt_trace_rays_device_noalloc(vrt_ctx* ctx, const vrt_scene* scene, const float* d_rays, uint64_t n,
+                                  float tmax, vrt_hit* d_hits) {
+  if (!ctx || !scene || (n && (!d_rays || !d_hits)))
+    return ctx ? ctx->fail(VRT_ERR_INVALID, "null argument") : VRT_ERR_INVALID;
+  if (n > VRT_NOALLOC_MAX_RAYS)
+    return ctx->fail(VRT_ERR_INVALID,
+                     "vrt_trace_rays_device_noalloc: %llu rays exceed its maximum of %llu (the ray-queue head "
+                     "in its fixed scratch is 32-bit)",
+                     (unsigned long long)n, (unsigned long long)VRT_NOALLOC_MAX_RAYS);
+  GUARD_BEGIN
+  std::lock_guard<std::mutex> lock_(ctx->mu);
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  if (n == 0) return VRT_OK;
+  if (!ctx->noalloc_scratch)
+    return ctx->fail(VRT_ERR_DEVICE, "hipGetSymbolAddress gave no scratch address on device %d", ctx->device);
+  // the counters and the queue head start at zero; the last call's values
+  // are still there (every call on this GPU shares the array)
+  HIP_TRY(ctx, vrt::launch_zero_words(ctx->stream, ctx->noalloc_scratch, NOALLOC_SCRATCH_WORDS));
+  const vrt::SceneDev s = scene->dev();
+  HIP_TRY(ctx, ctx->launch_trace(s, d_rays, n, tmax, d_hits, (unsigned long long*)ctx->noalloc_scratch,
+                                 (uint32_t*)ctx->noalloc_scratch + 8, false));
   return VRT_OK;
   GUARD_END(ctx)
 }

@@ -16,6 +16,7 @@ VOXEL_SIZE = 0.0625
 FACE_NONE = 6
 FACE_MISS = 255
 VOXEL_MISS = 0xFFFFFFFF
+NOALLOC_MAX_RAYS = 0x7FFFFFFF  # VRT_NOALLOC_MAX_RAYS
 FLAG_TEMPORAL = 1
 FLAG_DENOISE = 2
 ACCUM_RGBA16F = 0
@@ -168,6 +169,7 @@ def _declare(L):
     L.vrt_scene_dims.argtypes = [P, P, C.POINTER(C.c_uint64)]
     L.vrt_trace_rays.argtypes = [P, P, P, C.c_uint64, C.c_float, P]
     L.vrt_trace_rays_device.argtypes = [P, P, P, C.c_uint64, C.c_float, P, C.POINTER(C.c_float)]
+    L.vrt_trace_rays_device_noalloc.argtypes = [P, P, P, C.c_uint64, C.c_float, P]
     L.vrt_device_alloc.argtypes = [P, C.c_uint64, C.POINTER(P)]
     L.vrt_device_free.argtypes = [P, P]
     L.vrt_copy.argtypes = [P, P, P, C.c_uint64, C.c_uint32]
@@ -204,7 +206,7 @@ def _declare(L):
         getattr(L, name).restype = I
     for name in ("vrt_create", "vrt_device_count", "vrt_scene_upload", "vrt_scene_release",
                  "vrt_scene_bricks", "vrt_scene_dims", "vrt_trace_rays", "vrt_trace_rays_device",
-                 "vrt_device_alloc", "vrt_device_free", "vrt_copy",
+                 "vrt_trace_rays_device_noalloc", "vrt_device_alloc", "vrt_device_free", "vrt_copy",
                  "vrt_render", "vrt_view_release", "vrt_host_alloc", "vrt_host_free",
                  "vrt_temporal_pass", "vrt_atrous_pass",
                  "vrt_stats_get", "vrt_stats_reset", "vrt_generate_scene", "vrt_scene_spawn",
@@ -373,6 +375,13 @@ class Context:
         self._check(self.L.vrt_trace_rays_device(self.h, scene.handle, C.c_void_p(d_rays), n,
                                                  tmax, C.c_void_p(d_hits), C.byref(ms)))
         return ms.value
+
+    def trace_rays_device_noalloc(self, scene: Scene, d_rays: int, n: int, d_hits: int,
+                                  tmax: float = 1e4) -> None:
+        """vrt_trace_rays_device_noalloc: enqueued on the context's stream,
+        returns without waiting."""
+        self._check(self.L.vrt_trace_rays_device_noalloc(self.h, scene.handle, C.c_void_p(d_rays), n,
+                                                         tmax, C.c_void_p(d_hits)))
 
     def device_array(self, nbytes: int) -> "DeviceArray":
         """A device buffer owned by this context (vrt_device_alloc)."""
