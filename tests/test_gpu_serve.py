@@ -192,3 +192,36 @@ def test_admission_by_hbm_footprint(world):
             ctx.wait(t, None, timeout_ms=120000)
         assert e.value.code == -4  # VRT_ERR_BUDGET
         assert ctx.serve_stats()["failed"] == 1
+
+
+def test_tight_budget_keeps_the_pooled_result_buffer(world):
+    """Admission under a budget where freeing the whole result pool would
+    cover the new view's state but leave no room for a fresh result buffer:
+    the request takes the pooled buffer that fits (it is kept out of the
+    pool before room is made), evicts an idle view instead, and completes.
+
+    Sizes (fmt rgba16f, 76 B/px of view state, results 12 B/px rounded to
+    64 KiB): A and B 256x256 (state 4980736 B, result 786432 B), a big view
+    512x288 (state 11206656 B, result 1769472 B, too large to serve B).
+    Budget = scene + states of A, big and B + 393216 B of slack."""
+    descs, vox = world
+    with v.Context(0) as probe:
+        sc = probe.upload(vox[0])
+        scene_bytes = probe.stats()["hbm_resident"]
+        sc.release()
+    vs_small, vs_big, slack = 256 * 256 * 76, 512 * 288 * 76, 393216
+    budget = scene_bytes + 2 * vs_small + vs_big + slack
+    with v.Context(0, hbm_budget=budget) as ctx:
+        ctx.serve_configure(streams=1, max_batch_views=1)
+        scenes = [ctx.upload(vox[0])]
+        assert ctx.stats()["hbm_resident"] == scene_bytes
+        for k, (vid, w, h) in enumerate(((1, 256, 256), (2, 512, 288), (3, 256, 256))):
+            s = dict(si=0, w=w, h=h, nb=1, flags=0, fmt=0, spp=1, pose=k, frame=k)
+            view, _ = _view(scenes, descs, s, vid)
+            t = ctx.submit(view)
+            res = ctx.wait(t, np.zeros((h, w, 3), np.float32), timeout_ms=60000)
+            assert res is not None, f"request {k} (view {vid}) failed"
+        st = ctx.serve_stats()
+        assert st["completed"] == 3 and st["failed"] == 0, st
+        assert st["evictions"] == 1, st  # view 1's idle state made the room
+        assert ctx.stats()["hbm_resident"] <= budget

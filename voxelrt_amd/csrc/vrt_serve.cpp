@@ -222,10 +222,38 @@ bool Server::launch(uint32_t li) {
       }
     }
     const uint64_t rbytes = (px * 12 + RESULT_ALIGN - 1) / RESULT_ALIGN * RESULT_ALIGN;
-    auto pit = pool.lower_bound(rbytes);
-    const bool pooled = pit != pool.end() && pit->first <= 2 * rbytes;
-    if (!pooled) need += rbytes;
-    if (!make_room(need)) {
+    // a pooled result buffer that fits is taken out of the pool first, so
+    // that make_room (which frees pooled buffers) cannot free it
+    void* pooled = nullptr;
+    uint64_t pooled_bytes = 0;
+    {
+      auto pit = pool.lower_bound(rbytes);
+      if (pit != pool.end() && pit->first <= 2 * rbytes) {
+        pooled = pit->second;
+        pooled_bytes = pit->first;
+        pool.erase(pit);
+      } else {
+        need += rbytes;
+      }
+    }
+    bool room = make_room(need);
+    ViewState* state = nullptr;
+    int rc = VRT_OK;
+    if (room) {
+      rc = ctx->view_state(vid, w, h, v.accum_format, &state, L.lane.stream);
+      if (rc == VRT_OK && pooled) {
+        t.result = pooled;
+        t.result_bytes = pooled_bytes;
+        pooled = nullptr;
+      } else if (rc == VRT_OK) {
+        rc = acquire_result(rbytes, &t.result, &t.result_bytes);
+      }
+      // out of budget after all (e.g. a reserve elsewhere in between): the
+      // same as no room -- wait for running batches, or fail below
+      if (rc == VRT_ERR_BUDGET) room = false;
+    }
+    if (pooled) release_result(pooled, pooled_bytes);  // not used: back to the pool
+    if (!room) {
       // nothing left to evict: wait for running batches to finish, or fail
       // the request if nothing can ever free the space
       if (pick.empty() && running_lanes() == 0) {
@@ -244,9 +272,6 @@ bool Server::launch(uint32_t li) {
       ++it;
       continue;
     }
-    ViewState* state = nullptr;
-    int rc = ctx->view_state(vid, w, h, v.accum_format, &state, L.lane.stream);
-    if (rc == VRT_OK) rc = acquire_result(rbytes, &t.result, &t.result_bytes);
     if (rc != VRT_OK) {
       it = queue.erase(it);
       fail_ticket(id, rc, ctx->err);
