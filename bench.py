@@ -15,7 +15,7 @@ Prints one JSON line:
                (all host cores, OpenMP over rows)
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
-                [--workload frame_1080|traverse|batch|serve]
+                [--workload frame_1080|traverse|batch|serve|profile]
 """
 from __future__ import annotations
 
@@ -50,8 +50,8 @@ def _pt_kernel_name(t):
     """The path-tracer kernel a tuning launches at frame_1080 (vrt_tuning's
     pt_kernel / pt_tile; the single-loop DDA fits LDS at 256^3)."""
     k, f = t["pt_kernel"], t["pt_tile"]
-    if k in (1, 6):
-        return "k_pathtrace_persistent"
+    if k == 8:
+        return "k_pathtrace_wave"
     if (f in (2, 3) and k in (4, 7)) or (f in (4, 5) and k == 7):
         return "k_pathtrace_refill"
     if f == 1 and k == 7:
@@ -93,7 +93,6 @@ def traverse(args):
     occupancy p.  Algorithmic bytes per ray: 24 in + 16 out + 4 per brick step
     + 4 per voxel step + 4 per hit."""
     import numpy as np
-    import torch
 
     import voxelrt_amd as v
     from voxelrt_amd import configs as cf
@@ -101,16 +100,14 @@ def traverse(args):
     n_rays = 1 << 24
     rays = cf.traverse_rays(n_rays, seed=77)
     ctx = v.Context(0)
-    d_rays = torch.from_numpy(rays).to("cuda:0")
-    d_hits = torch.empty(n_rays * 4, dtype=torch.int32, device="cuda:0")
-    torch.cuda.synchronize()
-    variants = {"lane": dict(trace_kernel=0), "lane_flat": dict(trace_kernel=2), "lane_lds": dict(trace_kernel=3),
+    d_rays = ctx.device_array(rays.nbytes).upload(rays)
+    d_hits = ctx.device_array(n_rays * 16)
+    variants = {"lane": dict(trace_kernel=0),
                 "lane_fast": dict(trace_kernel=4, dda_batch=1),
                 "lane_fast_b16": dict(trace_kernel=4, dda_batch=16),
                 "lane_fast_b16_w6": dict(trace_kernel=4, dda_batch=16, trace_waves=6),
                 "persistent_fast_b16": dict(PROFILED_TRAVERSE),
-                "persistent_fast_b16_w6": dict(PROFILED_TRAVERSE, trace_waves=6),
-                "persistent": dict(trace_kernel=1, trace_blocks=2048, trace_steps=8, trace_min_refill=16)}
+                "persistent_fast_b16_w6": dict(PROFILED_TRAVERSE, trace_waves=6)}
     # every variant from the same base (an autotuned tuning of the previous
     # scene must not leak into the next scene's fixed variants)
     base = dict(trace_waves=0, empty_skip=0, dda_batch=1)
@@ -119,7 +116,7 @@ def traverse(args):
         sc = ctx.upload(vox)
         if not args.no_cpu:
             print(json.dumps(_traverse_cpu(vox, rays, p)), flush=True)
-        for name in ("lane", "lane_fast", "lane_fast_b16", "lane_fast_b16_w6", "persistent", "persistent_fast_b16",
+        for name in ("lane", "lane_fast", "lane_fast_b16", "lane_fast_b16_w6", "persistent_fast_b16",
                      "persistent_fast_b16_w6", "autotuned"):
             if name == "autotuned":
                 tuned = ctx.autotune(sc, 1 << 22).as_dict()
@@ -128,6 +125,56 @@ def traverse(args):
             _traverse_one(args, ctx, sc, p, name, d_rays, d_hits, n_rays,
                           tuned if name == "autotuned" else variants[name])
         sc.release()
+    ctx.close()
+
+
+def profile(args):
+    """Target of the committed rocprofv3 summaries (kernel traces and --pmc
+    passes): (1) --steps frames of frame_1080 with the path-tracer variant
+    given by --pt-kernel / --pt-waves / --pt-tile / --pt-order /
+    --pt-min-refill / --pt-blocks; (2) for each --traverse-p occupancy, two
+    launches over 2^24 rays of the traversal variant --trace-kernel
+    (--dda-batch, 1024 blocks / 8 steps / refill at 16 for the persistent
+    kernels), rays and hits resident in HBM; (3) two device-to-device copies
+    of 256 MiB (vrt_copy), the calibration for FETCH_SIZE / WRITE_SIZE."""
+    import numpy as np
+
+    import voxelrt_amd as v
+    from voxelrt_amd import configs as cf
+
+    print("profile target start", flush=True)
+    ctx = v.Context(0)
+    desc = dict(cf.FRAME_1080_SCENE)
+    sc = ctx.upload(cf.frame_1080_scene())
+    p = cf.FRAME_1080
+    r = v.Renderer(ctx, sc, p["width"], p["height"], bounces=p["bounces"], temporal=True, denoise=True,
+                   fclamp=p["fclamp"], max_history=p["max_history"], view_id=1)
+    kernel = 8 if args.pt_kernel == "auto" else int(args.pt_kernel)
+    ctx.set_tuning(pt_kernel=kernel, pt_waves=args.pt_waves, pt_order=args.pt_order, pt_tile=args.pt_tile,
+                   pt_min_refill=args.pt_min_refill, pt_blocks=args.pt_blocks, pt_shade_min=args.pt_shade_min)
+    for i in range(args.steps):
+        r.set_pose(*cf.world_orbit(desc, i))
+        r.frame()
+    ctx.stats()
+    print(f"{args.steps} frames done", flush=True)
+    if args.traverse_p:
+        n = 1 << 24
+        rays = cf.traverse_rays(n, seed=77)
+        d_rays = ctx.device_array(rays.nbytes).upload(rays)
+        d_hits = ctx.device_array(n * 16)
+        ctx.set_tuning(trace_kernel=args.trace_kernel, trace_blocks=1024, trace_steps=8, trace_min_refill=16,
+                       dda_batch=args.dda_batch)
+        for occ in args.traverse_p:
+            sc2 = ctx.upload(cf.traverse_scene(occ))
+            for _ in range(2):
+                ctx.trace_rays_device(sc2, d_rays.ptr, n, d_hits.ptr)
+            sc2.release()
+            print(f"traverse p={occ} done", flush=True)
+    a = ctx.device_array(256 << 20).upload(np.ones(256 << 20, np.uint8))
+    b = ctx.device_array(256 << 20)
+    for _ in range(2):
+        ctx.copy_device(b, a)
+    print("copy done (256 MiB device to device, x2)", flush=True)
     ctx.close()
 
 
@@ -172,12 +219,12 @@ PROFILED_TRAVERSE = dict(trace_kernel=5, trace_blocks=1024, trace_steps=8, trace
 def _traverse_one(args, ctx, sc, p, name, d_rays, d_hits, n_rays, tuning):
     import numpy as np
     for _ in range(args.warmup):
-        ctx.trace_rays_device(sc, d_rays.data_ptr(), n_rays, d_hits.data_ptr())
+        ctx.trace_rays_device(sc, d_rays.ptr, n_rays, d_hits.ptr)
     s0 = ctx.stats()
     ms = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        ms.append(ctx.trace_rays_device(sc, d_rays.data_ptr(), n_rays, d_hits.data_ptr()))
+        ms.append(ctx.trace_rays_device(sc, d_rays.ptr, n_rays, d_hits.ptr))
     s1 = ctx.stats()
     wall = time.perf_counter() - t0
     K = args.steps
@@ -445,7 +492,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-frames", type=int, default=8)
-    ap.add_argument("--workload", default="frame_1080", choices=["frame_1080", "traverse", "batch", "serve"])
+    ap.add_argument("--workload", default="frame_1080", choices=["frame_1080", "traverse", "batch", "serve", "profile"])
+    ap.add_argument("--pt-order", type=int, default=2, help="profile workload: path-tracer tile order")
+    ap.add_argument("--trace-kernel", type=int, default=4, help="profile workload: traversal variant")
+    ap.add_argument("--dda-batch", type=int, default=1, help="profile workload: brick-entry batch")
+    ap.add_argument("--traverse-p", type=float, nargs="*", default=[], help="profile workload: occupancies")
     ap.add_argument("--requests", type=int, default=256)
     ap.add_argument("--load", type=float, default=0.8, help="Poisson rate as a fraction of plain capacity")
     ap.add_argument("--serve-streams", type=int, default=2)
@@ -453,8 +504,11 @@ def main():
     ap.add_argument("--serve-pixels", type=int, default=16 << 20)
     ap.add_argument("--batch-mode", type=int, default=2, choices=[-1, 0, 1, 2],
                     help="served form's batch mode (-1: the batch autotuner's choice among all)")
-    ap.add_argument("--pt-kernel", default="auto", choices=["auto", "0", "1", "2", "3", "4", "5", "6", "7"])
-    ap.add_argument("--pt-waves", type=int, default=0, choices=[0, 4, 5])
+    ap.add_argument("--pt-kernel", default="auto", choices=["auto", "0", "4", "7", "8"])
+    ap.add_argument("--pt-waves", type=int, default=0, choices=[0, 4, 5, 6])
+    ap.add_argument("--pt-min-refill", type=int, default=16, help="pt-kernel 8: lanes waiting before a level-switch pass")
+    ap.add_argument("--pt-blocks", type=int, default=0, help="pt-kernel 8: persistent waves (0 = 16 per CU)")
+    ap.add_argument("--pt-shade-min", type=int, default=32, help="pt-kernel 8: finished rays shaded together")
     ap.add_argument("--pt-tile", type=int, default=0, choices=[0, 1, 2, 3, 4, 5],
                     help="path-tracer tile form with a fixed --pt-kernel")
     args = ap.parse_args()
@@ -469,6 +523,8 @@ def main():
         return batch(args)
     if args.workload == "serve":
         return serve(args)
+    if args.workload == "profile":
+        return profile(args)
 
     import voxelrt_amd as v
     from voxelrt_amd import configs as cf
@@ -493,7 +549,8 @@ def main():
         tuned = ctx.autotune(sc, 0, r.view().u).as_dict()
     else:
         tuned = ctx.set_tuning(pt_kernel=int(args.pt_kernel), pt_waves=args.pt_waves,
-                               pt_tile=args.pt_tile).as_dict()
+                               pt_tile=args.pt_tile, pt_min_refill=args.pt_min_refill,
+                               pt_blocks=args.pt_blocks, pt_shade_min=args.pt_shade_min).as_dict()
 
     for i in range(args.warmup):
         step(i)
@@ -534,8 +591,8 @@ def main():
                                              f"pt_kernel={tuned['pt_kernel']},pt_waves={tuned['pt_waves']},"
                                              f"pt_tile={tuned['pt_tile']}"),
                      "alg_bytes_per_launch": int(alg_bytes), "avg_launch_ms": round(pt_avg_ms, 4)},
-        "tuning": {k: tuned[k] for k in ("pt_kernel", "pt_waves", "pt_order", "pt_tile", "pt_blocks", "pt_steps",
-                                          "pt_min_refill", "pt_chunk", "dda_batch")},
+        "tuning": {k: tuned[k] for k in ("pt_kernel", "pt_waves", "pt_order", "pt_tile", "pt_blocks",
+                                          "pt_min_refill", "pt_shade_min")},
         "breakdown": {"gpu_ms_pathtrace": round(pt_avg_ms, 4), "gpu_ms_temporal": round(tp_ms / K, 4),
                       "gpu_ms_atrous3": round(at_ms / K, 4), "gpu_ms_frame": round(fr_ms / K, 4),
                       "rays_per_frame": rays // K, "brick_steps_per_ray": round(bsteps / max(rays, 1), 2),

@@ -272,33 +272,40 @@ int vrt_atrous_pass(vrt_ctx* ctx, const vrt_uniforms* u, const void* src,
  * given scene and keeps the fastest in the context.  No reference
  * counterpart: WebGPU dispatches fixed 8x8 workgroups (main.js:728-734). */
 typedef struct vrt_tuning {
-  uint32_t trace_kernel;      /* 0 = one ray per lane, 1 = persistent ray queue,
-                                 2 = one ray per lane, single-stream DDA,
-                                 3 = one ray per lane, occupancy staged in LDS,
+  uint32_t trace_kernel;      /* traversal: 0 = one ray per lane, nested brick / voxel
+                                 loops on global memory,
                                  4 = one ray per lane, single loop, both levels in LDS,
-                                 5 = persistent ray queue, single loop in LDS,
+                                 5 = persistent ray queue, single loop in LDS (default),
                                  6 = one ray per lane, single loop on global memory
-                                     (the form 4 falls back to when its LDS does not
-                                     fit, 512^3 and up) */
-  uint32_t trace_blocks;      /* persistent grid (256-thread blocks), 0 = default */
-  uint32_t trace_steps;       /* DDA iterations between queue refill checks */
-  uint32_t trace_min_refill;  /* idle lanes (of 64) that trigger a refill */
-  uint32_t pt_kernel;         /* path tracer: 0 = one pixel per lane, 1 = persistent,
-                                 2 = one pixel per lane, single-stream DDA,
-                                 3 = one pixel per lane, occupancy staged in LDS,
+                                     (the form 4 and 5 fall back to when their LDS
+                                     does not fit, 512^3 and up).
+                                 1, 2 and 3 are rejected. */
+  uint32_t trace_blocks;      /* trace 5: persistent grid (256-thread blocks), 0 = 1 */
+  uint32_t trace_steps;       /* trace 5: DDA iterations between queue refill checks */
+  uint32_t trace_min_refill;  /* trace 5: idle lanes (of 64) that trigger a refill */
+  uint32_t pt_kernel;         /* path tracer:
+                                 0 = one pixel per lane, nested loops on global memory,
                                  4 = one pixel per lane, single loop, both levels in LDS,
-                                 5 = as 4 with one trace call site per lane (shadow and
-                                     bounce rays of different lanes share the DDA loop),
-                                 6 = as 1 with the single-loop DDA in LDS (brick
-                                     entries batched per dda_batch),
-                                 7 = as 4 with the single loop on global memory */
-  uint32_t pt_blocks;
-  uint32_t pt_steps;
-  uint32_t pt_min_refill;
-  uint32_t pt_chunk;          /* pixels a wave takes per queue atomic */
-  uint32_t pt_waves;          /* one-pixel-per-lane kernels: minimum waves per
-                                 SIMD the register allocation must allow:
-                                 0 = compiler's choice, 4 or 5 */
+                                 7 = as 4 with the single loop on global memory,
+                                 8 = step-level refill on persistent waves (default):
+                                     pt_blocks waves (0 = 16 per CU) take 8x8 pixel
+                                     patches from a queue; lanes wait at brick
+                                     boundaries and at finished rays, and the wave
+                                     switches levels for pt_min_refill waiting lanes
+                                     and shades pt_shade_min finished rays at once;
+                                     the single-loop DDA in LDS where it fits (four
+                                     waves per block share it), pt_tile 1 = on global
+                                     memory always.
+                                 1, 2, 3, 5 and 6 are rejected.  The batched forms
+                                 (vrt_render_batch) run pt 0, 4 or 7; with pt 8 they
+                                 run pt 7. */
+  uint32_t pt_blocks;         /* pt 8: persistent waves, 0 = 16 per CU */
+  uint32_t unused0;           /* not read; keeps the layout */
+  uint32_t pt_min_refill;     /* pt 8: lanes (of 64) waiting at a brick boundary
+                                 before the wave switches their levels (0 = 1) */
+  uint32_t unused1;           /* not read; keeps the layout */
+  uint32_t pt_waves;          /* minimum waves per SIMD the register allocation must
+                                 allow: 0 = compiler's choice, 4 or 5; 6 for pt 8 */
   uint32_t batch_mode;        /* batched form (nviews > 1, serving):
                                  0 = one grid launch per pass over all tiles,
                                  1 = persistent path tracer on a shared tile
@@ -307,27 +314,33 @@ typedef struct vrt_tuning {
   uint32_t batch_blocks;      /* persistent path-tracer grid, 0 = 8 per CU */
   uint32_t dda_batch;         /* traversal kernels 4, 5 and 6: lanes of a wave that wait
                                  at occupied bricks before they enter them together
-                                 (1..64; 0 = 1, enter at once).  The path tracer
-                                 always enters at once. */
-  uint32_t empty_skip;        /* single-loop DDA (trace 4/5, pt 4): 1 = take the
-                                 coarse steps through runs of empty brick rows in
-                                 bulk (same results, fewer instructions on sparse
-                                 scenes), 0 = step by step */
-  uint32_t pt_order;          /* one-pixel-per-lane path tracer: order in which blocks
-                                 take tiles: 0 row-major, 1 one band of tiles per
-                                 XCD, 2 row-major from the last tile (default: the
-                                 bottom rows, usually the costliest, start first),
-                                 3 bands from their last tile */
-  uint32_t pt_tile;           /* one-pixel-per-lane path tracer: 0 = a 16x16 tile per
-                                 256-thread block, 1 = 8x8 tiles of one wave (pt 7),
-                                 2 / 3 = 32x32 / 32x16 tiles whose lanes take the next
-                                 pixel when theirs is done, one ray per lane per
-                                 round (pt 4 and 7), 4 / 5 = the same on one-wave
-                                 blocks of 16x16 / 16x8 tiles (pt 7) */
+                                 (1..64; 0 = 1, enter at once).  The path tracers
+                                 other than pt 8 enter at once. */
+  uint32_t empty_skip;        /* traversal kernels 4 and 5: 1 = take the coarse steps
+                                 through runs of empty brick rows in bulk (same
+                                 results, fewer instructions on sparse scenes),
+                                 0 = step by step.  The path tracers do not read it. */
+  uint32_t pt_order;          /* order in which blocks take tiles (pt 8: patches):
+                                 0 row-major, 1 one band of tiles per XCD (not pt 8),
+                                 2 row-major from the last tile (default: the bottom
+                                 rows, usually the costliest, start first), 3 bands
+                                 from their last tile (pt 8: as 2) */
+  uint32_t pt_tile;           /* pt 0, 4, 7: 0 = a 16x16 tile per 256-thread block,
+                                 1 = 8x8 tiles of one wave (pt 7), 2 / 3 = 32x32 /
+                                 32x16 tiles whose lanes take the next pixel when
+                                 theirs is done, one ray per lane per round (pt 4
+                                 and 7), 4 / 5 = the same on one-wave blocks of
+                                 16x16 / 16x8 tiles (pt 7).
+                                 pt 8: 0 = DDA in LDS where it fits, 1 = DDA on
+                                 global memory */
   uint32_t trace_waves;       /* traversal kernels 4, 5 and 6: minimum waves per SIMD
                                  the register allocation must allow: 0 = compiler's
                                  choice, 6 (pays on sparse scenes) */
-  uint32_t reserved[3];
+  uint32_t pt_shade_min;      /* pt 8: finished rays (of 64) a wave collects before it
+                                 shades them together (0 = 1: at once) */
+  uint32_t batch_tile;        /* pt 8 in the batched forms, which run pt 7 with 4
+                                 waves per SIMD: its pt_tile (0 = 5) */
+  uint32_t reserved;
 } vrt_tuning;
 int vrt_tuning_get(vrt_ctx* ctx, vrt_tuning* out);
 int vrt_tuning_set(vrt_ctx* ctx, const vrt_tuning* in);

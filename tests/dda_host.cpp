@@ -12,10 +12,32 @@
 #include <vector>
 
 #include "../voxelrt_amd/csrc/vrt_dda_fast.h"
-#include "../voxelrt_amd/csrc/vrt_dda_flat.h"
 #include "../voxelrt_amd/csrc/vrt_trace.h"
 
 namespace {
+
+// One lane of the step-level refill path tracer: step_ev() until the ray
+// ends, with the level switches it reports taken one at a time.
+template <class St>
+void step_ev_walk(const vrt::SceneDev& s, const vrt::LdsScene& ls, St& st, vrt::f3 ro, vrt::f3 rd, float tmax,
+                  vrt::TraceResult& o, uint64_t& nit) {
+  if (st.begin(s, ls, ro, rd, tmax, o)) return;
+  uint32_t r = vrt::DDA_RUN;
+  for (;;) {
+    if (r == vrt::DDA_ENTER) {
+      st.enter(s, ls);
+      r = vrt::DDA_RUN;
+    } else if (r == vrt::DDA_LEAVE) {
+      r = st.resume(s, ls);
+    }
+    if (r == vrt::DDA_RUN) {
+      r = st.step_ev(s, ls);
+      ++nit;
+    }
+    if (r == vrt::DDA_HIT || r == vrt::DDA_MISS) break;
+  }
+  st.finish(s, r, o);
+}
 
 struct HostScene {
   std::vector<uint32_t> bricks, pool, bits, prefix, pbits, pprefix;
@@ -149,9 +171,6 @@ extern "C" int dda_host_trace(const uint32_t* vox, int nx, int ny, int nz, const
   std::memcpy(lds.data() + 16 * 256 + 2 * pw, h.pdist.data(), h.pdist.size());
   uint64_t nit = 0;
   const uint32_t lane = 37;  // any slot
-  std::vector<uint32_t> lds2(16 * 256 + (size_t)h.dev.bit_words, 0u);
-  std::memcpy(lds2.data() + 16 * 256, h.bits.data(), h.dev.bit_words * 4);
-  vrt::LdsMem lm{lds2.data() + 16 * 256, lds2.data() + lane, 256u};
   vrt::LdsScene ls{lds.data(), lane, 16u * 256u, 16u * 256u + pw, 16u * 256u + 2u * pw};
   for (uint64_t i = 0; i < n; ++i) {
     const float* r = rays + 6 * i;
@@ -159,8 +178,6 @@ extern "C" int dda_host_trace(const uint32_t* vox, int nx, int ny, int nz, const
     vrt::TraceResult o;
     switch (variant) {
       case 0: vrt::trace(h.dev, ro, rd, tmax, o); break;
-      case 1: vrt::trace_flat(h.dev, ro, rd, tmax, o); break;
-      case 2: vrt::trace_m(h.dev, lm, ro, rd, tmax, o); break;
       case 3:
       case 4: {
         vrt::DdaFast st;
@@ -183,6 +200,17 @@ extern "C" int dda_host_trace(const uint32_t* vox, int nx, int ny, int nz, const
       case 7: vrt::trace_fast<true, false, true>(h.dev, ls, ro, rd, tmax, o); break;
       case 8: vrt::trace_fast<false, false, true, true>(h.dev, ls, ro, rd, tmax, o); break;
       case 9: vrt::trace_fast<false, false, false, true>(h.dev, ls, ro, rd, tmax, o); break;
+      case 10:  // the step-level refill path tracer's stepping: step_ev + enter / resume
+      case 11: {
+        if (variant == 10) {
+          vrt::DdaFastT<false, true> st;
+          step_ev_walk(h.dev, ls, st, ro, rd, tmax, o, nit);
+        } else {
+          vrt::DdaFastT<true, true> st;
+          step_ev_walk(h.dev, ls, st, ro, rd, tmax, o, nit);
+        }
+        break;
+      }
       default: return -1;
     }
     uint32_t tb;

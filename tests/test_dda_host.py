@@ -1,7 +1,9 @@
 """The library's DDA variants run on the CPU (a host build of the device
-headers, tests/dda_host.cpp) against the oracle: nested loops, single-stream,
-LDS-staged occupancy and the single-loop LDS form (step by step, with the
-bulk skip of empty brick rows, and its wave-batched form) must return the
+headers, tests/dda_host.cpp) against the oracle: nested loops and the
+single-loop form on LDS and on global memory (step by step, with the bulk
+skip of empty brick rows, its wave-batched form, and the step_ev stepping
+with caller-side level switches of the step-level refill path tracer) must
+return the
 same hit voxel, face, both step counts and the bits of t for every ray.  This
 checks the variants' logic without a GPU (the GPU tests check the same on
 the device).  Sparse scenes (isolated blocks, a floor, an empty grid, grids
@@ -21,8 +23,8 @@ from voxelrt_amd import configs as cf
 from tests.sparse import sparse_rays as _sparse_rays, sparse_scenes as _sparse_scenes
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-VARIANTS = {"nested": 0, "flat": 1, "lds": 2, "fast": 3, "fast_skip": 4, "fast_batched_skip": 5, "fast_global": 6,
-            "fast_global_batched": 7, "fast_global_km": 8, "fast_km": 9}
+VARIANTS = {"nested": 0, "fast": 3, "fast_skip": 4, "fast_batched_skip": 5, "fast_global": 6,
+            "fast_global_batched": 7, "fast_global_km": 8, "fast_km": 9, "step_ev_lds": 10, "step_ev_global": 11}
 
 
 @pytest.fixture(scope="module")

@@ -75,28 +75,28 @@ def traverse(request, ctx):
     sc.release()
 
 
-TRACE_VARIANTS = [dict(trace_kernel=0), dict(trace_kernel=2), dict(trace_kernel=3), dict(trace_kernel=4),
-                  dict(trace_kernel=4, dda_batch=8), dict(trace_kernel=4, dda_batch=64),
-                  dict(trace_kernel=4, empty_skip=1), dict(trace_kernel=4, dda_batch=16, empty_skip=1),
-                  dict(trace_kernel=6), dict(trace_kernel=6, dda_batch=16),
-                  dict(trace_kernel=1, trace_blocks=2048, trace_steps=8, trace_min_refill=16),
-                  dict(trace_kernel=1, trace_blocks=512, trace_steps=1, trace_min_refill=1),
-                  dict(trace_kernel=1, trace_blocks=4096, trace_steps=16, trace_min_refill=64),
-                  dict(trace_kernel=5, trace_blocks=2048, trace_steps=4, trace_min_refill=1),
-                  dict(trace_kernel=5, trace_blocks=333, trace_steps=1, trace_min_refill=64),
-                  dict(trace_kernel=5, trace_blocks=1024, trace_steps=8, trace_min_refill=16, dda_batch=16),
-                  dict(trace_kernel=5, trace_blocks=700, trace_steps=3, trace_min_refill=8, dda_batch=16,
-                       empty_skip=1),
-                  dict(trace_kernel=4, dda_batch=16, trace_waves=6), dict(trace_kernel=6, trace_waves=6),
-                  dict(trace_kernel=5, trace_blocks=1024, trace_steps=8, trace_min_refill=16, dda_batch=16,
-                       trace_waves=6)]
+TRACE_VARIANTS = {
+    "lane": dict(trace_kernel=0),
+    "lane-fast": dict(trace_kernel=4),
+    "lane-fast-b8": dict(trace_kernel=4, dda_batch=8),
+    "lane-fast-b64": dict(trace_kernel=4, dda_batch=64),
+    "lane-fast-skip": dict(trace_kernel=4, empty_skip=1),
+    "lane-fast-b16-skip": dict(trace_kernel=4, dda_batch=16, empty_skip=1),
+    "lane-fast-global": dict(trace_kernel=6),
+    "lane-fast-global-b16": dict(trace_kernel=6, dda_batch=16),
+    "persist-fast": dict(trace_kernel=5, trace_blocks=2048, trace_steps=4, trace_min_refill=1),
+    "persist-fast-333": dict(trace_kernel=5, trace_blocks=333, trace_steps=1, trace_min_refill=64),
+    "persist-fast-b16": dict(trace_kernel=5, trace_blocks=1024, trace_steps=8, trace_min_refill=16, dda_batch=16),
+    "persist-fast-skip": dict(trace_kernel=5, trace_blocks=700, trace_steps=3, trace_min_refill=8, dda_batch=16,
+                              empty_skip=1),
+    "lane-fast-b16-w6": dict(trace_kernel=4, dda_batch=16, trace_waves=6),
+    "lane-fast-global-w6": dict(trace_kernel=6, trace_waves=6),
+    "persist-fast-b16-w6": dict(trace_kernel=5, trace_blocks=1024, trace_steps=8, trace_min_refill=16, dda_batch=16,
+                                trace_waves=6),
+}
 
 
-@pytest.fixture(params=range(len(TRACE_VARIANTS)),
-                ids=["lane", "lane-flat", "lane-lds", "lane-fast", "lane-fast-b8", "lane-fast-b64", "lane-fast-skip",
-                     "lane-fast-b16-skip", "lane-fast-global", "lane-fast-global-b16", "persist", "persist-s1", "persist-s16", "persist-fast", "persist-fast-333",
-                     "persist-fast-b16", "persist-fast-skip", "lane-fast-b16-w6", "lane-fast-global-w6",
-                     "persist-fast-b16-w6"])
+@pytest.fixture(params=list(TRACE_VARIANTS))
 def trace_variant(request, ctx):
     old = ctx.tuning().as_dict()
     ctx.set_tuning(**TRACE_VARIANTS[request.param])
@@ -197,7 +197,7 @@ def test_autotune_keeps_results(ctx, traverse):
         pytest.skip("one occupancy is enough")
     old = ctx.tuning().as_dict()
     t = ctx.autotune(sc, 1 << 20)
-    assert t.trace_kernel in (0, 1, 2, 3, 4, 5, 6) and t.trace_waves in (0, 6)
+    assert t.trace_kernel in (0, 4, 5, 6) and t.trace_waves in (0, 6)
     rays = cf.traverse_rays(1 << 18, seed=5)
     _hits_equal(ctx.trace_rays(sc, rays), oracle.trace_rays(og, rays), f"autotuned {t.as_dict()}")
     ctx.set_tuning(**old)
@@ -214,36 +214,39 @@ def frame128(ctx):
     sc.release()
 
 
-PT_VARIANTS = [dict(pt_kernel=0), dict(pt_kernel=0, pt_waves=5), dict(pt_kernel=2, pt_waves=4),
-               dict(pt_kernel=3), dict(pt_kernel=3, pt_waves=4), dict(pt_kernel=3, pt_waves=5),
-               dict(pt_kernel=4), dict(pt_kernel=4, pt_waves=5), dict(pt_kernel=5), dict(pt_kernel=5, pt_waves=5),
-               dict(pt_kernel=4, pt_waves=5, empty_skip=1), dict(pt_kernel=4, pt_waves=0, empty_skip=1),
-               dict(pt_kernel=4, pt_waves=5, pt_order=1), dict(pt_kernel=5, pt_waves=5, pt_order=2),
-               dict(pt_kernel=3, pt_order=3), dict(pt_kernel=7), dict(pt_kernel=7, pt_waves=5),
-               dict(pt_kernel=7, pt_tile=1), dict(pt_kernel=7, pt_waves=4, pt_tile=1, pt_order=0),
-               dict(pt_kernel=4, pt_waves=4, pt_tile=2), dict(pt_kernel=7, pt_waves=5, pt_tile=3),
-               dict(pt_kernel=7, pt_tile=2, pt_order=0), dict(pt_kernel=7, pt_waves=4, pt_tile=4),
-               dict(pt_kernel=7, pt_tile=5, pt_order=0),
-               dict(pt_kernel=7, pt_waves=4, pt_tile=3), dict(pt_kernel=7, pt_waves=4, pt_tile=5),
-               dict(pt_kernel=1, pt_blocks=1024, pt_steps=8, pt_min_refill=32, pt_chunk=64),
-               dict(pt_kernel=1, pt_blocks=97, pt_steps=1, pt_min_refill=1, pt_chunk=64),
-               dict(pt_kernel=1, pt_blocks=2048, pt_steps=16, pt_min_refill=64, pt_chunk=256),
-               dict(pt_kernel=6, pt_blocks=1024, pt_steps=8, pt_min_refill=32, pt_chunk=64),
-               dict(pt_kernel=6, pt_waves=4, pt_blocks=97, pt_steps=1, pt_min_refill=1, pt_chunk=64, dda_batch=8),
-               dict(pt_kernel=6, pt_waves=5, pt_blocks=2048, pt_steps=16, pt_min_refill=64, pt_chunk=256,
-                    dda_batch=64)]
+PT_VARIANTS = {
+    "lane": dict(pt_kernel=0),
+    "lane-w5": dict(pt_kernel=0, pt_waves=5),
+    "lane-fast": dict(pt_kernel=4),
+    "lane-fast-w5": dict(pt_kernel=4, pt_waves=5),
+    "lane-fast-w5-xcd": dict(pt_kernel=4, pt_waves=5, pt_order=1),
+    "lane-xcdrev": dict(pt_kernel=0, pt_order=3),
+    "lane-fast-global": dict(pt_kernel=7),
+    "lane-fast-global-w5": dict(pt_kernel=7, pt_waves=5),
+    "lane-fast-global-8x8": dict(pt_kernel=7, pt_tile=1),
+    "lane-fast-global-8x8-w4-rowmajor": dict(pt_kernel=7, pt_waves=4, pt_tile=1, pt_order=0),
+    "refill-fast-32x32-w4": dict(pt_kernel=4, pt_waves=4, pt_tile=2),
+    "refill-global-32x16-w5": dict(pt_kernel=7, pt_waves=5, pt_tile=3),
+    "refill-global-32x32-rowmajor": dict(pt_kernel=7, pt_tile=2, pt_order=0),
+    "refill-wave-16x16-w4": dict(pt_kernel=7, pt_waves=4, pt_tile=4),
+    "refill-wave-16x8-rowmajor": dict(pt_kernel=7, pt_tile=5, pt_order=0),
+    "refill-global-32x16-w4": dict(pt_kernel=7, pt_waves=4, pt_tile=3),
+    "refill-wave-16x8-w4": dict(pt_kernel=7, pt_waves=4, pt_tile=5),
+    "wave-w4-m16-s32": dict(pt_kernel=8, pt_waves=4, pt_min_refill=16, pt_shade_min=32, pt_blocks=0, pt_tile=0),
+    "wave-w4-m32-s32": dict(pt_kernel=8, pt_waves=4, pt_min_refill=32, pt_shade_min=32, pt_blocks=0),
+    "wave-w4-m1-s1-97blocks-rowmajor": dict(pt_kernel=8, pt_waves=4, pt_min_refill=1, pt_shade_min=1, pt_blocks=97,
+                                            pt_order=0),
+    "wave-m64-s64-300blocks": dict(pt_kernel=8, pt_waves=0, pt_min_refill=64, pt_shade_min=64, pt_blocks=300),
+    "wave-w5-m16-s48-rowmajor": dict(pt_kernel=8, pt_waves=5, pt_min_refill=16, pt_shade_min=48, pt_blocks=0,
+                                     pt_order=0),
+    "wave-global-w4-m12-s24": dict(pt_kernel=8, pt_waves=4, pt_min_refill=12, pt_shade_min=24, pt_blocks=0,
+                                   pt_tile=1),
+    "wave-global-w6-m0-s0-rev": dict(pt_kernel=8, pt_waves=6, pt_min_refill=0, pt_shade_min=0, pt_blocks=1024,
+                                     pt_tile=1, pt_order=3),
+}
 
 
-@pytest.fixture(params=range(len(PT_VARIANTS)),
-                ids=["lane", "lane-w5", "lane-flat-w4", "lane-lds", "lane-lds-w4", "lane-lds-w5", "lane-fast",
-                     "lane-fast-w5", "lane-fast1", "lane-fast1-w5", "lane-fast-w5-skip", "lane-fast-skip",
-                     "lane-fast-w5-xcd", "lane-fast1-w5-rev", "lane-lds-xcdrev", "lane-fast-global",
-                     "lane-fast-global-w5", "lane-fast-global-8x8", "lane-fast-global-8x8-w4-rowmajor",
-                     "refill-fast-32x32-w4", "refill-global-32x16-w5", "refill-global-32x32-rowmajor",
-                     "refill-wave-16x16-w4", "refill-wave-16x8-rowmajor",
-                     "refill-global-32x16-w4", "refill-wave-16x8-w4",
-                     "persist",
-                     "persist-s1", "persist-s16", "persist-fast", "persist-fast-w4-b8", "persist-fast-w5-b64"])
+@pytest.fixture(params=list(PT_VARIANTS))
 def pt_variant(request, ctx):
     old = ctx.tuning().as_dict()
     ctx.set_tuning(**PT_VARIANTS[request.param])
@@ -418,32 +421,32 @@ def test_frame_sequence_bitexact(ctx, pt_variant):
 # ---------------------------------------------------------------------------
 # batched form: every view of a mixed batch equals the same view alone
 # ---------------------------------------------------------------------------
-@pytest.mark.parametrize("tuning", [dict(pt_kernel=0, pt_waves=0), dict(pt_kernel=3, pt_waves=5),
-                                    dict(pt_kernel=2, pt_waves=4), dict(pt_kernel=4, pt_waves=5),
-                                    dict(pt_kernel=4, pt_waves=5, batch_mode=1),
-                                    dict(pt_kernel=0, pt_waves=0, batch_mode=1, batch_blocks=7),
-                                    dict(pt_kernel=4, pt_waves=5, batch_mode=2, empty_skip=1),
-                                    dict(pt_kernel=5, pt_waves=4, batch_mode=1),
-                                    dict(pt_kernel=4, pt_waves=5, batch_mode=0, pt_order=0),
-                                    dict(pt_kernel=4, pt_waves=5, batch_mode=1, pt_order=0),
-                                    dict(pt_kernel=7, pt_waves=4, batch_mode=1),
-                                    dict(pt_kernel=3, pt_waves=4, batch_mode=2),
-                                    dict(pt_kernel=7, pt_waves=4, pt_tile=5, batch_mode=0),
-                                    dict(pt_kernel=7, pt_waves=5, pt_tile=4, batch_mode=0, pt_order=0),
-                                    dict(pt_kernel=7, pt_waves=4, pt_tile=5, batch_mode=1),
-                                    dict(pt_kernel=7, pt_waves=0, pt_tile=4, batch_mode=1, batch_blocks=5),
-                                    dict(pt_kernel=7, pt_waves=4, pt_tile=5, batch_mode=2, pt_order=1),
-                                    dict(pt_kernel=7, pt_waves=4, pt_tile=3, batch_mode=0),
-                                    dict(pt_kernel=4, pt_waves=5, pt_tile=2, batch_mode=1),
-                                    dict(pt_kernel=4, pt_waves=4, pt_tile=3, batch_mode=2, pt_order=0),
-                                    dict(pt_kernel=7, pt_waves=5, pt_tile=2, batch_mode=1, batch_blocks=3)],
-                         ids=["lane", "lds-w5", "flat-w4", "fast-w5", "queue-fast", "queue-7blocks",
-                              "graph-fast", "queue-fast1-w4", "grid-fast-rowmajor", "queue-fast-rowmajor",
-                              "queue-fast-global-w4",
-                              "graph-lds", "grid-refill16x8", "grid-refill16x16-rowmajor", "queue-refill16x8",
-                              "queue-refill16x16-5blocks", "graph-refill16x8", "grid-refill32x16-global",
-                              "queue-refill32x32-lds", "graph-refill32x16-lds-rowmajor",
-                              "queue-refill32x32-global-3blocks"])
+BATCH_VARIANTS = {
+    "lane": dict(pt_kernel=0, pt_waves=0),
+    "fast-w5": dict(pt_kernel=4, pt_waves=5),
+    "queue-fast": dict(pt_kernel=4, pt_waves=5, batch_mode=1),
+    "queue-7blocks": dict(pt_kernel=0, pt_waves=0, batch_mode=1, batch_blocks=7),
+    "graph-fast": dict(pt_kernel=4, pt_waves=5, batch_mode=2),
+    "grid-fast-rowmajor": dict(pt_kernel=4, pt_waves=5, batch_mode=0, pt_order=0),
+    "queue-fast-rowmajor": dict(pt_kernel=4, pt_waves=5, batch_mode=1, pt_order=0),
+    "queue-fast-global-w4": dict(pt_kernel=7, pt_waves=4, batch_mode=1),
+    "graph-lane-w4": dict(pt_kernel=0, pt_waves=4, batch_mode=2),
+    "grid-refill16x8": dict(pt_kernel=7, pt_waves=4, pt_tile=5, batch_mode=0),
+    "grid-refill16x16-rowmajor": dict(pt_kernel=7, pt_waves=5, pt_tile=4, batch_mode=0, pt_order=0),
+    "queue-refill16x8": dict(pt_kernel=7, pt_waves=4, pt_tile=5, batch_mode=1),
+    "queue-refill16x16-5blocks": dict(pt_kernel=7, pt_waves=0, pt_tile=4, batch_mode=1, batch_blocks=5),
+    "graph-refill16x8": dict(pt_kernel=7, pt_waves=4, pt_tile=5, batch_mode=2, pt_order=1),
+    "grid-refill32x16-global": dict(pt_kernel=7, pt_waves=4, pt_tile=3, batch_mode=0),
+    "queue-refill32x32-lds": dict(pt_kernel=4, pt_waves=5, pt_tile=2, batch_mode=1),
+    "graph-refill32x16-lds-rowmajor": dict(pt_kernel=4, pt_waves=4, pt_tile=3, batch_mode=2, pt_order=0),
+    "queue-refill32x32-global-3blocks": dict(pt_kernel=7, pt_waves=5, pt_tile=2, batch_mode=1, batch_blocks=3),
+    # the step-level refill kernel's batches run pt 7 on batch_tile
+    "wave-grid": dict(pt_kernel=8, pt_waves=4, batch_mode=0),
+    "wave-graph-refill32x16": dict(pt_kernel=8, pt_waves=6, pt_tile=1, batch_mode=2, batch_tile=3),
+}
+
+
+@pytest.mark.parametrize("tuning", list(BATCH_VARIANTS.values()), ids=list(BATCH_VARIANTS))
 def test_batched_views_match_single_views(ctx, tuning):
     descs = [cf.world_desc(64, 1), cf.world_desc(128, 2), dict(cf.world_desc(64, 3), ny=32)]
     scenes = [ctx.upload(v.generate_scene(**d)) for d in descs]

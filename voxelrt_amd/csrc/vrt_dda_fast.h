@@ -84,7 +84,7 @@ VRT_D uint32_t pack_left(const int p[3], const int n[3], const int sd[3]) {
 // the ray up; step() runs one iteration and returns DDA_RUN, DDA_MISS or
 // DDA_HIT; finish() writes the result record.  trace_fast() runs them in a
 // loop; the persistent traversal kernel interleaves rays per lane.
-enum : uint32_t { DDA_RUN = 0u, DDA_MISS = 1u, DDA_HIT = 2u, DDA_ENTER = 3u };
+enum : uint32_t { DDA_RUN = 0u, DDA_MISS = 1u, DDA_HIT = 2u, DDA_ENTER = 3u, DDA_LEAVE = 4u };
 
 // Bitwise selects (c ? a : b, bit for bit).  Written as masks so that the
 // compiler cannot turn a choice between two state-array elements into a load
@@ -403,6 +403,30 @@ struct DdaFastT {
     if (inc == 1u) return DDA_MISS;  // coarse: left the grid / out of brick iterations
     // fine: leave the brick, back to the coarse DDA and its advance
     // (voxel.wgsl:142-150 follows the fine loop)
+    leave(s, L);
+    return advance() ? DDA_MISS : DDA_RUN;
+  }
+
+  // step() with both level switches left to the caller: returns DDA_ENTER at
+  // an occupied brick of the coarse level (call enter()), DDA_LEAVE when the
+  // fine level has left its brick (call resume(), which returns the coarse
+  // level to the caller and takes its advance).  A caller that runs enter()
+  // and resume() for many lanes at once keeps those long branches out of the
+  // per-step code.  The same operations as step(), in the same order.
+  VRT_D uint32_t step_ev(const SceneDev& s, const LdsScene& L) {
+    if (t > tBound) return DDA_MISS;
+    uint32_t word;
+    if constexpr (GM)
+      word = *(const uint32_t*)(gbase + ((idx >> 5) << 2));
+    else
+      word = *(const uint32_t*)((const char*)L.lds + (((idx >> 5) << wsh) + base4));
+    steps += inc;
+    if (bit_at(word, idx)) return inc != 1u ? DDA_HIT : DDA_ENTER;
+    if (!advance()) return DDA_RUN;
+    return inc == 1u ? DDA_MISS : DDA_LEAVE;
+  }
+  // After DDA_LEAVE: back to the coarse level and its advance (voxel.wgsl:142-150).
+  VRT_D uint32_t resume(const SceneDev& s, const LdsScene& L) {
     leave(s, L);
     return advance() ? DDA_MISS : DDA_RUN;
   }

@@ -12,17 +12,16 @@ using Uniforms = vrt_uniforms;
 struct SceneDev;
 
 // Trace variant ids for the one-ray/pixel-per-lane kernels (launch_* `tv`).
-constexpr int TV_NESTED = 0, TV_FLAT = 1, TV_LDS = 2, TV_FAST = 3, TV_FAST1 = 4, TV_FASTSKIP = 5, TV_FASTG = 6;
-// Variants that trace with the single-loop DDA on LDS (TV_FAST1: the same
-// DDA, path tracer with one trace call site per lane).
-// TV_FASTSKIP: the same DDA taking runs of empty brick rows in bulk.
-// TV_FASTG: the single-loop DDA on global memory (no LDS), the form for
-// scenes whose bitmaps do not fit in LDS.
-constexpr bool tv_fast(int tv) { return tv == TV_FAST || tv == TV_FAST1 || tv == TV_FASTSKIP; }
-constexpr bool tv_uses_lds(int tv) { return tv == TV_LDS || tv_fast(tv); }
-// Largest occupancy bitmap staged in LDS (words): 48 KiB, 393216 bricks
-// (512^3 = 262144 bricks fits).  Larger scenes fall back to TV_NESTED.
-// TV_FAST stages the padded bitmap and its prefix counts within the same limit.
+// TV_NESTED: nested brick / voxel loops on global memory (vrt_trace.h).
+// TV_FAST: the single-loop DDA with both levels in LDS (vrt_dda_fast.h);
+// TV_FASTSKIP: the same DDA taking runs of empty brick rows in bulk
+// (traversal kernels only); TV_FASTG: the single-loop DDA on global memory
+// (no LDS), the form for scenes whose bitmaps do not fit in LDS.
+constexpr int TV_NESTED = 0, TV_FAST = 3, TV_FASTSKIP = 5, TV_FASTG = 6;
+constexpr bool tv_fast(int tv) { return tv == TV_FAST || tv == TV_FASTSKIP; }
+// LDS words the single-loop variants may stage beyond the mask slots: the
+// padded bitmap, its prefix counts and the row distances within 48 KiB.
+// Larger scenes run TV_FASTG.
 constexpr uint32_t LDS_BIT_WORDS_MAX = 12288;
 
 hipError_t launch_brick_masks(hipStream_t st, const uint32_t* vox, int nx, int ny, int bgx,
@@ -54,10 +53,15 @@ hipError_t launch_trace_persistent(hipStream_t st, const SceneDev& s, const floa
 hipError_t launch_pathtrace(hipStream_t st, const SceneDev& s, const Uniforms& u, float fclamp,
                             uint32_t spp, void* radiance, void* albedo, void* gbuf,
                             unsigned long long* counters, int tv, int waves, int form);
-hipError_t launch_pathtrace_persistent(hipStream_t st, const SceneDev& s, const Uniforms& u, float fclamp,
-                                       uint32_t spp, void* radiance, void* albedo, void* gbuf,
-                                       unsigned long long* counters, uint32_t* queue, int blocks,
-                                       int steps, int min_refill, int chunk, int tv, int waves);
+// Step-level refill path tracer on persistent one-wave blocks
+// (vrt_pathtrace_wave.h): `blocks` blocks of 64 threads; a pass when
+// `min_wait` lanes of a wave wait at a brick boundary or `shade_min` have
+// finished rays; waves = 0 / 4 / 5 / 6 minimum waves per SIMD; form 1 = the
+// global-memory DDA even where the LDS form fits; queue: one u32 patch
+// counter, cleared here by a kernel.
+hipError_t launch_pathtrace_wave(hipStream_t st, const SceneDev& s, const Uniforms& u, float fclamp, uint32_t spp,
+                                 void* radiance, void* albedo, void* gbuf, unsigned long long* counters,
+                                 uint32_t* queue, int blocks, int min_wait, int shade_min, int waves, int form);
 struct ViewDesc;
 // The refill tile form (2: 32x32, 3: 32x16 on 256-thread blocks; 4: 16x16,
 // 5: 16x8 on one-wave blocks) the path tracer runs for variant tv and tuning

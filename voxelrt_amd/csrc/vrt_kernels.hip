@@ -17,7 +17,6 @@
 #include "vrt_device.h"
 #include "vrt_kernels.h"
 #include "vrt_trace.h"
-#include "vrt_dda_flat.h"
 #include "vrt_dda_fast.h"
 #include "vrt_batch.h"
 
@@ -209,31 +208,22 @@ __global__ __launch_bounds__(256) void k_pack_pool(const uint32_t* __restrict__ 
   }
 }
 
-// Trace variants of the one-ray-per-lane kernels (all give identical results):
+// Trace variants of the one-ray-per-lane kernels (all give identical results;
+// ids in vrt_kernels.h):
 //   TV_NESTED  nested brick/voxel loops reading HBM/L2 (vrt_trace.h)
-//   TV_FLAT    single-stream DDA (vrt_dda_flat.h)
-//   TV_LDS     nested loops on the LDS-staged occupancy bitmap + per-lane
-//              brick mask slot (LdsMem); launched with lds_bytes() of dynamic LDS
-// (ids in vrt_kernels.h)
-
 //   TV_FAST    single-loop DDA with both levels in LDS (vrt_dda_fast.h):
-//              bitmap + prefix counts + per-lane mask slot
-//   TV_FAST1   TV_FAST traversal; the path tracer uses shade_pixel_1
-//   TV_FASTSKIP TV_FAST with the bulk skip of empty brick rows
+//              bitmap + prefix counts + per-lane mask slot; launched with
+//              lds_bytes() of dynamic LDS
+//   TV_FASTSKIP TV_FAST with the bulk skip of empty brick rows (traversal)
 //   TV_FASTG   the single-loop DDA on global memory (DdaFastT<true>)
 // What a lane needs to trace in a given variant.
 struct LaneMem {
-  LdsMem m;    // TV_LDS
   LdsScene f;  // TV_FAST
 };
 
 template <int TV>
 VRT_D void trace_sel(const SceneDev& s, const LaneMem& L, f3 ro, f3 rd, float tmax, TraceResult& h) {
-  if constexpr (TV == TV_FLAT)
-    trace_flat(s, ro, rd, tmax, h);
-  else if constexpr (TV == TV_LDS)
-    trace_m(s, L.m, ro, rd, tmax, h);
-  else if constexpr (TV == TV_FASTSKIP)
+  if constexpr (TV == TV_FASTSKIP)
     trace_fast<false, true>(s, L.f, ro, rd, tmax, h);
   else if constexpr (TV == TV_FASTG)
     trace_fast<false, false, true, true>(s, L.f, ro, rd, tmax, h);
@@ -243,21 +233,13 @@ VRT_D void trace_sel(const SceneDev& s, const LaneMem& L, f3 ro, f3 rd, float tm
     trace(s, ro, rd, tmax, h);
 }
 
-// Dynamic LDS of a TV_LDS / TV_FAST block of 256 threads: 16 mask words per
-// lane (word-major: word w of lane l at w * 256 + l), then the scene's
-// occupancy bitmap, then (TV_FAST) its prefix counts.  Every thread of the
-// block must call this.
+// Dynamic LDS of a TV_FAST block of 256 threads: 16 mask words per lane
+// (word-major: word w of lane l at w * 256 + l), then the scene's padded
+// occupancy bitmap, its prefix counts and its row distances.  Every thread of
+// the block must call this.
 template <int TV>
 VRT_D LaneMem lds_setup(const SceneDev& s, uint32_t* lds) {
-  LaneMem L{{nullptr, nullptr, 0u}, {nullptr, 0u, 0u, 0u, 0u}};
-  if constexpr (TV == TV_LDS) {
-    uint32_t* bits = lds + 16u * 256u;
-    for (uint32_t i = threadIdx.x; i < s.bit_words; i += blockDim.x) bits[i] = s.bits[i];
-    __syncthreads();
-    L.m.bits = bits;
-    L.m.mask = lds + threadIdx.x;
-    L.m.stride = 256u;
-  }
+  LaneMem L{{nullptr, 0u, 0u, 0u, 0u}};
   if constexpr (tv_fast(TV)) {
     uint32_t* bits = lds + 16u * 256u;
     for (uint32_t i = threadIdx.x; i < s.pwords; i += blockDim.x) {
@@ -319,7 +301,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WV))) void 
 // ---------------------------------------------------------------------------
 // traversal microbench, persistent form: each lane pulls rays from a global
 // queue (one atomic per wave refill) and advances a resumable DDA state one
-// loop iteration at a time (vrt_dda_flat.h).  `steps` iterations run between
+// loop iteration at a time (vrt_dda_fast.h).  `steps` iterations run between
 // refill checks; a refill happens once at least `min_refill` lanes are idle
 // (or the whole wave is).  Rays are taken from the global counter in chunks of
 // QUEUE_CHUNK per wave, so the counter sees one atomic per 256 rays instead of
@@ -338,17 +320,7 @@ VRT_D void store_hit(uint4* hits, uint32_t i, const TraceResult& h) {
   hits[i] = rec;
 }
 
-// The resumable DDA a persistent lane runs: single-stream HBM form
-// (vrt_dda_flat.h) or the single-loop LDS form (vrt_dda_fast.h).
-struct FlatRunner {
-  DdaFlat st;
-  VRT_D bool begin(const SceneDev& s, const LaneMem&, f3 ro, f3 rd, float tmax, TraceResult& o) {
-    return st.begin(s, ro, rd, tmax, o);
-  }
-  VRT_D bool step(const SceneDev& s, const LaneMem&, TraceResult& o) { return st.step(s, o); }
-  VRT_D void settle(const SceneDev&, const LaneMem&, bool) {}
-  VRT_D f3 dir() const { return mk3(st.rd[0], st.rd[1], st.rd[2]); }
-};
+// The resumable DDA a persistent lane runs (the single-loop LDS form).
 // A lane that reaches an occupied brick waits (pend) until entry_batch lanes
 // of its wave wait, or every lane walking a ray does; settle(), called by the
 // whole wave after each step, lets them enter together.
@@ -616,105 +588,6 @@ VRT_D PixelOut shade_pixel(const SceneDev& s, const LaneMem& L, const Uniforms& 
   return res;
 }
 
-// pathtrace.wgsl:248-330 with one trace call site.  Each lane walks its
-// pixel's rays one at a time -- primary, then per vertex the sun shadow ray
-// and the bounce ray -- and the shading between two traces runs as a
-// transition on the ray just finished.  Lanes of a wave therefore trace rays
-// of different kinds in the same DDA loop instead of waiting at separate
-// call sites.  Every lane performs exactly the operations of shade_pixel in
-// the same order (same RNG draws, same FP32 sequence): results are identical.
-template <int TV>
-VRT_D PixelOut shade_pixel_1(const SceneDev& s, const LaneMem& L, const Uniforms& u, float fclamp, uint32_t px,
-                             uint32_t py, uint32_t frame, uint32_t w, uint32_t h, RayCount& rc,
-                             const GbufOut& go) {
-  enum : uint32_t { PRIMARY, SHADOW, BOUNCE, DONE, SKY };
-  PixelOut res;
-  uint32_t rng = init_rng(px, py, frame);
-  const float jx = rand01(rng) - 0.5f;
-  const float jy = rand01(rng) - 0.5f;
-  const float uvx = (((float)px + 0.5f) + jx) / (float)w;
-  const float uvy = (((float)py + 0.5f) + jy) / (float)h;
-  const bool indirect = (u.params0[2] & VRT_FLAG_INDIRECT) != 0u;
-  const uint32_t nb = u.params0[1];
-  const f3 sunRad = mk3(u.sun_radiance[0], u.sun_radiance[1], u.sun_radiance[2]);
-  f3 ro = mk3(u.cam_pos[0], u.cam_pos[1], u.cam_pos[2]);
-  f3 rd = camera_ray(u, uvx, uvy);
-  f3 radiance = mk3(0.0f, 0.0f, 0.0f), tp = mk3(1.0f, 1.0f, 1.0f);
-  f3 pos = ro, n = mk3(0.0f, 0.0f, 0.0f), surf = ro, l = rd, alb1 = mk3(0.0f, 0.0f, 0.0f);
-  Material mat{};
-  float ndl = 0.0f;
-  uint32_t bounce = 0u, phase = PRIMARY;
-  while (phase < DONE) {
-    TraceResult hh;
-    trace_sel<TV>(s, L, ro, phase == SHADOW ? l : rd, 1e4f, hh);
-    rc.add(hh);
-    bool vertex = false, after_nee = false;
-    if (phase == PRIMARY) {
-      if (hh.t < 0.0f) {
-        res.radiance = indirect ? mk3(0.0f, 0.0f, 0.0f) : sky_color(u, rd, true);
-        go.store(mk3(1.0f, 1.0f, 1.0f), make_float4(-rd.x, -rd.y, -rd.z, -1.0f));
-        phase = SKY;
-        break;
-      }
-      mat = unpack_material(hh.mat);
-      alb1 = mat.albedo;
-      pos = ro + rd * hh.t;
-      n = hh.n;
-      go.store(mk3(vsqrt(alb1.x), vsqrt(alb1.y), vsqrt(alb1.z)), make_float4(n.x, n.y, n.z, hh.t));
-      vertex = true;
-    } else if (phase == SHADOW) {
-      if (!(hh.t >= 0.0f)) radiance = radiance + (((tp * mat.albedo) * INV_PI_F) * ndl) * sunRad;
-      after_nee = true;
-    } else {  // BOUNCE
-      if (hh.t < 0.0f) {
-        radiance = radiance + tp * sky_color(u, rd, false);
-        phase = DONE;
-      } else {
-        pos = surf + rd * hh.t;
-        n = hh.n;
-        mat = unpack_material(hh.mat);
-        ++bounce;
-        vertex = true;
-      }
-    }
-    if (vertex) {  // emission + sun NEE at the path vertex
-      const bool direct = !(indirect && bounce == 0u);
-      if (mat.emissive > 0.0f && direct) radiance = radiance + (tp * mat.albedo) * (mat.emissive * EMISSIVE_SCALE);
-      surf = pos + n * 1e-3f;
-      l = sample_sun_dir(u, rng);
-      ndl = dot(n, l);
-      if (ndl > 0.0f && l.y > 0.0f && direct) {
-        phase = SHADOW;
-        ro = surf;
-      } else {
-        after_nee = true;
-      }
-    }
-    if (after_nee) {  // cosine bounce + Russian roulette
-      if (bounce == nb) {
-        phase = DONE;
-      } else {
-        rd = cosine_hemisphere(n, rng);
-        tp = tp * mat.albedo;
-        bool alive = true;
-        if (bounce >= 2u) {
-          const float p = vclamp(vmax(tp.x, vmax(tp.y, tp.z)), 0.05f, 0.95f);
-          if (rand01(rng) > p) alive = false;
-          else tp = tp / p;
-        }
-        phase = alive ? BOUNCE : DONE;
-        ro = surf;
-      }
-    }
-  }
-  if (phase == SKY) return res;
-  if (!(fclamp <= 0.0f))
-    radiance = mk3(vmin(radiance.x, fclamp), vmin(radiance.y, fclamp), vmin(radiance.z, fclamp));
-  const f3 den = mk3(vmax(alb1.x, 1e-3f), vmax(alb1.y, 1e-3f), vmax(alb1.z, 1e-3f));
-  res.radiance = radiance / den;
-  return res;
-}
-
 // pathtrace.wgsl:248-330 as a resumable per-lane sample path: start() sets
 // up a sample's primary ray, after() consumes the result of the ray just
 // traced and either sets up the next ray (shadow or bounce) or ends the
@@ -846,10 +719,7 @@ VRT_D void pathtrace_tile(const SceneDev& s, const LaneMem& L, const Uniforms& u
     for (uint32_t smp = 0; smp < spp; ++smp) {
       const GbufOut go{albedo, gbuf, i, smp == 0u};  // the first sample's albedo / G-buffer
       PixelOut r;
-      if constexpr (TV == TV_FAST1)
-        r = shade_pixel_1<TV>(s, L, u, fclamp, px, py, u.params0[0] * spp + smp, w, h, rc, go);
-      else
-        r = shade_pixel<TV>(s, L, u, fclamp, px, py, u.params0[0] * spp + smp, w, h, rc, go);
+      r = shade_pixel<TV>(s, L, u, fclamp, px, py, u.params0[0] * spp + smp, w, h, rc, go);
       acc = acc + r.radiance;
     }
     const f3 o = acc / (float)spp;
@@ -891,7 +761,7 @@ template <int WV>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WV))) void k_pathtrace8(
     SceneDev s, Uniforms u, float fclamp, uint32_t spp, ushort4* __restrict__ radiance, uchar4* __restrict__ albedo,
     float4* __restrict__ gbuf, unsigned long long* __restrict__ counters) {
-  const LaneMem L{{nullptr, nullptr, 0u}, {nullptr, 0u, 0u, 0u, 0u}};
+  const LaneMem L{{nullptr, 0u, 0u, 0u, 0u}};
   RayCount rc{0u, 0u, 0u, 0u};
   pathtrace_tile<TV_FASTG, 8>(s, L, u, fclamp, spp, map_tile(blockIdx.x, gridDim.x, s.tile_order), radiance, albedo,
                               gbuf, rc);
@@ -1270,7 +1140,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WV))) void 
   __shared__ uint32_t s_tile;
   const uint32_t tiles = head->tiles, nviews = head->nviews;
   const void* loaded = nullptr;  // scene whose occupancy is in LDS
-  LaneMem L{{nullptr, nullptr, 0u}, {nullptr, 0u, 0u, 0u, 0u}};
+  LaneMem L{{nullptr, 0u, 0u, 0u, 0u}};
   for (;;) {
     __syncthreads();  // every lane is done with the previous tile (s_tile, LDS)
     if (threadIdx.x == 0u) s_tile = atomicAdd(&head->queue, 1u);
@@ -1281,7 +1151,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WV))) void 
     if (b >= tiles) break;
     const uint32_t vi = find_view(views, nviews, b);
     const ViewDesc& d = views[vi];
-    if (tv_uses_lds(TV)) {
+    if (tv_fast(TV)) {
       if (d.s.pool != loaded) {
         L = lds_setup<TV>(d.s, lds_dyn);
         loaded = d.s.pool;
@@ -1341,7 +1211,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WV))) void k
   __shared__ uint32_t s_tile, s_next;
   const uint32_t tiles = head->rtiles, nviews = head->nviews;
   const void* loaded = nullptr;  // scene whose occupancy is in LDS
-  LaneMem L{{nullptr, nullptr, 0u}, {nullptr, 0u, 0u, 0u, 0u}};
+  LaneMem L{{nullptr, 0u, 0u, 0u, 0u}};
   for (;;) {
     // b is block-uniform; readfirstlane puts it in a scalar register, so the
     // view's descriptor (scene, uniforms) loads as scalars instead of into
@@ -1362,7 +1232,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WV))) void k
     }
     if (b >= tiles) break;
     const ViewDesc& d = views[find_view_r(views, nviews, b)];
-    if (tv_uses_lds(TV)) {
+    if (tv_fast(TV)) {
       if (d.s.pool != loaded) {
         L = lds_setup<TV>(d.s, lds_dyn);
         loaded = d.s.pool;
@@ -1496,7 +1366,7 @@ __global__ __launch_bounds__(256) void k_hdr_batch_gs(const BatchHead* __restric
 
 }  // namespace vrt
 
-#include "vrt_pathtrace_persistent.h"
+#include "vrt_pathtrace_wave.h"
 
 // ---------------------------------------------------------------------------
 // host-side launchers (declared in vrt_kernels.h)
@@ -1505,22 +1375,21 @@ namespace vrt {
 
 static inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
-// TV_LDS launches: 16 mask words per lane (256-thread blocks) + the bitmap,
-// kept within 64 KiB of LDS per block.
-// LDS words a variant stages beyond the mask slots: TV_LDS the bitmap,
-// TV_FAST the padded bitmap, its prefix counts and the row distances (bytes).
-static inline uint32_t lds_words(int tv, uint32_t bit_words, uint32_t pwords) {
-  return tv_fast(tv) ? 2u * pwords + (pwords + 3u) / 4u : (tv == TV_LDS ? bit_words : 0u);
+// LDS of the single-loop variants: 16 mask words per lane (256-thread
+// blocks), then the padded bitmap, its prefix counts and the row distances
+// (bytes), kept within 64 KiB per block.  bit_words (the unpadded bitmap) is
+// not staged by any variant; the argument keeps the launchers' signatures.
+static inline uint32_t lds_words(int tv, uint32_t, uint32_t pwords) {
+  return tv_fast(tv) ? 2u * pwords + (pwords + 3u) / 4u : 0u;
 }
 static inline uint32_t lds_bytes(int tv, uint32_t bit_words, uint32_t pwords) {
-  return tv_uses_lds(tv) ? (16u * 256u + lds_words(tv, bit_words, pwords)) * 4u : 0u;
+  return tv_fast(tv) ? (16u * 256u + lds_words(tv, bit_words, pwords)) * 4u : 0u;
 }
-// Variant actually launched: LDS variants fall back when what they stage
-// does not fit -- the single-loop ones to their global-memory form, the
-// nested one to nested loops on HBM.
+// Variant actually launched: the LDS variants fall back to the single loop on
+// global memory when what they stage does not fit.
 static inline int lds_variant(int tv, uint32_t bit_words, uint32_t pwords) {
-  if (!tv_uses_lds(tv) || lds_words(tv, bit_words, pwords) <= LDS_BIT_WORDS_MAX) return tv;
-  return tv_fast(tv) ? TV_FASTG : TV_NESTED;
+  if (!tv_fast(tv) || lds_words(tv, bit_words, pwords) <= LDS_BIT_WORDS_MAX) return tv;
+  return TV_FASTG;
 }
 
 hipError_t launch_brick_masks(hipStream_t st, const uint32_t* vox, int nx, int ny, int bgx,
@@ -1597,10 +1466,6 @@ hipError_t launch_trace_rays(hipStream_t st, const SceneDev& s, const float* ray
   else if (tv == TV_FASTSKIP)
     hipLaunchKernelGGL(k_trace_rays<TV_FASTSKIP>, g, dim3(256), shmem, st, s, rays, n, tmax, (uint4*)hits,
                        counters);
-  else if (tv == TV_LDS)
-    hipLaunchKernelGGL(k_trace_rays<TV_LDS>, g, dim3(256), shmem, st, s, rays, n, tmax, (uint4*)hits, counters);
-  else if (tv == TV_FLAT)
-    hipLaunchKernelGGL(k_trace_rays<TV_FLAT>, g, dim3(256), 0, st, s, rays, n, tmax, (uint4*)hits, counters);
   else
     hipLaunchKernelGGL(k_trace_rays<TV_NESTED>, g, dim3(256), 0, st, s, rays, n, tmax, (uint4*)hits, counters);
   return hipGetLastError();
@@ -1621,25 +1486,25 @@ hipError_t launch_trace_persistent(hipStream_t st, const SceneDev& s, const floa
                                   float tmax, void* hits, unsigned long long* counters,
                                   uint32_t* queue, int blocks, int steps, int min_refill, int tv, int waves,
                                   bool clear_queue) {
+  // the persistent kernel runs the single loop in LDS; where the scene's
+  // bitmaps do not fit, the one-ray-per-lane kernel on global memory
+  tv = lds_variant(tv_fast(tv) ? tv : TV_FAST, s.bit_words, s.pwords);
+  if (!tv_fast(tv)) return launch_trace_rays(st, s, rays, n, tmax, hits, counters, 0, TV_FASTG, waves);
   if (clear_queue) {
     const hipError_t e = hipMemsetAsync(queue, 0, sizeof(uint32_t), st);
     if (e != hipSuccess) return e;
   }
   const dim3 g(blocks > 0 ? blocks : 1);
   const int sp = steps < 1 ? 1 : steps, mr = min_refill < 1 ? 1 : min_refill;
-  tv = tv_fast(tv) ? lds_variant(tv, s.bit_words, s.pwords) : TV_NESTED;
   const uint32_t shmem = lds_bytes(tv, s.bit_words, s.pwords);
   if (tv == TV_FASTSKIP)
     hipLaunchKernelGGL((k_trace_persistent<FastRunner<true>, TV_FASTSKIP>), g, dim3(256), shmem, st, s, rays, n,
                        tmax, (uint4*)hits, counters, queue, sp, mr);
-  else if (tv_fast(tv) && waves == 6)
+  else if (waves == 6)
     hipLaunchKernelGGL((k_trace_persistent<FastRunner<false>, TV_FAST, 6>), g, dim3(256), shmem, st, s, rays, n,
                        tmax, (uint4*)hits, counters, queue, sp, mr);
-  else if (tv_fast(tv))
-    hipLaunchKernelGGL((k_trace_persistent<FastRunner<false>, TV_FAST>), g, dim3(256), shmem, st, s, rays, n, tmax,
-                       (uint4*)hits, counters, queue, sp, mr);
   else
-    hipLaunchKernelGGL((k_trace_persistent<FlatRunner, TV_NESTED>), g, dim3(256), 0, st, s, rays, n, tmax,
+    hipLaunchKernelGGL((k_trace_persistent<FastRunner<false>, TV_FAST>), g, dim3(256), shmem, st, s, rays, n, tmax,
                        (uint4*)hits, counters, queue, sp, mr);
   return hipGetLastError();
 }
@@ -1712,24 +1577,12 @@ hipError_t launch_pathtrace(hipStream_t st, const SceneDev& s, const Uniforms& u
                        (uchar4*)albedo, (float4*)gbuf, counters);
   };
   switch (tv * 8 + (waves == 4 || waves == 5 ? waves : 1)) {
-    case TV_FAST1 * 8 + 4: go(k_pathtrace<TV_FAST1, 4>); break;
-    case TV_FAST1 * 8 + 5: go(k_pathtrace<TV_FAST1, 5>); break;
-    case TV_FAST1 * 8 + 1: go(k_pathtrace<TV_FAST1, 1>); break;
-    case TV_FASTSKIP * 8 + 4: go(k_pathtrace<TV_FASTSKIP, 4>); break;
-    case TV_FASTSKIP * 8 + 5: go(k_pathtrace<TV_FASTSKIP, 5>); break;
-    case TV_FASTSKIP * 8 + 1: go(k_pathtrace<TV_FASTSKIP, 1>); break;
     case TV_FASTG * 8 + 4: go(k_pathtrace<TV_FASTG, 4>); break;
     case TV_FASTG * 8 + 5: go(k_pathtrace<TV_FASTG, 5>); break;
     case TV_FASTG * 8 + 1: go(k_pathtrace<TV_FASTG, 1>); break;
     case TV_FAST * 8 + 4: go(k_pathtrace<TV_FAST, 4>); break;
     case TV_FAST * 8 + 5: go(k_pathtrace<TV_FAST, 5>); break;
     case TV_FAST * 8 + 1: go(k_pathtrace<TV_FAST, 1>); break;
-    case TV_LDS * 8 + 4: go(k_pathtrace<TV_LDS, 4>); break;
-    case TV_LDS * 8 + 5: go(k_pathtrace<TV_LDS, 5>); break;
-    case TV_LDS * 8 + 1: go(k_pathtrace<TV_LDS, 1>); break;
-    case TV_FLAT * 8 + 4: go(k_pathtrace<TV_FLAT, 4>); break;
-    case TV_FLAT * 8 + 5: go(k_pathtrace<TV_FLAT, 5>); break;
-    case TV_FLAT * 8 + 1: go(k_pathtrace<TV_FLAT, 1>); break;
     case TV_NESTED * 8 + 4: go(k_pathtrace<TV_NESTED, 4>); break;
     case TV_NESTED * 8 + 5: go(k_pathtrace<TV_NESTED, 5>); break;
     default: go(k_pathtrace<TV_NESTED, 1>); break;
@@ -1737,25 +1590,34 @@ hipError_t launch_pathtrace(hipStream_t st, const SceneDev& s, const Uniforms& u
   return hipGetLastError();
 }
 
-hipError_t launch_pathtrace_persistent(hipStream_t st, const SceneDev& s, const Uniforms& u, float fclamp,
-                                       uint32_t spp, void* radiance, void* albedo, void* gbuf,
-                                       unsigned long long* counters, uint32_t* queue, int blocks,
-                                       int steps, int min_refill, int chunk, int tv, int waves) {
-  hipError_t e = hipMemsetAsync(queue, 0, sizeof(uint32_t), st);
+hipError_t launch_pathtrace_wave(hipStream_t st, const SceneDev& s, const Uniforms& u, float fclamp, uint32_t spp,
+                                 void* radiance, void* albedo, void* gbuf, unsigned long long* counters,
+                                 uint32_t* queue, int blocks, int min_wait, int shade_min, int waves,
+                                 int form) {
+  hipError_t e = launch_zero_words(st, queue, 1u);
   if (e != hipSuccess) return e;
-  tv = tv == TV_FAST ? lds_variant(TV_FAST, s.bit_words, s.pwords) : TV_NESTED;
-  const uint32_t shmem = lds_bytes(tv, s.bit_words, s.pwords);
+  const uint32_t mw = min_wait < 1 ? 1u : (min_wait > 64 ? 64u : (uint32_t)min_wait);
+  const uint32_t sm = shade_min < 1 ? 1u : (shade_min > 64 ? 64u : (uint32_t)shade_min);
+  // the LDS form when the scene's bitmaps fit (four waves per block, so the
+  // grid has a quarter of the blocks), else the global-memory form
+  const bool lds = form != 1 && lds_variant(TV_FAST, s.bit_words, s.pwords) == TV_FAST;
+  const uint32_t nt = lds ? 256u : 64u;
+  const uint32_t shmem = lds ? lds_bytes(TV_FAST, s.bit_words, s.pwords) : 0u;
+  const int nb = blocks > 0 ? (lds ? (blocks + 3) / 4 : blocks) : 1;
   auto go = [&](auto kernel) {
-    hipLaunchKernelGGL(kernel, dim3(blocks > 0 ? blocks : 1), dim3(256), shmem, st, s, u, fclamp, spp,
-                       (ushort4*)radiance, (uchar4*)albedo, (float4*)gbuf, counters, queue, steps < 1 ? 1 : steps,
-                       min_refill < 1 ? 1 : min_refill, (uint32_t)(chunk < 64 ? 64 : chunk));
+    hipLaunchKernelGGL(kernel, dim3(nb), dim3(nt), shmem, st, s, u, fclamp, spp, (ushort4*)radiance,
+                       (uchar4*)albedo, (float4*)gbuf, counters, queue, mw, sm);
   };
-  if (tv == TV_FAST) {
-    if (waves == 4) go(k_pathtrace_persistent<FastRunner<false>, TV_FAST, 4>);
-    else if (waves == 5) go(k_pathtrace_persistent<FastRunner<false>, TV_FAST, 5>);
-    else go(k_pathtrace_persistent<FastRunner<false>, TV_FAST, 1>);
-  } else {
-    go(k_pathtrace_persistent<FlatRunner, TV_NESTED, 1>);
+  const int wv = waves == 4 || waves == 5 || waves == 6 ? waves : 1;
+  switch (wv * 1000 + (int)nt) {
+    case 4064: go(k_pathtrace_wave<4, 64>); break;
+    case 5064: go(k_pathtrace_wave<5, 64>); break;
+    case 6064: go(k_pathtrace_wave<6, 64>); break;
+    case 1064: go(k_pathtrace_wave<1, 64>); break;
+    case 4256: go(k_pathtrace_wave<4, 256>); break;
+    case 5256: go(k_pathtrace_wave<5, 256>); break;
+    case 6256: go(k_pathtrace_wave<6, 256>); break;
+    default: go(k_pathtrace_wave<1, 256>); break;
   }
   return hipGetLastError();
 }
@@ -1828,24 +1690,12 @@ hipError_t launch_pathtrace_batch(hipStream_t st, const ViewDesc* views, uint32_
     hipLaunchKernelGGL(kernel, dim3(tiles), dim3(256), shmem, st, views, nviews, counters);
   };
   switch (tv * 8 + (waves == 4 || waves == 5 ? waves : 1)) {
-    case TV_FAST1 * 8 + 4: go(k_pathtrace_batch<TV_FAST1, 4>); break;
-    case TV_FAST1 * 8 + 5: go(k_pathtrace_batch<TV_FAST1, 5>); break;
-    case TV_FAST1 * 8 + 1: go(k_pathtrace_batch<TV_FAST1, 1>); break;
-    case TV_FASTSKIP * 8 + 4: go(k_pathtrace_batch<TV_FASTSKIP, 4>); break;
-    case TV_FASTSKIP * 8 + 5: go(k_pathtrace_batch<TV_FASTSKIP, 5>); break;
-    case TV_FASTSKIP * 8 + 1: go(k_pathtrace_batch<TV_FASTSKIP, 1>); break;
     case TV_FASTG * 8 + 4: go(k_pathtrace_batch<TV_FASTG, 4>); break;
     case TV_FASTG * 8 + 5: go(k_pathtrace_batch<TV_FASTG, 5>); break;
     case TV_FASTG * 8 + 1: go(k_pathtrace_batch<TV_FASTG, 1>); break;
     case TV_FAST * 8 + 4: go(k_pathtrace_batch<TV_FAST, 4>); break;
     case TV_FAST * 8 + 5: go(k_pathtrace_batch<TV_FAST, 5>); break;
     case TV_FAST * 8 + 1: go(k_pathtrace_batch<TV_FAST, 1>); break;
-    case TV_LDS * 8 + 4: go(k_pathtrace_batch<TV_LDS, 4>); break;
-    case TV_LDS * 8 + 5: go(k_pathtrace_batch<TV_LDS, 5>); break;
-    case TV_LDS * 8 + 1: go(k_pathtrace_batch<TV_LDS, 1>); break;
-    case TV_FLAT * 8 + 4: go(k_pathtrace_batch<TV_FLAT, 4>); break;
-    case TV_FLAT * 8 + 5: go(k_pathtrace_batch<TV_FLAT, 5>); break;
-    case TV_FLAT * 8 + 1: go(k_pathtrace_batch<TV_FLAT, 1>); break;
     case TV_NESTED * 8 + 4: go(k_pathtrace_batch<TV_NESTED, 4>); break;
     case TV_NESTED * 8 + 5: go(k_pathtrace_batch<TV_NESTED, 5>); break;
     default: go(k_pathtrace_batch<TV_NESTED, 1>); break;
@@ -1868,24 +1718,12 @@ hipError_t launch_batch_persistent(hipStream_t st, BatchHead* head, const ViewDe
       hipLaunchKernelGGL(K::fn, gp, dim3(K::nt), tv == TV_FASTG ? 0u : shmem, st, head, views, counters);
     });
   else switch (tv * 8 + wvr) {
-    case TV_FAST1 * 8 + 4: go(k_pathtrace_batch_q<TV_FAST1, 4>); break;
-    case TV_FAST1 * 8 + 5: go(k_pathtrace_batch_q<TV_FAST1, 5>); break;
-    case TV_FAST1 * 8 + 1: go(k_pathtrace_batch_q<TV_FAST1, 1>); break;
-    case TV_FASTSKIP * 8 + 4: go(k_pathtrace_batch_q<TV_FASTSKIP, 4>); break;
-    case TV_FASTSKIP * 8 + 5: go(k_pathtrace_batch_q<TV_FASTSKIP, 5>); break;
-    case TV_FASTSKIP * 8 + 1: go(k_pathtrace_batch_q<TV_FASTSKIP, 1>); break;
     case TV_FASTG * 8 + 4: go(k_pathtrace_batch_q<TV_FASTG, 4>); break;
     case TV_FASTG * 8 + 5: go(k_pathtrace_batch_q<TV_FASTG, 5>); break;
     case TV_FASTG * 8 + 1: go(k_pathtrace_batch_q<TV_FASTG, 1>); break;
     case TV_FAST * 8 + 4: go(k_pathtrace_batch_q<TV_FAST, 4>); break;
     case TV_FAST * 8 + 5: go(k_pathtrace_batch_q<TV_FAST, 5>); break;
     case TV_FAST * 8 + 1: go(k_pathtrace_batch_q<TV_FAST, 1>); break;
-    case TV_LDS * 8 + 4: go(k_pathtrace_batch_q<TV_LDS, 4>); break;
-    case TV_LDS * 8 + 5: go(k_pathtrace_batch_q<TV_LDS, 5>); break;
-    case TV_LDS * 8 + 1: go(k_pathtrace_batch_q<TV_LDS, 1>); break;
-    case TV_FLAT * 8 + 4: go(k_pathtrace_batch_q<TV_FLAT, 4>); break;
-    case TV_FLAT * 8 + 5: go(k_pathtrace_batch_q<TV_FLAT, 5>); break;
-    case TV_FLAT * 8 + 1: go(k_pathtrace_batch_q<TV_FLAT, 1>); break;
     case TV_NESTED * 8 + 4: go(k_pathtrace_batch_q<TV_NESTED, 4>); break;
     case TV_NESTED * 8 + 5: go(k_pathtrace_batch_q<TV_NESTED, 5>); break;
     default: go(k_pathtrace_batch_q<TV_NESTED, 1>); break;

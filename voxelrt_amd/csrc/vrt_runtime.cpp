@@ -286,16 +286,17 @@ int vrt_create(const vrt_create_info* info, vrt_ctx** out) {
   size_t freeb = 0, total = 0;
   (void)hipMemGetInfo(&freeb, &total);
   ctx->budget = (info && info->hbm_budget) ? info->hbm_budget : (uint64_t)(0.9 * (double)freeb);
-  ctx->tuning.trace_kernel = 1;
-  ctx->tuning.trace_blocks = 2048;
+  ctx->tuning.trace_kernel = 5;
+  ctx->tuning.trace_blocks = 1024;
   ctx->tuning.trace_steps = 8;
   ctx->tuning.trace_min_refill = 16;
-  ctx->tuning.pt_kernel = 0;
+  ctx->tuning.dda_batch = 16;
+  ctx->tuning.pt_kernel = 8;
   ctx->tuning.pt_order = 2;  // last tile first: the bottom rows (near ground) usually cost most
-  ctx->tuning.pt_blocks = 1024;
-  ctx->tuning.pt_steps = 8;
-  ctx->tuning.pt_min_refill = 32;
-  ctx->tuning.pt_chunk = 64;
+  ctx->tuning.pt_blocks = 0;
+  ctx->tuning.pt_waves = 4;
+  ctx->tuning.pt_min_refill = 16;
+  ctx->tuning.pt_shade_min = 32;
   if (ctx->d_queue.alloc(64 * sizeof(uint32_t)) != hipSuccess ||
       ctx->d_counters.alloc(4 * sizeof(uint64_t)) != hipSuccess ||
       hipMemsetAsync(ctx->d_counters.ptr, 0, 4 * sizeof(uint64_t), ctx->stream) != hipSuccess) {
@@ -597,13 +598,10 @@ int vrt_scene_bricks(vrt_ctx* ctx, const vrt_scene* scene, uint32_t* bricks, uin
 // traversal
 // ---------------------------------------------------------------------------
 // vrt_tuning.trace_kernel -> trace variant of the one-ray-per-lane kernel:
-// 0 nested, 2 single-stream (flat), 3 LDS-staged occupancy, 4 single loop
-// with both levels in LDS (with the empty-row skip when skip is set), 6 the
-// single loop on global memory.
+// 0 nested, 4 single loop with both levels in LDS (with the empty-row skip
+// when skip is set), 6 the single loop on global memory.
 static int trace_variant(uint32_t kernel, uint32_t skip) {
   switch (kernel) {
-    case 2: return vrt::TV_FLAT;
-    case 3: return vrt::TV_LDS;
     case 4: return skip ? vrt::TV_FASTSKIP : vrt::TV_FAST;
     case 6: return vrt::TV_FASTG;
     default: return vrt::TV_NESTED;
@@ -611,18 +609,14 @@ static int trace_variant(uint32_t kernel, uint32_t skip) {
 }
 
 // vrt_tuning.pt_kernel -> trace variant of the one-pixel-per-lane path
-// tracers (the single-view kernel and the batched forms): 0 nested, 2 flat,
-// 3 LDS-staged occupancy, 4 single loop in LDS (empty-row skip when skip is
-// set), 5 as 4 with one trace call site, 6 (persistent in the single-view
-// form) as 4, 7 the single loop on global memory.
-static int pt_variant(uint32_t kernel, uint32_t skip) {
+// tracers (the single-view kernel and the batched forms): 0 nested, 4 single
+// loop in LDS, 7 (and 8 in the batched forms) the single loop on global
+// memory.
+static int pt_variant(uint32_t kernel) {
   switch (kernel) {
-    case 2: return vrt::TV_FLAT;
-    case 3: return vrt::TV_LDS;
-    case 4: return skip ? vrt::TV_FASTSKIP : vrt::TV_FAST;
-    case 5: return vrt::TV_FAST1;
-    case 6: return vrt::TV_FAST;
-    case 7: return vrt::TV_FASTG;
+    case 4: return vrt::TV_FAST;
+    case 7:
+    case 8: return vrt::TV_FASTG;
     default: return vrt::TV_NESTED;
   }
 }
@@ -630,15 +624,14 @@ static int pt_variant(uint32_t kernel, uint32_t skip) {
 hipError_t vrt_ctx::launch_pathtrace(const vrt::SceneDev& s0, const vrt_uniforms& u, float fclamp,
                                      uint32_t spp, void* radiance, void* albedo, void* gbuf) {
   const vrt::SceneDev s = tuned(s0);
-  if (tuning.pt_kernel == 1 || tuning.pt_kernel == 6)
-    return vrt::launch_pathtrace_persistent(stream, s, u, fclamp, spp, radiance, albedo, gbuf,
-                                            (unsigned long long*)d_counters.ptr, (uint32_t*)d_queue.ptr + 1,
-                                            (int)tuning.pt_blocks, (int)tuning.pt_steps,
-                                            (int)tuning.pt_min_refill, (int)tuning.pt_chunk,
-                                            tuning.pt_kernel == 6 ? vrt::TV_FAST : vrt::TV_NESTED,
-                                            (int)tuning.pt_waves);
+  if (tuning.pt_kernel == 8)
+    return vrt::launch_pathtrace_wave(stream, s, u, fclamp, spp, radiance, albedo, gbuf,
+                                      (unsigned long long*)d_counters.ptr, (uint32_t*)d_queue.ptr + 2,
+                                      tuning.pt_blocks ? (int)tuning.pt_blocks : cus * 16,
+                                      (int)tuning.pt_min_refill, (int)tuning.pt_shade_min, (int)tuning.pt_waves,
+                                      (int)tuning.pt_tile);
   return vrt::launch_pathtrace(stream, s, u, fclamp, spp, radiance, albedo, gbuf,
-                               (unsigned long long*)d_counters.ptr, pt_variant(tuning.pt_kernel, tuning.empty_skip),
+                               (unsigned long long*)d_counters.ptr, pt_variant(tuning.pt_kernel),
                                (int)tuning.pt_waves, (int)tuning.pt_tile);
 }
 
@@ -648,12 +641,11 @@ hipError_t vrt_ctx::launch_trace(const vrt::SceneDev& s0, const float* rays, uin
   const vrt::SceneDev s = tuned(s0);
   if (!counters) counters = (unsigned long long*)d_counters.ptr;
   if (!queue) queue = (uint32_t*)d_queue.ptr;
-  if ((tuning.trace_kernel == 1 || tuning.trace_kernel == 5) && n < 0x80000000ull)
+  if (tuning.trace_kernel == 5 && n < 0x80000000ull)
     return vrt::launch_trace_persistent(stream, s, rays, (uint32_t)n, tmax, hits, counters, queue,
                                         (int)tuning.trace_blocks, (int)tuning.trace_steps,
                                         (int)tuning.trace_min_refill,
-                                        tuning.trace_kernel != 5 ? vrt::TV_NESTED
-                                        : (tuning.empty_skip ? vrt::TV_FASTSKIP : vrt::TV_FAST),
+                                        tuning.empty_skip ? vrt::TV_FASTSKIP : vrt::TV_FAST,
                                         (int)tuning.trace_waves, clear_queue);
   return vrt::launch_trace_rays(stream, s, rays, n, tmax, hits, counters, trace_blocks,
                                 trace_variant(tuning.trace_kernel, tuning.empty_skip), (int)tuning.trace_waves);
@@ -1034,12 +1026,15 @@ int vrt_tuning_get(vrt_ctx* ctx, vrt_tuning* out) {
 int vrt_tuning_set(vrt_ctx* ctx, const vrt_tuning* in) {
   if (!ctx || !in) return VRT_ERR_INVALID;
   std::lock_guard<std::mutex> lock_(ctx->mu);
-  if (in->trace_kernel > 6 || in->pt_kernel > 7 || in->trace_blocks > (1u << 20) ||
-      in->pt_blocks > (1u << 20) || in->trace_steps > 1024 || in->pt_steps > 1024 ||
-      in->trace_min_refill > 64 || in->pt_min_refill > 64 || in->pt_chunk > (1u << 16) ||
-      !(in->pt_waves == 0 || in->pt_waves == 4 || in->pt_waves == 5) || in->batch_mode > 2 ||
+  const uint32_t tk = in->trace_kernel, pk = in->pt_kernel;
+  if (!(tk == 0 || tk == 4 || tk == 5 || tk == 6) || !(pk == 0 || pk == 4 || pk == 7 || pk == 8) ||
+      in->trace_blocks > (1u << 20) || in->pt_blocks > (1u << 20) || in->trace_steps > 1024 ||
+      in->trace_min_refill > 64 || in->pt_min_refill > 64 ||
+      !(in->pt_waves == 0 || in->pt_waves == 4 || in->pt_waves == 5 || (in->pt_waves == 6 && pk == 8)) ||
+      in->batch_mode > 2 ||
       in->batch_blocks > (1u << 16) || in->dda_batch > 64 ||
-      in->empty_skip > 1 || in->pt_order > 3 || in->pt_tile > 5 || !(in->trace_waves == 0 || in->trace_waves == 6))
+      in->empty_skip > 1 || in->pt_order > 3 || in->pt_tile > 5 || !(in->trace_waves == 0 || in->trace_waves == 6) ||
+      in->pt_shade_min > 64 || in->batch_tile > 5)
     return ctx->fail(VRT_ERR_INVALID, "tuning value out of range");
   ctx->tuning = *in;
   return VRT_OK;
@@ -1052,8 +1047,23 @@ int vrt_autotune(vrt_ctx* ctx, const vrt_scene* scene, uint64_t nrays, const vrt
   std::lock_guard<std::mutex> lock_(ctx->mu);
   HIP_TRY(ctx, hipSetDevice(ctx->device));
   const vrt::SceneDev s = scene->dev();
-  const vrt_tuning start = ctx->tuning;
-  vrt_tuning best = start;
+  uint64_t before[4];
+  int rc = ctx->counters_to_host(before);
+  if (rc) return rc;
+  // On every return: the sweep's rays leave the statistics, and the tuning
+  // is the one in effect before the call unless the sweep completed.
+  struct Restore {
+    vrt_ctx* c;
+    const uint64_t* cnt;
+    vrt_tuning keep;
+    ~Restore() {
+      c->tuning = keep;
+      if (hipMemcpyAsync(c->d_counters.ptr, cnt, 4 * sizeof(uint64_t), hipMemcpyHostToDevice, c->stream) ==
+          hipSuccess)
+        (void)hipStreamSynchronize(c->stream);
+    }
+  } restore{ctx, before, ctx->tuning};
+  vrt_tuning best = ctx->tuning;
   // a warm-up launch, then the fastest of three rounds of three launches:
   // a transient slowdown of one round (clocks, a neighbour on the device)
   // does not decide the choice
@@ -1072,9 +1082,6 @@ int vrt_autotune(vrt_ctx* ctx, const vrt_scene* scene, uint64_t nrays, const vrt
     *out_ms = best_round;
     return VRT_OK;
   };
-  uint64_t before[4];
-  int rc = ctx->counters_to_host(before);
-  if (rc) return rc;
   if (nrays > 0) {
     // random rays inside the grid's box, directions spread over the sphere
     const uint64_t n = std::min<uint64_t>(nrays, 1ull << 24);
@@ -1101,65 +1108,48 @@ int vrt_autotune(vrt_ctx* ctx, const vrt_scene* scene, uint64_t nrays, const vrt
     HIP_TRY(ctx, ctx->scratch_rays.ensure(n * 24));
     HIP_TRY(ctx, ctx->scratch_hits.ensure(n * 16));
     HIP_TRY(ctx, hipMemcpyAsync(ctx->scratch_rays.ptr, rays.data(), n * 24, hipMemcpyHostToDevice, ctx->stream));
-    const uint32_t blocks[] = {1024, 2048, 4096, 8192};
-    const uint32_t steps[] = {1, 4, 8, 16};
-    const uint32_t refill[] = {1, 16, 32};
     float best_ms = 1e30f;
-    ctx->tuning.trace_kernel = 0;
-    float ms = 0;
-    rc = time_it([&] { return ctx->launch_trace(s, (const float*)ctx->scratch_rays.ptr, n, 1e4f,
-                                                ctx->scratch_hits.ptr); }, &ms);
-    if (rc) return rc;
-    best_ms = ms;
-    best.trace_kernel = 0;
-    best.dda_batch = 1;
-    // one-ray-per-lane variants; the single-loop one also over its brick
-    // entry batch
-    for (uint32_t k : {2u, 3u, 4u, 6u})
+    auto try_trace = [&](const vrt_tuning& cand) -> int {
+      ctx->tuning = cand;
+      float ms = 0;
+      const int r = time_it([&] { return ctx->launch_trace(s, (const float*)ctx->scratch_rays.ptr, n, 1e4f,
+                                                            ctx->scratch_hits.ptr); }, &ms);
+      if (r) return r;
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = cand;
+      }
+      return VRT_OK;
+    };
+    vrt_tuning c = best;
+    c.trace_kernel = 0;
+    c.dda_batch = 1;
+    c.trace_waves = 0;
+    if ((rc = try_trace(c))) return rc;
+    // one-ray-per-lane single-loop variants over their brick entry batch and
+    // register budget
+    for (uint32_t k : {4u, 6u})
       for (uint32_t db : {1u, 8u, 16u, 32u})
         for (uint32_t wv : {0u, 6u}) {
-          if (k < 4u && (db != 1u || wv != 0u)) continue;
-          ctx->tuning.trace_kernel = k;
-          ctx->tuning.dda_batch = db;
-          ctx->tuning.trace_waves = wv;
-          rc = time_it([&] { return ctx->launch_trace(s, (const float*)ctx->scratch_rays.ptr, n, 1e4f,
-                                                      ctx->scratch_hits.ptr); }, &ms);
-          if (rc) return rc;
-          if (ms < best_ms) {
-            best_ms = ms;
-            best.trace_kernel = k;
-            best.dda_batch = db;
-            best.trace_waves = wv;
-          }
+          c.trace_kernel = k;
+          c.dda_batch = db;
+          c.trace_waves = wv;
+          if ((rc = try_trace(c))) return rc;
         }
-    ctx->tuning.trace_waves = 0;
-    for (uint32_t pk : {1u, 5u})
-      for (uint32_t db : {1u, 16u})
-        for (uint32_t wv : {0u, 6u}) {
-          if (pk == 1u && (db != 1u || wv != 0u)) continue;
-          for (uint32_t b : blocks)
-            for (uint32_t st : steps)
-              for (uint32_t rf : refill) {
-                ctx->tuning.trace_kernel = pk;
-                ctx->tuning.trace_blocks = b;
-                ctx->tuning.trace_steps = st;
-                ctx->tuning.trace_min_refill = rf;
-                ctx->tuning.dda_batch = db;
-                ctx->tuning.trace_waves = wv;
-                rc = time_it([&] { return ctx->launch_trace(s, (const float*)ctx->scratch_rays.ptr, n, 1e4f,
-                                                            ctx->scratch_hits.ptr); }, &ms);
-                if (rc) return rc;
-                if (ms < best_ms) {
-                  best_ms = ms;
-                  best.trace_kernel = pk;
-                  best.trace_blocks = b;
-                  best.trace_steps = st;
-                  best.trace_min_refill = rf;
-                  best.dda_batch = db;
-                  best.trace_waves = wv;
-                }
-              }
-        }
+    // the persistent ray queue over its launch shape
+    c.trace_kernel = 5;
+    for (uint32_t db : {1u, 16u})
+      for (uint32_t wv : {0u, 6u})
+        for (uint32_t b : {1024u, 2048u, 4096u, 8192u})
+          for (uint32_t st : {1u, 4u, 8u, 16u})
+            for (uint32_t rf : {1u, 16u, 32u}) {
+              c.dda_batch = db;
+              c.trace_waves = wv;
+              c.trace_blocks = b;
+              c.trace_steps = st;
+              c.trace_min_refill = rf;
+              if ((rc = try_trace(c))) return rc;
+            }
   }
   if (u != nullptr) {
     // path tracer on the given view (its size, pose and bounce count)
@@ -1170,140 +1160,70 @@ int vrt_autotune(vrt_ctx* ctx, const vrt_scene* scene, uint64_t nrays, const vrt
     HIP_TRY(ctx, alb.alloc(px * 4));
     HIP_TRY(ctx, gb.alloc(px * 16));
     auto launch = [&] { return ctx->launch_pathtrace(s, *u, 48.0f, 1, rad.ptr, alb.ptr, gb.ptr); };
-    ctx->tuning = best;
-    ctx->tuning.pt_kernel = 0;
-    ctx->tuning.pt_waves = 0;
-    float ms = 0;
-    rc = time_it(launch, &ms);
-    if (rc) return rc;
-    float best_ms = ms;
-    best.pt_kernel = 0;
-    best.pt_waves = 0;
-    // one-pixel-per-lane variants x register budgets (waves per SIMD)
-    for (uint32_t k : {0u, 2u, 3u, 4u, 5u, 7u})
-      for (uint32_t wv : {0u, 4u, 5u}) {
-        if (k == 0 && wv == 0) continue;  // measured above
-        ctx->tuning.pt_kernel = k;
-        ctx->tuning.pt_waves = wv;
-        rc = time_it(launch, &ms);
-        if (rc) return rc;
-        if (ms < best_ms) {
-          best_ms = ms;
-          best.pt_kernel = k;
-          best.pt_waves = wv;
-        }
-      }
-    // tile order of the best one-pixel-per-lane variant (costly tiles last
-    // leave a tail of under-filled CUs)
-    ctx->tuning.pt_kernel = best.pt_kernel;
-    ctx->tuning.pt_waves = best.pt_waves;
-    for (uint32_t o : {0u, 1u, 2u, 3u}) {
-      if (o == best.pt_order) continue;  // measured above
-      ctx->tuning.pt_order = o;
-      rc = time_it(launch, &ms);
-      if (rc) return rc;
-      // another order must win clearly: the tuning view is one pose, and the
-      // default (costly bottom rows first) is the better one over a camera
-      // path unless the pose says otherwise by more than the noise
-      if (ms < best_ms * 0.98f) {
+    float best_ms = 1e30f;
+    // margin < 1: a candidate must beat the best by that factor (tile orders:
+    // the tuning view is one pose, and the default order -- costly bottom
+    // rows first -- is the better one over a camera path unless the pose
+    // says otherwise by more than the noise)
+    auto try_pt = [&](const vrt_tuning& cand, float margin) -> int {
+      ctx->tuning = cand;
+      float ms = 0;
+      const int r = time_it(launch, &ms);
+      if (r) return r;
+      if (ms < best_ms * margin) {
         best_ms = ms;
-        best.pt_order = o;
+        best = cand;
       }
-    }
-    ctx->tuning.pt_order = best.pt_order;
-    // the global-memory variant on 8x8 tiles of one wave; refill tiles
-    // (32x32, 32x16) of the single-loop variants
+      return VRT_OK;
+    };
+    vrt_tuning c = best;
+    c.pt_tile = 0;
+    // one-pixel-per-lane variants x register budgets (waves per SIMD)
+    for (uint32_t k : {0u, 4u, 7u})
+      for (uint32_t wv : {0u, 4u, 5u}) {
+        c.pt_kernel = k;
+        c.pt_waves = wv;
+        if ((rc = try_pt(c, 1.0f))) return rc;
+      }
+    // the global-memory variant on 8x8 tiles of one wave; refill tiles of
+    // the single-loop variants (32x32, 32x16; 16x16, 16x8 on one-wave blocks)
     for (uint32_t k : {4u, 7u})
       for (uint32_t form : {1u, 2u, 3u, 4u, 5u})
         for (uint32_t wv : {0u, 4u, 5u}) {
           if ((form == 1u || form >= 4u) && k != 7u) continue;
-          ctx->tuning.pt_kernel = k;
-          ctx->tuning.pt_waves = wv;
-          ctx->tuning.pt_tile = form;
-          rc = time_it(launch, &ms);
-          if (rc) return rc;
-          if (ms < best_ms) {
-            best_ms = ms;
-            best.pt_kernel = k;
-            best.pt_waves = wv;
-            best.pt_tile = form;
-          }
+          c.pt_kernel = k;
+          c.pt_waves = wv;
+          c.pt_tile = form;
+          if ((rc = try_pt(c, 1.0f))) return rc;
         }
-    ctx->tuning.pt_tile = best.pt_tile;
-    // tile order again for the chosen tile form (refill tiles finish in a
-    // different order than one-pixel-per-lane tiles)
-    ctx->tuning.pt_kernel = best.pt_kernel;
-    ctx->tuning.pt_waves = best.pt_waves;
+    // tile order of the best of these
+    c = best;
     for (uint32_t o : {0u, 1u, 2u, 3u}) {
-      if (o == best.pt_order) continue;  // measured above
-      ctx->tuning.pt_order = o;
-      rc = time_it(launch, &ms);
-      if (rc) return rc;
-      // another order must win clearly: the tuning view is one pose, and the
-      // default (costly bottom rows first) is the better one over a camera
-      // path unless the pose says otherwise by more than the noise
-      if (ms < best_ms * 0.98f) {
-        best_ms = ms;
-        best.pt_order = o;
-      }
+      if (o == best.pt_order) continue;
+      c.pt_order = o;
+      if ((rc = try_pt(c, 0.98f))) return rc;
     }
-    ctx->tuning.pt_order = best.pt_order;
-    const uint32_t blocks[] = {512, 1024, 2048};
-    const uint32_t steps[] = {4, 8, 16};
-    const uint32_t refill[] = {16, 32, 48};
-    const uint32_t chunks[] = {64, 256};
-    for (uint32_t b : blocks)
-      for (uint32_t st : steps)
-        for (uint32_t rf : refill)
-          for (uint32_t ch : chunks) {
-            ctx->tuning.pt_kernel = 1;
-            ctx->tuning.pt_blocks = b;
-            ctx->tuning.pt_steps = st;
-            ctx->tuning.pt_min_refill = rf;
-            ctx->tuning.pt_chunk = ch;
-            rc = time_it(launch, &ms);
-            if (rc) return rc;
-            if (ms < best_ms) {
-              best_ms = ms;
-              best.pt_kernel = 1;
-              best.pt_blocks = b;
-              best.pt_steps = st;
-              best.pt_min_refill = rf;
-              best.pt_chunk = ch;
-            }
-          }
-    // persistent with the single-loop DDA: register budget x launch shape x
-    // brick-entry batch
-    for (uint32_t wv : {0u, 4u})
-      for (uint32_t b : {1024u, 2048u})
-        for (uint32_t st : {4u, 8u, 16u})
-          for (uint32_t rf : {16u, 32u})
-            for (uint32_t db : {1u, 8u}) {
-              ctx->tuning.pt_kernel = 6;
-              ctx->tuning.pt_waves = wv;
-              ctx->tuning.pt_blocks = b;
-              ctx->tuning.pt_steps = st;
-              ctx->tuning.pt_min_refill = rf;
-              ctx->tuning.pt_chunk = 64;
-              ctx->tuning.dda_batch = db;
-              rc = time_it(launch, &ms);
-              if (rc) return rc;
-              if (ms < best_ms) {
-                best_ms = ms;
-                best.pt_kernel = 6;
-                best.pt_waves = wv;
-                best.pt_blocks = b;
-                best.pt_steps = st;
-                best.pt_min_refill = rf;
-                best.pt_chunk = 64;
-                best.dda_batch = db;
-              }
-            }
+    // step-level refill: level-switch and shading thresholds, DDA memory
+    c = best;
+    c.pt_kernel = 8;
+    c.pt_blocks = 0;
+    c.pt_waves = 4;
+    c.pt_order = 2;
+    for (uint32_t form : {0u, 1u})
+      for (uint32_t mw : {12u, 16u, 24u})
+        for (uint32_t sm : {24u, 32u}) {
+          c.pt_tile = form;
+          c.pt_min_refill = mw;
+          c.pt_shade_min = sm;
+          if ((rc = try_pt(c, 1.0f))) return rc;
+        }
+    if (best.pt_kernel == 8) {
+      c = best;
+      c.pt_order = 0;
+      if ((rc = try_pt(c, 0.98f))) return rc;
+    }
   }
-  ctx->tuning = best;
-  // the sweep's rays do not count in the statistics
-  HIP_TRY(ctx, hipMemcpyAsync(ctx->d_counters.ptr, before, sizeof before, hipMemcpyHostToDevice, ctx->stream));
-  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  restore.keep = best;
   if (chosen) *chosen = best;
   return VRT_OK;
   GUARD_END(ctx)
@@ -1352,6 +1272,21 @@ int vrt_autotune_batch(vrt_ctx* ctx, const vrt_view* views, uint32_t nviews, uin
   int rc = ctx->counters_to_host(before);
   if (rc) return rc;
   const vrt_tuning start = ctx->tuning;
+  // On every return: the sweep's rays leave the statistics, the graphs it
+  // captured are dropped (recaptured on first use), and the tuning is the
+  // one in effect before the call unless the sweep completed.
+  struct Restore {
+    vrt_ctx* c;
+    const uint64_t* cnt;
+    vrt_tuning keep;
+    ~Restore() {
+      c->tuning = keep;
+      if (hipMemcpyAsync(c->d_counters.ptr, cnt, 4 * sizeof(uint64_t), hipMemcpyHostToDevice, c->stream) ==
+          hipSuccess)
+        (void)hipStreamSynchronize(c->stream);
+      c->lane0.drop_graphs();
+    }
+  } restore{ctx, before, start};
   vrt_tuning best = start;
   float best_ms = 1e30f;
   auto time_batch = [&](float* out_ms) -> int {
@@ -1377,25 +1312,24 @@ int vrt_autotune_batch(vrt_ctx* ctx, const vrt_view* views, uint32_t nviews, uin
         if (!(modes >> mode & 1u) || (mode == 0u && b != 0u)) continue;
         ctx->tuning = start;
         ctx->tuning.batch_mode = mode;
-        ctx->tuning.pt_tile = form;
+        // pt 8 keeps its own pt_tile (the single-view kernel's DDA memory)
+        if (start.pt_kernel == 8) {
+          if (form == 0u) continue;  // batch_tile 0 is form 5
+          ctx->tuning.batch_tile = form;
+        } else {
+          ctx->tuning.pt_tile = form;
+        }
         // half or twice the default grid
         ctx->tuning.batch_blocks = b == 0u ? 0u : cus * (form >= 4u ? 32u : 8u) * (b == 1u ? 1u : 4u) / 2u;
         float ms = 0;
         rc = time_batch(&ms);
-        if (rc) {
-          ctx->tuning = start;
-          return rc;
-        }
+        if (rc) return rc;
         if (ms < best_ms) {
           best_ms = ms;
           best = ctx->tuning;
         }
       }
-  ctx->tuning = best;
-  // the sweep's rays do not count in the statistics
-  HIP_TRY(ctx, hipMemcpyAsync(ctx->d_counters.ptr, before, sizeof before, hipMemcpyHostToDevice, ctx->stream));
-  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-  ctx->lane0.drop_graphs();  // captured by the sweep; recaptured on first use
+  restore.keep = best;
   if (chosen) *chosen = best;
   return VRT_OK;
   GUARD_END(ctx)
@@ -1543,9 +1477,15 @@ int vrt_ctx::render_batch(vrt::BatchLane& lane, const vrt_view* views, uint32_t 
     any_hdr |= d[k].hdr != nullptr;
     tiles += d[k].tiles_x * ((vs.h + 15u) / 16u);
   }
-  // refill forms: the path tracer's own 16 x rth tiles, numbered per view
-  const int ptv = pt_variant(tuning.pt_kernel, tuning.empty_skip);
-  const int rform = vrt::refill_form(ptv, (int)tuning.pt_tile, max_bit_words, max_pwords);
+  // refill forms: the path tracer's own 16 x rth tiles, numbered per view.
+  // The step-level refill kernel (pt 8) has no batched form: its batches run
+  // pt 7 with 4 waves per SIMD on the tile form batch_tile (0 = one-wave
+  // blocks of 16x8 refill tiles).
+  const bool wave_pt = tuning.pt_kernel == 8;
+  const int ptv = pt_variant(tuning.pt_kernel);
+  const int pt_waves = wave_pt ? 4 : (int)tuning.pt_waves;
+  const int tile = wave_pt ? (tuning.batch_tile ? (int)tuning.batch_tile : 5) : (int)tuning.pt_tile;
+  const int rform = vrt::refill_form(ptv, tile, max_bit_words, max_pwords);
   uint32_t rtiles = 0;
   if (rform != 0) {
     uint32_t tw = 0, th = 0;
@@ -1568,7 +1508,7 @@ int vrt_ctx::render_batch(vrt::BatchLane& lane, const vrt_view* views, uint32_t 
   HIP_TRY(this, hipEventRecord(lane.desc_done[stage], st));
   HIP_TRY(this, hipEventRecord(ev[0], st));
   if (tuning.batch_mode == 1 || tuning.batch_mode == 2) {
-    const int tv = ptv, wv = (int)tuning.pt_waves;
+    const int tv = ptv, wv = pt_waves;
     // one-wave blocks for the 16-wide refill tiles: four times as many for the same waves
     const uint32_t ptb = tuning.batch_blocks ? tuning.batch_blocks : (uint32_t)cus * (rform >= 4 ? 32u : 8u);
     const uint32_t gsb = (uint32_t)cus * 8u;
@@ -1609,7 +1549,7 @@ int vrt_ctx::render_batch(vrt::BatchLane& lane, const vrt_view* views, uint32_t 
     return VRT_OK;
   }
   HIP_TRY(this, vrt::launch_pathtrace_batch(st, dd, nviews, tiles, (unsigned long long*)d_counters.ptr,
-                                            ptv, (int)tuning.pt_waves, max_bit_words, max_pwords, rform, rtiles));
+                                            ptv, pt_waves, max_bit_words, max_pwords, rform, rtiles));
   HIP_TRY(this, hipEventRecord(ev[1], st));
   HIP_TRY(this, vrt::launch_temporal_batch(st, dd, nviews, tiles));
   HIP_TRY(this, hipEventRecord(ev[2], st));

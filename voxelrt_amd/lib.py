@@ -83,14 +83,15 @@ class Stats(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("trace_kernel", C.c_uint32), ("trace_blocks", C.c_uint32),
                 ("trace_steps", C.c_uint32), ("trace_min_refill", C.c_uint32),
-                ("pt_kernel", C.c_uint32), ("pt_blocks", C.c_uint32), ("pt_steps", C.c_uint32),
-                ("pt_min_refill", C.c_uint32), ("pt_chunk", C.c_uint32), ("pt_waves", C.c_uint32),
+                ("pt_kernel", C.c_uint32), ("pt_blocks", C.c_uint32), ("unused0", C.c_uint32),
+                ("pt_min_refill", C.c_uint32), ("unused1", C.c_uint32), ("pt_waves", C.c_uint32),
                 ("batch_mode", C.c_uint32), ("batch_blocks", C.c_uint32), ("dda_batch", C.c_uint32),
                 ("empty_skip", C.c_uint32), ("pt_order", C.c_uint32), ("pt_tile", C.c_uint32),
-                ("trace_waves", C.c_uint32), ("reserved", C.c_uint32 * 3)]
+                ("trace_waves", C.c_uint32), ("pt_shade_min", C.c_uint32), ("batch_tile", C.c_uint32),
+                ("reserved", C.c_uint32)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+        return {k: getattr(self, k) for k, _ in self._fields_ if k not in ("reserved", "unused0", "unused1")}
 
 
 class Request(C.Structure):
@@ -382,6 +383,11 @@ class Context:
         returns without waiting."""
         self._check(self.L.vrt_trace_rays_device_noalloc(self.h, scene.handle, C.c_void_p(d_rays), n,
                                                          tmax, C.c_void_p(d_hits)))
+
+    def copy_device(self, dst: "DeviceArray", src: "DeviceArray", nbytes: int = 0):
+        """Device-to-device copy (vrt_copy), waits for it."""
+        n = nbytes or min(dst.nbytes, src.nbytes)
+        self._check(self.L.vrt_copy(self.h, C.c_void_p(dst.ptr), C.c_void_p(src.ptr), n, 3))
 
     def device_array(self, nbytes: int) -> "DeviceArray":
         """A device buffer owned by this context (vrt_device_alloc)."""
