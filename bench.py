@@ -11,10 +11,18 @@ Prints one JSON line:
   roofline     the path-trace kernel (dominant): algorithmic bytes per launch
                (4 B per brick step + 4 B per voxel step + 4 B per hit voxel +
                28 B of stores per pixel) / its HIP-event duration, vs 8 TB/s
+  batched      the batched form (64 views of 8 scenes in one call per pass)
+               vs the same views rendered one call each (--workload batch)
+  served       the served form (continuous batching of a request stream over
+               16 scenes up to 1024^3): saturated Mrays/s vs serial plain
+               rendering, p50/p99 frame latency at 0.8 x plain capacity
+               (--workload serve)
   cpu_baseline the CPU oracle on --cpu-frames (8) frames of the same orbit
-               (all host cores, OpenMP over rows)
+               (all host cores, OpenMP over rows); beside it the oracle on one
+               core and the reference's VoxelBench trace() on the traversal
+               workload
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-forms]
                 [--workload frame_1080|traverse|batch|serve|profile]
 """
 from __future__ import annotations
@@ -59,16 +67,13 @@ def _pt_kernel_name(t):
     return "k_pathtrace"
 
 
-def cpu_baseline(vox, frames, w, h, fclamp, desc):
-    """The CPU oracle on `frames` frames of the workload (all host cores)."""
-    import numpy as np  # noqa: F401
-
+def _oracle_frames(g, frames, w, h, fclamp, desc, threads):
+    """Mrays/s and ms/frame of the CPU oracle chain on the first `frames`
+    frames of the orbit."""
     import oracle
     from tests.chain import OracleChain
     from voxelrt_amd import configs as cf
 
-    threads = min(os.cpu_count() or 1, 64)
-    g = oracle.OracleGrid(vox)
     chain = OracleChain(g, w, h, 0, threads=threads)
     prev_vp = prev_pos = None
     rays = 0
@@ -81,10 +86,34 @@ def cpu_baseline(vox, frames, w, h, fclamp, desc):
         rays += out["counters"].rays
         prev_vp, prev_pos = vp, pos
     dt = time.perf_counter() - t0
-    return {"value": round(rays / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "ms_per_frame": round(dt / frames * 1e3, 1),
-            "sample": f"{frames} frames of the 1920x1080 orbit (pathtrace + temporal + 3 a-trous), "
-                      f"oracle C++ FP32, OpenMP over rows"}
+    return rays / dt / 1e6, dt / frames * 1e3
+
+
+def cpu_baseline(vox, frames, w, h, fclamp, desc):
+    """CPU baselines of the workload: the oracle chain on `frames` frames on
+    all host cores (the headline figure) and on one frame on one core, and
+    the reference's own VoxelBench brick DDA, trace() (core.cpp:52-130,
+    compiled from its sources into oracle/_ref), on 2^20 of the 2^24-ray
+    traversal workload at p = 0.25 on all host cores."""
+    import oracle
+    from voxelrt_amd import configs as cf
+
+    threads = min(os.cpu_count() or 1, 64)
+    g = oracle.OracleGrid(vox)
+    mrays, ms = _oracle_frames(g, frames, w, h, fclamp, desc, threads)
+    out = {"value": round(mrays, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+           "ms_per_frame": round(ms, 1),
+           "sample": f"{frames} frames of the 1920x1080 orbit (pathtrace + temporal + 3 a-trous), "
+                     f"oracle C++ FP32, OpenMP over rows"}
+    mrays1, ms1 = _oracle_frames(g, 1, w, h, fclamp, desc, 1)
+    out["one_core"] = {"value": round(mrays1, 3), "unit": "Mrays/s", "cores": 1, "kind": "port",
+                       "ms_per_frame": round(ms1, 1), "sample": "frame 0 of the same orbit, one thread"}
+    rays = cf.traverse_rays(1 << 24, seed=77)
+    tr = _traverse_cpu(cf.traverse_scene(0.25), rays, 0.25, sample=1 << 20, one_core=False)
+    for b in tr["cpu_baseline"]:
+        if b["kind"] == "reference":
+            out["reference_trace"] = dict(b, workload="traversal 256^3, p = 0.25")
+    return out
 
 
 def traverse(args):
@@ -178,7 +207,7 @@ def profile(args):
     ctx.close()
 
 
-def _traverse_cpu(vox, rays, p, sample=1 << 20):
+def _traverse_cpu(vox, rays, p, sample=1 << 20, one_core=True):
     """CPU baselines on the first `sample` rays of the same workload: the
     oracle's FP32 DDA (all host cores and one core) and the reference's own
     VoxelBench brick DDA (core.cpp:52-130, double t, 4^3 bricks) compiled from
@@ -192,7 +221,7 @@ def _traverse_cpu(vox, rays, p, sample=1 << 20):
     cores = os.cpu_count() or 1
     out = {"metric": "traversal CPU baseline Mrays/s (256^3)", "p": p, "sample": f"first {len(r)} of the 2^24 rays",
            "cpu_baseline": []}
-    for threads in (cores, 1):
+    for threads in ((cores, 1) if one_core else (cores,)):
         rr = r if threads > 1 else r[: len(r) // 8]
         t0 = time.perf_counter()
         oracle.trace_rays(g, rr, threads=threads)
@@ -248,7 +277,7 @@ def _traverse_one(args, ctx, sc, p, name, d_rays, d_hits, n_rays, tuning):
                                    "alg_bytes_per_launch": int(alg)}}), flush=True)
 
 
-def batch(args):
+def batch(args, emit=True):
     """Batched form: 8 scenes of {64,128,256,512}^3 and 64 views of uneven
     size (256x256 .. 1920x1080, aspect 1:1 .. 21:9) and bounce count (1..6),
     temporal + a-trous on.  'plain' renders the views one call each (one
@@ -286,12 +315,14 @@ def batch(args):
     # batched form: launch mode, tile form and persistent grid timed on the
     # batch itself (its own view states; outside the timed region)
     tuned_b = ctx.autotune_batch(views_for(0, 90000)).as_dict()
-    # plain per-view baseline: the tile form that renders the 64 views one by
-    # one fastest (the single-view autotuner saw one view only)
-    plain_ms = {}
-    for form in (0, 2, 3, 4, 5):
-        ctx.set_tuning(**dict(tuned, pt_tile=form))
-        vs = views_for(0, 80000 + 100 * form)
+    # plain per-view baseline: whichever of the single-view autotuner's choice
+    # (it saw one view only) and the refill-tile kernels renders the 64 views
+    # one by one fastest
+    cands = [tuned] + [dict(tuned, pt_kernel=7, pt_waves=4, pt_tile=f) for f in (3, 5)]
+    plain_ms = []
+    for c, cand in enumerate(cands):
+        ctx.set_tuning(**cand)
+        vs = views_for(0, 80000 + 100 * c)
         for view in vs:
             ctx.render([view])
         ctx.stats()
@@ -299,10 +330,10 @@ def batch(args):
         for view in vs:
             ctx.render([view])
         ctx.stats()
-        plain_ms[form] = time.perf_counter() - t0
+        plain_ms.append(time.perf_counter() - t0)
         for view in vs:
             ctx.release_view(view.view_id)
-    tuned_p = dict(tuned, pt_tile=min(plain_ms, key=plain_ms.get))
+    tuned_p = cands[min(range(len(cands)), key=plain_ms.__getitem__)]
     modes = {"plain": 0, "batched": 0, "batched_queue": 1, "batched_graph": 2}
     for i, (mode, bm) in enumerate(modes.items()):
         ctx.set_tuning(**(tuned_p if mode == "plain" else dict(tuned_b, batch_mode=bm)))
@@ -335,13 +366,16 @@ def batch(args):
            "speedup": round(results[best]["mrays_s"] / max(results["plain"]["mrays_s"], 1e-9), 2),
            "tuning": {k: tuned[k] for k in ("pt_kernel", "pt_waves", "pt_tile")},
            "tuning_plain": {k: tuned_p[k] for k in ("pt_kernel", "pt_waves", "pt_tile")},
-           "tuning_batched": {k: tuned_b[k] for k in ("batch_mode", "pt_tile", "batch_blocks")},
+           "tuning_batched": {k: tuned_b[k] for k in ("batch_mode", "pt_kernel", "pt_tile", "batch_tile",
+                                                      "batch_blocks")},
            "views": len(specs), "pixels_per_batch": pixels, "scene_upload_s": round(upload_s, 2),
            "steps": args.steps, "warmup": args.warmup}
-    print(json.dumps(out), flush=True)
     for sc in scenes:
         sc.release()
     ctx.close()
+    if emit:
+        print(json.dumps(out), flush=True)
+    return out
 
 
 def _pct(xs, q):
@@ -349,7 +383,7 @@ def _pct(xs, q):
     return round(float(np.percentile(np.asarray(xs, dtype=np.float64), q)), 3) if len(xs) else None
 
 
-def serve(args):
+def serve(args, emit=True):
     """Served form vs plain serial rendering on one arrival trace.
 
     16 scenes (64^3 .. 1024^3), 32 camera streams (360p .. 1440p, spp 1..4,
@@ -479,10 +513,13 @@ def serve(args):
            "camera_streams": len(cams), "scene_upload_s": round(upload_s, 2),
            "serve_config": {"streams": args.serve_streams, "max_batch_views": args.serve_views,
                             "max_batch_pixels": args.serve_pixels},
-           "tuning_served": {k: served_tuning[k] for k in ("batch_mode", "pt_tile", "batch_blocks")},
+           "tuning_served": {k: served_tuning[k] for k in ("batch_mode", "pt_kernel", "pt_tile", "batch_tile",
+                                                           "batch_blocks")},
            "tuning": {k: tuned[k] for k in ("pt_kernel", "pt_waves", "pt_tile")}}
-    print(json.dumps(out), flush=True)
     ctx.close()
+    if emit:
+        print(json.dumps(out), flush=True)
+    return out
 
 
 def main():
@@ -491,6 +528,7 @@ def main():
     ap.add_argument("--steps", type=int, default=32)
     ap.add_argument("--warmup", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-forms", action="store_true", help="frame_1080: leave out the batched and served forms")
     ap.add_argument("--cpu-frames", type=int, default=8)
     ap.add_argument("--workload", default="frame_1080", choices=["frame_1080", "traverse", "batch", "serve", "profile"])
     ap.add_argument("--pt-order", type=int, default=2, help="profile workload: path-tracer tile order")
@@ -600,6 +638,28 @@ def main():
     }
     sc.release()
     ctx.close()
+    if not args.no_forms:
+        # the batched and served forms on their own workloads (a few seconds
+        # each), summarised; --workload batch / serve print them in full
+        ns = argparse.Namespace(**vars(args))
+        ns.steps, ns.warmup = 8, 2
+        b = batch(ns, emit=False)
+        out["batched"] = {"value": b["value"], "unit": "Mrays/s", "plain_mrays_s": b["plain"]["mrays_s"],
+                          "speedup_vs_plain": b["speedup"], "best_mode": b["best_mode"],
+                          "ms_per_batch": b[b["best_mode"]]["ms_per_batch"],
+                          "plain_ms_per_batch": b["plain"]["ms_per_batch"],
+                          "workload": f"{b['views']} views of 8 scenes (64^3..512^3), 256x256..1920x1080, "
+                                      f"bounces 1..6, temporal + a-trous, {b['steps']} timed batches"}
+        sv = serve(ns, emit=False)
+        runs = sv["runs"]
+        out["served"] = {"value": sv["value"], "unit": "Mrays/s",
+                         "saturated_speedup_vs_plain": sv["speedup_saturated"],
+                         "p50_ms": runs["served_poisson"]["p50_ms"], "p99_ms": runs["served_poisson"]["p99_ms"],
+                         "plain_p50_ms": runs["plain_poisson"]["p50_ms"],
+                         "plain_p99_ms": runs["plain_poisson"]["p99_ms"],
+                         "load_vs_plain": sv["load_vs_plain"], "poisson_rate_per_s": sv["poisson_rate_per_s"],
+                         "workload": f"{sv['requests']} requests over {sv['camera_streams']} camera streams of "
+                                     f"{sv['scenes']} scenes (64^3..1024^3), 360p..1440p, spp 1..4, bounces 1..4"}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(vox, args.cpu_frames, w, h, p["fclamp"], desc)
     print(json.dumps(out), flush=True)
