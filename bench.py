@@ -135,8 +135,11 @@ def traverse(args):
                 "lane_fast": dict(trace_kernel=4, dda_batch=1),
                 "lane_fast_b16": dict(trace_kernel=4, dda_batch=16),
                 "lane_fast_b16_w6": dict(trace_kernel=4, dda_batch=16, trace_waves=6),
-                "persistent_fast_b16": dict(PROFILED_TRAVERSE),
-                "persistent_fast_b16_w6": dict(PROFILED_TRAVERSE, trace_waves=6)}
+                "persistent_fast_b16": dict(trace_kernel=5, trace_blocks=1024, trace_steps=8, trace_min_refill=16,
+                                            dda_batch=16),
+                "wave_m16_f32": dict(PROFILED_TRAVERSE),
+                "wave_m16_f32_768": dict(PROFILED_TRAVERSE, trace_blocks=768),
+                "wave_m16_f32_w6": dict(PROFILED_TRAVERSE, trace_waves=6)}
     # every variant from the same base (an autotuned tuning of the previous
     # scene must not leak into the next scene's fixed variants)
     base = dict(trace_waves=0, empty_skip=0, dda_batch=1)
@@ -145,8 +148,7 @@ def traverse(args):
         sc = ctx.upload(vox)
         if not args.no_cpu:
             print(json.dumps(_traverse_cpu(vox, rays, p)), flush=True)
-        for name in ("lane", "lane_fast", "lane_fast_b16", "lane_fast_b16_w6", "persistent_fast_b16",
-                     "persistent_fast_b16_w6", "autotuned"):
+        for name in list(variants) + ["autotuned"]:
             if name == "autotuned":
                 tuned = ctx.autotune(sc, 1 << 22).as_dict()
             else:
@@ -163,8 +165,8 @@ def profile(args):
     given by --pt-kernel / --pt-waves / --pt-tile / --pt-order /
     --pt-min-refill / --pt-blocks; (2) for each --traverse-p occupancy, two
     launches over 2^24 rays of the traversal variant --trace-kernel
-    (--dda-batch, 1024 blocks / 8 steps / refill at 16 for the persistent
-    kernels), rays and hits resident in HBM; (3) two device-to-device copies
+    (--dda-batch, 1024 blocks, --trace-steps / --trace-min-refill for the
+    persistent kernels), rays and hits resident in HBM; (3) two device-to-device copies
     of 256 MiB (vrt_copy), the calibration for FETCH_SIZE / WRITE_SIZE."""
     import numpy as np
 
@@ -191,8 +193,8 @@ def profile(args):
         rays = cf.traverse_rays(n, seed=77)
         d_rays = ctx.device_array(rays.nbytes).upload(rays)
         d_hits = ctx.device_array(n * 16)
-        ctx.set_tuning(trace_kernel=args.trace_kernel, trace_blocks=1024, trace_steps=8, trace_min_refill=16,
-                       dda_batch=args.dda_batch)
+        ctx.set_tuning(trace_kernel=args.trace_kernel, trace_blocks=1024, trace_steps=args.trace_steps,
+                       trace_min_refill=args.trace_min_refill, dda_batch=args.dda_batch)
         for occ in args.traverse_p:
             sc2 = ctx.upload(cf.traverse_scene(occ))
             for _ in range(2):
@@ -242,7 +244,7 @@ def _traverse_cpu(vox, rays, p, sample=1 << 20, one_core=True):
 
 
 # the traversal variant whose HBM traffic profiles/traverse_pmc.txt holds
-PROFILED_TRAVERSE = dict(trace_kernel=5, trace_blocks=1024, trace_steps=8, trace_min_refill=16, dda_batch=16)
+PROFILED_TRAVERSE = dict(trace_kernel=7, trace_blocks=1024, trace_steps=32, trace_min_refill=16)
 
 
 def _traverse_one(args, ctx, sc, p, name, d_rays, d_hits, n_rays, tuning):
@@ -269,10 +271,11 @@ def _traverse_one(args, ctx, sc, p, name, d_rays, d_hits, n_rays, tuning):
                       "kernel_ms": round(kms, 4), "wall_ms_per_launch": round(wall / K * 1e3, 4),
                       "brick_steps_per_ray": round(bs / n_rays, 3),
                       "voxel_steps_per_ray": round(vs / n_rays, 3), "hit_frac": round(hits / n_rays, 4),
-                      "roofline": {"bound": "hbm", "kernel": "k_trace_rays", "achieved": round(gbs, 1),
+                      "roofline": {"bound": "hbm", "kernel": {5: "k_trace_persistent", 7: "k_trace_wave"}.get(
+                                       tuning.get("trace_kernel"), "k_trace_rays"), "achieved": round(gbs, 1),
                                    "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                    "frac": round(gbs / HBM_PEAK_GBS, 4),
-                                   "traffic": _pmc_traffic("traverse_pmc.txt", f"trace_kernel=5,p={p}")
+                                   "traffic": _pmc_traffic("traverse_pmc.txt", f"trace_kernel=7,p={p}")
                                    if tuning == PROFILED_TRAVERSE else None,
                                    "alg_bytes_per_launch": int(alg)}}), flush=True)
 
@@ -532,8 +535,12 @@ def main():
     ap.add_argument("--cpu-frames", type=int, default=8)
     ap.add_argument("--workload", default="frame_1080", choices=["frame_1080", "traverse", "batch", "serve", "profile"])
     ap.add_argument("--pt-order", type=int, default=2, help="profile workload: path-tracer tile order")
-    ap.add_argument("--trace-kernel", type=int, default=4, help="profile workload: traversal variant")
+    ap.add_argument("--trace-kernel", type=int, default=7, help="profile workload: traversal variant")
     ap.add_argument("--dda-batch", type=int, default=1, help="profile workload: brick-entry batch")
+    ap.add_argument("--trace-steps", type=int, default=32,
+                    help="profile workload: trace 5 steps per refill check, trace 7 finished rays per pass")
+    ap.add_argument("--trace-min-refill", type=int, default=16,
+                    help="profile workload: trace 5 idle lanes per refill, trace 7 lanes waiting per level switch")
     ap.add_argument("--traverse-p", type=float, nargs="*", default=[], help="profile workload: occupancies")
     ap.add_argument("--requests", type=int, default=256)
     ap.add_argument("--load", type=float, default=0.8, help="Poisson rate as a fraction of plain capacity")

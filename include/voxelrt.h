@@ -212,7 +212,7 @@ int vrt_trace_rays_device(vrt_ctx* ctx, const vrt_scene* scene,
  * capture, tight serving loops, harnesses that forbid hipMalloc.  The same
  * kernels and the same hits as vrt_trace_rays_device with the context's
  * tuning; their scratch (the ray / step / hit counters and the persistent
- * kernels' ray-queue head) is a fixed 256-byte device array inside the
+ * kernels' ray queue) is a fixed 1280-byte device array inside the
  * library, one per GPU, held until the process ends and cleared by a kernel
  * at the start of each call (no hipMalloc, no hipFree, no memset node).
  * n is at most VRT_NOALLOC_MAX_RAYS (the queue head is 32-bit); larger n is
@@ -275,14 +275,24 @@ typedef struct vrt_tuning {
   uint32_t trace_kernel;      /* traversal: 0 = one ray per lane, nested brick / voxel
                                  loops on global memory,
                                  4 = one ray per lane, single loop, both levels in LDS,
-                                 5 = persistent ray queue, single loop in LDS (default),
+                                 5 = persistent ray queue, single loop in LDS,
                                  6 = one ray per lane, single loop on global memory
                                      (the form 4 and 5 fall back to when their LDS
-                                     does not fit, 512^3 and up).
+                                     does not fit, 512^3 and up),
+                                 7 = step-level refill on persistent waves (default):
+                                     lanes wait at brick boundaries and at finished
+                                     rays; the wave switches levels for
+                                     trace_min_refill waiting lanes and writes
+                                     records / starts rays for trace_steps finished
+                                     lanes at once; rays come from a queue split over
+                                     eight counters; the DDA in LDS where it fits,
+                                     else on global memory.
                                  1, 2 and 3 are rejected. */
-  uint32_t trace_blocks;      /* trace 5: persistent grid (256-thread blocks), 0 = 1 */
-  uint32_t trace_steps;       /* trace 5: DDA iterations between queue refill checks */
-  uint32_t trace_min_refill;  /* trace 5: idle lanes (of 64) that trigger a refill */
+  uint32_t trace_blocks;      /* trace 5 and 7: persistent grid (256-thread blocks), 0 = 1 */
+  uint32_t trace_steps;       /* trace 5: DDA iterations between queue refill checks;
+                                 trace 7: finished lanes (of 64) collected per pass */
+  uint32_t trace_min_refill;  /* trace 5: idle lanes (of 64) that trigger a refill;
+                                 trace 7: lanes waiting at a brick boundary per pass */
   uint32_t pt_kernel;         /* path tracer:
                                  0 = one pixel per lane, nested loops on global memory,
                                  4 = one pixel per lane, single loop, both levels in LDS,
@@ -333,7 +343,7 @@ typedef struct vrt_tuning {
                                  16x16 / 16x8 tiles (pt 7).
                                  pt 8: 0 = DDA in LDS where it fits, 1 = DDA on
                                  global memory */
-  uint32_t trace_waves;       /* traversal kernels 4, 5 and 6: minimum waves per SIMD
+  uint32_t trace_waves;       /* traversal kernels 4 to 7: minimum waves per SIMD
                                  the register allocation must allow: 0 = compiler's
                                  choice, 6 (pays on sparse scenes) */
   uint32_t pt_shade_min;      /* pt 8: finished rays (of 64) a wave collects before it

@@ -93,6 +93,9 @@ TRACE_VARIANTS = {
     "lane-fast-global-w6": dict(trace_kernel=6, trace_waves=6),
     "persist-fast-b16-w6": dict(trace_kernel=5, trace_blocks=1024, trace_steps=8, trace_min_refill=16, dda_batch=16,
                                 trace_waves=6),
+    "wave-m16-f32": dict(trace_kernel=7, trace_blocks=1024, trace_min_refill=16, trace_steps=32),
+    "wave-m1-f1-333": dict(trace_kernel=7, trace_blocks=333, trace_min_refill=1, trace_steps=1),
+    "wave-m64-f64-w6": dict(trace_kernel=7, trace_blocks=2048, trace_min_refill=64, trace_steps=64, trace_waves=6),
 }
 
 
@@ -197,7 +200,7 @@ def test_autotune_keeps_results(ctx, traverse):
         pytest.skip("one occupancy is enough")
     old = ctx.tuning().as_dict()
     t = ctx.autotune(sc, 1 << 20)
-    assert t.trace_kernel in (0, 4, 5, 6) and t.trace_waves in (0, 6)
+    assert t.trace_kernel in (0, 4, 5, 6, 7) and t.trace_waves in (0, 6)
     rays = cf.traverse_rays(1 << 18, seed=5)
     _hits_equal(ctx.trace_rays(sc, rays), oracle.trace_rays(og, rays), f"autotuned {t.as_dict()}")
     ctx.set_tuning(**old)

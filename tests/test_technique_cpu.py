@@ -11,6 +11,7 @@ oracle chain on the grid the adapter uploaded, as in the GPU test.
 
 Built here from the reference sources (make -C oracle asan); skipped where
 they are absent."""
+import fcntl
 import os
 import subprocess
 
@@ -30,8 +31,11 @@ W, H = 240, 135  # VoxelBench core.h:138
 def driver():
     if not os.path.exists(REF):
         pytest.skip("VoxelBench sources absent")
-    r = subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "asan"], capture_output=True, text=True,
-                       timeout=600)
+    os.makedirs(os.path.dirname(DRIVER), exist_ok=True)
+    with open(DRIVER + ".lock", "w") as lk:  # parallel test workers build it once
+        fcntl.flock(lk, fcntl.LOCK_EX)
+        r = subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "asan"], capture_output=True,
+                           text=True, timeout=600)
     assert r.returncode == 0, r.stdout + r.stderr
     return DRIVER
 

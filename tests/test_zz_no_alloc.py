@@ -22,8 +22,9 @@ def scene(ctx):
 @pytest.mark.parametrize("tuning", [dict(trace_kernel=5, trace_blocks=512, trace_steps=1, trace_min_refill=1),
                                     dict(trace_kernel=5, trace_blocks=1024, trace_steps=8, trace_min_refill=16,
                                          dda_batch=16),
-                                    dict(trace_kernel=4), dict(trace_kernel=6)],
-                         ids=["persist-fast-s1", "persist-fast", "lane-fast", "lane-global"])
+                                    dict(trace_kernel=4), dict(trace_kernel=6),
+                                    dict(trace_kernel=7, trace_blocks=512, trace_min_refill=16, trace_steps=32)],
+                         ids=["persist-fast-s1", "persist-fast", "lane-fast", "lane-global", "wave"])
 def test_noalloc_equals_trace_rays_device(ctx, scene, tuning):
     old = ctx.tuning().as_dict()
     ctx.set_tuning(**tuning)

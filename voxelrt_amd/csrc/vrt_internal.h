@@ -138,7 +138,7 @@ struct vrt_ctx {
   int cus = 256;         // compute units of the device (persistent grids)
   vrt_tuning tuning{};   // kernel variants + launch shapes (vrt_tuning_set / vrt_autotune)
   vrt::DeviceBuffer d_counters;  // u64 x4: rays, brick steps, voxel steps, hits
-  vrt::DeviceBuffer d_queue;     // u32 work-queue counters of the persistent kernels
+  vrt::DeviceBuffer d_queue;     // work queues of the persistent kernels: traversal (words 0..255), path tracer (256..511)
   vrt::DeviceBuffer scratch_rays, scratch_hits, scratch_hdr, scratch_ldr;
   std::vector<vrt_scene*> scenes;
   std::unordered_map<uint32_t, ViewState> views;

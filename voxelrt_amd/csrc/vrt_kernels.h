@@ -23,6 +23,9 @@ constexpr bool tv_fast(int tv) { return tv == TV_FAST || tv == TV_FASTSKIP; }
 // padded bitmap, its prefix counts and the row distances within 48 KiB.
 // Larger scenes run TV_FASTG.
 constexpr uint32_t LDS_BIT_WORDS_MAX = 12288;
+// u32 words of a persistent kernel's work queue (the split queue's eight
+// counters, one per 128-byte line; the single-counter kernels use word 0).
+constexpr uint32_t QUEUE_WORDS = 256;
 
 hipError_t launch_brick_masks(hipStream_t st, const uint32_t* vox, int nx, int ny, int bgx,
                               int bgy, int bz0, int nbz, uint32_t* masks, uint32_t* flags);
@@ -50,6 +53,15 @@ hipError_t launch_trace_persistent(hipStream_t st, const SceneDev& s, const floa
                                   float tmax, void* hits, unsigned long long* counters,
                                   uint32_t* queue, int blocks, int steps, int min_refill, int tv, int waves,
                                   bool clear_queue = true);
+// Step-level refill traversal (vrt_trace_wave.h): `blocks` blocks of 256
+// threads (LDS form) or 4 x blocks one-wave blocks (global-memory form, for
+// scenes whose bitmaps do not fit in LDS); a pass when `min_wait` lanes wait
+// at a brick boundary or `fin_min` have finished; waves 6 = built for >= 6
+// waves per SIMD.  queue: the chunk queue's 256 u32 words (QWORDS), cleared
+// here by a kernel when clear_queue is set (else the caller has zeroed them).
+hipError_t launch_trace_wave(hipStream_t st, const SceneDev& s, const float* rays, uint32_t n, float tmax,
+                             void* hits, unsigned long long* counters, uint32_t* queue, int blocks, int min_wait,
+                             int fin_min, int waves, bool clear_queue = true);
 hipError_t launch_pathtrace(hipStream_t st, const SceneDev& s, const Uniforms& u, float fclamp,
                             uint32_t spp, void* radiance, void* albedo, void* gbuf,
                             unsigned long long* counters, int tv, int waves, int form);
@@ -57,8 +69,8 @@ hipError_t launch_pathtrace(hipStream_t st, const SceneDev& s, const Uniforms& u
 // (vrt_pathtrace_wave.h): `blocks` blocks of 64 threads; a pass when
 // `min_wait` lanes of a wave wait at a brick boundary or `shade_min` have
 // finished rays; waves = 0 / 4 / 5 / 6 minimum waves per SIMD; form 1 = the
-// global-memory DDA even where the LDS form fits; queue: one u32 patch
-// counter, cleared here by a kernel.
+// global-memory DDA even where the LDS form fits; queue: the patch queue's
+// 256 u32 words (QWORDS), cleared here by a kernel.
 hipError_t launch_pathtrace_wave(hipStream_t st, const SceneDev& s, const Uniforms& u, float fclamp, uint32_t spp,
                                  void* radiance, void* albedo, void* gbuf, unsigned long long* counters,
                                  uint32_t* queue, int blocks, int min_wait, int shade_min, int waves, int form);

@@ -311,6 +311,38 @@ constexpr uint32_t RAY_NONE = 0xFFFFFFFFu;  // lane wants a ray
 constexpr uint32_t RAY_DONE = 0xFFFFFFFEu;  // queue exhausted for this lane
 constexpr uint32_t QUEUE_CHUNK = 256u;      // rays a wave takes per queue atomic
 
+// A work queue spread over QSPLIT counters QSTRIDE words apart (one 128-byte
+// line each): item i is drawn from counter i % QSPLIT as the k-th draw there
+// (i = k * QSPLIT + counter).  Atomics on one address are serialised in the
+// memory system (about 12 ns each on MI355X), which capped a single-counter
+// queue near 80 M draws/s; each wave draws from its home counter (its block
+// index mod QSPLIT) and, when that is exhausted, from each of the others
+// once in turn (a draw past the end tells the wave that counter is done).
+// Items still come out in about increasing order, so a queue ordered
+// costly-first stays so.  The QSPLIT counters start at zero; each counts at
+// most nitems / QSPLIT + 1 + (waves of the launch) draws.
+constexpr uint32_t QSPLIT = 8u, QSTRIDE = 32u, QWORDS = QSPLIT * QSTRIDE;
+static_assert(QWORDS == QUEUE_WORDS, "queue size");
+struct SplitQueue {
+  uint32_t* q;
+  uint32_t nitems, home, tried;
+  VRT_D SplitQueue(uint32_t* queue, uint32_t n, uint32_t block) : q(queue), nitems(n), home(block % QSPLIT), tried(0u) {}
+  // The next item for the calling wave (the same value in every lane), or
+  // nitems when every counter is exhausted.  Called by the whole wave.
+  VRT_D uint32_t next(uint32_t lane) {
+    while (tried < QSPLIT) {
+      const uint32_t c = (home + tried) % QSPLIT;
+      uint32_t k = 0u;
+      if (lane == 0u) k = atomicAdd(q + c * QSTRIDE, 1u);
+      k = __builtin_amdgcn_readfirstlane(__shfl(k, 0, 64));
+      const uint64_t item = (uint64_t)k * QSPLIT + c;
+      if (item < nitems) return (uint32_t)item;
+      ++tried;
+    }
+    return nitems;
+  }
+};
+
 VRT_D void store_hit(uint4* hits, uint32_t i, const TraceResult& h) {
   uint4 rec;
   rec.x = __builtin_bit_cast(uint32_t, h.t);
@@ -1367,6 +1399,7 @@ __global__ __launch_bounds__(256) void k_hdr_batch_gs(const BatchHead* __restric
 }  // namespace vrt
 
 #include "vrt_pathtrace_wave.h"
+#include "vrt_trace_wave.h"
 
 // ---------------------------------------------------------------------------
 // host-side launchers (declared in vrt_kernels.h)
@@ -1594,7 +1627,7 @@ hipError_t launch_pathtrace_wave(hipStream_t st, const SceneDev& s, const Unifor
                                  void* radiance, void* albedo, void* gbuf, unsigned long long* counters,
                                  uint32_t* queue, int blocks, int min_wait, int shade_min, int waves,
                                  int form) {
-  hipError_t e = launch_zero_words(st, queue, 1u);
+  hipError_t e = launch_zero_words(st, queue, QWORDS);
   if (e != hipSuccess) return e;
   const uint32_t mw = min_wait < 1 ? 1u : (min_wait > 64 ? 64u : (uint32_t)min_wait);
   const uint32_t sm = shade_min < 1 ? 1u : (shade_min > 64 ? 64u : (uint32_t)shade_min);
@@ -1618,6 +1651,35 @@ hipError_t launch_pathtrace_wave(hipStream_t st, const SceneDev& s, const Unifor
     case 5256: go(k_pathtrace_wave<5, 256>); break;
     case 6256: go(k_pathtrace_wave<6, 256>); break;
     default: go(k_pathtrace_wave<1, 256>); break;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_trace_wave(hipStream_t st, const SceneDev& s, const float* rays, uint32_t n, float tmax,
+                             void* hits, unsigned long long* counters, uint32_t* queue, int blocks, int min_wait,
+                             int fin_min, int waves, bool clear_queue) {
+  if (clear_queue) {
+    const hipError_t e = launch_zero_words(st, queue, QWORDS);
+    if (e != hipSuccess) return e;
+  }
+  const uint32_t mw = min_wait < 1 ? 1u : (min_wait > 64 ? 64u : (uint32_t)min_wait);
+  const uint32_t fm = fin_min < 1 ? 1u : (fin_min > 64 ? 64u : (uint32_t)fin_min);
+  // the LDS form when the scene's bitmaps fit (blocks of four waves), else
+  // the global-memory form on one-wave blocks (four times as many)
+  const bool lds = lds_variant(TV_FAST, s.bit_words, s.pwords) == TV_FAST;
+  const uint32_t nt = lds ? 256u : 64u;
+  const uint32_t shmem = lds ? lds_bytes(TV_FAST, s.bit_words, s.pwords) : 0u;
+  const int nb = blocks > 0 ? (lds ? blocks : 4 * blocks) : 1;
+  auto go = [&](auto kernel) {
+    hipLaunchKernelGGL(kernel, dim3(nb), dim3(nt), shmem, st, s, rays, n, tmax, (uint4*)hits, counters, queue, mw,
+                       fm);
+  };
+  if (lds) {
+    if (waves == 6) go(k_trace_wave<6, 256>);
+    else go(k_trace_wave<1, 256>);
+  } else {
+    if (waves == 6) go(k_trace_wave<6, 64>);
+    else go(k_trace_wave<1, 64>);
   }
   return hipGetLastError();
 }

@@ -209,13 +209,15 @@ int vrt_device_count(int32_t* count) {
 // Scratch of the allocation-free traversal entry point
 // (vrt_trace_rays_device_noalloc): the four 64-bit counters its kernels add
 // rays, brick steps, voxel steps and hits into (bytes 0..31) and the
-// persistent kernels' 32-bit ray-queue head (bytes 32..35); 256 bytes, one
-// copy per device, held until the process ends.  Host code takes its device
+// persistent kernels' ray queue (bytes 256..1279: eight 32-bit counters, one
+// per 128-byte line); 1280 bytes, one copy per device, held until the
+// process ends.  Host code takes its device
 // address per device once (scratch_address, from vrt_create, with the device
 // current) and uses only that address afterwards.
 // ---------------------------------------------------------------------------
-static __device__ __align__(256) unsigned long long scratch[32];
-constexpr uint32_t NOALLOC_SCRATCH_WORDS = 9;  // u32 words the entry point clears per call
+static __device__ __align__(256) unsigned long long scratch[160];
+constexpr uint32_t NOALLOC_QUEUE_WORD = 64;      // u32 index of the ray queue in scratch
+constexpr uint32_t NOALLOC_SCRATCH_WORDS = 320;  // u32 words the entry point clears per call
 
 static std::mutex g_scratch_mu;
 static std::unordered_map<int, void*> g_scratch_addr;  // device ordinal -> address of scratch
@@ -286,9 +288,9 @@ int vrt_create(const vrt_create_info* info, vrt_ctx** out) {
   size_t freeb = 0, total = 0;
   (void)hipMemGetInfo(&freeb, &total);
   ctx->budget = (info && info->hbm_budget) ? info->hbm_budget : (uint64_t)(0.9 * (double)freeb);
-  ctx->tuning.trace_kernel = 5;
+  ctx->tuning.trace_kernel = 7;
   ctx->tuning.trace_blocks = 1024;
-  ctx->tuning.trace_steps = 8;
+  ctx->tuning.trace_steps = 32;
   ctx->tuning.trace_min_refill = 16;
   ctx->tuning.dda_batch = 16;
   ctx->tuning.pt_kernel = 8;
@@ -297,7 +299,7 @@ int vrt_create(const vrt_create_info* info, vrt_ctx** out) {
   ctx->tuning.pt_waves = 4;
   ctx->tuning.pt_min_refill = 16;
   ctx->tuning.pt_shade_min = 32;
-  if (ctx->d_queue.alloc(64 * sizeof(uint32_t)) != hipSuccess ||
+  if (ctx->d_queue.alloc(2 * vrt::QUEUE_WORDS * sizeof(uint32_t)) != hipSuccess ||
       ctx->d_counters.alloc(4 * sizeof(uint64_t)) != hipSuccess ||
       hipMemsetAsync(ctx->d_counters.ptr, 0, 4 * sizeof(uint64_t), ctx->stream) != hipSuccess) {
     delete ctx;
@@ -599,10 +601,13 @@ int vrt_scene_bricks(vrt_ctx* ctx, const vrt_scene* scene, uint32_t* bricks, uin
 // ---------------------------------------------------------------------------
 // vrt_tuning.trace_kernel -> trace variant of the one-ray-per-lane kernel:
 // 0 nested, 4 single loop with both levels in LDS (with the empty-row skip
-// when skip is set), 6 the single loop on global memory.
+// when skip is set), 6 the single loop on global memory; the persistent
+// kernels 5 and 7 run 4 for ray lists of 2^31 rays and more.
 static int trace_variant(uint32_t kernel, uint32_t skip) {
   switch (kernel) {
-    case 4: return skip ? vrt::TV_FASTSKIP : vrt::TV_FAST;
+    case 4:
+    case 5:
+    case 7: return skip ? vrt::TV_FASTSKIP : vrt::TV_FAST;
     case 6: return vrt::TV_FASTG;
     default: return vrt::TV_NESTED;
   }
@@ -626,7 +631,7 @@ hipError_t vrt_ctx::launch_pathtrace(const vrt::SceneDev& s0, const vrt_uniforms
   const vrt::SceneDev s = tuned(s0);
   if (tuning.pt_kernel == 8)
     return vrt::launch_pathtrace_wave(stream, s, u, fclamp, spp, radiance, albedo, gbuf,
-                                      (unsigned long long*)d_counters.ptr, (uint32_t*)d_queue.ptr + 2,
+                                      (unsigned long long*)d_counters.ptr, (uint32_t*)d_queue.ptr + vrt::QUEUE_WORDS,
                                       tuning.pt_blocks ? (int)tuning.pt_blocks : cus * 16,
                                       (int)tuning.pt_min_refill, (int)tuning.pt_shade_min, (int)tuning.pt_waves,
                                       (int)tuning.pt_tile);
@@ -641,6 +646,10 @@ hipError_t vrt_ctx::launch_trace(const vrt::SceneDev& s0, const float* rays, uin
   const vrt::SceneDev s = tuned(s0);
   if (!counters) counters = (unsigned long long*)d_counters.ptr;
   if (!queue) queue = (uint32_t*)d_queue.ptr;
+  if (tuning.trace_kernel == 7 && n < 0x80000000ull)
+    return vrt::launch_trace_wave(stream, s, rays, (uint32_t)n, tmax, hits, counters, queue,
+                                  (int)tuning.trace_blocks, (int)tuning.trace_min_refill, (int)tuning.trace_steps,
+                                  (int)tuning.trace_waves, clear_queue);
   if (tuning.trace_kernel == 5 && n < 0x80000000ull)
     return vrt::launch_trace_persistent(stream, s, rays, (uint32_t)n, tmax, hits, counters, queue,
                                         (int)tuning.trace_blocks, (int)tuning.trace_steps,
@@ -691,7 +700,7 @@ int vrt_trace_rays_device_noalloc(vrt_ctx* ctx, const vrt_scene* scene, const fl
   HIP_TRY(ctx, vrt::launch_zero_words(ctx->stream, ctx->noalloc_scratch, NOALLOC_SCRATCH_WORDS));
   const vrt::SceneDev s = scene->dev();
   HIP_TRY(ctx, ctx->launch_trace(s, d_rays, n, tmax, d_hits, (unsigned long long*)ctx->noalloc_scratch,
-                                 (uint32_t*)ctx->noalloc_scratch + 8, false));
+                                 (uint32_t*)ctx->noalloc_scratch + NOALLOC_QUEUE_WORD, false));
   return VRT_OK;
   GUARD_END(ctx)
 }
@@ -1027,7 +1036,7 @@ int vrt_tuning_set(vrt_ctx* ctx, const vrt_tuning* in) {
   if (!ctx || !in) return VRT_ERR_INVALID;
   std::lock_guard<std::mutex> lock_(ctx->mu);
   const uint32_t tk = in->trace_kernel, pk = in->pt_kernel;
-  if (!(tk == 0 || tk == 4 || tk == 5 || tk == 6) || !(pk == 0 || pk == 4 || pk == 7 || pk == 8) ||
+  if (!(tk == 0 || tk == 4 || tk == 5 || tk == 6 || tk == 7) || !(pk == 0 || pk == 4 || pk == 7 || pk == 8) ||
       in->trace_blocks > (1u << 20) || in->pt_blocks > (1u << 20) || in->trace_steps > 1024 ||
       in->trace_min_refill > 64 || in->pt_min_refill > 64 ||
       !(in->pt_waves == 0 || in->pt_waves == 4 || in->pt_waves == 5 || (in->pt_waves == 6 && pk == 8)) ||
@@ -1136,6 +1145,18 @@ int vrt_autotune(vrt_ctx* ctx, const vrt_scene* scene, uint64_t nrays, const vrt
           c.trace_waves = wv;
           if ((rc = try_trace(c))) return rc;
         }
+    // step-level refill over its launch shape and pass thresholds
+    c.trace_kernel = 7;
+    c.dda_batch = 1;
+    c.trace_waves = 0;
+    for (uint32_t b : {768u, 1024u, 1536u})
+      for (uint32_t mw : {12u, 16u})
+        for (uint32_t fm : {24u, 32u}) {
+          c.trace_blocks = b;
+          c.trace_min_refill = mw;
+          c.trace_steps = fm;
+          if ((rc = try_trace(c))) return rc;
+        }
     // the persistent ray queue over its launch shape
     c.trace_kernel = 5;
     for (uint32_t db : {1u, 16u})
@@ -1210,7 +1231,7 @@ int vrt_autotune(vrt_ctx* ctx, const vrt_scene* scene, uint64_t nrays, const vrt
     c.pt_waves = 4;
     c.pt_order = 2;
     for (uint32_t form : {0u, 1u})
-      for (uint32_t mw : {12u, 16u, 24u})
+      for (uint32_t mw : {8u, 12u, 16u, 24u})
         for (uint32_t sm : {24u, 32u}) {
           c.pt_tile = form;
           c.pt_min_refill = mw;
