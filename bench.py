@@ -40,18 +40,25 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "Mrays/s & ms/frame @1080p 256³; batched/served p50/p99 latency vs plain"
 
 
-def _pmc_traffic(name: str, key: str):
-    """HBM bytes per launch of one kernel variant from a committed rocprofv3
-    --pmc summary, if it has that variant (profiles/<name>: a line
-    'traffic_bytes_per_launch[<key>]: <int>'), else None."""
+def _profile_value(name: str, field: str, key: str):
+    """A figure of one kernel variant from a committed rocprofv3 --pmc summary
+    (profiles/<name>: a line '<field>[<key>]: <number>'), or None when the
+    summary does not hold that variant.  Fields: traffic_bytes_per_launch
+    (FETCH_SIZE + WRITE_SIZE, calibrated), lanes_per_valu_inst,
+    valu_lane_op_fraction."""
     path = os.path.join(ROOT, "profiles", name)
     try:
         for line in open(path):
-            if line.startswith(f"traffic_bytes_per_launch[{key}]:"):
-                return int(float(line.split(":", 1)[1]))
+            if line.startswith(f"{field}[{key}]:"):
+                v = float(line.split(":", 1)[1])
+                return int(v) if field == "traffic_bytes_per_launch" else v
     except OSError:
         pass
     return None
+
+
+def _pmc_traffic(name: str, key: str):
+    return _profile_value(name, "traffic_bytes_per_launch", key)
 
 
 def _pt_kernel_name(t):
@@ -276,6 +283,9 @@ def _traverse_one(args, ctx, sc, p, name, d_rays, d_hits, n_rays, tuning):
                                    "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                    "frac": round(gbs / HBM_PEAK_GBS, 4),
                                    "traffic": _pmc_traffic("traverse_pmc.txt", f"trace_kernel=7,p={p}")
+                                   if tuning == PROFILED_TRAVERSE else None,
+                                   "valu_lane_op_fraction": _profile_value("traverse_pmc.txt", "valu_lane_op_fraction",
+                                                                           f"trace_kernel=7,p={p}")
                                    if tuning == PROFILED_TRAVERSE else None,
                                    "alg_bytes_per_launch": int(alg)}}), flush=True)
 
@@ -550,7 +560,7 @@ def main():
     ap.add_argument("--batch-mode", type=int, default=2, choices=[-1, 0, 1, 2],
                     help="served form's batch mode (-1: the batch autotuner's choice among all)")
     ap.add_argument("--pt-kernel", default="auto", choices=["auto", "0", "4", "7", "8"])
-    ap.add_argument("--pt-waves", type=int, default=0, choices=[0, 4, 5, 6])
+    ap.add_argument("--pt-waves", type=int, default=4, choices=[0, 4, 5, 6])
     ap.add_argument("--pt-min-refill", type=int, default=16, help="pt-kernel 8: lanes waiting before a level-switch pass")
     ap.add_argument("--pt-blocks", type=int, default=0, help="pt-kernel 8: persistent waves (0 = 16 per CU)")
     ap.add_argument("--pt-shade-min", type=int, default=32, help="pt-kernel 8: finished rays shaded together")
@@ -618,6 +628,7 @@ def main():
     hits = s1["hits"] - s0["hits"]
     ms_frame = wall / K * 1e3
     mrays = rays / wall / 1e6
+    pkey = f"pt_kernel={tuned['pt_kernel']},pt_waves={tuned['pt_waves']},pt_tile={tuned['pt_tile']}"
     # path-trace kernel: algorithmic bytes per launch
     alg_bytes = (4 * bsteps + 4 * vsteps + 4 * hits) / K + 28 * w * h
     pt_avg_ms = pt_ms / K
@@ -632,9 +643,10 @@ def main():
                    "width": w, "height": h, "grid": 256, "spp": 1, "bounces": 2},
         "roofline": {"bound": "hbm", "kernel": _pt_kernel_name(tuned), "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": _pmc_traffic("pathtrace_frame1080_pmc.txt",
-                                             f"pt_kernel={tuned['pt_kernel']},pt_waves={tuned['pt_waves']},"
-                                             f"pt_tile={tuned['pt_tile']}"),
+                     "traffic": _pmc_traffic("pathtrace_frame1080_pmc.txt", pkey),
+                     "valu_lanes_per_inst": _profile_value("pathtrace_frame1080_pmc.txt", "lanes_per_valu_inst", pkey),
+                     "valu_lane_op_fraction": _profile_value("pathtrace_frame1080_pmc.txt", "valu_lane_op_fraction",
+                                                             pkey),
                      "alg_bytes_per_launch": int(alg_bytes), "avg_launch_ms": round(pt_avg_ms, 4)},
         "tuning": {k: tuned[k] for k in ("pt_kernel", "pt_waves", "pt_order", "pt_tile", "pt_blocks",
                                           "pt_min_refill", "pt_shade_min")},
