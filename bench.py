@@ -421,7 +421,9 @@ def serve(args, emit=True):
     ctx.serve_configure(streams=args.serve_streams, max_batch_views=args.serve_views,
                         max_batch_pixels=args.serve_pixels)
     t0 = time.perf_counter()
-    scenes = [ctx.upload(v.generate_scene(**d)) for d in descs]
+    # evictable: under budget pressure the least recently used idle grids
+    # leave the device and come back on their next request
+    scenes = [ctx.upload(v.generate_scene(**d), evictable=True) for d in descs]
     upload_s = time.perf_counter() - t0
     # kernel variant: the project's autotuner on a 256^3 scene at 1080p (both
     # plain and served use it)
@@ -523,6 +525,8 @@ def serve(args, emit=True):
            "speedup_saturated": round(res["served_saturated"]["mrays_s"] / res["plain_saturated"]["mrays_s"], 2),
            "poisson_rate_per_s": round(rate, 2), "load_vs_plain": args.load,
            "runs": res, "serve_stats": st, "requests": len(order), "scenes": len(descs),
+           "scene_bytes": sum(sc.nbytes() for sc in scenes),
+           "scene_evictions": ctx.stats()["scene_evictions"],
            "camera_streams": len(cams), "scene_upload_s": round(upload_s, 2),
            "serve_config": {"streams": args.serve_streams, "max_batch_views": args.serve_views,
                             "max_batch_pixels": args.serve_pixels},

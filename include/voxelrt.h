@@ -163,6 +163,8 @@ typedef struct vrt_stats {
   double   sum_ms_temporal;
   double   sum_ms_atrous;
   double   sum_ms_frame;
+  uint64_t scene_evictions;  /* VRT_SCENE_EVICTABLE scenes moved to host memory */
+  uint64_t scene_restores;   /* ... and brought back */
 } vrt_stats;
 
 /* ---- context -------------------------------------------------------------- */
@@ -179,18 +181,37 @@ int vrt_scene_upload(vrt_ctx* ctx, const uint32_t* vox, uint32_t nx, uint32_t ny
                      uint32_t nz, uint64_t scene_key, vrt_scene** out);
 /* Editable scenes (VRT_SCENE_EDITABLE) also keep the dense grid resident
  * (4 B per voxel, counted in the HBM budget); vrt_scene_edit writes a box of
- * voxels (sx*sy*sz words, x fastest) into it and rebuilds the bricks, masks,
- * pool and occupancy bitmaps on the device (a full rebuild: slots follow
- * brick order, so a change of occupancy anywhere moves later slots).  Work
+ * voxels (sx*sy*sz words, x fastest) into it and updates the scene on the
+ * device: when no brick the box touches changes occupancy (empty / not),
+ * only those bricks' masks and pool entries are rewritten in place;
+ * otherwise the bricks, masks, pool and occupancy bitmaps are rebuilt (slots
+ * follow brick order, so a change of occupancy anywhere moves later slots).  Work
  * already enqueued that reads the scene completes first; frames launched
  * after the call see the edit.  Replaces VoxelBench's setVox + brick
  * rebuild on a SceneEvent geomEdit (core.h:78-82, scenes.h:8-12). */
 #define VRT_SCENE_EDITABLE 1u
+/* Evictable scenes (no reference counterpart) may leave the device when the
+ * context's HBM budget is needed -- by a scene upload, a view's state, a
+ * served request -- and nothing uses them: no queued or running served
+ * request, no call in progress.  The least recently used goes first; its
+ * device buffers are copied to host memory and the budget is released.  Any
+ * later use restores it (copied back, not rebuilt), so results do not change;
+ * vrt_trace_rays_device_noalloc, which must not allocate, rejects an evicted
+ * scene (vrt_scene_restore brings it back first).  Scenes without the flag
+ * stay resident until released.  vrt_stats counts evictions and restores. */
+#define VRT_SCENE_EVICTABLE 2u
 int vrt_scene_upload_ex(vrt_ctx* ctx, const uint32_t* vox, uint32_t nx, uint32_t ny,
                         uint32_t nz, uint64_t scene_key, uint32_t flags, vrt_scene** out);
 int vrt_scene_edit(vrt_ctx* ctx, vrt_scene* scene, const uint32_t* vox, uint32_t x0,
                    uint32_t y0, uint32_t z0, uint32_t sx, uint32_t sy, uint32_t sz);
 int vrt_scene_release(vrt_ctx* ctx, vrt_scene* scene);
+/* Bring an evicted scene back to the device (no-op when resident); *resident
+ * of vrt_scene_resident says where it is now.  No reference counterpart. */
+int vrt_scene_restore(vrt_ctx* ctx, vrt_scene* scene);
+int vrt_scene_resident(const vrt_scene* scene, int32_t* resident);
+/* How the scene's vrt_scene_edit calls were applied so far: in place (only
+ * the touched bricks) or by a rebuild.  No reference counterpart. */
+int vrt_scene_edit_counts(const vrt_scene* scene, uint64_t* in_place, uint64_t* rebuilt);
 /* Read back the device-built brick words (nbricks) and masks (nbricks*16). */
 int vrt_scene_bricks(vrt_ctx* ctx, const vrt_scene* scene, uint32_t* bricks,
                      uint32_t* masks);

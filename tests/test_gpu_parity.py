@@ -545,23 +545,42 @@ def test_autotune_batch_keeps_histories(ctx):
 # editable scenes: device rebuild after a box edit
 # ---------------------------------------------------------------------------
 def test_scene_edits_rebuild_on_device(ctx):
-    """Carve a hole, fill a block and recolour voxels of an editable scene:
-    after each edit the device bricks/masks, traced hits and a rendered frame
-    equal the oracle's on the edited dense grid, and the pool grows when the
-    occupied-brick count does."""
+    """Recolour solid voxels, clear single voxels of occupied bricks, carve a
+    hole, fill a block and rewrite everything in an editable scene: after each
+    edit the device bricks/masks, traced hits and a rendered frame equal the
+    oracle's on the edited dense grid; the edits that keep every touched
+    brick's occupancy are applied in place, the others rebuild the scene (and
+    the pool grows when the occupied-brick count does)."""
     desc = cf.world_desc(64, 7)
     vox = v.generate_scene(**desc)
     sc = ctx.upload(vox, editable=True)
     rng = np.random.default_rng(11)
-    edits = [((8, 0, 8), np.zeros((20, 24, 30), np.uint32)),                          # carve
-             ((30, 30, 30), np.full((10, 17, 9), 0x10406080, np.uint32)),             # fill into empty air
-             ((0, 0, 0), rng.integers(0, 2, (64, 64, 64), dtype=np.uint32) * 0x01808080)]  # rewrite all
+    recolour = np.where(vox[0:16, 0:16, 0:64] != 0, np.uint32(0x10A0B0C0), np.uint32(0))
+    # one voxel cleared in each of a few occupied bricks that keep others solid
+    nz, ny, nx = vox.shape
+    keep = np.argwhere(vox.reshape(nz // 8, 8, ny // 8, 8, nx // 8, 8).sum(axis=(1, 3, 5), dtype=np.uint64) > 0)
+    solid = None
+    for bz, by, bx in keep:
+        blk = vox[bz * 8:bz * 8 + 8, by * 8:by * 8 + 8, bx * 8:bx * 8 + 8]
+        if np.count_nonzero(blk) >= 2:
+            z, y, x = np.argwhere(blk != 0)[0]
+            solid = (int(bx * 8 + x), int(by * 8 + y), int(bz * 8 + z))
+            break
+    assert solid is not None
+    edits = [((0, 0, 0), recolour, True),                                            # in place
+             (solid, np.zeros((1, 1, 1), np.uint32), True),                          # in place
+             ((8, 0, 8), np.zeros((20, 24, 30), np.uint32), False),                  # carve
+             ((30, 30, 30), np.full((10, 17, 9), 0x10406080, np.uint32), False),      # fill into empty air
+             ((0, 0, 0), rng.integers(0, 2, (64, 64, 64), dtype=np.uint32) * 0x01808080, False)]  # rewrite all
     rays = cf.traverse_rays(1 << 15, n=64, seed=12)
-    for k, (origin, box) in enumerate(edits):
+    for k, (origin, box, in_place) in enumerate(edits):
         x0, y0, z0 = origin
         sz, sy, sx = box.shape
         vox[z0:z0 + sz, y0:y0 + sy, x0:x0 + sx] = box
+        before = sc.edit_counts()
         sc.edit(box, origin)
+        after = sc.edit_counts()
+        assert after == ((before[0] + 1, before[1]) if in_place else (before[0], before[1] + 1)), (k, before, after)
         words, masks = sc.bricks()
         ob, om = oracle.build_bricks(vox)
         assert np.array_equal(words != 0, ob != 0), k

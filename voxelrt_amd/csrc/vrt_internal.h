@@ -1,5 +1,6 @@
 // vrt_internal.h -- host-side state of libvoxelrt_hip (not part of the ABI).
 #pragma once
+#include <array>
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -77,6 +78,11 @@ struct vrt_scene {
   uint64_t key = 0;
   uint64_t lru = 0;
   uint64_t uses = 0;  // queued + running serve requests
+  uint64_t edits_in_place = 0, edits_rebuilt = 0;  // vrt_scene_edit calls by how they were applied
+  uint64_t pins = 0;       // C ABI calls using the scene right now (not evictable meanwhile)
+  bool evictable = false;  // VRT_SCENE_EVICTABLE
+  bool evicted = false;    // device buffers moved to `host` (budget released)
+  std::vector<std::vector<uint8_t>> host;  // an evicted scene's buffers, in buffers() order
   vrt::DeviceBuffer d_bricks;  // one word per 8^3 brick: 0 or 1 + pool slot
   vrt::DeviceBuffer d_pool;    // nocc x 528 words (mask + voxels)
   vrt::DeviceBuffer d_bits;    // one occupancy bit per brick (LDS-staged by the kernels)
@@ -86,6 +92,10 @@ struct vrt_scene {
   vrt::DeviceBuffer d_dense;  // editable scenes: the dense packed grid, rebuilt from on edits
   uint32_t psh1 = 0, psh2 = 0, psh3 = 0;  // padded bit index = bx | by << psh1 | bz << psh2; 2^psh3 bits
   vrt::SceneDev dev() const;
+  // every device buffer of the scene (those of size 0 are not allocated)
+  std::array<vrt::DeviceBuffer*, 8> buffers() {
+    return {&d_bricks, &d_pool, &d_bits, &d_prefix, &d_pbits, &d_pprefix, &d_pdist, &d_dense};
+  }
 };
 
 // Per-view resident frame targets (main.js:341-349): radiance rgba16f,
@@ -138,14 +148,26 @@ struct vrt_ctx {
   int cus = 256;         // compute units of the device (persistent grids)
   vrt_tuning tuning{};   // kernel variants + launch shapes (vrt_tuning_set / vrt_autotune)
   vrt::DeviceBuffer d_counters;  // u64 x4: rays, brick steps, voxel steps, hits
-  vrt::DeviceBuffer d_queue;     // work queues of the persistent kernels: traversal (words 0..255), path tracer (256..511)
+  vrt::DeviceBuffer d_queue;     // work queues of the persistent kernels: traversal (words 0..255), path
+                                 // tracer (256..511); word 512: the scene edit's occupancy-changed flag
   vrt::DeviceBuffer scratch_rays, scratch_hits, scratch_hdr, scratch_ldr;
   std::vector<vrt_scene*> scenes;
   std::unordered_map<uint32_t, ViewState> views;
 
   ~vrt_ctx();
   int fail(int code, const char* fmt, ...);
+  // budget: reserve evicts idle evictable scenes (least recently used first)
+  // when the bytes do not fit otherwise
   int reserve(uint64_t bytes);
+  // scene residency (VRT_SCENE_EVICTABLE): evict = device buffers to host
+  // memory and the budget released; restore = back to the device;
+  // make_scene_room evicts idle evictable scenes (no serve request queued or
+  // running, no C ABI call using them, not `keep`) least recently used first
+  // until `need` more bytes fit, and says whether they do
+  int evict_scene(vrt_scene* s);
+  int restore_scene(vrt_scene* s);
+  bool make_scene_room(uint64_t need, const vrt_scene* keep);
+  uint64_t scene_evictions = 0, scene_restores = 0;
   void unreserve(uint64_t bytes);
   int counters_to_host(uint64_t out[4]);
   int validate_view(const vrt_view& v);

@@ -29,6 +29,12 @@ constexpr uint32_t QUEUE_WORDS = 256;
 
 hipError_t launch_brick_masks(hipStream_t st, const uint32_t* vox, int nx, int ny, int bgx,
                               int bgy, int bz0, int nbz, uint32_t* masks, uint32_t* flags);
+// An edit's touched bricks [b0, b0 + e) per axis from the dense grid vox:
+// pool entries rewritten in place where occupancy is unchanged, *changed set
+// (atomicOr 1) where it changed (the caller rebuilds).
+hipError_t launch_edit_bricks(hipStream_t st, const uint32_t* vox, int nx, int ny, int bgx, int bgy, int bx0,
+                              int by0, int bz0, int ex, int ey, int ez, const uint32_t* bricks, uint32_t* pool,
+                              uint32_t* changed);
 hipError_t launch_brick_words(hipStream_t st, const uint32_t* flags, const uint32_t* slots,
                               uint64_t nb, uint32_t* bricks);
 hipError_t launch_brick_bits(hipStream_t st, const uint32_t* bricks, uint64_t nb, uint32_t* bits);

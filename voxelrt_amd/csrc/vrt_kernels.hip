@@ -78,6 +78,49 @@ __global__ __launch_bounds__(256) void k_brick_masks(const uint32_t* __restrict_
   if (lane == 0u) flags[gbi] = any;
 }
 
+// Edit of the bricks a box of voxels touched (bricks [b0, b0 + ext) per
+// axis), one 64-lane wave per brick: the brick's mask from the dense grid as
+// k_brick_masks builds it; where its occupancy (empty / not) is what the
+// brick word says, an occupied brick's pool entry (mask + voxels) is
+// rewritten in place at its slot, as k_pack_pool writes it; where the
+// occupancy changed, *changed is set and nothing of that brick is written
+// (the caller then rebuilds the scene: slots follow brick order).
+__global__ __launch_bounds__(256) void k_edit_bricks(const uint32_t* __restrict__ vox, int nx, int ny, int bgx,
+                                                     int bgy, int bx0, int by0, int bz0, int ex, int ey, int ez,
+                                                     const uint32_t* __restrict__ bricks, uint32_t* __restrict__ pool,
+                                                     uint32_t* __restrict__ changed) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint64_t wave = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (wave >= (uint64_t)ex * ey * ez) return;
+  const int bx = bx0 + (int)(wave % (uint64_t)ex);
+  const int by = by0 + (int)((wave / (uint64_t)ex) % (uint64_t)ey);
+  const int bz = bz0 + (int)(wave / ((uint64_t)ex * ey));
+  const uint64_t gbi = (uint64_t)bx + (uint64_t)by * bgx + (uint64_t)bz * bgx * bgy;
+  const int lx = (int)(lane & 7u), ly = (int)(lane >> 3);
+  uint32_t v[8];
+  uint32_t any = 0, mylo = 0, myhi = 0;
+#pragma unroll
+  for (int lz = 0; lz < 8; ++lz) {
+    const uint64_t vi = (uint64_t)(bx * 8 + lx) + (uint64_t)(by * 8 + ly) * nx +
+                        (uint64_t)(bz * 8 + lz) * (uint64_t)nx * ny;
+    v[lz] = vox[vi];
+    const uint64_t b = __ballot(v[lz] != 0u);
+    if (lane == (uint32_t)(2 * lz)) mylo = (uint32_t)b;
+    if (lane == (uint32_t)(2 * lz + 1)) myhi = (uint32_t)(b >> 32);
+    any |= (b != 0ull) ? 1u : 0u;
+  }
+  const uint32_t w = bricks[gbi];
+  if ((w != 0u) != (any != 0u)) {
+    if (lane == 0u) atomicOr(changed, 1u);
+    return;
+  }
+  if (w == 0u) return;
+  uint32_t* dst = pool + (uint64_t)(w - 1u) * POOL_WORDS;
+  if (lane < 16u) dst[lane] = (lane & 1u) ? myhi : mylo;
+#pragma unroll
+  for (int lz = 0; lz < 8; ++lz) dst[16 + lz * 64 + (int)lane] = v[lz];
+}
+
 // bricks[i] = flags[i] ? 1 + slot[i] : 0, slot = exclusive prefix of flags
 __global__ __launch_bounds__(256) void k_brick_words(const uint32_t* __restrict__ flags,
                                                      const uint32_t* __restrict__ slots,
@@ -1430,6 +1473,15 @@ hipError_t launch_brick_masks(hipStream_t st, const uint32_t* vox, int nx, int n
   const uint64_t waves = (uint64_t)bgx * bgy * nbz;
   hipLaunchKernelGGL(k_brick_masks, dim3(div_up(waves, 4)), dim3(256), 0, st, vox, nx, ny, bgx,
                      bgy, bz0, nbz, masks, flags);
+  return hipGetLastError();
+}
+
+hipError_t launch_edit_bricks(hipStream_t st, const uint32_t* vox, int nx, int ny, int bgx, int bgy, int bx0,
+                              int by0, int bz0, int ex, int ey, int ez, const uint32_t* bricks, uint32_t* pool,
+                              uint32_t* changed) {
+  const uint64_t waves = (uint64_t)ex * ey * ez;
+  hipLaunchKernelGGL(k_edit_bricks, dim3(div_up(waves, 4)), dim3(256), 0, st, vox, nx, ny, bgx, bgy, bx0, by0, bz0,
+                     ex, ey, ez, bricks, pool, changed);
   return hipGetLastError();
 }
 

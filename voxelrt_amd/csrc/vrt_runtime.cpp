@@ -94,11 +94,92 @@ static uint64_t view_bytes(uint32_t w, uint32_t h, uint32_t fmt) {
 }
 
 int vrt_ctx::reserve(uint64_t bytes) {
-  if (resident + bytes > budget)
+  if (resident + bytes > budget && !make_scene_room(bytes, nullptr))
     return fail(VRT_ERR_BUDGET, "HBM budget exceeded: %llu resident + %llu requested > %llu",
                 (unsigned long long)resident, (unsigned long long)bytes, (unsigned long long)budget);
   resident += bytes;
   return VRT_OK;
+}
+
+int vrt_ctx::evict_scene(vrt_scene* s) {
+  if (s->evicted) return VRT_OK;
+  // frames enqueued on the context's stream may still read it (serve batches
+  // cannot: the scene has no queued or running request)
+  HIP_TRY(this, hipStreamSynchronize(stream));
+  auto b = s->buffers();
+  std::vector<std::vector<uint8_t>> host(b.size());
+  for (size_t i = 0; i < b.size(); ++i)
+    if (b[i]->ptr) {
+      host[i].resize(b[i]->bytes);
+      HIP_TRY(this, hipMemcpyAsync(host[i].data(), b[i]->ptr, b[i]->bytes, hipMemcpyDeviceToHost, stream));
+    }
+  HIP_TRY(this, hipStreamSynchronize(stream));
+  for (auto* d : b) d->release();
+  s->host = std::move(host);
+  s->evicted = true;
+  unreserve(s->bytes);
+  ++scene_evictions;
+  return VRT_OK;
+}
+
+int vrt_ctx::restore_scene(vrt_scene* s) {
+  if (!s->evicted) return VRT_OK;
+  ++s->pins;  // not a candidate while room is made for it
+  const int rc = reserve(s->bytes);
+  --s->pins;
+  if (rc) return rc;
+  auto b = s->buffers();
+  for (size_t i = 0; i < b.size(); ++i) {
+    if (s->host[i].empty()) continue;
+    hipError_t e = b[i]->alloc(s->host[i].size());
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(b[i]->ptr, s->host[i].data(), s->host[i].size(), hipMemcpyHostToDevice, stream);
+    if (e != hipSuccess) {
+      for (auto* d : b) d->release();
+      unreserve(s->bytes);
+      return fail(e == hipErrorOutOfMemory ? VRT_ERR_NO_MEMORY : VRT_ERR_DEVICE, "scene restore: %s",
+                  hipGetErrorString(e));
+    }
+  }
+  HIP_TRY(this, hipStreamSynchronize(stream));  // before the host copies go
+  s->host.clear();
+  s->evicted = false;
+  s->lru = ++clock;
+  ++scene_restores;
+  return VRT_OK;
+}
+
+// The scenes a C ABI call uses, restored to the device when evicted and
+// pinned (not evictable) until the call returns.
+struct ScenePins {
+  vrt_ctx* ctx;
+  std::vector<vrt_scene*> held;
+  explicit ScenePins(vrt_ctx* c) : ctx(c) {}
+  ScenePins(const ScenePins&) = delete;
+  ScenePins& operator=(const ScenePins&) = delete;
+  int add(const vrt_scene* sc) {
+    vrt_scene* s = const_cast<vrt_scene*>(sc);
+    if (std::find(held.begin(), held.end(), s) != held.end()) return VRT_OK;
+    held.push_back(s);
+    ++s->pins;
+    s->lru = ++ctx->clock;
+    return ctx->restore_scene(s);
+  }
+  ~ScenePins() {
+    for (vrt_scene* s : held) --s->pins;
+  }
+};
+
+bool vrt_ctx::make_scene_room(uint64_t need, const vrt_scene* keep) {
+  while (resident + need > budget) {
+    vrt_scene* victim = nullptr;
+    for (vrt_scene* s : scenes)
+      if (s->evictable && !s->evicted && s->uses == 0 && s->pins == 0 && s != keep &&
+          (!victim || s->lru < victim->lru))
+        victim = s;
+    if (!victim || evict_scene(victim) != VRT_OK) return false;
+  }
+  return true;
 }
 void vrt_ctx::unreserve(uint64_t bytes) { resident = bytes > resident ? 0 : resident - bytes; }
 
@@ -299,7 +380,7 @@ int vrt_create(const vrt_create_info* info, vrt_ctx** out) {
   ctx->tuning.pt_waves = 4;
   ctx->tuning.pt_min_refill = 16;
   ctx->tuning.pt_shade_min = 32;
-  if (ctx->d_queue.alloc(2 * vrt::QUEUE_WORDS * sizeof(uint32_t)) != hipSuccess ||
+  if (ctx->d_queue.alloc((2 * vrt::QUEUE_WORDS + 64) * sizeof(uint32_t)) != hipSuccess ||
       ctx->d_counters.alloc(4 * sizeof(uint64_t)) != hipSuccess ||
       hipMemsetAsync(ctx->d_counters.ptr, 0, 4 * sizeof(uint64_t), ctx->stream) != hipSuccess) {
     delete ctx;
@@ -475,7 +556,8 @@ int vrt_scene_upload_ex(vrt_ctx* ctx, const uint32_t* vox, uint32_t nx, uint32_t
   GUARD_BEGIN
   std::lock_guard<std::mutex> lock_(ctx->mu);
   *out = nullptr;
-  if (flags & ~VRT_SCENE_EDITABLE) return ctx->fail(VRT_ERR_INVALID, "unknown scene flags 0x%x", flags);
+  if (flags & ~(VRT_SCENE_EDITABLE | VRT_SCENE_EVICTABLE))
+    return ctx->fail(VRT_ERR_INVALID, "unknown scene flags 0x%x", flags);
   if (nx == 0 || ny == 0 || nz == 0 || nx % 8 || ny % 8 || nz % 8)
     return ctx->fail(VRT_ERR_INVALID, "grid dims %ux%ux%u must be positive multiples of 8", nx, ny, nz);
   if ((uint64_t)nx * ny * nz > 0xFFFFFFFEull)
@@ -487,6 +569,7 @@ int vrt_scene_upload_ex(vrt_ctx* ctx, const uint32_t* vox, uint32_t nx, uint32_t
   sc->nz = nz;
   sc->key = scene_key;
   sc->nbricks = (uint64_t)(nx / 8) * (ny / 8) * (nz / 8);
+  sc->pins = 1;  // building: not evictable (joins ctx->scenes at the end)
   const uint64_t dense = (uint64_t)nx * ny * nz * 4;
   if (flags & VRT_SCENE_EDITABLE) {
     int rc = ctx->reserve(dense);
@@ -506,6 +589,8 @@ int vrt_scene_upload_ex(vrt_ctx* ctx, const uint32_t* vox, uint32_t nx, uint32_t
     return rc;
   }
   sc->lru = ++ctx->clock;
+  sc->pins = 0;
+  sc->evictable = (flags & VRT_SCENE_EVICTABLE) != 0;
   vrt_scene* raw = sc.release();
   ctx->scenes.push_back(raw);
   *out = raw;
@@ -525,7 +610,8 @@ int vrt_scene_edit(vrt_ctx* ctx, vrt_scene* scene, const uint32_t* vox, uint32_t
   std::lock_guard<std::mutex> lock_(ctx->mu);
   if (std::find(ctx->scenes.begin(), ctx->scenes.end(), scene) == ctx->scenes.end())
     return ctx->fail(VRT_ERR_INVALID, "unknown scene handle");
-  if (!scene->d_dense.ptr) return ctx->fail(VRT_ERR_INVALID, "scene was not uploaded with VRT_SCENE_EDITABLE");
+  if (!scene->d_dense.ptr && !(scene->evicted && !scene->host[7].empty()))
+    return ctx->fail(VRT_ERR_INVALID, "scene was not uploaded with VRT_SCENE_EDITABLE");
   if (sx == 0 || sy == 0 || sz == 0) return VRT_OK;
   if ((uint64_t)x0 + sx > scene->nx || (uint64_t)y0 + sy > scene->ny || (uint64_t)z0 + sz > scene->nz)
     return ctx->fail(VRT_ERR_INVALID, "edit box %u+%u, %u+%u, %u+%u outside the %ux%ux%u grid", x0, sx, y0, sy, z0,
@@ -536,11 +622,31 @@ int vrt_scene_edit(vrt_ctx* ctx, vrt_scene* scene, const uint32_t* vox, uint32_t
     int rc = ctx->order_after_serve();
     if (rc) return rc;
   }
+  ScenePins pins(ctx);
+  if (int rc = pins.add(scene)) return rc;
   uint32_t* dense = (uint32_t*)scene->d_dense.ptr;
   for (uint32_t z = 0; z < sz; ++z)
     HIP_TRY(ctx, hipMemcpy2DAsync(dense + ((uint64_t)(z0 + z) * scene->ny + y0) * scene->nx + x0,
                                   (size_t)scene->nx * 4, vox + (uint64_t)z * sx * sy, (size_t)sx * 4,
                                   (size_t)sx * 4, sy, hipMemcpyHostToDevice, ctx->stream));
+  // only the bricks the box touches: their pool entries are rewritten in
+  // place when none of them changes occupancy (the brick words, bitmaps,
+  // prefix counts and row distances then stay as they are); a change of
+  // occupancy moves the slots of later bricks, and the scene is rebuilt
+  const int bx0 = (int)(x0 / 8), by0 = (int)(y0 / 8), bz0 = (int)(z0 / 8);
+  const int ex = (int)((x0 + sx - 1) / 8) - bx0 + 1, ey = (int)((y0 + sy - 1) / 8) - by0 + 1,
+            ez = (int)((z0 + sz - 1) / 8) - bz0 + 1;
+  uint32_t* flag = (uint32_t*)ctx->d_queue.ptr + 2 * vrt::QUEUE_WORDS;  // one word past the work queues
+  HIP_TRY(ctx, vrt::launch_zero_words(ctx->stream, flag, 1u));
+  HIP_TRY(ctx, vrt::launch_edit_bricks(ctx->stream, dense, (int)scene->nx, (int)scene->ny, (int)scene->nx / 8,
+                                       (int)scene->ny / 8, bx0, by0, bz0, ex, ey, ez,
+                                       (const uint32_t*)scene->d_bricks.ptr, (uint32_t*)scene->d_pool.ptr, flag));
+  uint32_t changed = 0;
+  HIP_TRY(ctx, hipMemcpyAsync(&changed, flag, 4, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  scene->edits_in_place += changed ? 0u : 1u;
+  scene->edits_rebuilt += changed ? 1u : 0u;
+  if (!changed) return VRT_OK;
   return scene_build(ctx, scene, nullptr, dense);
   GUARD_END(ctx)
 }
@@ -556,11 +662,36 @@ int vrt_scene_release(vrt_ctx* ctx, vrt_scene* scene) {
                      (unsigned long long)scene->uses);
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
-  ctx->unreserve(scene->bytes);
+  if (!scene->evicted) ctx->unreserve(scene->bytes);  // an evicted scene holds no budget
   ctx->scenes.erase(it);
   delete scene;
   return VRT_OK;
   GUARD_END(ctx)
+}
+
+int vrt_scene_restore(vrt_ctx* ctx, vrt_scene* scene) {
+  if (!ctx || !scene) return ctx ? ctx->fail(VRT_ERR_INVALID, "null argument") : VRT_ERR_INVALID;
+  GUARD_BEGIN
+  std::lock_guard<std::mutex> lock_(ctx->mu);
+  if (std::find(ctx->scenes.begin(), ctx->scenes.end(), scene) == ctx->scenes.end())
+    return ctx->fail(VRT_ERR_INVALID, "unknown scene handle");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  ScenePins pins(ctx);
+  return pins.add(scene);
+  GUARD_END(ctx)
+}
+
+int vrt_scene_resident(const vrt_scene* scene, int32_t* resident) {
+  if (!scene || !resident) return VRT_ERR_INVALID;
+  *resident = scene->evicted ? 0 : 1;
+  return VRT_OK;
+}
+
+int vrt_scene_edit_counts(const vrt_scene* scene, uint64_t* in_place, uint64_t* rebuilt) {
+  if (!scene) return VRT_ERR_INVALID;
+  if (in_place) *in_place = scene->edits_in_place;
+  if (rebuilt) *rebuilt = scene->edits_rebuilt;
+  return VRT_OK;
 }
 
 int vrt_scene_dims(const vrt_scene* scene, uint32_t dims[3], uint64_t* bytes) {
@@ -575,6 +706,8 @@ int vrt_scene_bricks(vrt_ctx* ctx, const vrt_scene* scene, uint32_t* bricks, uin
   GUARD_BEGIN
   std::lock_guard<std::mutex> lock_(ctx->mu);
   HIP_TRY(ctx, hipSetDevice(ctx->device));
+  ScenePins pins(ctx);
+  if (int rc = pins.add(scene)) return rc;
   const uint64_t nb = scene->nbricks;
   std::vector<uint32_t> words(nb);
   HIP_TRY(ctx, hipMemcpyAsync(words.data(), scene->d_bricks.ptr, nb * 4, hipMemcpyDeviceToHost, ctx->stream));
@@ -668,6 +801,8 @@ int vrt_trace_rays_device(vrt_ctx* ctx, const vrt_scene* scene, const float* d_r
   std::lock_guard<std::mutex> lock_(ctx->mu);
   HIP_TRY(ctx, hipSetDevice(ctx->device));
   if (n == 0) { if (ms) *ms = 0.0f; return VRT_OK; }
+  ScenePins pins(ctx);
+  if (int rc = pins.add(scene)) return rc;
   const vrt::SceneDev s = scene->dev();
   HIP_TRY(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
   HIP_TRY(ctx, ctx->launch_trace(s, d_rays, n, tmax, d_hits));
@@ -695,6 +830,8 @@ int vrt_trace_rays_device_noalloc(vrt_ctx* ctx, const vrt_scene* scene, const fl
   if (n == 0) return VRT_OK;
   if (!ctx->noalloc_scratch)
     return ctx->fail(VRT_ERR_DEVICE, "hipGetSymbolAddress gave no scratch address on device %d", ctx->device);
+  if (scene->evicted)  // restoring it would allocate
+    return ctx->fail(VRT_ERR_INVALID, "scene is evicted to host memory; vrt_scene_restore it first");
   // the counters and the queue head start at zero; the last call's values
   // are still there (every call on this GPU shares the array)
   HIP_TRY(ctx, vrt::launch_zero_words(ctx->stream, ctx->noalloc_scratch, NOALLOC_SCRATCH_WORDS));
@@ -762,6 +899,8 @@ int vrt_trace_rays(vrt_ctx* ctx, const vrt_scene* scene, const float* rays, uint
   std::lock_guard<std::mutex> lock_(ctx->mu);
   HIP_TRY(ctx, hipSetDevice(ctx->device));
   if (n == 0) return VRT_OK;
+  ScenePins pins(ctx);
+  if (int rc = pins.add(scene)) return rc;
   HIP_TRY(ctx, ctx->scratch_rays.ensure(n * 24));
   HIP_TRY(ctx, ctx->scratch_hits.ensure(n * 16));
   HIP_TRY(ctx, hipMemcpyAsync(ctx->scratch_rays.ptr, rays, n * 24, hipMemcpyHostToDevice, ctx->stream));
@@ -844,6 +983,9 @@ int vrt_render(vrt_ctx* ctx, const vrt_view* views, uint32_t nviews, vrt_frame* 
     int rc = ctx->order_after_serve();
     if (rc) return rc;
   }
+  ScenePins pins(ctx);
+  for (uint32_t k = 0; k < nviews; ++k)
+    if (int rc = pins.add(views[k].scene)) return rc;
   std::vector<ViewState*> states(nviews);
   for (uint32_t k = 0; k < nviews; ++k) {
     const vrt_view& v = views[k];
@@ -1055,6 +1197,8 @@ int vrt_autotune(vrt_ctx* ctx, const vrt_scene* scene, uint64_t nrays, const vrt
   GUARD_BEGIN
   std::lock_guard<std::mutex> lock_(ctx->mu);
   HIP_TRY(ctx, hipSetDevice(ctx->device));
+  ScenePins pins(ctx);
+  if (int rc = pins.add(scene)) return rc;
   const vrt::SceneDev s = scene->dev();
   uint64_t before[4];
   int rc = ctx->counters_to_host(before);
@@ -1268,6 +1412,9 @@ int vrt_autotune_batch(vrt_ctx* ctx, const vrt_view* views, uint32_t nviews, uin
     int rc = ctx->order_after_serve();
     if (rc) return rc;
   }
+  ScenePins pins(ctx);
+  for (uint32_t k = 0; k < nviews; ++k)
+    if (int rc = pins.add(views[k].scene)) return rc;
   // the sweep renders into view states of its own: the callers' histories
   // are not touched
   std::vector<ViewState> tmp(nviews);
@@ -1372,6 +1519,8 @@ int vrt_stats_get(vrt_ctx* ctx, vrt_stats* out) {
   out->sum_ms_temporal = ctx->sum_temporal;
   out->sum_ms_atrous = ctx->sum_atrous;
   out->sum_ms_frame = ctx->sum_frame;
+  out->scene_evictions = ctx->scene_evictions;
+  out->scene_restores = ctx->scene_restores;
   out->rays = c[0];
   out->brick_steps = c[1];
   out->voxel_steps = c[2];

@@ -175,7 +175,7 @@ bool Server::make_room(uint64_t need) {
         found = true;
       }
     }
-    if (!found) return false;
+    if (!found) return ctx->make_scene_room(need, nullptr);  // last: idle evictable scenes
     auto it = ctx->views.find(victim);
     ctx->unreserve(it->second.bytes);
     ctx->views.erase(it);
@@ -236,10 +236,14 @@ bool Server::launch(uint32_t li) {
         need += rbytes;
       }
     }
-    bool room = make_room(need);
+    // an evicted scene comes back first (it has queued requests, so nothing
+    // evicts it again before the launch)
+    vrt_scene* scn = const_cast<vrt_scene*>(v.scene);
+    bool room = make_room(need + (scn->evicted ? scn->bytes : 0));
     ViewState* state = nullptr;
     int rc = VRT_OK;
-    if (room) {
+    if (room && scn->evicted) rc = ctx->restore_scene(scn);
+    if (room && rc == VRT_OK) {
       rc = ctx->view_state(vid, w, h, v.accum_format, &state, L.lane.stream);
       if (rc == VRT_OK && pooled) {
         t.result = pooled;
@@ -248,10 +252,10 @@ bool Server::launch(uint32_t li) {
       } else if (rc == VRT_OK) {
         rc = acquire_result(rbytes, &t.result, &t.result_bytes);
       }
-      // out of budget after all (e.g. a reserve elsewhere in between): the
-      // same as no room -- wait for running batches, or fail below
-      if (rc == VRT_ERR_BUDGET) room = false;
     }
+    // out of budget after all (e.g. a reserve elsewhere in between): the
+    // same as no room -- wait for running batches, or fail below
+    if (rc == VRT_ERR_BUDGET) room = false;
     if (pooled) release_result(pooled, pooled_bytes);  // not used: back to the pool
     if (!room) {
       // nothing left to evict: wait for running batches to finish, or fail

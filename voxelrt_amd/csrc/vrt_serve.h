@@ -5,7 +5,8 @@
 // the serve streams ("lanes") has no batch on the device, the worker takes
 // queued requests in order -- at most one frame per view, at most
 // max_batch_views requests and max_batch_pixels pixel-samples -- admits each
-// by HBM footprint (evicting idle view state and pooled result buffers when
+// by HBM footprint (evicting pooled result buffers, idle view state and idle
+// evictable scenes, in that order, when
 // needed), and enqueues the batched passes for them on that lane.  Lanes run
 // concurrently on the device, so a request that arrives while one batch runs
 // starts in the next free lane without waiting for the running batch.  The

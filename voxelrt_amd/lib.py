@@ -25,6 +25,7 @@ SCENE_RANDOM_BRICKS = 1
 SCENE_WORLD = 2
 SCENE_VOXELRT = 3
 SCENE_EDITABLE = 1
+SCENE_EVICTABLE = 2
 
 ERRORS = {-1: "VRT_ERR_INVALID", -2: "VRT_ERR_DEVICE", -3: "VRT_ERR_NO_MEMORY",
           -4: "VRT_ERR_BUDGET", -5: "VRT_ERR_NOT_READY", -6: "VRT_ERR_CANCELLED",
@@ -77,7 +78,8 @@ class Stats(C.Structure):
                 ("gpu_ms_frame", C.c_double), ("hbm_resident", C.c_uint64),
                 ("hbm_budget", C.c_uint64), ("timed_calls", C.c_uint64),
                 ("sum_ms_pathtrace", C.c_double), ("sum_ms_temporal", C.c_double),
-                ("sum_ms_atrous", C.c_double), ("sum_ms_frame", C.c_double)]
+                ("sum_ms_atrous", C.c_double), ("sum_ms_frame", C.c_double),
+                ("scene_evictions", C.c_uint64), ("scene_restores", C.c_uint64)]
 
 
 class Tuning(C.Structure):
@@ -166,6 +168,12 @@ def _declare(L):
                                  C.c_uint32]
     L.vrt_scene_upload_ex.restype = I
     L.vrt_scene_edit.restype = I
+    L.vrt_scene_edit_counts.argtypes = [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+    L.vrt_scene_edit_counts.restype = I
+    L.vrt_scene_restore.argtypes = [P, P]
+    L.vrt_scene_restore.restype = I
+    L.vrt_scene_resident.argtypes = [P, C.POINTER(C.c_int32)]
+    L.vrt_scene_resident.restype = I
     L.vrt_scene_bricks.argtypes = [P, P, P, P]
     L.vrt_scene_dims.argtypes = [P, P, C.POINTER(C.c_uint64)]
     L.vrt_trace_rays.argtypes = [P, P, P, C.c_uint64, C.c_float, P]
@@ -311,11 +319,28 @@ class Scene:
 
     def edit(self, box: np.ndarray, origin):
         """Write box (sz, sy, sx) uint32 at voxel origin (x0, y0, z0) of an
-        editable scene and rebuild it on the device."""
+        editable scene and update it on the device (the touched bricks in
+        place, or a rebuild when their occupancy changes)."""
         box = np.ascontiguousarray(box, np.uint32)
         sz, sy, sx = box.shape
         x0, y0, z0 = (int(c) for c in origin)
         self.ctx._check(self.ctx.L.vrt_scene_edit(self.ctx.h, self.handle, _ptr(box), x0, y0, z0, sx, sy, sz))
+
+    def restore(self):
+        """Bring the scene back to the device if it was evicted."""
+        self.ctx._check(self.ctx.L.vrt_scene_restore(self.ctx.h, self.handle))
+
+    def resident(self) -> bool:
+        r = C.c_int32()
+        self.ctx._check(self.ctx.L.vrt_scene_resident(self.handle, C.byref(r)))
+        return bool(r.value)
+
+    def edit_counts(self):
+        """(edits applied in place to the touched bricks, edits that rebuilt
+        the scene)."""
+        a, b = C.c_uint64(), C.c_uint64()
+        self.ctx._check(self.ctx.L.vrt_scene_edit_counts(self.handle, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     def release(self):
         if self.handle:
@@ -355,12 +380,12 @@ class Context:
         if rc != 0:
             raise VrtError(rc, self.L.vrt_last_error(self.h).decode())
 
-    def upload(self, vox: np.ndarray, key: int = 0, editable: bool = False) -> Scene:
+    def upload(self, vox: np.ndarray, key: int = 0, editable: bool = False, evictable: bool = False) -> Scene:
         vox = np.ascontiguousarray(vox, np.uint32)
         nz, ny, nx = vox.shape
         out = C.c_void_p()
-        self._check(self.L.vrt_scene_upload_ex(self.h, _ptr(vox), nx, ny, nz, key,
-                                               SCENE_EDITABLE if editable else 0, C.byref(out)))
+        flags = (SCENE_EDITABLE if editable else 0) | (SCENE_EVICTABLE if evictable else 0)
+        self._check(self.L.vrt_scene_upload_ex(self.h, _ptr(vox), nx, ny, nz, key, flags, C.byref(out)))
         return Scene(self, out.value, (nx, ny, nz))
 
     def trace_rays(self, scene: Scene, rays: np.ndarray, tmax: float = 1e4) -> np.ndarray:
