@@ -19,7 +19,11 @@ formula in float64 (numpy), not from the oracle's code:
   most paths end early), and a bounce that misses adds the sky without the
   sun disc (an open sunlit floor at 1 bounce has mean
   E n.l / pi + (1/pi) * integral of sky(w) cos(theta) over the hemisphere,
-  the integral by float64 quadrature).
+  the integral by float64 quadrature);
+
+  the a-trous weights' exponential exp_w against float64 exp (error bounded
+  by the rounding of x log2 e and the polynomial), its special values, and
+  the bilinear filter position sub_texel (8 fractional bits, nearest-even).
 """
 import math
 
@@ -308,3 +312,58 @@ def test_bounce_miss_adds_sky_without_disc():
     assert np.all(np.abs(mean - want) < 5 * se + 2e-4 * want), (mean, want, se)
     # a disc on the bounce ray would add about the direct term once more
     assert np.all(direct > 100 * se)
+
+
+# ---------------------------------------------------------------------------
+# the weights' exponential and the bilinear filter position
+# ---------------------------------------------------------------------------
+def test_exp_w_against_float64_exp():
+    """exp_w (e^x for the a-trous weights, x <= 0) against float64 exp: the
+    rounding of x log2(e) to float costs up to |x| 2^-24 relative, the
+    degree-7 polynomial ~1e-8, so |rel| <= 1.2e-7 for |x| < 1 and
+    1e-7 (|x| + 1) up to the smallest normal result; a dropped last
+    polynomial term (1.7e-7 at x = -ln2/2) or a wrong ln 2 fails it."""
+    xs = np.concatenate([-np.logspace(-9, math.log10(87.0), 4000), np.linspace(-87.0, 0.0, 4000),
+                         [-0.34657359, -0.5, -0.6931472, -1.0]]).astype(np.float32)
+    got = np.array([oracle.exp_w(float(x)) for x in xs], np.float64)
+    want = np.exp(xs.astype(np.float64))
+    rel = np.abs(got - want) / want
+    small = np.abs(xs) < 1.0
+    assert rel[small].max() <= 1.2e-7, rel[small].max()
+    assert (rel <= 1e-7 * (np.abs(xs.astype(np.float64)) + 1.0)).all(), rel.max()
+
+
+def test_exp_w_special_values():
+    """e^0 = 1 exactly; NaN, -inf and arguments whose power of two is below
+    2^-160 give 0; results below the smallest normal are within a few
+    denormal steps."""
+    assert oracle.exp_w(0.0) == 1.0 and oracle.exp_w(-0.0) == 1.0
+    assert oracle.exp_w(float("nan")) == 0.0
+    assert oracle.exp_w(-float("inf")) == 0.0
+    assert oracle.exp_w(-120.0) == 0.0
+    for x in (-90.0, -95.0, -100.0):
+        assert abs(oracle.exp_w(x) - math.exp(x)) <= 32 * 2.0 ** -149
+
+
+def test_sub_texel_is_nearest_eighth_bit_position():
+    """The bilinear filter position at 8 fractional bits: texel coordinates
+    on the 1/256 grid come back exactly as (floor, fraction); midpoints
+    between grid points round to the even grid point; the rounded position
+    is within 1/512 of the coordinate and never decreases."""
+    ks = np.arange(-1024, 1024 * 4)
+    for k in ks[::7]:
+        tc = float(k) / 256.0
+        fl, fr = oracle.sub_texel(tc)
+        assert fl == math.floor(tc) and fr == tc - math.floor(tc), (tc, fl, fr)
+    for k in (-3, -2, 1, 2, 5, 6, 255, 256, 257):
+        tc = (k + 0.5) / 256.0
+        q = k if k % 2 == 0 else k + 1
+        fl, fr = oracle.sub_texel(tc)
+        assert fl + fr == q / 256.0, (tc, fl, fr)
+    rng = np.random.default_rng(3)
+    tcs = np.sort(rng.uniform(-4.0, 1920.0, 5000).astype(np.float32))
+    pos = np.array([sum(oracle.sub_texel(float(t))) for t in tcs])
+    assert (np.abs(pos - tcs.astype(np.float64)) <= 1.0 / 512 + 1e-4).all()
+    assert (np.diff(pos) >= 0).all()
+    fr = np.array([oracle.sub_texel(float(t))[1] for t in tcs])
+    assert ((fr >= 0) & (fr < 1)).all() and np.allclose(fr * 256, np.round(fr * 256))
