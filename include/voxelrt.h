@@ -328,8 +328,7 @@ typedef struct vrt_tuning {
                                      waves per block share it), pt_tile 1 = on global
                                      memory always.
                                  1, 2, 3, 5 and 6 are rejected.  The batched forms
-                                 (vrt_render_batch) run pt 0, 4 or 7; with pt 8 they
-                                 run pt 7. */
+                                 run pt 0, 4, 7 or 8 (see batch_tile). */
   uint32_t pt_blocks;         /* pt 8: persistent waves, 0 = 16 per CU */
   uint32_t unused0;           /* not read; keeps the layout */
   uint32_t pt_min_refill;     /* pt 8: lanes (of 64) waiting at a brick boundary
@@ -369,8 +368,12 @@ typedef struct vrt_tuning {
                                  choice, 6 (pays on sparse scenes) */
   uint32_t pt_shade_min;      /* pt 8: finished rays (of 64) a wave collects before it
                                  shades them together (0 = 1: at once) */
-  uint32_t batch_tile;        /* pt 8 in the batched forms, which run pt 7 with 4
-                                 waves per SIMD: its pt_tile (0 = 5) */
+  uint32_t batch_tile;        /* pt 8 in the batched forms: 0 = its batched form (one-wave
+                                 blocks, the DDA on global memory, each wave on one
+                                 view at a time: chunks of 8 patches of 8x8 pixels
+                                 from a queue split over eight counters; pt_blocks
+                                 waves, 0 = 16 per CU); 2..5 = pt 7 with 4 waves per
+                                 SIMD on that pt_tile form */
   uint32_t reserved;
 } vrt_tuning;
 int vrt_tuning_get(vrt_ctx* ctx, vrt_tuning* out);

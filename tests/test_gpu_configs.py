@@ -73,8 +73,9 @@ def batch_scenes(ctx):
         sc.release()
 
 
-@pytest.mark.parametrize("mode", [0, 1, 2], ids=["grid", "queue", "graph"])
-def test_batch_views_match_oracle(ctx, batch_scenes, mode):
+@pytest.mark.parametrize("mode,kernel", [(0, 7), (1, 7), (2, 7), (0, 8), (2, 8)],
+                         ids=["grid", "queue", "graph", "wave-grid", "wave-graph"])
+def test_batch_views_match_oracle(ctx, batch_scenes, mode, kernel):
     descs, scenes, grids = batch_scenes
     rng = np.random.default_rng(40 + mode)
     specs = []  # one view per scene: size, bounces 1..6, accumulation format
@@ -83,7 +84,11 @@ def test_batch_views_match_oracle(ctx, batch_scenes, mode):
         w = min(480, int(round(h * [1.0, 4 / 3, 16 / 9, 21 / 9][si % 4])))
         specs.append(dict(si=si, w=w, h=h, nb=1 + si % 6, fmt=si % 2, seed=7717 * si))
     old = ctx.tuning().as_dict()
-    ctx.set_tuning(pt_kernel=7, pt_waves=4, pt_tile=5, batch_mode=mode)
+    if kernel == 8:  # the step-level refill kernel's batched form
+        ctx.set_tuning(pt_kernel=8, pt_waves=4, pt_tile=0, batch_tile=0, batch_mode=mode, pt_blocks=0,
+                       pt_min_refill=16, pt_shade_min=32)
+    else:
+        ctx.set_tuning(pt_kernel=7, pt_waves=4, pt_tile=5, batch_mode=mode)
     chains = [OracleChain(grids[s["si"]], s["w"], s["h"], s["fmt"]) for s in specs]
     prev = [(None, None)] * len(specs)
     try:
@@ -124,7 +129,7 @@ def test_served_mix_with_1024_scene_matches_oracle():
              (0, 160, 90, 1, 2, 3, 0, 0), (1, 64, 36, 4, 4, 0, 0, 3), (0, 160, 90, 1, 2, 3, 0, 0),
              (0, 200, 112, 2, 4, 3, 1, 4), (2, 48, 48, 3, 2, 3, 0, 5)]
     with v.Context(0) as ctx:
-        ctx.set_tuning(pt_kernel=7, pt_waves=4, pt_tile=5, batch_mode=2)
+        ctx.set_tuning(pt_kernel=8, pt_waves=4, batch_tile=0, batch_mode=2)
         ctx.serve_configure(streams=2, max_batch_views=4)
         scenes = [ctx.upload(x) for x in vox]
         chains, prev, frames, tickets, wants = {}, {}, {}, [], []

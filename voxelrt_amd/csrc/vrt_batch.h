@@ -9,6 +9,7 @@
 // the per-pixel code of the single-view kernels: results are bit-identical
 // to rendering each view alone.
 #pragma once
+#include "vrt_kernels.h"
 #include "vrt_trace.h"
 
 namespace vrt {
@@ -21,6 +22,8 @@ struct ViewDesc {
   uint32_t w, h, tiles_x, denoise;
   uint32_t rtile0;  // first of the view's path-trace tiles of the refill forms
   uint32_t rtx;     // their count across the view
+  uint32_t wchunk0; // first of the view's patch chunks of the step-level refill form
+  uint32_t pad0;
   void* radiance;    // rgba16f
   void* albedo;      // rgba8unorm
   void* gbuf_cur;    // rgba32f
@@ -35,15 +38,18 @@ struct ViewDesc {
 
 // Header of the device descriptor table, at offset 0; the ViewDesc array
 // follows at BATCH_HEAD_BYTES.  `queue` is the shared tile queue of the
-// persistent path-trace launch; the host uploads it as 0 with every table.
+// persistent path-trace launch and `wq` the split queue of the step-level
+// refill form's chunks; the host uploads both as 0 with every table.
 struct BatchHead {
   uint32_t nviews;
   uint32_t tiles;
   uint32_t queue;
-  uint32_t rtiles;  // path-trace tiles of the refill forms (32x32, 32x16, 16x16 or 16x8)
-  uint32_t pad[12];
+  uint32_t rtiles;   // path-trace tiles of the refill forms (32x32, 32x16, 16x16 or 16x8)
+  uint32_t wchunks;  // patch chunks of the step-level refill form
+  uint32_t pad[11];
+  uint32_t wq[QUEUE_WORDS];
 };
-constexpr uint32_t BATCH_HEAD_BYTES = 64;
+constexpr uint32_t BATCH_HEAD_BYTES = 64 + QUEUE_WORDS * 4;
 static_assert(sizeof(BatchHead) == BATCH_HEAD_BYTES, "BatchHead layout");
 
 // Index of the view owning tile `b` (views sorted by tile0, tile0[0] = 0).

@@ -26,6 +26,9 @@ constexpr uint32_t LDS_BIT_WORDS_MAX = 12288;
 // u32 words of a persistent kernel's work queue (the split queue's eight
 // counters, one per 128-byte line; the single-counter kernels use word 0).
 constexpr uint32_t QUEUE_WORDS = 256;
+// 8x8 patches per work item of the batched step-level refill path tracer (a
+// wave finishes its paths before it moves to another view's chunk).
+constexpr uint32_t WAVE_CHUNK = 8;
 
 hipError_t launch_brick_masks(hipStream_t st, const uint32_t* vox, int nx, int ny, int bgx,
                               int bgy, int bz0, int nbz, uint32_t* masks, uint32_t* flags);
@@ -106,6 +109,15 @@ hipError_t launch_batch_persistent(hipStream_t st, BatchHead* head, const ViewDe
                                    uint32_t max_pwords, uint32_t pt_blocks, uint32_t gs_blocks, bool denoise,
                                    bool hdr, int rform);
 hipError_t launch_atrous_batch(hipStream_t st, const ViewDesc* views, uint32_t nviews, uint32_t tiles, int pass);
+// The step-level refill path tracer over a batch table (vrt_pathtrace_wave.h
+// k_pathtrace_wave_batch): `blocks` one-wave blocks drawing chunks of the
+// views' 8x8 patches from head->wq (zero at launch; the table upload zeroes
+// it); then the filter passes on a fixed grid-stride grid of gs_blocks.
+hipError_t launch_pathtrace_wave_batch(hipStream_t st, BatchHead* head, const ViewDesc* views,
+                                       unsigned long long* counters, int blocks, int min_wait, int shade_min,
+                                       int waves);
+hipError_t launch_batch_filters(hipStream_t st, BatchHead* head, const ViewDesc* views, uint32_t gs_blocks,
+                                bool denoise, bool hdr);
 hipError_t launch_temporal(hipStream_t st, const Uniforms& u, const void* radiance,
                            const void* gcur, const void* aprev, const void* gprev, void* aout,
                            int fmt);

@@ -1736,6 +1736,32 @@ hipError_t launch_trace_wave(hipStream_t st, const SceneDev& s, const float* ray
   return hipGetLastError();
 }
 
+hipError_t launch_pathtrace_wave_batch(hipStream_t st, BatchHead* head, const ViewDesc* views,
+                                       unsigned long long* counters, int blocks, int min_wait, int shade_min,
+                                       int waves) {
+  const uint32_t mw = min_wait < 1 ? 1u : (min_wait > 64 ? 64u : (uint32_t)min_wait);
+  const uint32_t sm = shade_min < 1 ? 1u : (shade_min > 64 ? 64u : (uint32_t)shade_min);
+  const dim3 g(blocks > 0 ? blocks : 1);
+  if (waves == 5) hipLaunchKernelGGL(k_pathtrace_wave_batch<5>, g, dim3(64), 0, st, head, views, counters, mw, sm);
+  else if (waves == 6) hipLaunchKernelGGL(k_pathtrace_wave_batch<6>, g, dim3(64), 0, st, head, views, counters, mw, sm);
+  else if (waves == 4) hipLaunchKernelGGL(k_pathtrace_wave_batch<4>, g, dim3(64), 0, st, head, views, counters, mw, sm);
+  else hipLaunchKernelGGL(k_pathtrace_wave_batch<1>, g, dim3(64), 0, st, head, views, counters, mw, sm);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_filters(hipStream_t st, BatchHead* head, const ViewDesc* views, uint32_t gs_blocks,
+                                bool denoise, bool hdr) {
+  const dim3 gg(gs_blocks > 0 ? gs_blocks : 1);
+  hipLaunchKernelGGL(k_temporal_batch_gs, gg, dim3(256), 0, st, head, views);
+  if (denoise) {
+    hipLaunchKernelGGL(k_atrous_batch_gs<0>, gg, dim3(256), 0, st, head, views);
+    hipLaunchKernelGGL(k_atrous_batch_gs<1>, gg, dim3(256), 0, st, head, views);
+    hipLaunchKernelGGL(k_atrous_batch_gs<2>, gg, dim3(256), 0, st, head, views);
+  }
+  if (hdr) hipLaunchKernelGGL(k_hdr_batch_gs, gg, dim3(256), 0, st, head, views);
+  return hipGetLastError();
+}
+
 int refill_form(int tv, int form, uint32_t max_bit_words, uint32_t max_pwords) {
   tv = lds_variant(tv, max_bit_words, max_pwords);
   if ((form == 4 || form == 5) && tv == TV_FASTG) return form;

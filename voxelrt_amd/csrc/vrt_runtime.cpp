@@ -1480,9 +1480,9 @@ int vrt_autotune_batch(vrt_ctx* ctx, const vrt_view* views, uint32_t nviews, uin
         if (!(modes >> mode & 1u) || (mode == 0u && b != 0u)) continue;
         ctx->tuning = start;
         ctx->tuning.batch_mode = mode;
-        // pt 8 keeps its own pt_tile (the single-view kernel's DDA memory)
+        // pt 8 keeps its own pt_tile (the single-view kernel's DDA memory);
+        // its batch_tile 0 is the step-level refill batch kernel
         if (start.pt_kernel == 8) {
-          if (form == 0u) continue;  // batch_tile 0 is form 5
           ctx->tuning.batch_tile = form;
         } else {
           ctx->tuning.pt_tile = form;
@@ -1648,14 +1648,22 @@ int vrt_ctx::render_batch(vrt::BatchLane& lane, const vrt_view* views, uint32_t 
     tiles += d[k].tiles_x * ((vs.h + 15u) / 16u);
   }
   // refill forms: the path tracer's own 16 x rth tiles, numbered per view.
-  // The step-level refill kernel (pt 8) has no batched form: its batches run
-  // pt 7 with 4 waves per SIMD on the tile form batch_tile (0 = one-wave
-  // blocks of 16x8 refill tiles).
+  // The step-level refill kernel (pt 8) runs its batched form
+  // (k_pathtrace_wave_batch) with batch_tile 0, or pt 7 with 4 waves per
+  // SIMD on the tile form batch_tile.
   const bool wave_pt = tuning.pt_kernel == 8;
+  const bool wave_batch = wave_pt && tuning.batch_tile == 0;
   const int ptv = pt_variant(tuning.pt_kernel);
-  const int pt_waves = wave_pt ? 4 : (int)tuning.pt_waves;
-  const int tile = wave_pt ? (tuning.batch_tile ? (int)tuning.batch_tile : 5) : (int)tuning.pt_tile;
-  const int rform = vrt::refill_form(ptv, tile, max_bit_words, max_pwords);
+  const int pt_waves = wave_pt && !wave_batch ? 4 : (int)tuning.pt_waves;
+  const int tile = wave_pt ? (int)tuning.batch_tile : (int)tuning.pt_tile;
+  const int rform = wave_batch ? 0 : vrt::refill_form(ptv, tile, max_bit_words, max_pwords);
+  uint32_t wchunks = 0;
+  for (uint32_t k = 0; k < nviews; ++k) {
+    d[k].wchunk0 = wchunks;
+    const uint32_t np = ((d[k].w + 7u) / 8u) * ((d[k].h + 7u) / 8u);
+    wchunks += (np + vrt::WAVE_CHUNK - 1u) / vrt::WAVE_CHUNK;
+  }
+  const uint32_t wave_blocks = tuning.pt_blocks ? tuning.pt_blocks : (uint32_t)cus * 16u;
   uint32_t rtiles = 0;
   if (rform != 0) {
     uint32_t tw = 0, th = 0;
@@ -1671,6 +1679,7 @@ int vrt_ctx::render_batch(vrt::BatchLane& lane, const vrt_view* views, uint32_t 
   hh->tiles = tiles;
   hh->queue = 0;
   hh->rtiles = rtiles;
+  hh->wchunks = wchunks;
   vrt::BatchHead* dh = (vrt::BatchHead*)lane.desc_dev.ptr;
   const vrt::ViewDesc* dd = (const vrt::ViewDesc*)((char*)lane.desc_dev.ptr + vrt::BATCH_HEAD_BYTES);
   HIP_TRY(this, hipMemcpyAsync(lane.desc_dev.ptr, lane.desc_host[stage],
@@ -1683,6 +1692,12 @@ int vrt_ctx::render_batch(vrt::BatchLane& lane, const vrt_view* views, uint32_t 
     const uint32_t ptb = tuning.batch_blocks ? tuning.batch_blocks : (uint32_t)cus * (rform >= 4 ? 32u : 8u);
     const uint32_t gsb = (uint32_t)cus * 8u;
     auto enqueue = [&] {
+      if (wave_batch) {
+        const hipError_t e = vrt::launch_pathtrace_wave_batch(st, dh, dd, (unsigned long long*)d_counters.ptr,
+                                                              (int)wave_blocks, (int)tuning.pt_min_refill,
+                                                              (int)tuning.pt_shade_min, wv);
+        return e != hipSuccess ? e : vrt::launch_batch_filters(st, dh, dd, gsb, any_denoise, any_hdr);
+      }
       return vrt::launch_batch_persistent(st, dh, dd, (unsigned long long*)d_counters.ptr, tv, wv, max_bit_words,
                                           max_pwords, ptb, gsb, any_denoise, any_hdr, rform);
     };
@@ -1690,9 +1705,12 @@ int vrt_ctx::render_batch(vrt::BatchLane& lane, const vrt_view* views, uint32_t 
       HIP_TRY(this, enqueue());
     } else {
       // one graph per launch shape; the table it reads is rewritten per batch
+      const uint64_t shape = wave_batch ? ((uint64_t)wave_blocks | (uint64_t)std::min(tuning.pt_min_refill, 64u) << 21 |
+                                           (uint64_t)std::min(tuning.pt_shade_min, 64u) << 28)
+                                        : (uint64_t)ptb;
       const std::pair<uint64_t, uint64_t> key{
           (uint64_t)tv | (uint64_t)wv << 4 | (uint64_t)rform << 8 | (uint64_t)any_denoise << 12 |
-              (uint64_t)any_hdr << 13 | (uint64_t)ptb << 16,
+              (uint64_t)any_hdr << 13 | (uint64_t)wave_batch << 14 | shape << 16,
           (uint64_t)max_bit_words | (uint64_t)max_pwords << 32};
       auto it = lane.graphs.find(key);
       if (it == lane.graphs.end()) {
@@ -1718,8 +1736,12 @@ int vrt_ctx::render_batch(vrt::BatchLane& lane, const vrt_view* views, uint32_t 
     HIP_TRY(this, hipEventRecord(ev[3], st));
     return VRT_OK;
   }
-  HIP_TRY(this, vrt::launch_pathtrace_batch(st, dd, nviews, tiles, (unsigned long long*)d_counters.ptr,
-                                            ptv, pt_waves, max_bit_words, max_pwords, rform, rtiles));
+  if (wave_batch)
+    HIP_TRY(this, vrt::launch_pathtrace_wave_batch(st, dh, dd, (unsigned long long*)d_counters.ptr, (int)wave_blocks,
+                                                   (int)tuning.pt_min_refill, (int)tuning.pt_shade_min, pt_waves));
+  else
+    HIP_TRY(this, vrt::launch_pathtrace_batch(st, dd, nviews, tiles, (unsigned long long*)d_counters.ptr,
+                                              ptv, pt_waves, max_bit_words, max_pwords, rform, rtiles));
   HIP_TRY(this, hipEventRecord(ev[1], st));
   HIP_TRY(this, vrt::launch_temporal_batch(st, dd, nviews, tiles));
   HIP_TRY(this, hipEventRecord(ev[2], st));
