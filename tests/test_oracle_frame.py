@@ -175,3 +175,28 @@ def test_present_reads_f16_history():
     a = oracle.present(f.view(np.uint16), 0, alb, 1.0)
     b = oracle.present(f.astype(np.float32), 1, alb, 1.0)
     assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("fmt", [0, 1], ids=["f16", "f32"])
+def test_hdr_capture_matches_float64_formula(fmt):
+    """HDR capture (main.js:628-659): each channel is ir * a * a in double
+    with a = albedo byte / 255, stored to a Float32Array -- restated here
+    with numpy float64 from the raw texels (f16 history read as halves);
+    includes bytes 0 and 255, zero, huge and denormal-range illumination."""
+    rng = np.random.default_rng(77 + fmt)
+    h, w = 9, 13
+    alb = rng.integers(0, 256, (h, w, 4), dtype=np.uint8)
+    alb[0, 0, :3] = (0, 255, 1)
+    if fmt == 0:
+        bits = rng.integers(0, 0x7BFF, (h, w, 4), dtype=np.uint16)  # finite non-negative halves
+        bits[0, 0, :3] = (0, 0x7BFF, 0x0001)                          # 0, max, smallest denormal
+        illum = bits
+        ir = bits.view(np.float16).astype(np.float64)
+    else:
+        illum = (rng.random((h, w, 4)) * 50.0).astype(np.float32)
+        illum[0, 0, :3] = (0.0, 3.0e38, 1.0e-40)
+        ir = illum.astype(np.float64)
+    a = alb[..., :3].astype(np.float64) / 255.0
+    want = (ir[..., :3] * a * a).astype(np.float32)
+    got = oracle.hdr(np.ascontiguousarray(illum), fmt, alb)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
