@@ -3,8 +3,9 @@
   frame_1080  1920x1080 on the 256^3 frame scene (partial tile rows: 1080 is
               67.5 tiles of 16), three orbit frames through the whole chain
               (pathtrace -> temporal -> a-trous x3 -> HDR), with the path
-              tracer forms the benchmark's autotuner picks (the single loop on
-              global memory, 4 waves per SIMD, 32x16 and 16x8 refill tiles);
+              tracer the benchmark times (step-level refill, 4 waves per SIMD,
+              the DDA in LDS, min_wait 12 / shade_min 24), its global-memory
+              form, and the refill-tile forms (32x16 and 16x8 tiles);
   batch       the eight batch scenes (64^3 .. 512^3, two of each) with one
               view each at reduced size, bounces 1..6, mixed accumulation
               formats, rendered as ONE batched call (grid launches, the
@@ -36,9 +37,13 @@ def frame1080(ctx):
     sc.release()
 
 
-@pytest.mark.parametrize("tuning", [dict(pt_kernel=7, pt_waves=4, pt_tile=3, pt_order=2),
+@pytest.mark.parametrize("tuning", [dict(pt_kernel=8, pt_waves=4, pt_tile=0, pt_order=2, pt_blocks=0,
+                                         pt_min_refill=12, pt_shade_min=24),
+                                    dict(pt_kernel=8, pt_waves=4, pt_tile=1, pt_order=2, pt_blocks=0,
+                                         pt_min_refill=16, pt_shade_min=32),
+                                    dict(pt_kernel=7, pt_waves=4, pt_tile=3, pt_order=2),
                                     dict(pt_kernel=7, pt_waves=4, pt_tile=5, pt_order=2)],
-                         ids=["refill32x16", "refill16x8"])
+                         ids=["wave-lds", "wave-global", "refill32x16", "refill16x8"])
 def test_frame_1080_chain_bitexact(ctx, frame1080, tuning):
     vox, sc, og = frame1080
     p = cf.FRAME_1080

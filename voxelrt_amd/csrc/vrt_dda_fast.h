@@ -127,8 +127,9 @@ VRT_D bool bit_at(uint32_t w, uint32_t i) {
 // entry) with a 4-byte stride, so the level still only switches registers.
 // KM: the brick-entry cell counts through two per-ray constants (kfine,
 // mfine) instead of per-axis selects: fewer instructions on the entry path,
-// two more live registers -- the path tracer's choice; the traversal kernels,
-// tighter on registers, keep pack_left.
+// two more live registers -- the step-level kernels' choice (path tracer and
+// traversal, whose occupancy LDS, not registers, bounds); the one-ray-per-lane
+// kernels keep pack_left.
 template <bool GM, bool KM = false>
 struct DdaFastT {
   float ro[3], rdv[3], inv[3];

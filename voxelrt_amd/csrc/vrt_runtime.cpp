@@ -370,7 +370,7 @@ int vrt_create(const vrt_create_info* info, vrt_ctx** out) {
   (void)hipMemGetInfo(&freeb, &total);
   ctx->budget = (info && info->hbm_budget) ? info->hbm_budget : (uint64_t)(0.9 * (double)freeb);
   ctx->tuning.trace_kernel = 7;
-  ctx->tuning.trace_blocks = 1024;
+  ctx->tuning.trace_blocks = 1280;  // five 256-thread blocks per CU: five waves per SIMD
   ctx->tuning.trace_steps = 32;
   ctx->tuning.trace_min_refill = 16;
   ctx->tuning.dda_batch = 16;
@@ -1293,7 +1293,7 @@ int vrt_autotune(vrt_ctx* ctx, const vrt_scene* scene, uint64_t nrays, const vrt
     c.trace_kernel = 7;
     c.dda_batch = 1;
     c.trace_waves = 0;
-    for (uint32_t b : {768u, 1024u, 1536u})
+    for (uint32_t b : {1024u, 1280u, 1536u})
       for (uint32_t mw : {12u, 16u})
         for (uint32_t fm : {24u, 32u}) {
           c.trace_blocks = b;

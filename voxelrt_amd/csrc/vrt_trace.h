@@ -68,9 +68,23 @@ struct TraceResult {
   uint32_t face;    // VoxelBench face id, 6 = zero normal, 255 = miss
 };
 
-VRT_D float safe_inv(float v) {  // voxel.wgsl:65-71
-  float s = v >= 0.0f ? 1.0f : -1.0f;
-  return s / vmax(vabs(v), 1e-8f);
+// voxel.wgsl:65-71: s / max(|v|, 1e-8) with s = +-1, correctly rounded.  On
+// the device, for divisors up to 2^64, as v_rcp_f32 refined by one Newton
+// step (e = 1 - b*r, r + e*r, each one FMA): that equals the IEEE quotient
+// 1/b for every float b in [1e-8, 2^64] (checked exhaustively on gfx950 by
+// tests/rcp_exhaustive.hip), and -1/b is its negation.  Larger divisors and
+// NaN take the division.
+VRT_D float safe_inv(float v) {
+  const float b = vmax(vabs(v), 1e-8f);
+#if defined(__HIP_DEVICE_COMPILE__)
+  if (b <= 0x1p64f) {
+    const float r = __builtin_amdgcn_rcpf(b);
+    const float q = __builtin_fmaf(__builtin_fmaf(-b, r, 1.0f), r, r);
+    return v >= 0.0f ? q : -q;
+  }
+#endif
+  const float s = v >= 0.0f ? 1.0f : -1.0f;
+  return s / b;
 }
 VRT_D int sign_i(float v) { return v > 0.0f ? 1 : (v < 0.0f ? -1 : 0); }
 VRT_D int floor_clamp(float x, int lo, int hi) {

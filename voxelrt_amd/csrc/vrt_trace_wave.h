@@ -37,7 +37,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WV))) void k
   if constexpr (!GM) L0 = lds_setup<TV_FAST>(s, lds_dyn).f;  // every thread of the block, one barrier
   const uint32_t lane = threadIdx.x & 63u;
   const uint64_t lt_mask = lane == 0u ? 0ull : (~0ull >> (64u - lane));
-  DdaFastT<GM, false> st;
+  DdaFastT<GM, true> st;
   uint32_t status = LS_NEED, ray = 0u;
   uint32_t pf_id = RAY_NONE;  // the lane's prefetched ray, RAY_NONE when its slot is empty
   float pf[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 1.0f};
