@@ -421,6 +421,28 @@ def test_frame_sequence_bitexact(ctx, pt_variant):
     sc.release()
 
 
+@pytest.mark.parametrize("fmt", [0, 1], ids=["rgba16f", "rgba32f"])
+def test_filter_chain_ragged(ctx, frame128, fmt):
+    """Temporal + a-trous on a ragged size (partial 16x16 tiles at the right
+    and bottom edges, halos leaving the image) over four moving frames, in
+    both accumulation formats: every target equals the oracle chain."""
+    vox, sc, og = frame128
+    pos, yaw, pitch = cf.frame_128_pose()
+    w, h = 157, 83
+    try:
+        r = v.Renderer(ctx, sc, w, h, bounces=2, temporal=True, denoise=True, fclamp=48.0, accum_format=fmt,
+                       view_id=960)
+        chain = OracleChain(og, w, h, fmt)
+        for i in range(4):
+            r.set_pose((pos[0] + 0.04 * i, pos[1], pos[2] - 0.03 * i), yaw + 0.01 * i, pitch)
+            got = r.frame(("accum", "denoised", "hdr", "ldr"))
+            want = chain.frame(to_oracle_uniforms(r.last_view.u), 48.0)
+            for k in ("accum", "denoised", "hdr", "ldr"):
+                assert_bits_equal(f"frame {i} {k}", got[k], want[k])
+    finally:
+        ctx.release_view(960)
+
+
 # ---------------------------------------------------------------------------
 # batched form: every view of a mixed batch equals the same view alone
 # ---------------------------------------------------------------------------

@@ -951,22 +951,28 @@ VRT_D void store_accum(void* p, uint64_t i, float4 v) {
   }
 }
 
+// The stored value of an accumulation texel: FP32, or rounded to f16 (RNE) per
+// component -- what load_accum returns after store_accum.
 template <int FMT>
-VRT_D void temporal_tile(const Uniforms& u, uint32_t tile, const ushort4* __restrict__ radiance,
-                         const float4* __restrict__ gcur, const void* __restrict__ aprev,
-                         const float4* __restrict__ gprev, void* __restrict__ aout) {
+VRT_D float4 stored_accum(float4 v) {
+  if (FMT == 1) return v;
+  return make_float4(f16_to_f32(f32_to_f16(v.x)), f16_to_f32(f32_to_f16(v.y)), f16_to_f32(f32_to_f16(v.z)),
+                     f16_to_f32(f32_to_f16(v.w)));
+}
+
+// temporal.wgsl:17-77 for pixel (px, py): the accumulated colour and history
+// count before the store's rounding.
+template <int FMT>
+VRT_D float4 temporal_value(const Uniforms& u, uint32_t px, uint32_t py, const ushort4* __restrict__ radiance,
+                            const float4* __restrict__ gcur, const void* __restrict__ aprev,
+                            const float4* __restrict__ gprev) {
   const uint32_t w = (uint32_t)u.params1[0], h = (uint32_t)u.params1[1];
-  uint32_t px, py;
-  if (!tile_pixel(w, h, tile, px, py)) return;
   const uint64_t i = (uint64_t)py * w + px;
   const ushort4 rv = radiance[i];
   const f3 cur = mk3(f16_to_f32(rv.x), f16_to_f32(rv.y), f16_to_f32(rv.z));
   const float4 g = gcur[i];
   const bool temporalOn = (u.params0[2] & 1u) != 0u;
-  if (g.w < 0.0f || !temporalOn) {
-    store_accum<FMT>(aout, i, make_float4(cur.x, cur.y, cur.z, 1.0f));
-    return;
-  }
+  if (g.w < 0.0f || !temporalOn) return make_float4(cur.x, cur.y, cur.z, 1.0f);
   const float uvx = ((float)px + 0.5f) / (float)w;
   const float uvy = ((float)py + 0.5f) / (float)h;
   const f3 worldPos = mk3(u.cam_pos[0], u.cam_pos[1], u.cam_pos[2]) + camera_ray(u, uvx, uvy) * g.w;
@@ -1016,7 +1022,17 @@ VRT_D void temporal_tile(const Uniforms& u, uint32_t tile, const ushort4* __rest
       }
     }
   }
-  store_accum<FMT>(aout, i, make_float4(color.x, color.y, color.z, history));
+  return make_float4(color.x, color.y, color.z, history);
+}
+
+template <int FMT>
+VRT_D void temporal_tile(const Uniforms& u, uint32_t tile, const ushort4* __restrict__ radiance,
+                         const float4* __restrict__ gcur, const void* __restrict__ aprev,
+                         const float4* __restrict__ gprev, void* __restrict__ aout) {
+  const uint32_t w = (uint32_t)u.params1[0], h = (uint32_t)u.params1[1];
+  uint32_t px, py;
+  if (!tile_pixel(w, h, tile, px, py)) return;
+  store_accum<FMT>(aout, (uint64_t)py * w + px, temporal_value<FMT>(u, px, py, radiance, gcur, aprev, gprev));
 }
 
 template <int FMT>
@@ -1067,6 +1083,10 @@ struct AtrousLds {
   float2 g[S * S];  // t, normal code (as float)
 };
 
+template <int STEP>
+VRT_D void atrous_filter(const Uniforms& u, const AtrousLds<STEP>& lds, bool general, int ox, int oy, float4 center,
+                         const float4* __restrict__ gbuf, ushort4* __restrict__ dst);
+
 template <int FMT, int STEP>
 VRT_D void atrous_tile(const Uniforms& u, uint32_t tile, const void* __restrict__ src,
                        const float4* __restrict__ gbuf, ushort4* __restrict__ dst, AtrousLds<STEP>& lds) {
@@ -1101,12 +1121,27 @@ VRT_D void atrous_tile(const Uniforms& u, uint32_t tile, const void* __restrict_
     sg[k] = tg;
   }
   general = __syncthreads_or(general);
+  const int x = ox + HALO + (int)(threadIdx.x & 15u), y = oy + HALO + (int)(threadIdx.x >> 4);
+  if (x >= w || y >= h) return;
+  atrous_filter<STEP>(u, lds, general != 0, ox, oy, load_accum<FMT>(src, (uint64_t)y * w + x), gbuf, dst);
+}
+
+// The filter of atrous_tile for the calling thread's pixel, from the staged
+// neighbourhood (tile origin ox, oy including the halo); `center` is the
+// pixel's accumulation value with its history count.  The pixel is inside
+// the image.
+template <int STEP>
+VRT_D void atrous_filter(const Uniforms& u, const AtrousLds<STEP>& lds, bool general, int ox, int oy, float4 center,
+                         const float4* __restrict__ gbuf, ushort4* __restrict__ dst) {
+  constexpr int HALO = 2 * STEP;
+  constexpr int S = 16 + 2 * HALO;
+  const float4* sc = lds.c;
+  const float2* sg = lds.g;
+  const int w = (int)u.params1[0];
   const int tx = (int)(threadIdx.x & 15u), ty = (int)(threadIdx.x >> 4);
   const int x = ox + HALO + tx, y = oy + HALO + ty;
-  if (x >= w || y >= h) return;
   const int cidx = (ty + HALO) * S + (tx + HALO);
   const uint64_t i = (uint64_t)y * w + x;
-  const float4 center = load_accum<FMT>(src, i);  // with its history count
   const float2 g = sg[cidx];
   if (g.x < 0.0f) {  // sky: copy (atrous.wgsl:38-41)
     dst[i] = make_ushort4(f32_to_f16(center.x), f32_to_f16(center.y), f32_to_f16(center.z),
