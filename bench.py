@@ -172,7 +172,7 @@ def profile(args):
     given by --pt-kernel / --pt-waves / --pt-tile / --pt-order /
     --pt-min-refill / --pt-blocks; (2) for each --traverse-p occupancy, two
     launches over 2^24 rays of the traversal variant --trace-kernel
-    (--dda-batch, 1280 blocks, --trace-steps / --trace-min-refill for the
+    (--dda-batch, 1024 blocks, --trace-steps / --trace-min-refill for the
     persistent kernels), rays and hits resident in HBM; (3) two device-to-device copies
     of 256 MiB (vrt_copy), the calibration for FETCH_SIZE / WRITE_SIZE."""
     import numpy as np
@@ -200,7 +200,7 @@ def profile(args):
         rays = cf.traverse_rays(n, seed=77)
         d_rays = ctx.device_array(rays.nbytes).upload(rays)
         d_hits = ctx.device_array(n * 16)
-        ctx.set_tuning(trace_kernel=args.trace_kernel, trace_blocks=1280, trace_steps=args.trace_steps,
+        ctx.set_tuning(trace_kernel=args.trace_kernel, trace_blocks=1024, trace_steps=args.trace_steps,
                        trace_min_refill=args.trace_min_refill, dda_batch=args.dda_batch)
         for occ in args.traverse_p:
             sc2 = ctx.upload(cf.traverse_scene(occ))
@@ -251,7 +251,7 @@ def _traverse_cpu(vox, rays, p, sample=1 << 20, one_core=True):
 
 
 # the traversal variant whose HBM traffic profiles/traverse_pmc.txt holds
-PROFILED_TRAVERSE = dict(trace_kernel=7, trace_blocks=1280, trace_steps=32, trace_min_refill=16)
+PROFILED_TRAVERSE = dict(trace_kernel=7, trace_blocks=1024, trace_steps=32, trace_min_refill=16)
 
 
 def _traverse_one(args, ctx, sc, p, name, d_rays, d_hits, n_rays, tuning):

@@ -370,7 +370,7 @@ int vrt_create(const vrt_create_info* info, vrt_ctx** out) {
   (void)hipMemGetInfo(&freeb, &total);
   ctx->budget = (info && info->hbm_budget) ? info->hbm_budget : (uint64_t)(0.9 * (double)freeb);
   ctx->tuning.trace_kernel = 7;
-  ctx->tuning.trace_blocks = 1280;  // five 256-thread blocks per CU: five waves per SIMD
+  ctx->tuning.trace_blocks = 1024;
   ctx->tuning.trace_steps = 32;
   ctx->tuning.trace_min_refill = 16;
   ctx->tuning.dda_batch = 16;
