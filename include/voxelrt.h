@@ -428,9 +428,9 @@ typedef struct vrt_request {
 } vrt_request;
 
 typedef struct vrt_serve_config {
-  uint32_t streams;         /* device streams batches overlap on, 1..4 (default 2) */
-  uint32_t max_batch_views; /* requests per launch, 1..1024 (default 64) */
-  uint64_t max_batch_pixels;/* pixels x spp per launch (default 16M) */
+  uint32_t streams;         /* device streams batches overlap on, 1..4 (default 4) */
+  uint32_t max_batch_views; /* requests per launch, 1..1024 (default 8) */
+  uint64_t max_batch_pixels;/* pixels x spp per launch (default 4M) */
 } vrt_serve_config;
 
 /* Host steady-clock times in ms since vrt_create. */

@@ -35,10 +35,14 @@ ServeLane::~ServeLane() {
   if (h_vcount) (void)hipHostFree(h_vcount);
 }
 
+// Defaults: four streams and small batches keep a request's wait behind a
+// running batch short (measured on 16 scenes 64^3..1024^3 at 0.8 x plain
+// capacity: the same saturated rate as two streams with 64-view batches, and
+// p99 at the heaviest requests' own service time instead of above it).
 Server::Server(vrt_ctx* c) : ctx(c) {
-  cfg.streams = 2;
-  cfg.max_batch_views = 64;
-  cfg.max_batch_pixels = 16ull << 20;
+  cfg.streams = 4;
+  cfg.max_batch_views = 8;
+  cfg.max_batch_pixels = 4ull << 20;
 }
 
 Server::~Server() {

@@ -505,8 +505,8 @@ class Context:
         self._check(self.L.vrt_stats_reset(self.h))
 
     # ---- serving (continuous batching) ----
-    def serve_configure(self, streams: int = 2, max_batch_views: int = 64,
-                        max_batch_pixels: int = 16 << 20):
+    def serve_configure(self, streams: int = 4, max_batch_views: int = 8,
+                        max_batch_pixels: int = 4 << 20):
         cfg = ServeConfig(streams, max_batch_views, max_batch_pixels)
         self._check(self.L.vrt_serve_configure(self.h, C.byref(cfg)))
 
