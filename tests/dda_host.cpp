@@ -18,7 +18,7 @@ namespace {
 
 // One lane of the step-level refill path tracer: step_ev() until the ray
 // ends, with the level switches it reports taken one at a time.
-template <class St>
+template <bool PEEK = false, class St>
 void step_ev_walk(const vrt::SceneDev& s, const vrt::LdsScene& ls, St& st, vrt::f3 ro, vrt::f3 rd, float tmax,
                   vrt::TraceResult& o, uint64_t& nit) {
   if (st.begin(s, ls, ro, rd, tmax, o)) return;
@@ -29,6 +29,13 @@ void step_ev_walk(const vrt::SceneDev& s, const vrt::LdsScene& ls, St& st, vrt::
       r = vrt::DDA_RUN;
     } else if (r == vrt::DDA_LEAVE) {
       r = st.resume(s, ls);
+      if (PEEK && r == vrt::DDA_RUN) {
+        r = st.peek_enter(s, ls);
+        if (r == vrt::DDA_ENTER) {
+          st.enter(s, ls);
+          r = vrt::DDA_RUN;
+        }
+      }
     }
     if (r == vrt::DDA_RUN) {
       r = st.step_ev(s, ls);
@@ -153,7 +160,9 @@ void build(const uint32_t* vox, int nx, int ny, int nz, HostScene& h) {
 // with both levels in LDS, 4 the same with the empty-row skip, 5 the batched
 // form of 4 (trace_fast<true>; one lane here), 6 the single loop on global
 // memory (DdaFastT<true>), 7 its batched form, 8 and 9 the path tracer's
-// forms of 6 and 3 (brick-entry cell counts from kfine / mfine, KM = true).
+// forms of 6 and 3 (brick-entry cell counts from kfine / mfine, KM = true),
+// 10 and 11 the step-level refill kernels' stepping (step_ev, level switches one
+// at a time), 12 and 13 the same with the re-entry check after resume.
 // hits: n records of 4 u32
 // (t bits, voxel, brick_steps | voxel_steps << 16, face), as vrt_trace_rays
 // writes them.  *iters: loop iterations (step() calls) of variants 3 and 4.
@@ -208,6 +217,17 @@ extern "C" int dda_host_trace(const uint32_t* vox, int nx, int ny, int nz, const
         } else {
           vrt::DdaFastT<true, true> st;
           step_ev_walk(h.dev, ls, st, ro, rd, tmax, o, nit);
+        }
+        break;
+      }
+      case 12:  // 10 and 11 with the kernels' re-entry check after resume
+      case 13: {
+        if (variant == 12) {
+          vrt::DdaFastT<false, true> st;
+          step_ev_walk<true>(h.dev, ls, st, ro, rd, tmax, o, nit);
+        } else {
+          vrt::DdaFastT<true, true> st;
+          step_ev_walk<true>(h.dev, ls, st, ro, rd, tmax, o, nit);
         }
         break;
       }

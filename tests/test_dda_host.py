@@ -24,7 +24,8 @@ from tests.sparse import sparse_rays as _sparse_rays, sparse_scenes as _sparse_s
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 VARIANTS = {"nested": 0, "fast": 3, "fast_skip": 4, "fast_batched_skip": 5, "fast_global": 6,
-            "fast_global_batched": 7, "fast_global_km": 8, "fast_km": 9, "step_ev_lds": 10, "step_ev_global": 11}
+            "fast_global_batched": 7, "fast_global_km": 8, "fast_km": 9, "step_ev_lds": 10, "step_ev_global": 11,
+            "step_ev_peek_lds": 12, "step_ev_peek_global": 13}
 
 
 @pytest.fixture(scope="module")

@@ -431,6 +431,22 @@ struct DdaFastT {
     leave(s, L);
     return advance() ? DDA_MISS : DDA_RUN;
   }
+  // The first half of the coarse level's next step_ev, for a caller that can
+  // enter at once: DDA_MISS when t is past the bound, DDA_ENTER (the step
+  // counted) at an occupied brick, else DDA_RUN with nothing done -- the
+  // next step_ev then takes the whole step.  Either way the same operations
+  // as step_ev.
+  VRT_D uint32_t peek_enter(const SceneDev& s, const LdsScene& L) {
+    if (t > tBound) return DDA_MISS;
+    uint32_t word;
+    if constexpr (GM)
+      word = *(const uint32_t*)(gbase + ((idx >> 5) << 2));
+    else
+      word = *(const uint32_t*)((const char*)L.lds + (((idx >> 5) << wsh) + base4));
+    if (!bit_at(word, idx)) return DDA_RUN;
+    steps += inc;
+    return DDA_ENTER;
+  }
 
   // Walk to the end (DDA_HIT or DDA_MISS).  Batched: enter bricks in groups
   // of up to s.entry_batch lanes, and every lane of the wave that walks a ray

@@ -207,11 +207,16 @@ VRT_D void wave_paths(V& v, const LdsScene& L0, uint32_t min_wait, uint32_t shad
 
   for (;;) {
     // ---------------- level switches (vrt_dda_fast.h step_ev) -------------
+    // a lane back at the coarse level whose next brick is occupied enters it
+    // in this pass (peek_enter), with the lanes that stopped at one
+    if (status == DDA_LEAVE) {
+      status = st.resume(v.s(), L0);
+      if (status == DDA_RUN) status = st.peek_enter(v.s(), L0);
+    }
     if (status == DDA_ENTER) {
       st.enter(v.s(), L0);
       status = DDA_RUN;
     }
-    if (status == DDA_LEAVE) status = st.resume(v.s(), L0);
     // ---------------- service: results of finished rays -------------------
     // when shade_min lanes have finished rays, or no lane is left stepping
     const uint32_t nfin = (uint32_t)__popcll(__ballot(status == DDA_HIT || status == DDA_MISS));
