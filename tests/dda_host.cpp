@@ -22,8 +22,13 @@ template <bool PEEK = false, class St>
 void step_ev_walk(const vrt::SceneDev& s, const vrt::LdsScene& ls, St& st, vrt::f3 ro, vrt::f3 rd, float tmax,
                   vrt::TraceResult& o, uint64_t& nit) {
   if (st.begin(s, ls, ro, rd, tmax, o)) return;
-  uint32_t r = vrt::DDA_RUN;
+  uint32_t r = PEEK ? st.peek_enter(s, ls) : vrt::DDA_RUN;
+  if (r == vrt::DDA_ENTER) {
+    st.enter(s, ls);
+    r = vrt::DDA_RUN;
+  }
   for (;;) {
+    if (r == vrt::DDA_MISS) break;
     if (r == vrt::DDA_ENTER) {
       st.enter(s, ls);
       r = vrt::DDA_RUN;
@@ -162,7 +167,7 @@ void build(const uint32_t* vox, int nx, int ny, int nz, HostScene& h) {
 // memory (DdaFastT<true>), 7 its batched form, 8 and 9 the path tracer's
 // forms of 6 and 3 (brick-entry cell counts from kfine / mfine, KM = true),
 // 10 and 11 the step-level refill kernels' stepping (step_ev, level switches one
-// at a time), 12 and 13 the same with the re-entry check after resume.
+// at a time), 12 and 13 the same with the entry check after begin and resume.
 // hits: n records of 4 u32
 // (t bits, voxel, brick_steps | voxel_steps << 16, face), as vrt_trace_rays
 // writes them.  *iters: loop iterations (step() calls) of variants 3 and 4.

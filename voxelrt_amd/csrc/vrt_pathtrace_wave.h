@@ -206,21 +206,12 @@ VRT_D void wave_paths(V& v, const LdsScene& L0, uint32_t min_wait, uint32_t shad
   RayCount total{0u, 0u, 0u, 0u};
 
   for (;;) {
-    // ---------------- level switches (vrt_dda_fast.h step_ev) -------------
-    // a lane back at the coarse level whose next brick is occupied enters it
-    // in this pass (peek_enter), with the lanes that stopped at one
-    if (status == DDA_LEAVE) {
-      status = st.resume(v.s(), L0);
-      if (status == DDA_RUN) status = st.peek_enter(v.s(), L0);
-    }
-    if (status == DDA_ENTER) {
-      st.enter(v.s(), L0);
-      status = DDA_RUN;
-    }
     // ---------------- service: results of finished rays -------------------
     // when shade_min lanes have finished rays, or no lane is left stepping
+    // (lanes stopped at a level switch step on after this service)
     const uint32_t nfin = (uint32_t)__popcll(__ballot(status == DDA_HIT || status == DDA_MISS));
-    const bool shade = nfin >= shade_min || __ballot(status == DDA_RUN) == 0ull;
+    const bool shade =
+        nfin >= shade_min || __ballot(status == DDA_RUN || status == DDA_ENTER || status == DDA_LEAVE) == 0ull;
     const bool fin = shade && (status == DDA_HIT || status == DDA_MISS);
     const uint32_t kind = P.pf & 3u;
     bool hit = false;
@@ -410,6 +401,21 @@ VRT_D void wave_paths(V& v, const LdsScene& L0, uint32_t min_wait, uint32_t shad
       }
     }
     if (__ballot(status != LS_IDLE) == 0ull) break;
+    // ---------------- level switches (vrt_dda_fast.h step_ev) -------------
+    // a lane whose ray has just begun, or is back at the coarse level, and
+    // whose next brick is occupied enters it in this pass (peek_enter), with
+    // the lanes that stopped at one: secondary rays start beside the surface
+    // they leave, mostly in an occupied brick
+    bool peek = begin && status == DDA_RUN;
+    if (status == DDA_LEAVE) {
+      status = st.resume(v.s(), L0);
+      peek = status == DDA_RUN;
+    }
+    if (peek) status = st.peek_enter(v.s(), L0);
+    if (status == DDA_ENTER) {
+      st.enter(v.s(), L0);
+      status = DDA_RUN;
+    }
     // ---------------- stepping --------------------------------------------
     // until min_wait lanes wait at a brick boundary, shade_min lanes have a
     // finished ray, or no lane steps; the counts are taken every
