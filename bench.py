@@ -454,6 +454,28 @@ def serve(args, emit=True):
     # streams' views (about a saturated batch), in --batch-mode (-1: any mode)
     modes = (0, 1, 2) if args.batch_mode < 0 else (args.batch_mode,)
     served_tuning = ctx.autotune_batch(views_for(warm[:8], 90000), modes).as_dict()
+    # the server's batches are mostly of one or two views (its limits and the
+    # arrival rate cut them), not the eight the batch autotuner timed: keep
+    # whichever of its choice and the step-level refill batch form (batch_tile
+    # 0) serves a saturated burst of 48 requests faster
+    cands = [served_tuning]
+    if served_tuning["pt_kernel"] == 8 and served_tuning["batch_tile"] != 0:
+        cands.append(dict(served_tuning, batch_tile=0, batch_blocks=0))
+    if len(cands) > 1:
+        burst_s = []
+        for c, cand in enumerate(cands):
+            ctx.set_tuning(**cand)
+            vs = views_for(warm + order[:48], 95000 + 100 * c)
+            for k in range(len(warm)):
+                ctx.wait(ctx.submit(vs[k]), hdr[warm[k]])
+            t0 = time.perf_counter()
+            tk = [ctx.submit(view) for view in vs[len(warm):]]
+            for t, s in zip(tk, order[:48]):
+                ctx.wait(t, hdr[s])
+            burst_s.append(time.perf_counter() - t0)
+            for k in range(len(warm)):
+                ctx.release_view(vs[k].view_id)
+        served_tuning = cands[min(range(len(cands)), key=burst_s.__getitem__)]
 
     def run_plain(base, arrivals=None):
         ctx.set_tuning(**tuned)

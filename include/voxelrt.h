@@ -370,10 +370,11 @@ typedef struct vrt_tuning {
                                  shades them together (0 = 1: at once) */
   uint32_t batch_tile;        /* pt 8 in the batched forms: 0 = its batched form (one-wave
                                  blocks, the DDA on global memory, each wave on one
-                                 view at a time: chunks of 8 patches of 8x8 pixels
-                                 from a queue split over eight counters; pt_blocks
-                                 waves, 0 = 16 per CU); 2..5 = pt 7 with 4 waves per
-                                 SIMD on that pt_tile form */
+                                 view at a time: chunks of 1..8 patches of 8x8 pixels,
+                                 two or more chunks per wave, from a queue split over
+                                 eight counters; pt_blocks waves, 0 = 16 per CU);
+                                 2..5 = pt 7 with 4 waves per SIMD on that pt_tile
+                                 form */
   uint32_t reserved;
 } vrt_tuning;
 int vrt_tuning_get(vrt_ctx* ctx, vrt_tuning* out);

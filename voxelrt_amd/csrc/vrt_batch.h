@@ -46,7 +46,8 @@ struct BatchHead {
   uint32_t queue;
   uint32_t rtiles;   // path-trace tiles of the refill forms (32x32, 32x16, 16x16 or 16x8)
   uint32_t wchunks;  // patch chunks of the step-level refill form
-  uint32_t pad[11];
+  uint32_t wchunk;   // 8x8 patches per chunk, 1..WAVE_CHUNK
+  uint32_t pad[10];
   uint32_t wq[QUEUE_WORDS];
 };
 constexpr uint32_t BATCH_HEAD_BYTES = 64 + QUEUE_WORDS * 4;

@@ -26,9 +26,17 @@ constexpr uint32_t LDS_BIT_WORDS_MAX = 12288;
 // u32 words of a persistent kernel's work queue (the split queue's eight
 // counters, one per 128-byte line; the single-counter kernels use word 0).
 constexpr uint32_t QUEUE_WORDS = 256;
-// 8x8 patches per work item of the batched step-level refill path tracer (a
-// wave finishes its paths before it moves to another view's chunk).
+// Most 8x8 patches per work item of the batched step-level refill path
+// tracer (a wave finishes its paths before it moves to another view's chunk,
+// so large batches take large chunks).  A launch takes
+// wave_chunk(patches, waves) per item: two or more items per wave, so a small
+// batch (one view of 360p: 3600 patches for 4096 waves) is not left to a few
+// waves that walk eight patches each.
 constexpr uint32_t WAVE_CHUNK = 8;
+inline uint32_t wave_chunk(uint64_t patches, uint32_t waves) {
+  const uint64_t c = patches / (2ull * (waves ? waves : 1u));
+  return c < 1u ? 1u : (c > WAVE_CHUNK ? WAVE_CHUNK : (uint32_t)c);
+}
 
 hipError_t launch_brick_masks(hipStream_t st, const uint32_t* vox, int nx, int ny, int bgx,
                               int bgy, int bz0, int nbz, uint32_t* masks, uint32_t* flags);

@@ -100,7 +100,7 @@ struct WaveView1 {
   VRT_D void flush(RayCount&, uint32_t) {}
 };
 
-// The batched form's view source.  Work items are chunks of WAVE_CHUNK
+// The batched form's view source.  Work items are chunks of BatchHead::wchunk
 // patches of one view (ViewDesc::wchunk0: a view's first chunk), drawn from a
 // SplitQueue over the table's BatchHead::wchunks; a wave stays on one view
 // until its next chunk belongs to another.
@@ -118,7 +118,7 @@ VRT_D uint32_t find_view_w(const ViewDesc* __restrict__ views, uint32_t nviews, 
 struct WaveViewBatch {
   static constexpr bool MULTI = true;
   const ViewDesc* __restrict__ views;
-  uint32_t nviews, nchunks;
+  uint32_t nviews, nchunks, chunk;
   uint32_t vi = 0u, pend = 0u, pend_first = 0u;  // current view; the next one and its first patch
   uint32_t next_p = 0u, left = 0u;               // next patch of the current chunk, patches left in it
   float fclamp = 0.0f;
@@ -130,7 +130,8 @@ struct WaveViewBatch {
   bool indirect = false;
   SplitQueue sq;
   VRT_D WaveViewBatch(const BatchHead* head, const ViewDesc* v, uint32_t* queue)
-      : views(v), nviews(head->nviews), nchunks(head->wchunks), sq(queue, head->wchunks, blockIdx.x) {}
+      : views(v), nviews(head->nviews), nchunks(head->wchunks), chunk(head->wchunk),
+        sq(queue, head->wchunks, blockIdx.x) {}
   VRT_D const ViewDesc& d() const { return views[vi]; }
   VRT_D const SceneDev& s() const { return d().s; }
   VRT_D const Uniforms& u() const { return d().u; }
@@ -149,21 +150,21 @@ struct WaveViewBatch {
     nb = dv.u.params0[1];
     indirect = (dv.u.params0[2] & VRT_FLAG_INDIRECT) != 0u;
     next_p = first;
-    left = np - first < WAVE_CHUNK ? np - first : WAVE_CHUNK;
+    left = np - first < chunk ? np - first : chunk;
   }
   VRT_D uint32_t next(uint32_t lane, uint32_t& patch) {
     if (left == 0u) {
       const uint32_t c = sq.next(lane);
       if (c >= nchunks) return PATCH_NONE;
       const uint32_t k = __builtin_amdgcn_readfirstlane(find_view_w(views, nviews, c));
-      const uint32_t first = (c - views[k].wchunk0) * WAVE_CHUNK;
+      const uint32_t first = (c - views[k].wchunk0) * chunk;
       if (k != vi || np == 0u) {
         pend = k;
         pend_first = first;
         return PATCH_SWITCH;
       }
       next_p = first;
-      left = np - first < WAVE_CHUNK ? np - first : WAVE_CHUNK;
+      left = np - first < chunk ? np - first : chunk;
     }
     patch = wave_patch(next_p, np, s().tile_order);
     ++next_p;

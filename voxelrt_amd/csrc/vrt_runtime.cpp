@@ -1657,13 +1657,16 @@ int vrt_ctx::render_batch(vrt::BatchLane& lane, const vrt_view* views, uint32_t 
   const int pt_waves = wave_pt && !wave_batch ? 4 : (int)tuning.pt_waves;
   const int tile = wave_pt ? (int)tuning.batch_tile : (int)tuning.pt_tile;
   const int rform = wave_batch ? 0 : vrt::refill_form(ptv, tile, max_bit_words, max_pwords);
+  const uint32_t wave_blocks = tuning.pt_blocks ? tuning.pt_blocks : (uint32_t)cus * 16u;
+  uint64_t patches = 0;
+  for (uint32_t k = 0; k < nviews; ++k) patches += ((d[k].w + 7u) / 8u) * ((d[k].h + 7u) / 8u);
+  const uint32_t wchunk = vrt::wave_chunk(patches, wave_blocks);
   uint32_t wchunks = 0;
   for (uint32_t k = 0; k < nviews; ++k) {
     d[k].wchunk0 = wchunks;
     const uint32_t np = ((d[k].w + 7u) / 8u) * ((d[k].h + 7u) / 8u);
-    wchunks += (np + vrt::WAVE_CHUNK - 1u) / vrt::WAVE_CHUNK;
+    wchunks += (np + wchunk - 1u) / wchunk;
   }
-  const uint32_t wave_blocks = tuning.pt_blocks ? tuning.pt_blocks : (uint32_t)cus * 16u;
   uint32_t rtiles = 0;
   if (rform != 0) {
     uint32_t tw = 0, th = 0;
@@ -1680,6 +1683,7 @@ int vrt_ctx::render_batch(vrt::BatchLane& lane, const vrt_view* views, uint32_t 
   hh->queue = 0;
   hh->rtiles = rtiles;
   hh->wchunks = wchunks;
+  hh->wchunk = wchunk;
   vrt::BatchHead* dh = (vrt::BatchHead*)lane.desc_dev.ptr;
   const vrt::ViewDesc* dd = (const vrt::ViewDesc*)((char*)lane.desc_dev.ptr + vrt::BATCH_HEAD_BYTES);
   HIP_TRY(this, hipMemcpyAsync(lane.desc_dev.ptr, lane.desc_host[stage],
