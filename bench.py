@@ -396,6 +396,39 @@ def _pct(xs, q):
     return round(float(np.percentile(np.asarray(xs, dtype=np.float64), q)), 3) if len(xs) else None
 
 
+def _sample_stats(xs):
+    """n, mean, median, sample stddev, cv, min, p95 (linear between order
+    statistics) and max of one figure over repeats, as the reference's
+    harness summarises them (test/bench.mjs:533-553)."""
+    import numpy as np
+    a = np.asarray([x for x in xs if np.isfinite(x)], dtype=np.float64)
+    if a.size == 0:
+        return None
+    mean = float(a.mean())
+    sd = float(a.std(ddof=1)) if a.size > 1 else 0.0
+    r = lambda x: round(float(x), 4)
+    return {"n": int(a.size), "mean": r(mean), "median": r(np.percentile(a, 50)), "stddev": r(sd),
+            "cv": r(sd / mean if mean else 0.0), "min": r(a.min()), "p95": r(np.percentile(a, 95)),
+            "max": r(a.max())}
+
+
+def _perf_stats(frames, w, h, wall_ms_per_frame, pass_ms):
+    """Per-frame figures of one timed run in the shape of the reference's
+    performanceStats (test/bench.mjs:480-512): wall and GPU ms and fps,
+    GPU ms per megapixel, the run's GPU cost, each pass's share of the GPU
+    frame and the dominant pass.  pass_ms: mean GPU ms per frame by pass."""
+    mp = w * h / 1e6
+    total = sum(pass_ms.values())
+    pct = {k: round(100.0 * ms / total, 2) for k, ms in pass_ms.items()}
+    dom = max(pass_ms, key=pass_ms.get)
+    return {"frames": frames, "pixels": w * h, "megapixels": round(mp, 4),
+            "wall_ms_per_frame": round(wall_ms_per_frame, 4), "wall_fps": round(1e3 / wall_ms_per_frame, 1),
+            "wall_run_ms": round(wall_ms_per_frame * frames, 2), "gpu_ms_per_frame": round(total, 4),
+            "gpu_fps": round(1e3 / total, 1), "gpu_ms_per_megapixel": round(total / mp, 4),
+            "gpu_run_ms": round(total * frames, 2), "timing_frames": frames,
+            "gpu_pass_percent": pct, "dominant_pass": dom, "dominant_pass_percent": pct[dom]}
+
+
 def serve(args, emit=True):
     """Served form vs plain serial rendering on one arrival trace.
 
@@ -569,6 +602,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-forms", action="store_true", help="frame_1080: leave out the batched and served forms")
     ap.add_argument("--cpu-frames", type=int, default=8)
+    ap.add_argument("--repeats", type=int, default=3,
+                    help="frame_1080: runs of --steps frames summarised in perf (the timed run is the first)")
     ap.add_argument("--workload", default="frame_1080", choices=["frame_1080", "traverse", "batch", "serve", "profile"])
     ap.add_argument("--pt-order", type=int, default=2, help="profile workload: path-tracer tile order")
     ap.add_argument("--trace-kernel", type=int, default=7, help="profile workload: traversal variant")
@@ -684,6 +719,27 @@ def main():
                       "rays_per_frame": rays // K, "brick_steps_per_ray": round(bsteps / max(rays, 1), 2),
                       "voxel_steps_per_ray": round(vsteps / max(rays, 1), 2)},
     }
+    # the reference harness's per-run figures for the timed run, and their
+    # spread over repeats: the timed run above and args.repeats - 1 more runs
+    # of K frames continuing the orbit (after the timed region; they do not
+    # enter `value`)
+    pass_ms = {"pathtrace": pt_ms / K, "temporal": tp_ms / K, "atrous": at_ms / K}
+    perf = _perf_stats(K, w, h, ms_frame, pass_ms)
+    walls, gpus = [ms_frame], [perf["gpu_ms_per_frame"]]
+    nxt = args.warmup + K
+    for _ in range(max(args.repeats, 1) - 1):
+        a = ctx.stats()
+        t1 = time.perf_counter()
+        for k in range(K):
+            step(nxt + k)
+        b = ctx.stats()
+        nxt += K
+        walls.append((time.perf_counter() - t1) / K * 1e3)
+        gpus.append(sum(b[f] - a[f] for f in ("sum_ms_pathtrace", "sum_ms_temporal", "sum_ms_atrous")) / K)
+    perf["repeats"] = len(walls)
+    perf["wall_ms_per_frame_stats"] = _sample_stats(walls)
+    perf["gpu_ms_per_frame_stats"] = _sample_stats(gpus)
+    out["perf"] = perf
     sc.release()
     ctx.close()
     if not args.no_forms:

@@ -12,7 +12,10 @@
 //   <out>/orient.txt    primary-visibility agreement between VoxelBench's
 //                       G-buffer (pixDir, core.cpp:180-184) and a G-buffer
 //                       rendered through the C ABI with MethodHIP's camera
-// Usage: vb_technique_driver <bunker|courtyard|cavern> <frames> <event_frame> <out_dir>
+// Usage: vb_technique_driver <bunker|courtyard|cavern> <frames> <event_frame> <out_dir> [light_frame]
+// light_frame: also a light change at that frame -- the sun moves to
+// norm(-0.3, 0.9, 0.25) with radiance (14, 9, 4.5) and onEvent() gets an event
+// with only lightChange set (as courtyard's own event, scenes.h:75-77,121)
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
@@ -44,14 +47,15 @@ static bool write_grid(const std::string& path, const MethodHIP& m) {
 }
 
 int main(int argc, char** argv) {
-  if (argc != 5) {
-    std::fprintf(stderr, "usage: %s <scene> <frames> <event_frame> <out_dir>\n", argv[0]);
+  if (argc != 5 && argc != 6) {
+    std::fprintf(stderr, "usage: %s <scene> <frames> <event_frame> <out_dir> [light_frame]\n", argv[0]);
     return 2;
   }
   setvbuf(stdout, nullptr, _IOLBF, 0);  // progress lines survive an abnormal end
   auto s = make_scene(argv[1]);
   const int frames = std::atoi(argv[2]), event_frame = std::atoi(argv[3]);
   const std::string out = argv[4];
+  const int light_frame = argc == 6 ? std::atoi(argv[5]) : -1;
   if (!s || frames < 1) return 2;
   s->build();
   collectEmitters();
@@ -70,6 +74,15 @@ int main(int argc, char** argv) {
       buildGBuffer();
       m.onEvent(s->event);
       if (s->event.geomEdit && !write_grid(out + "/grid1.bin", m)) return 4;
+    }
+    if (f == light_frame) {
+      SKY.enabled = true;
+      SKY.sunDir = norm(V3(-0.3f, 0.9f, 0.25f));
+      SKY.sunE = V3(14, 9, 4.5f);
+      SceneEvent le;
+      le.lightChange = true;
+      m.onEvent(le);
+      std::printf("light change before render %d\n", f);
     }
     const int64_t rays = m.frame(4L * W * H);
     if (rays <= 0) return 5;

@@ -70,3 +70,34 @@ def test_technique_adapter_matches_oracle(tmp_path, scene, frames, event):
     agree = int(kv["agree"]) / int(kv["pixels"])
     assert agree >= 0.97, kv
     assert 0.0 <= float(kv["median_rel_depth"]) < 0.02, kv
+
+
+def test_technique_light_change_matches_oracle(tmp_path):
+    """onEvent(lightChange) on the GPU: cavern's collapse at frame 1, then the
+    sun moved at frame 2 (the driver's light_frame; courtyard's own light
+    event stays skipped above).  Every frame equals the oracle chain, and the
+    frames from the light change on carry the new sun in their uniforms."""
+    if not os.path.exists(DRIVER):
+        pytest.skip("oracle/_ref/vb_technique_driver not built (needs the VoxelBench sources)")
+    frames, event, light = 4, 1, 2
+    print(f"technique driver: cavern frames={frames} event={event} light={light}", flush=True)
+    r = subprocess.run([DRIVER, "cavern", str(frames), str(event), str(tmp_path), str(light)], capture_output=True,
+                       text=True, timeout=180)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert f"light change before render {light}" in r.stdout, r.stdout
+    chain = OracleChain(oracle.OracleGrid(_grid(tmp_path / "grid0.bin")), W, H, 1)
+    want_dir = np.array([-0.3, 0.9, 0.25]) / np.linalg.norm([-0.3, 0.9, 0.25])
+    for f in range(frames):
+        if f == event:
+            chain.g = oracle.OracleGrid(_grid(tmp_path / "grid1.bin"))
+        raw = (tmp_path / f"frame{f:03d}.bin").read_bytes()
+        u = oracle.Uniforms.from_buffer_copy(raw[:304])
+        d = np.array(u.sun_dir[:3], np.float64)
+        d /= np.linalg.norm(d)
+        moved = np.allclose(d, want_dir, atol=1e-5) and np.allclose(u.sun_radiance[:4], [14.0, 9.0, 4.5, 1.0])
+        assert moved == (f >= light), (f, d, list(u.sun_radiance))
+        rays = int(np.frombuffer(raw[304:312], np.int64)[0])
+        out = np.frombuffer(raw[312:], np.float32).reshape(H, W, 3)
+        want = chain.frame(u, 48.0)
+        assert_bits_equal(f"cavern light-change frame {f} out", out, want["hdr"])
+        assert rays == want["counters"].rays, (rays, want["counters"].rays)

@@ -70,3 +70,37 @@ def test_technique_adapter_clean_under_sanitizers(driver, tmp_path, scene, frame
         want = chain.frame(u, 48.0)
         assert_bits_equal(f"{scene} frame {f} out", out, want["hdr"])
         assert rays == want["counters"].rays
+
+
+def _sun_after(u):
+    d = np.array(u.sun_dir[:3], np.float64)
+    return d / np.linalg.norm(d), np.array(u.sun_radiance[:4], np.float64)
+
+
+def test_technique_light_change_under_sanitizers(driver, tmp_path):
+    """onEvent(lightChange) after a geometry event: cavern's collapse at frame
+    1, then the sun moved (the driver's light_frame, frame 2).  Every frame
+    equals the oracle chain, and the frames from the light change on carry
+    the new sun in their uniforms (the adapter reads SKY every frame)."""
+    frames, event, light = 4, 1, 2
+    env = dict(os.environ, OMP_NUM_THREADS="4",
+               ASAN_OPTIONS="detect_leaks=1:halt_on_error=1:abort_on_error=0",
+               UBSAN_OPTIONS="halt_on_error=1:print_stacktrace=1")
+    r = subprocess.run([driver, "cavern", str(frames), str(event), str(tmp_path), str(light)], capture_output=True,
+                       text=True, timeout=300, env=env)
+    log = r.stdout + r.stderr
+    assert r.returncode == 0, log
+    assert "runtime error" not in log and "Sanitizer" not in log, log
+    assert f"light change before render {light}" in r.stdout, log
+    chain = OracleChain(oracle.OracleGrid(_grid(tmp_path / "grid0.bin")), W, H, 1)
+    want_dir = np.array([-0.3, 0.9, 0.25]) / np.linalg.norm([-0.3, 0.9, 0.25])
+    for f in range(frames):
+        if f == event:
+            chain.g = oracle.OracleGrid(_grid(tmp_path / "grid1.bin"))
+        raw = (tmp_path / f"frame{f:03d}.bin").read_bytes()
+        u = oracle.Uniforms.from_buffer_copy(raw[:304])
+        d, e = _sun_after(u)
+        moved = np.allclose(d, want_dir, atol=1e-5) and np.allclose(e, [14.0, 9.0, 4.5, 1.0])
+        assert moved == (f >= light), (f, d, e)
+        out = np.frombuffer(raw[312:], np.float32).reshape(H, W, 3)
+        assert_bits_equal(f"cavern light-change frame {f} out", out, chain.frame(u, 48.0)["hdr"])
