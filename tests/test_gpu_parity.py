@@ -466,12 +466,13 @@ BATCH_VARIANTS = {
     "graph-refill32x16-lds-rowmajor": dict(pt_kernel=4, pt_waves=4, pt_tile=3, batch_mode=2, pt_order=0),
     "queue-refill32x32-global-3blocks": dict(pt_kernel=7, pt_waves=5, pt_tile=2, batch_mode=1, batch_blocks=3),
     # the step-level refill kernel's batched form (batch_tile 0), or pt 7 on
-    # batch_tile.  Its work items are chunks of patches // (2 x waves) 8x8
+    # batch_tile.  Its work items are chunks of patches // (8 x waves) 8x8
     # patches, 1..8: the five views below have 495 patches, so the default
-    # grid takes chunks of 1, 97 waves of 2, 60 of 4 and 16 of 8
+    # grid and 97 waves take chunks of 1, 30 waves of 2, 15 of 4 and 7 of 8
     "wave-grid": dict(pt_kernel=8, pt_waves=4, batch_mode=0, batch_tile=0),
-    "wave-grid-16blocks-chunk8": dict(pt_kernel=8, pt_waves=4, batch_mode=0, batch_tile=0, pt_blocks=16),
-    "wave-graph-60blocks-chunk4": dict(pt_kernel=8, pt_waves=4, batch_mode=2, batch_tile=0, pt_blocks=60),
+    "wave-grid-7blocks-chunk8": dict(pt_kernel=8, pt_waves=4, batch_mode=0, batch_tile=0, pt_blocks=7),
+    "wave-graph-15blocks-chunk4": dict(pt_kernel=8, pt_waves=4, batch_mode=2, batch_tile=0, pt_blocks=15),
+    "wave-queue-30blocks-chunk2": dict(pt_kernel=8, pt_waves=4, batch_mode=1, batch_tile=0, pt_blocks=30),
     "wave-queue-97blocks-m1-s1": dict(pt_kernel=8, pt_waves=4, batch_mode=1, batch_tile=0, pt_blocks=97,
                                       pt_min_refill=1, pt_shade_min=1),
     "wave-graph-w6-rowmajor": dict(pt_kernel=8, pt_waves=6, batch_mode=2, batch_tile=0, pt_order=0),

@@ -1660,7 +1660,7 @@ int vrt_ctx::render_batch(vrt::BatchLane& lane, const vrt_view* views, uint32_t 
   const uint32_t wave_blocks = tuning.pt_blocks ? tuning.pt_blocks : (uint32_t)cus * 16u;
   uint64_t patches = 0;
   for (uint32_t k = 0; k < nviews; ++k) patches += ((d[k].w + 7u) / 8u) * ((d[k].h + 7u) / 8u);
-  const uint32_t wchunk = vrt::wave_chunk(patches, wave_blocks);
+  const uint32_t wchunk = vrt::wave_chunk(patches, wave_blocks, nviews);
   uint32_t wchunks = 0;
   for (uint32_t k = 0; k < nviews; ++k) {
     d[k].wchunk0 = wchunks;
