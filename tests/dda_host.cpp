@@ -156,6 +156,21 @@ void build(const uint32_t* vox, int nx, int ny, int nz, HostScene& h) {
   s.voxel_size = 0.0625f;
   s.inv_voxel = 1.0f / 0.0625f;
   s.entry_batch = 1u;
+  // the box of occupied bricks (brute force over the brick words)
+  for (int c = 0; c < 3; ++c) {
+    s.box_lo[c] = c == 0 ? bgx : (c == 1 ? bgy : bgz);
+    s.box_hi[c] = -1;
+  }
+  for (int bz = 0; bz < bgz; ++bz)
+    for (int by = 0; by < bgy; ++by)
+      for (int bx = 0; bx < bgx; ++bx)
+        if (h.bricks[(uint64_t)bx + (uint64_t)by * bgx + (uint64_t)bz * bgx * bgy]) {
+          const int b[3] = {bx, by, bz};
+          for (int c = 0; c < 3; ++c) {
+            s.box_lo[c] = std::min(s.box_lo[c], b[c]);
+            s.box_hi[c] = std::max(s.box_hi[c], b[c]);
+          }
+        }
   s.zero = 0.0f;
 }
 
@@ -167,7 +182,8 @@ void build(const uint32_t* vox, int nx, int ny, int nz, HostScene& h) {
 // memory (DdaFastT<true>), 7 its batched form, 8 and 9 the path tracer's
 // forms of 6 and 3 (brick-entry cell counts from kfine / mfine, KM = true),
 // 10 and 11 the step-level refill kernels' stepping (step_ev, level switches one
-// at a time), 12 and 13 the same with the entry check after begin and resume.
+// at a time), 12 and 13 the same with the entry check after begin and resume,
+// 14 and 15 those ending rays at the box of occupied bricks (BOX = true).
 // hits: n records of 4 u32
 // (t bits, voxel, brick_steps | voxel_steps << 16, face), as vrt_trace_rays
 // writes them.  *iters: loop iterations (step() calls) of variants 3 and 4.
@@ -232,6 +248,17 @@ extern "C" int dda_host_trace(const uint32_t* vox, int nx, int ny, int nz, const
           step_ev_walk<true>(h.dev, ls, st, ro, rd, tmax, o, nit);
         } else {
           vrt::DdaFastT<true, true> st;
+          step_ev_walk<true>(h.dev, ls, st, ro, rd, tmax, o, nit);
+        }
+        break;
+      }
+      case 14:  // the path tracer's stepping: 12 and 13 ending rays at the occupied box
+      case 15: {
+        if (variant == 14) {
+          vrt::DdaFastT<false, true, true> st;
+          step_ev_walk<true>(h.dev, ls, st, ro, rd, tmax, o, nit);
+        } else {
+          vrt::DdaFastT<true, true, true> st;
           step_ev_walk<true>(h.dev, ls, st, ro, rd, tmax, o, nit);
         }
         break;

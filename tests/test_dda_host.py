@@ -160,3 +160,47 @@ def test_div_rcp_is_ieee_division(dda):
     L.div_rcp_check.argtypes = [C.c_uint64, C.c_uint64]
     L.div_rcp_check.restype = C.c_uint64
     assert L.div_rcp_check(20_000_000, 7) == 0
+
+
+BOX_VARIANTS = {"step_ev_peek_box_lds": 14, "step_ev_peek_box_global": 15}
+
+
+def _box_scene(kind):
+    if kind == "world":
+        return v.generate_scene(**cf.world_desc(64, 4))
+    vox = v.generate_scene(v.SCENE_RANDOM_BRICKS, n=64, seed=31, p_brick=0.3, p_voxel=0.5)
+    if kind == "subbox":  # occupied bricks only in bricks 2..4 (x), 1..5 (y), 3..6 (z)
+        keep = np.zeros_like(vox, bool)
+        keep[24:56, 8:48, 16:40] = True
+        vox[~keep] = 0
+    elif kind == "empty":
+        vox[:] = 0
+    return vox
+
+
+@pytest.mark.parametrize("variant", list(BOX_VARIANTS))
+@pytest.mark.parametrize("kind", ["world", "subbox", "empty", "full"])
+def test_box_variants_end_misses_early(dda, variant, kind):
+    """The path tracer's DDA (BOX = true: rays end at the box of occupied
+    bricks) against the oracle's grid walk: every record's voxel, face, t
+    bits and voxel steps are equal; a hit's brick steps are equal; a miss
+    takes at most as many brick steps, and fewer wherever the box is smaller
+    than the grid."""
+    vox = _box_scene(kind)
+    rays = np.concatenate([cf.traverse_rays(20000, n=64, seed=5), _edge_rays(64)])
+    want = oracle.trace_rays(oracle.OracleGrid(vox), rays)
+    got = dda(vox, rays, BOX_VARIANTS[variant])
+    for f in ("voxel", "face", "voxel_steps"):
+        bad = np.flatnonzero(got[f] != want[f])
+        assert len(bad) == 0, f"{kind} {f}: {len(bad)} mismatches, first {bad[:5]}"
+    assert np.array_equal(got["t"].view(np.uint32), want["t"].view(np.uint32))
+    hit = want["t"] >= 0
+    assert np.array_equal(got["brick_steps"][hit], want["brick_steps"][hit])
+    assert np.all(got["brick_steps"][~hit] <= want["brick_steps"][~hit])
+    saved = int(want["brick_steps"][~hit].sum()) - int(got["brick_steps"][~hit].sum())
+    if kind == "full":
+        assert saved == 0  # the box is the grid
+    else:
+        assert saved > 0 and (~hit).sum() > 0
+    if kind == "empty":
+        assert not hit.any() and np.all(got["brick_steps"] == 0)

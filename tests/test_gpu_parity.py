@@ -278,7 +278,14 @@ def test_pathtrace_frame128_bitexact(ctx, frame128, pt_variant):
     assert_bits_equal("gbuf", outs["gbuf"], gb)
     assert_bits_equal("albedo", outs["albedo"], alb)
     assert_bits_equal("radiance", outs["radiance"], rad)
-    assert (fr.rays, fr.brick_steps, fr.voxel_steps) == (cnt.rays, cnt.brick_steps, cnt.voxel_steps)
+    assert (fr.rays, fr.voxel_steps) == (cnt.rays, cnt.voxel_steps)
+    if pt_variant.get("pt_kernel") == 8:
+        # the step-level kernel ends rays at the box of occupied bricks: a
+        # miss takes fewer brick steps than the grid walk (the scene's
+        # bricks stop well below the grid's top), a hit the same
+        assert 0 < fr.brick_steps < cnt.brick_steps
+    else:
+        assert fr.brick_steps == cnt.brick_steps
 
 
 @pytest.mark.parametrize("bounces,spp,fclamp,frame", [(4, 3, 0.0, 7), (0, 1, 48.0, 123456789), (6, 2, 10.0, 3)])

@@ -130,7 +130,14 @@ VRT_D bool bit_at(uint32_t w, uint32_t i) {
 // two more live registers -- the step-level kernels' choice (path tracer and
 // traversal, whose occupancy LDS, not registers, bounds); the one-ray-per-lane
 // kernels keep pack_left.
-template <bool GM, bool KM = false>
+// BOX: the coarse level's cells-left counts run to the box of occupied
+// bricks (SceneDev::box_lo / box_hi) instead of the grid: every brick outside
+// it is empty, and along each axis the walk is monotone, so a ray that leaves
+// the box (or starts outside it moving away) cannot hit; it ends there as a
+// miss.  Hits, their t and both step counts are those of the grid walk; a
+// miss records fewer brick steps (the path tracer's choice, whose step counts
+// are statistics; the traversal's miss records keep the grid walk's counts).
+template <bool GM, bool KM = false, bool BOX = false>
 struct DdaFastT {
   float ro[3], rdv[3], inv[3];
   int sd[3];
@@ -199,7 +206,23 @@ struct DdaFastT {
     di[1] = sd[1] * (int)(1u << s.psh1);
     di[2] = sd[2] * (int)(1u << s.psh2);
     idx = (uint32_t)bp[0] | ((uint32_t)bp[1] << s.psh1) | ((uint32_t)bp[2] << s.psh2);
-    cnt = pack_left(bp, BG, sd);
+    if constexpr (BOX) {
+      // cells left to the box's far side along each moving axis; negative
+      // (outside and moving away, or outside on an axis it does not move
+      // along) means the ray never reaches an occupied brick
+      cnt = 0u;
+      bool out = false;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const int lo = s.box_lo[c], hi = s.box_hi[c];
+        const int left = sd[c] > 0 ? hi - bp[c] : (sd[c] < 0 ? bp[c] - lo : (bp[c] < lo || bp[c] > hi ? -1 : 255));
+        out |= left < 0;
+        cnt |= (uint32_t)(left & 255) << (8 * c);
+      }
+      if (out) return true;
+    } else {
+      cnt = pack_left(bp, BG, sd);
+    }
     kfine = mfine = 0u;
     if constexpr (KM) {
 #pragma unroll

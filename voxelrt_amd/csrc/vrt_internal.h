@@ -91,6 +91,7 @@ struct vrt_scene {
   vrt::DeviceBuffer d_pdist;             // row distance per padded bitmap word (bytes)
   vrt::DeviceBuffer d_dense;  // editable scenes: the dense packed grid, rebuilt from on edits
   uint32_t psh1 = 0, psh2 = 0, psh3 = 0;  // padded bit index = bx | by << psh1 | bz << psh2; 2^psh3 bits
+  int box_lo[3] = {0, 0, 0}, box_hi[3] = {-1, -1, -1};  // occupied bricks' box (scene_build)
   vrt::SceneDev dev() const;
   // every device buffer of the scene (those of size 0 are not allocated)
   std::array<vrt::DeviceBuffer*, 8> buffers() {

@@ -57,6 +57,9 @@ hipError_t launch_brick_pbits(hipStream_t st, const uint32_t* bricks, int bgx, i
 hipError_t launch_popc(hipStream_t st, const uint32_t* words, uint64_t n, uint32_t* counts);
 // Row distances of a scene (SceneDev::pdist) from its brick words; tmp holds
 // 3 * bgy * bgz bytes, pdist (pwords + 3) & ~3 bytes.
+// Per brick row (by, bz), r = by + bz * bgy: first | last << 16 of its
+// occupied bx, 0xFFFF when empty (bgy * bgz words).
+hipError_t launch_row_span(hipStream_t st, const uint32_t* bricks, int bgx, int bgy, int bgz, uint32_t* span);
 hipError_t launch_row_dist(hipStream_t st, const uint32_t* bricks, int bgx, int bgy, int bgz, uint32_t sh1,
                            uint32_t sh2, uint64_t pwords, uint8_t* tmp, uint8_t* pdist);
 hipError_t launch_pack_pool(hipStream_t st, const uint32_t* vox, int nx, int ny, int bgx, int bgy,

@@ -179,6 +179,23 @@ __global__ __launch_bounds__(256) void k_row_occ(const uint32_t* __restrict__ br
   occ[r] = o != 0u ? 1 : 0;
 }
 
+// span[r]: the first and last occupied bx of row r (first | last << 16), or
+// 0xFFFF when the row is empty: the host reduces them to the box of occupied
+// bricks (SceneDev::box_lo / box_hi).
+__global__ __launch_bounds__(256) void k_row_span(const uint32_t* __restrict__ bricks, int bgx, int bgy, int bgz,
+                                                  uint32_t* __restrict__ span) {
+  const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= (uint64_t)bgy * (uint64_t)bgz) return;
+  const uint32_t* row = bricks + r * (uint64_t)bgx;
+  uint32_t first = 0xFFFFu, last = 0u;
+  for (int x = 0; x < bgx; ++x)
+    if (row[x] != 0u) {
+      first = first == 0xFFFFu ? (uint32_t)x : first;
+      last = (uint32_t)x;
+    }
+  span[r] = first == 0xFFFFu ? 0xFFFFu : (first | last << 16);
+}
+
 // dy[r]: distance along y, within the row's z, to the nearest occupied row
 // (255 when there is none or it is farther).
 __global__ __launch_bounds__(256) void k_row_dy(const uint8_t* __restrict__ occ, int bgy, int bgz,
@@ -1548,6 +1565,12 @@ hipError_t launch_row_dist(hipStream_t st, const uint32_t* bricks, int bgx, int 
   hipLaunchKernelGGL(k_row_dist, dim3(div_up(rows, 256)), dim3(256), 0, st, dy, bgy, bgz, dist);
   hipLaunchKernelGGL(k_word_dist, dim3(div_up(nw, 256)), dim3(256), 0, st, dist, bgy, bgz, sh1, sh2, pwords, nw,
                      pdist);
+  return hipGetLastError();
+}
+
+hipError_t launch_row_span(hipStream_t st, const uint32_t* bricks, int bgx, int bgy, int bgz, uint32_t* span) {
+  const uint64_t rows = (uint64_t)bgy * (uint64_t)bgz;
+  hipLaunchKernelGGL(k_row_span, dim3(div_up(rows, 256)), dim3(256), 0, st, bricks, bgx, bgy, bgz, span);
   return hipGetLastError();
 }
 

@@ -191,7 +191,7 @@ VRT_D void wave_paths(V& v, const LdsScene& L0, uint32_t min_wait, uint32_t shad
                       unsigned long long* __restrict__ counters) {
   const uint32_t lane = threadIdx.x & 63u;
   const uint64_t lt_mask = lane == 0u ? 0ull : (~0ull >> (64u - lane));
-  DdaFastT<GM, true> st;
+  DdaFastT<GM, true, true> st;  // rays end at the box of occupied bricks
   WavePath P;
   P.pix = 0u;
   P.smp = 0u;

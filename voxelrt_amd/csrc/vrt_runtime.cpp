@@ -214,6 +214,10 @@ vrt::SceneDev vrt_scene::dev() const {
   s.voxel_size = VRT_VOXEL_SIZE;
   s.inv_voxel = 1.0f / VRT_VOXEL_SIZE;
   s.entry_batch = 1u;
+  for (int c = 0; c < 3; ++c) {
+    s.box_lo[c] = box_lo[c];
+    s.box_hi[c] = box_hi[c];
+  }
   s.zero = 0.0f;
   return s;
 }
@@ -527,8 +531,32 @@ static int scene_build(vrt_ctx* ctx, vrt_scene* sc, const uint32_t* hvox, const 
     if (e == hipSuccess)
       e = vrt::launch_row_dist(ctx->stream, (const uint32_t*)sc->d_bricks.ptr, bgx, bgy, bgz, sc->psh1, sc->psh2, pw,
                                (uint8_t*)rows.ptr, (uint8_t*)sc->d_pdist.ptr);
-    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);  // before rows is freed
+    // the box of occupied bricks, from each row's first and last occupied bx
+    vrt::DeviceBuffer span;
+    std::vector<uint32_t> hspan((size_t)bgy * (size_t)bgz);
+    if (e == hipSuccess) e = span.alloc(hspan.size() * 4);
+    if (e == hipSuccess)
+      e = vrt::launch_row_span(ctx->stream, (const uint32_t*)sc->d_bricks.ptr, bgx, bgy, bgz, (uint32_t*)span.ptr);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(hspan.data(), span.ptr, hspan.size() * 4, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);  // before rows and span are freed
     if (e != hipSuccess) return err("row distances", e);
+    int lo[3] = {bgx, bgy, bgz}, hi[3] = {-1, -1, -1};
+    for (int bz = 0; bz < bgz; ++bz)
+      for (int by = 0; by < bgy; ++by) {
+        const uint32_t sp = hspan[(size_t)by + (size_t)bz * (size_t)bgy];
+        if (sp == 0xFFFFu) continue;
+        lo[0] = std::min(lo[0], (int)(sp & 0xFFFFu));
+        hi[0] = std::max(hi[0], (int)(sp >> 16));
+        lo[1] = std::min(lo[1], by);
+        hi[1] = std::max(hi[1], by);
+        lo[2] = std::min(lo[2], bz);
+        hi[2] = std::max(hi[2], bz);
+      }
+    for (int c = 0; c < 3; ++c) {
+      sc->box_lo[c] = lo[c];
+      sc->box_hi[c] = hi[c];
+    }
   }
   // pass 2: pack occupied bricks into the pool
   for (int bz0 = 0; bz0 < bgz; bz0 += slab) {

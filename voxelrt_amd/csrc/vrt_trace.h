@@ -52,6 +52,11 @@ struct SceneDev {
   // each XCD's L2 sees the bricks of one band), 2 row-major from the last
   // tile, 3 bands from their last tile.
   uint32_t tile_order;
+  // The box of occupied bricks, in brick coordinates (inclusive; lo > hi on
+  // every axis for a scene with none): outside it every brick is empty, so
+  // a ray that leaves it cannot hit.  The path tracer's DDA ends rays there
+  // (DdaFastT<.., BOX = true>).
+  int box_lo[3], box_hi[3];
   // Always 0.0f.  Passed at run time so that (zero - x) stays a real
   // subtraction: the gfx950 instruction selector folds a literal (0.0f - x)
   // feeding a multiply into a negated operand, which turns +0 into -0.
