@@ -22,24 +22,20 @@ template <bool PEEK = false, class St>
 void step_ev_walk(const vrt::SceneDev& s, const vrt::LdsScene& ls, St& st, vrt::f3 ro, vrt::f3 rd, float tmax,
                   vrt::TraceResult& o, uint64_t& nit) {
   if (st.begin(s, ls, ro, rd, tmax, o)) return;
+  // enter() reports a brick whose solid voxels the ray cannot reach (FBOX):
+  // the lane then leaves it at once
+  auto enter = [&]() { return st.enter(s, ls) ? vrt::DDA_RUN : vrt::DDA_LEAVE; };
   uint32_t r = PEEK ? st.peek_enter(s, ls) : vrt::DDA_RUN;
-  if (r == vrt::DDA_ENTER) {
-    st.enter(s, ls);
-    r = vrt::DDA_RUN;
-  }
+  if (r == vrt::DDA_ENTER) r = enter();
   for (;;) {
     if (r == vrt::DDA_MISS) break;
     if (r == vrt::DDA_ENTER) {
-      st.enter(s, ls);
-      r = vrt::DDA_RUN;
+      r = enter();
     } else if (r == vrt::DDA_LEAVE) {
       r = st.resume(s, ls);
       if (PEEK && r == vrt::DDA_RUN) {
         r = st.peek_enter(s, ls);
-        if (r == vrt::DDA_ENTER) {
-          st.enter(s, ls);
-          r = vrt::DDA_RUN;
-        }
+        if (r == vrt::DDA_ENTER) r = enter();
       }
     }
     if (r == vrt::DDA_RUN) {
@@ -52,7 +48,7 @@ void step_ev_walk(const vrt::SceneDev& s, const vrt::LdsScene& ls, St& st, vrt::
 }
 
 struct HostScene {
-  std::vector<uint32_t> bricks, pool, bits, prefix, pbits, pprefix;
+  std::vector<uint32_t> bricks, pool, bits, prefix, pbits, pprefix, vbox;
   std::vector<uint8_t> pdist;
   vrt::SceneDev dev{};
 };
@@ -86,8 +82,23 @@ void build(const uint32_t* vox, int nx, int ny, int nz, HostScene& h) {
         h.bricks[bi] = ++slot;
         h.pool.insert(h.pool.end(), mask, mask + 16);
         h.pool.insert(h.pool.end(), words, words + 512);
+        {  // the brick's box of solid voxels (packed as SceneDev::vbox)
+          int lo[3] = {7, 7, 7}, hi[3] = {0, 0, 0};
+          for (int lz = 0; lz < 8; ++lz)
+            for (int ly = 0; ly < 8; ++ly)
+              for (int lx = 0; lx < 8; ++lx)
+                if (mask[(lx + 8 * ly + 64 * lz) >> 5] >> ((lx + 8 * ly + 64 * lz) & 31) & 1u) {
+                  const int l[3] = {lx, ly, lz};
+                  for (int c = 0; c < 3; ++c) {
+                    lo[c] = std::min(lo[c], l[c]);
+                    hi[c] = std::max(hi[c], l[c]);
+                  }
+                }
+          h.vbox.push_back((uint32_t)(lo[0] | lo[1] << 3 | lo[2] << 6 | hi[0] << 9 | hi[1] << 12 | hi[2] << 15));
+        }
       }
   if (h.pool.empty()) h.pool.assign(vrt::POOL_WORDS, 0u);
+  if (h.vbox.empty()) h.vbox.assign(1, 0u);
   const uint64_t nw = (nb + 31) / 32;
   h.bits.assign(nw, 0u);
   for (uint64_t i = 0; i < nb; ++i)
@@ -144,6 +155,7 @@ void build(const uint32_t* vox, int nx, int ny, int nz, HostScene& h) {
   s.psh2 = sh2;
   s.bricks = h.bricks.data();
   s.pool = h.pool.data();
+  s.vbox = h.vbox.data();
   s.bits = h.bits.data();
   s.prefix = h.prefix.data();
   s.bit_words = (uint32_t)nw;
@@ -248,6 +260,17 @@ extern "C" int dda_host_trace(const uint32_t* vox, int nx, int ny, int nz, const
           step_ev_walk<true>(h.dev, ls, st, ro, rd, tmax, o, nit);
         } else {
           vrt::DdaFastT<true, true> st;
+          step_ev_walk<true>(h.dev, ls, st, ro, rd, tmax, o, nit);
+        }
+        break;
+      }
+      case 16:  // 14 and 15 also ending fine walks at the entered brick's box of solid voxels
+      case 17: {
+        if (variant == 16) {
+          vrt::DdaFastT<false, true, true, true> st;
+          step_ev_walk<true>(h.dev, ls, st, ro, rd, tmax, o, nit);
+        } else {
+          vrt::DdaFastT<true, true, true, true> st;
           step_ev_walk<true>(h.dev, ls, st, ro, rd, tmax, o, nit);
         }
         break;

@@ -204,3 +204,48 @@ def test_box_variants_end_misses_early(dda, variant, kind):
         assert saved > 0 and (~hit).sum() > 0
     if kind == "empty":
         assert not hit.any() and np.all(got["brick_steps"] == 0)
+
+
+FBOX_VARIANTS = {"step_ev_peek_box_fbox_lds": 16, "step_ev_peek_box_fbox_global": 17}
+
+
+@pytest.mark.parametrize("variant", list(FBOX_VARIANTS))
+@pytest.mark.parametrize("kind", ["world", "subbox", "full"])
+def test_fbox_variants_end_fine_walks_early(dda, variant, kind):
+    """The path tracer's DDA with both boxes (rays end at the box of occupied
+    bricks; a fine walk ends at the entered brick's box of solid voxels, and
+    a brick whose box the ray cannot reach is left at once) against the
+    oracle's grid walk: every record's voxel, face and t bits are equal, a
+    hit's brick steps are equal, a miss's at most equal, and the voxel steps
+    at most equal (fewer in the world scene, whose bricks leave empty
+    margins; random bricks at half fill span their whole brick)."""
+    vox = _box_scene(kind)
+    rays = np.concatenate([cf.traverse_rays(20000, n=64, seed=5), _edge_rays(64)])
+    want = oracle.trace_rays(oracle.OracleGrid(vox), rays)
+    got = dda(vox, rays, FBOX_VARIANTS[variant])
+    for f in ("voxel", "face"):
+        bad = np.flatnonzero(got[f] != want[f])
+        assert len(bad) == 0, f"{kind} {f}: {len(bad)} mismatches, first {bad[:5]}"
+    assert np.array_equal(got["t"].view(np.uint32), want["t"].view(np.uint32))
+    hit = want["t"] >= 0
+    assert np.array_equal(got["brick_steps"][hit], want["brick_steps"][hit])
+    assert np.all(got["brick_steps"][~hit] <= want["brick_steps"][~hit])
+    assert np.all(got["voxel_steps"] <= want["voxel_steps"])
+    if kind == "world":
+        assert got["voxel_steps"].sum() < want["voxel_steps"].sum()
+
+
+def test_fbox_on_sparse_bricks_skips_whole_bricks(dda):
+    """Bricks holding one solid voxel in a corner: most rays that enter such a
+    brick cannot reach that voxel and leave it at once (no voxel step)."""
+    vox = np.zeros((64, 64, 64), np.uint32)
+    rng = np.random.default_rng(2)
+    for bz, by, bx in rng.integers(0, 8, (120, 3)):
+        vox[bz * 8 + 7, by * 8 + 7, bx * 8 + 7] = 0xFF8080FF
+    rays = cf.traverse_rays(20000, n=64, seed=6)
+    want = oracle.trace_rays(oracle.OracleGrid(vox), rays)
+    for variant in FBOX_VARIANTS.values():
+        got = dda(vox, rays, variant)
+        assert np.array_equal(got["voxel"], want["voxel"]) and np.array_equal(got["face"], want["face"])
+        assert np.array_equal(got["t"].view(np.uint32), want["t"].view(np.uint32))
+        assert got["voxel_steps"].sum() * 4 < want["voxel_steps"].sum()

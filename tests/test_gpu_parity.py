@@ -278,14 +278,17 @@ def test_pathtrace_frame128_bitexact(ctx, frame128, pt_variant):
     assert_bits_equal("gbuf", outs["gbuf"], gb)
     assert_bits_equal("albedo", outs["albedo"], alb)
     assert_bits_equal("radiance", outs["radiance"], rad)
-    assert (fr.rays, fr.voxel_steps) == (cnt.rays, cnt.voxel_steps)
+    assert fr.rays == cnt.rays
     if pt_variant.get("pt_kernel") == 8:
-        # the step-level kernel ends rays at the box of occupied bricks: a
-        # miss takes fewer brick steps than the grid walk (the scene's
-        # bricks stop well below the grid's top), a hit the same
+        # the step-level kernel ends rays at the box of occupied bricks (a
+        # miss takes fewer brick steps than the grid walk: the scene's bricks
+        # stop well below the grid's top) and fine walks at the entered
+        # brick's box of solid voxels (fewer voxel steps in bricks passed
+        # without a hit); the images above are the grid walk's
         assert 0 < fr.brick_steps < cnt.brick_steps
+        assert 0 < fr.voxel_steps < cnt.voxel_steps
     else:
-        assert fr.brick_steps == cnt.brick_steps
+        assert (fr.brick_steps, fr.voxel_steps) == (cnt.brick_steps, cnt.voxel_steps)
 
 
 @pytest.mark.parametrize("bounces,spp,fclamp,frame", [(4, 3, 0.0, 7), (0, 1, 48.0, 123456789), (6, 2, 10.0, 3)])

@@ -137,7 +137,14 @@ VRT_D bool bit_at(uint32_t w, uint32_t i) {
 // miss.  Hits, their t and both step counts are those of the grid walk; a
 // miss records fewer brick steps (the path tracer's choice, whose step counts
 // are statistics; the traversal's miss records keep the grid walk's counts).
-template <bool GM, bool KM = false, bool BOX = false>
+// FBOX: the same at the fine level, against the entered brick's box of solid
+// voxels (SceneDev::vbox): the fine walk ends where the ray leaves that box,
+// and enter() returns false for a ray that cannot reach it (the caller then
+// leaves the brick at once: resume()).  The coarse walk resumes from its own
+// saved state either way, so every hit (voxel, face, t) and the brick steps
+// are unchanged; the voxel steps spent in bricks passed without a hit are
+// fewer.
+template <bool GM, bool KM = false, bool BOX = false, bool FBOX = false>
 struct DdaFastT {
   float ro[3], rdv[3], inv[3];
   int sd[3];
@@ -274,16 +281,17 @@ struct DdaFastT {
   // prefix[w] + the set bits below it in bitmap word w; its mask goes into
   // the lane's LDS slot; the fine level starts at the voxel containing
   // ro + rd * (t + 1e-4), clamped into the brick, with tMax from the origin.
-  VRT_D void enter(const SceneDev& s, const LdsScene& L) {
+  VRT_D bool enter(const SceneDev& s, const LdsScene& L) {
     const uint32_t w = idx >> 5;
+    uint32_t slot;
     if constexpr (GM) {
       const uint32_t word = s.pbits[w];
-      const uint32_t slot = s.pprefix[w] + (uint32_t)__builtin_popcount(word & ((1u << (idx & 31u)) - 1u));
+      slot = s.pprefix[w] + (uint32_t)__builtin_popcount(word & ((1u << (idx & 31u)) - 1u));
       blk = slot * (uint32_t)POOL_WORDS;
       gbase = (const char*)(s.pool + blk);  // mask words 0..15 of the entry
     } else {
       const uint32_t word = L.lds[L.bits_off + w];
-      const uint32_t slot = L.lds[L.prefix_off + w] + (uint32_t)__builtin_popcount(word & ((1u << (idx & 31u)) - 1u));
+      slot = L.lds[L.prefix_off + w] + (uint32_t)__builtin_popcount(word & ((1u << (idx & 31u)) - 1u));
       blk = slot * (uint32_t)POOL_WORDS;
       const uint4* q = (const uint4*)(s.pool + blk);
 #pragma unroll
@@ -317,7 +325,25 @@ struct DdaFastT {
     cntB = cnt;
     iterB = iter;
     idx = (uint32_t)lp[0] + 8u * (uint32_t)lp[1] + 64u * (uint32_t)lp[2];
-    if constexpr (KM) {
+    bool reach = true;
+    if constexpr (FBOX) {
+      // the three axes at once, a byte each (values 0..7, so 8 + a - b never
+      // borrows across bytes): A = 8 + hi - lp, B = 8 + lp - lo; an axis
+      // moving up needs A >= 8 (left = A - 8), one moving down B >= 8, one
+      // that does not move lo <= lp <= hi (both >= 8; left 255).  The axis
+      // signs come from the ray's kfine / mfine (7 / 0 / 255, 7 / 0).
+      const uint32_t vb = s.vbox[slot];
+      const uint32_t lpk = (uint32_t)lp[0] | ((uint32_t)lp[1] << 8) | ((uint32_t)lp[2] << 16);
+      const uint32_t lok = (vb & 7u) | ((vb << 5) & 0x700u) | ((vb << 10) & 0x70000u);
+      const uint32_t hik = ((vb >> 9) & 7u) | ((vb >> 4) & 0x700u) | ((vb << 1) & 0x70000u);
+      const uint32_t A = hik + 0x080808u - lpk, B = lpk + 0x080808u - lok;
+      const uint32_t up = kfine & mfine, down = mfine & ~kfine;      // 7 per byte
+      const uint32_t mu = up | (up << 1), md = down | (down << 1);   // 15 per byte
+      const uint32_t S = (A & mu) | (B & md);
+      const uint32_t mov = (mu | md) & 0x080808u, still = ~(mu | md) & 0x080808u;
+      reach = (S & mov) == mov && (A & B & still) == still;
+      cnt = (S - mov) | ((still >> 3) * 0xFFu);
+    } else if constexpr (KM) {
       cnt = kfine ^ (((uint32_t)lp[0] | ((uint32_t)lp[1] << 8) | ((uint32_t)lp[2] << 16)) & mfine);
     } else {
       const int n8[3] = {8, 8, 8};
@@ -330,6 +356,7 @@ struct DdaFastT {
     }
     inc = 0x10000u;
     tBound = __builtin_inff();
+    return reach;
   }
 
   // Leave the brick: back to the coarse registers saved on entry.

@@ -90,12 +90,13 @@ struct vrt_scene {
   vrt::DeviceBuffer d_pbits, d_pprefix;  // the same over the power-of-two padded brick grid
   vrt::DeviceBuffer d_pdist;             // row distance per padded bitmap word (bytes)
   vrt::DeviceBuffer d_dense;  // editable scenes: the dense packed grid, rebuilt from on edits
+  vrt::DeviceBuffer d_vbox;   // per pool slot: the box of the brick's solid voxels (SceneDev::vbox)
   uint32_t psh1 = 0, psh2 = 0, psh3 = 0;  // padded bit index = bx | by << psh1 | bz << psh2; 2^psh3 bits
   int box_lo[3] = {0, 0, 0}, box_hi[3] = {-1, -1, -1};  // occupied bricks' box (scene_build)
   vrt::SceneDev dev() const;
   // every device buffer of the scene (those of size 0 are not allocated)
-  std::array<vrt::DeviceBuffer*, 8> buffers() {
-    return {&d_bricks, &d_pool, &d_bits, &d_prefix, &d_pbits, &d_pprefix, &d_pdist, &d_dense};
+  std::array<vrt::DeviceBuffer*, 9> buffers() {
+    return {&d_bricks, &d_pool, &d_bits, &d_prefix, &d_pbits, &d_pprefix, &d_pdist, &d_dense, &d_vbox};
   }
 };
 

@@ -57,6 +57,9 @@ hipError_t launch_brick_pbits(hipStream_t st, const uint32_t* bricks, int bgx, i
 hipError_t launch_popc(hipStream_t st, const uint32_t* words, uint64_t n, uint32_t* counts);
 // Row distances of a scene (SceneDev::pdist) from its brick words; tmp holds
 // 3 * bgy * bgz bytes, pdist (pwords + 3) & ~3 bytes.
+// Per pool slot [0, nslots): the packed box of the brick's solid voxels
+// (SceneDev::vbox), from the entry's mask words.
+hipError_t launch_brick_vbox(hipStream_t st, const uint32_t* pool, uint64_t nslots, uint32_t* vbox);
 // Per brick row (by, bz), r = by + bz * bgy: first | last << 16 of its
 // occupied bx, 0xFFFF when empty (bgy * bgz words).
 hipError_t launch_row_span(hipStream_t st, const uint32_t* bricks, int bgx, int bgy, int bgz, uint32_t* span);

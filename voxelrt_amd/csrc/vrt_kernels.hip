@@ -196,6 +196,40 @@ __global__ __launch_bounds__(256) void k_row_span(const uint32_t* __restrict__ b
   span[r] = first == 0xFFFFu ? 0xFFFFu : (first | last << 16);
 }
 
+// vbox[slot]: the box of the solid voxels of the brick in pool slot `slot`,
+// from its 16 mask words (bit x + 8 y + 64 z; word k holds z = k >> 1, rows
+// y = 4 (k & 1) .. + 3 in its four bytes): lo x | lo y << 3 | lo z << 6 |
+// hi x << 9 | hi y << 12 | hi z << 15.
+__global__ __launch_bounds__(256) void k_brick_vbox(const uint32_t* __restrict__ pool, uint64_t nslots,
+                                                    uint32_t* __restrict__ vbox) {
+  const uint64_t slot = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot >= nslots) return;
+  const uint4* m = (const uint4*)(pool + slot * (uint64_t)POOL_WORDS);
+  uint32_t xs = 0u, ys = 0u, zs = 0u;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const uint4 v = m[q];
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int k = 4 * q + j;
+      const uint32_t word = w[j];
+      xs |= (word | word >> 8 | word >> 16 | word >> 24) & 0xFFu;
+#pragma unroll
+      for (int b = 0; b < 4; ++b) ys |= ((word >> (8 * b)) & 0xFFu) != 0u ? 1u << (4 * (k & 1) + b) : 0u;
+      zs |= word != 0u ? 1u << (k >> 1) : 0u;
+    }
+  }
+  if (xs == 0u) {  // no solid voxel (not an occupied brick's entry)
+    vbox[slot] = 0u;
+    return;
+  }
+  const uint32_t lo[3] = {(uint32_t)__builtin_ctz(xs), (uint32_t)__builtin_ctz(ys), (uint32_t)__builtin_ctz(zs)};
+  const uint32_t hi[3] = {31u - (uint32_t)__builtin_clz(xs), 31u - (uint32_t)__builtin_clz(ys),
+                          31u - (uint32_t)__builtin_clz(zs)};
+  vbox[slot] = lo[0] | lo[1] << 3 | lo[2] << 6 | hi[0] << 9 | hi[1] << 12 | hi[2] << 15;
+}
+
 // dy[r]: distance along y, within the row's z, to the nearest occupied row
 // (255 when there is none or it is farther).
 __global__ __launch_bounds__(256) void k_row_dy(const uint8_t* __restrict__ occ, int bgy, int bgz,
@@ -1565,6 +1599,12 @@ hipError_t launch_row_dist(hipStream_t st, const uint32_t* bricks, int bgx, int 
   hipLaunchKernelGGL(k_row_dist, dim3(div_up(rows, 256)), dim3(256), 0, st, dy, bgy, bgz, dist);
   hipLaunchKernelGGL(k_word_dist, dim3(div_up(nw, 256)), dim3(256), 0, st, dist, bgy, bgz, sh1, sh2, pwords, nw,
                      pdist);
+  return hipGetLastError();
+}
+
+hipError_t launch_brick_vbox(hipStream_t st, const uint32_t* pool, uint64_t nslots, uint32_t* vbox) {
+  if (nslots == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_brick_vbox, dim3(div_up(nslots, 256)), dim3(256), 0, st, pool, nslots, vbox);
   return hipGetLastError();
 }
 

@@ -191,7 +191,9 @@ VRT_D void wave_paths(V& v, const LdsScene& L0, uint32_t min_wait, uint32_t shad
                       unsigned long long* __restrict__ counters) {
   const uint32_t lane = threadIdx.x & 63u;
   const uint64_t lt_mask = lane == 0u ? 0ull : (~0ull >> (64u - lane));
-  DdaFastT<GM, true, true> st;  // rays end at the box of occupied bricks
+  // rays end at the box of occupied bricks, fine walks at the entered
+  // brick's box of solid voxels
+  DdaFastT<GM, true, true, true> st;
   WavePath P;
   P.pix = 0u;
   P.smp = 0u;
@@ -413,10 +415,8 @@ VRT_D void wave_paths(V& v, const LdsScene& L0, uint32_t min_wait, uint32_t shad
       peek = status == DDA_RUN;
     }
     if (peek) status = st.peek_enter(v.s(), L0);
-    if (status == DDA_ENTER) {
-      st.enter(v.s(), L0);
-      status = DDA_RUN;
-    }
+    if (status == DDA_ENTER)  // a brick whose solid voxels the ray cannot reach is left at once
+      status = st.enter(v.s(), L0) ? DDA_RUN : st.resume(v.s(), L0);
     // ---------------- stepping --------------------------------------------
     // until min_wait lanes wait at a brick boundary, shade_min lanes have a
     // finished ray, or no lane steps; the counts are taken every

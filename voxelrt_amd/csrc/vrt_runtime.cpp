@@ -199,6 +199,7 @@ vrt::SceneDev vrt_scene::dev() const {
   s.pbits = (const uint32_t*)d_pbits.ptr;
   s.pprefix = (const uint32_t*)d_pprefix.ptr;
   s.pdist = (const uint8_t*)d_pdist.ptr;
+  s.vbox = (const uint32_t*)d_vbox.ptr;
   s.entry_batch = 1u;
   s.tile_order = 0u;
   s.pwords = psh3 >= 5 ? (1u << (psh3 - 5)) : 1u;
@@ -473,8 +474,9 @@ static int scene_build(vrt_ctx* ctx, vrt_scene* sc, const uint32_t* hvox, const 
   const uint64_t pool_bytes = sc->d_pool.bytes >= pool_need ? sc->d_pool.bytes
                               : (sc->d_dense.ptr ? pool_need + pool_need / 4 : pool_need);
   const uint64_t pdist_bytes = (((pbits_n + 31) / 32) + 3) & ~3ull;
+  const uint64_t vbox_bytes = pool_bytes / (vrt::POOL_WORDS * 4) * 4;  // a word per pool slot
   const uint64_t bytes = sc->d_dense.bytes + nb * 4 + pool_bytes + (nb + 63) / 64 * 8 + (nb + 31) / 32 * 4 +
-                         (pbits_n + 63) / 64 * 8 + (pbits_n + 31) / 32 * 4 + pdist_bytes;
+                         (pbits_n + 63) / 64 * 8 + (pbits_n + 31) / 32 * 4 + pdist_bytes + vbox_bytes;
   if (bytes > sc->bytes) {
     int rc = ctx->reserve(bytes - sc->bytes);
     if (rc) return rc;
@@ -488,6 +490,7 @@ static int scene_build(vrt_ctx* ctx, vrt_scene* sc, const uint32_t* hvox, const 
   if (e == hipSuccess && !sc->d_pbits.ptr) e = sc->d_pbits.alloc(((pbits_n + 63) / 64) * 8);
   if (e == hipSuccess && !sc->d_pprefix.ptr) e = sc->d_pprefix.alloc((pbits_n + 31) / 32 * 4);
   if (e == hipSuccess && !sc->d_pdist.ptr) e = sc->d_pdist.alloc(pdist_bytes);
+  if (e == hipSuccess && sc->d_vbox.bytes < vbox_bytes) e = sc->d_vbox.alloc(vbox_bytes);
   if (e != hipSuccess)
     return ctx->fail(e == hipErrorOutOfMemory ? VRT_ERR_NO_MEMORY : VRT_ERR_DEVICE, "scene allocation failed: %s",
                      hipGetErrorString(e));
@@ -571,6 +574,8 @@ static int scene_build(vrt_ctx* ctx, vrt_scene* sc, const uint32_t* hvox, const 
                               (uint32_t*)sc->d_pool.ptr);
     if (e != hipSuccess) return err("pack", e);
   }
+  e = vrt::launch_brick_vbox(ctx->stream, (const uint32_t*)sc->d_pool.ptr, nocc, (uint32_t*)sc->d_vbox.ptr);
+  if (e != hipSuccess) return err("voxel boxes", e);
   e = hipStreamSynchronize(ctx->stream);
   if (e != hipSuccess) return err("scene build", e);
   return VRT_OK;
@@ -674,8 +679,13 @@ int vrt_scene_edit(vrt_ctx* ctx, vrt_scene* scene, const uint32_t* vox, uint32_t
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   scene->edits_in_place += changed ? 0u : 1u;
   scene->edits_rebuilt += changed ? 1u : 0u;
-  if (!changed) return VRT_OK;
-  return scene_build(ctx, scene, nullptr, dense);
+  if (changed) return scene_build(ctx, scene, nullptr, dense);
+  // in place: the touched bricks' masks may have changed, and with them
+  // their boxes of solid voxels
+  HIP_TRY(ctx, vrt::launch_brick_vbox(ctx->stream, (const uint32_t*)scene->d_pool.ptr, scene->nocc,
+                                      (uint32_t*)scene->d_vbox.ptr));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return VRT_OK;
   GUARD_END(ctx)
 }
 

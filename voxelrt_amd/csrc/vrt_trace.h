@@ -57,6 +57,11 @@ struct SceneDev {
   // a ray that leaves it cannot hit.  The path tracer's DDA ends rays there
   // (DdaFastT<.., BOX = true>).
   int box_lo[3], box_hi[3];
+  // Per pool slot, the box of the brick's solid voxels: lo x | lo y << 3 |
+  // lo z << 6 | hi x << 9 | hi y << 12 | hi z << 15 (brick-local, inclusive).
+  // A ray entering the brick that cannot reach it walks no voxel of it
+  // (DdaFastT<.., FBOX = true>).
+  const uint32_t* vbox;
   // Always 0.0f.  Passed at run time so that (zero - x) stays a real
   // subtraction: the gfx950 instruction selector folds a literal (0.0f - x)
   // feeding a multiply into a negated operand, which turns +0 into -0.
