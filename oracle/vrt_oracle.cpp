@@ -161,24 +161,23 @@ inline float pow_own(float x, float y) {
 }
 inline float exp_own(float x) { return exp2_own(x * LOG2E_F); }
 
-// e^x for the a-trous weights (atrous.wgsl:74-75), whose arguments are <= 0
-// or NaN: z = x log2(e), floored at -160 (a NaN argument becomes -160, so
-// the weight is 0), k = rint(z) (nearest, ties to even), then
-// 2^k * e^((z - k) ln 2) with the degree-7 Taylor polynomial of exp2_own in
-// fused multiply-add steps, scaled by ldexp (correctly rounded, subnormal
-// results included).  Every step is exact or correctly rounded, so the
-// device's exp_w (vrt_device.h) gives the same bits.
+// e^x for the a-trous weights (atrous.wgsl:74-75; WGSL allows exp 3 + 2|x|
+// ulps), whose arguments are <= 0 or NaN: z = x log2(e), floored at -160 (a
+// NaN argument becomes -160, so the weight is 0), k = rint(z) (nearest, ties
+// to even), f = z - k in [-1/2, 1/2] (exact), then 2^k * 2^f with a degree-5
+// polynomial for 2^f fitted for relative error on [-1/2, 1/2] (constant term
+// 1; within 2.1 ulps) in fused multiply-add steps, scaled by ldexp
+// (correctly rounded, subnormal results included).  Every step is exact or
+// correctly rounded, so the device's exp_w (vrt_device.h) gives the same bits.
 __attribute__((target("fma"))) float exp_w(float x) {
   const float z = std::fmax(x * LOG2E_F, -160.0f);
   const float k = std::rint(z);
-  const float y = (z - k) * LN2_F;
-  float p = std::fmaf((float)(1.0 / 5040.0), y, (float)(1.0 / 720.0));
-  p = std::fmaf(p, y, (float)(1.0 / 120.0));
-  p = std::fmaf(p, y, (float)(1.0 / 24.0));
-  p = std::fmaf(p, y, (float)(1.0 / 6.0));
-  p = std::fmaf(p, y, 0.5f);
-  p = std::fmaf(p, y, 1.0f);
-  p = std::fmaf(p, y, 1.0f);
+  const float f = z - k;  // exact, in [-1/2, 1/2]
+  float p = std::fmaf(0x1.5bb9fep-10f, f, 0x1.3cea88p-7f);
+  p = std::fmaf(p, f, 0x1.c6b754p-5f);
+  p = std::fmaf(p, f, 0x1.ebf9bcp-3f);
+  p = std::fmaf(p, f, 0x1.62e42ap-1f);
+  p = std::fmaf(p, f, 1.0f);
   return std::ldexp(p, (int)k);
 }
 inline float pow8(float x) {

@@ -163,20 +163,21 @@ VRT_D float exp_r(float x) { return exp2_r(x * 0x1.715476p+0f); }
 
 // e^x for the a-trous weights, whose arguments are <= 0 or NaN:
 // z = max(x log2(e), -160) (IEEE maximum: a NaN argument becomes -160 and
-// the weight 0), k = rint(z) in [-160, 0], 2^k * e^((z - k) ln 2) with the
-// degree-7 Taylor polynomial of exp2_r in fused multiply-add steps, scaled
-// by ldexp.  Each step is exact or correctly rounded (14 instructions).
+// the weight 0), k = rint(z) in [-160, 0], f = z - k in [-1/2, 1/2] (exact),
+// e^x = 2^k 2^f with 2^f from a degree-5 polynomial fitted for relative
+// error on [-1/2, 1/2] (constant term 1, so e^0 = 1 exactly) in fused
+// multiply-add steps, scaled by ldexp: within 2.1 ulps of 2^f, plus the
+// rounding of x log2(e) (|x| 2^-24 relative), inside the 3 + 2|x| ulps WGSL
+// allows exp (11 instructions).
 VRT_D float exp_w(float x) {
   const float z = __builtin_fmaxf(x * 0x1.715476p+0f, -160.0f);
   const float k = __builtin_rintf(z);
-  const float y = (z - k) * 0x1.62e43p-1f;
-  float p = __builtin_fmaf((float)(1.0 / 5040.0), y, (float)(1.0 / 720.0));
-  p = __builtin_fmaf(p, y, (float)(1.0 / 120.0));
-  p = __builtin_fmaf(p, y, (float)(1.0 / 24.0));
-  p = __builtin_fmaf(p, y, (float)(1.0 / 6.0));
-  p = __builtin_fmaf(p, y, 0.5f);
-  p = __builtin_fmaf(p, y, 1.0f);
-  p = __builtin_fmaf(p, y, 1.0f);
+  const float f = z - k;
+  float p = __builtin_fmaf(0x1.5bb9fep-10f, f, 0x1.3cea88p-7f);
+  p = __builtin_fmaf(p, f, 0x1.c6b754p-5f);
+  p = __builtin_fmaf(p, f, 0x1.ebf9bcp-3f);
+  p = __builtin_fmaf(p, f, 0x1.62e42ap-1f);
+  p = __builtin_fmaf(p, f, 1.0f);
   return __builtin_ldexpf(p, (int)k);
 }
 VRT_D float pow8(float x) {
