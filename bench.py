@@ -627,7 +627,7 @@ def main():
     ap.add_argument("--pt-waves", type=int, default=4, choices=[0, 4, 5, 6])
     ap.add_argument("--pt-min-refill", type=int, default=16, help="pt-kernel 8: lanes waiting before a level-switch pass")
     ap.add_argument("--pt-blocks", type=int, default=0, help="pt-kernel 8: persistent waves (0 = 16 per CU)")
-    ap.add_argument("--pt-shade-min", type=int, default=32, help="pt-kernel 8: finished rays shaded together")
+    ap.add_argument("--pt-shade-min", type=int, default=40, help="pt-kernel 8: finished rays shaded together")
     ap.add_argument("--pt-tile", type=int, default=0, choices=[0, 1, 2, 3, 4, 5],
                     help="path-tracer tile form with a fixed --pt-kernel")
     args = ap.parse_args()

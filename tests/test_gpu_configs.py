@@ -38,7 +38,7 @@ def frame1080(ctx):
 
 
 @pytest.mark.parametrize("tuning", [dict(pt_kernel=8, pt_waves=4, pt_tile=0, pt_order=2, pt_blocks=0,
-                                         pt_min_refill=12, pt_shade_min=24),
+                                         pt_min_refill=16, pt_shade_min=40),
                                     dict(pt_kernel=8, pt_waves=4, pt_tile=1, pt_order=2, pt_blocks=0,
                                          pt_min_refill=16, pt_shade_min=32),
                                     dict(pt_kernel=7, pt_waves=4, pt_tile=3, pt_order=2),

@@ -41,8 +41,9 @@
 namespace vrt {
 
 // DDA steps a lane takes between two counts of the wave's waiting and
-// finished lanes (4 measured fastest at 1080p, against 1, 2, 3, 6 and 8).
-constexpr int STEPS_PER_CHECK = 4;
+// finished lanes (6 measured fastest at 1080p, against 3, 4, 5 and 8, once
+// rays ended at the occupied boxes; 4 before, when walks were longer).
+constexpr int STEPS_PER_CHECK = 6;
 
 // Lane status: a DDA status while a ray is in flight or just finished
 // (DDA_RUN / DDA_MISS / DDA_HIT), or one of these.

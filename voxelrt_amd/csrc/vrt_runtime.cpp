@@ -384,7 +384,7 @@ int vrt_create(const vrt_create_info* info, vrt_ctx** out) {
   ctx->tuning.pt_blocks = 0;
   ctx->tuning.pt_waves = 4;
   ctx->tuning.pt_min_refill = 16;
-  ctx->tuning.pt_shade_min = 32;
+  ctx->tuning.pt_shade_min = 40;
   if (ctx->d_queue.alloc((2 * vrt::QUEUE_WORDS + 64) * sizeof(uint32_t)) != hipSuccess ||
       ctx->d_counters.alloc(4 * sizeof(uint64_t)) != hipSuccess ||
       hipMemsetAsync(ctx->d_counters.ptr, 0, 4 * sizeof(uint64_t), ctx->stream) != hipSuccess) {
@@ -1414,7 +1414,7 @@ int vrt_autotune(vrt_ctx* ctx, const vrt_scene* scene, uint64_t nrays, const vrt
     c.pt_order = 2;
     for (uint32_t form : {0u, 1u})
       for (uint32_t mw : {8u, 12u, 16u, 24u})
-        for (uint32_t sm : {24u, 32u}) {
+        for (uint32_t sm : {32u, 40u}) {
           c.pt_tile = form;
           c.pt_min_refill = mw;
           c.pt_shade_min = sm;
