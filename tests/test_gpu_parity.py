@@ -6,6 +6,8 @@ accumulation, f16 a-trous output, f32 HDR) must match bit for bit: both sides
 compute FP32 in the same operation order without fused multiply-add and with
 the same own transcendental functions.
 """
+import math
+
 import numpy as np
 import pytest
 
@@ -645,6 +647,40 @@ def test_scene_edits_rebuild_on_device(ctx):
     with pytest.raises(v.VrtError):
         plain.edit(np.zeros((1, 1, 1), np.uint32), (0, 0, 0))  # not editable
     plain.release()
+
+
+def test_inplace_edit_grows_voxel_boxes(ctx):
+    """An in-place edit that fills voxels of already occupied bricks outside
+    their boxes of solid voxels.  The path tracer ends fine walks at each
+    brick's box, so the edit must recompute the boxes (a stale one would let
+    rays pass the new voxels): frames before and after equal the oracle's, and
+    the camera sees the new voxels."""
+    vox = np.zeros((64, 64, 64), np.uint32)
+    vox[0:64, 0:4, 0:64] = 0x10508050  # floor
+    for bz in range(2, 6):
+        for bx in range(2, 6):
+            vox[bz * 8, 24, bx * 8] = 0x10B04040  # one voxel in each brick's low corner
+    sc = ctx.upload(vox, editable=True)
+    # looking along +z and down at the bricks (32, 28, 32) from (32, 48, -8), voxel units
+    u, _ = v.make_uniforms((2.0, 3.0, -0.5), 0.0, math.atan2(-1.25, 2.5), 96, 64, frame=1, bounces=2, flags=0)
+    frames = []
+    for k in range(2):
+        if k == 1:
+            slab = np.full((32, 4, 32), 0x10B0B040, np.uint32)  # y 28..31 of bricks bx, bz 2..5, by 3
+            vox[16:48, 28:32, 16:48] = slab
+            before = sc.edit_counts()
+            sc.edit(slab, (16, 28, 16))
+            assert sc.edit_counts() == (before[0] + 1, before[1])  # in place: occupancy unchanged
+        got, fr = _render_one(ctx, sc, 970 + k, u, 48.0, 1, 96, 64)
+        og = oracle.OracleGrid(vox)
+        rad, alb, gb, cnt = oracle.pathtrace(og, to_oracle_uniforms(u), 48.0, 1)
+        assert_bits_equal(f"box edit {k} gbuf", got["gbuf"], gb)
+        assert_bits_equal(f"box edit {k} radiance", got["radiance"], rad)
+        assert fr.rays == cnt.rays
+        frames.append(got["gbuf"])
+    changed = np.any(frames[0] != frames[1], axis=-1).mean()
+    assert changed > 0.05, changed  # the slabs cover a good part of the view
+    sc.release()
 
 
 def test_large_scene_runs_the_global_dda(ctx):
