@@ -649,6 +649,42 @@ def test_scene_edits_rebuild_on_device(ctx):
     plain.release()
 
 
+@pytest.mark.parametrize("kind", ["empty", "corner-noncubic"])
+def test_pathtrace_occupied_box_edges(ctx, kind):
+    """Scenes at the edges of the path tracer's occupied boxes: an empty grid
+    (the box is empty: every ray ends at setup and sees the sky) and a
+    non-cubic grid whose occupied bricks fill only one corner region; frames
+    equal the oracle's for every step-level variant form."""
+    if kind == "empty":
+        vox = np.zeros((64, 64, 64), np.uint32)
+        pos, yaw, pitch = (2.0, 2.0, 2.0), 0.7, -0.3
+    else:
+        vox = v.generate_scene(v.SCENE_RANDOM_BRICKS, n=96, ny=32, nz=64, seed=17, p_brick=0.5, p_voxel=0.4)
+        vox[:, :, 40:] = 0  # x >= 40
+        vox[:, 16:, :] = 0  # y >= 16
+        vox[:24, :, :] = 0  # z < 24
+        pos, yaw, pitch = (0.5, 1.5, 0.5), 0.6, -0.2  # above the box, looking down into it
+    sc = ctx.upload(vox)
+    og = oracle.OracleGrid(vox)
+    u, _ = v.make_uniforms(pos, yaw, pitch, 80, 48, frame=3, bounces=3, flags=0)
+    rad, alb, gb, cnt = oracle.pathtrace(og, to_oracle_uniforms(u), 48.0, 1)
+    old = ctx.tuning().as_dict()
+    try:
+        for tile in (0, 1):  # LDS and global-memory DDA
+            ctx.set_tuning(pt_kernel=8, pt_waves=4, pt_tile=tile, pt_blocks=0)
+            got, fr = _render_one(ctx, sc, 975 + tile, u, 48.0, 1, 80, 48)
+            assert_bits_equal(f"{kind} tile {tile} gbuf", got["gbuf"], gb)
+            assert_bits_equal(f"{kind} tile {tile} radiance", got["radiance"], rad)
+            assert fr.rays == cnt.rays
+            if kind == "empty":
+                assert fr.brick_steps == 0 and fr.voxel_steps == 0 and (gb[..., 3] < 0).all()
+    finally:
+        ctx.set_tuning(**old)
+    if kind != "empty":
+        assert (gb[..., 3] >= 0).mean() > 0.1  # the corner region is in view
+    sc.release()
+
+
 def test_inplace_edit_grows_voxel_boxes(ctx):
     """An in-place edit that fills voxels of already occupied bricks outside
     their boxes of solid voxels.  The path tracer ends fine walks at each
