@@ -936,8 +936,10 @@ void orc_atrous(const vrt_uniforms* u, const void* src, uint32_t src_format, con
                      load_tex(src, src_format, j * 4 + 2));
           float wKernel = K[dx < 0 ? -dx : dx] * K[dy < 0 ? -dy : dy];
           float wNormal = pow32(fmax_(dot3(gn, mk(gq[0], gq[1], gq[2])), 0.0f));
-          float wDepth = exp_w(-std::fabs(g[3] - gq[3]) / (0.05f * g[3] + 0.01f));
-          float wLum = exp_w(-std::fabs(lumC - dot3(cq, lw)) / sigmaL);
+          // WGSL's divisions (2.5 ulps allowed) as products with the
+          // correctly rounded reciprocals of the per-pixel divisors
+          float wDepth = exp_w(-(std::fabs(g[3] - gq[3]) * (1.0f / (0.05f * g[3] + 0.01f))));
+          float wLum = exp_w(-(std::fabs(lumC - dot3(cq, lw)) * (1.0f / sigmaL)));
           float wt = ((wKernel * wNormal) * wDepth) * wLum;
           sum = add(sum, muls(cq, wt));
           wSum += wt;

@@ -310,33 +310,3 @@ extern "C" void exp_w_table(uint64_t start, uint64_t stride, uint64_t n, float* 
   }
 }
 
-// vrt::div_rcp against IEEE division on n random pairs (divisors in
-// [2^-8, 2^10), dividends in [2^-37, 2^20) and zero): the number of results
-// whose bits differ.
-extern "C" uint64_t div_rcp_check(uint64_t n, uint64_t seed) {
-  uint64_t z = seed, bad = 0;
-  auto next = [&z]() {
-    uint64_t x = (z += 0x9E3779B97F4A7C15ull);
-    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
-    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
-    return x ^ (x >> 31);
-  };
-  for (uint64_t i = 0; i < n; ++i) {
-    const uint64_t r1 = next(), r2 = next();
-    const uint32_t bb = ((119u + (uint32_t)(r1 % 18)) << 23) | (uint32_t)((r1 >> 8) & 0x7fffffu);
-    const uint32_t ab = ((90u + (uint32_t)(r2 % 57)) << 23) | (uint32_t)((r2 >> 8) & 0x7fffffu);
-    float a, b;
-    std::memcpy(&a, &ab, 4);
-    std::memcpy(&b, &bb, 4);
-    if ((r2 & 0xffu) == 0u) a = 0.0f;
-    volatile float q = a / b;
-    const float y = 1.0f / b;
-    const float q1 = vrt::div_rcp(a, b, y);
-    uint32_t u1, u2;
-    const float qq = q;
-    std::memcpy(&u1, &qq, 4);
-    std::memcpy(&u2, &q1, 4);
-    bad += u1 != u2;
-  }
-  return bad;
-}

@@ -43,18 +43,6 @@ VRT_D uint64_t wave_ballot(bool p) {
 
 VRT_D float vmin(float a, float b) { return b < a ? b : a; }
 
-// a / b rounded to nearest, given y = 1 / b rounded to nearest (one IEEE
-// division per divisor instead of one per quotient): q = RN(a*y), the
-// remainder a - b*q is exact in one fused multiply-add, and RN(q + rem*y) is
-// the correctly rounded quotient (Markstein's correction step) for normal
-// operands whose quotient neither overflows nor underflows -- the case of
-// every use here (divisors >= 0.01, dividends finite).  Explicit fmaf, so
-// -ffp-contract=off does not change it.
-VRT_D float div_rcp(float a, float b, float y) {
-  const float q = a * y;
-  const float rem = __builtin_fmaf(-q, b, a);
-  return __builtin_fmaf(rem, y, q);
-}
 VRT_D float vmax(float a, float b) { return a < b ? b : a; }
 VRT_D float vclamp(float x, float lo, float hi) { return vmin(vmax(x, lo), hi); }
 VRT_D float vabs(float x) { return __builtin_fabsf(x); }

@@ -1113,7 +1113,9 @@ __global__ __launch_bounds__(256) void k_temporal(Uniforms u, const ushort4* __r
 //
 // Weights: ((wKernel * wNormal) * wDepth) * wLum as atrous.wgsl:76 multiplies
 // them, wDepth and wLum each through exp_w; the divisions by the per-pixel
-// depth scale and sigmaL as div_rcp (bit-identical to IEEE division).
+// depth scale and sigmaL as products with their correctly rounded
+// reciprocals, taken once per pixel (within 1.5 ulps of the quotient, inside
+// the 2.5 ulps WGSL allows division).
 // ---------------------------------------------------------------------------
 constexpr uint32_t NCODE_GENERAL = 7u, NCODE_SKY = 15u, NCODE_NONE = 31u;
 
@@ -1222,8 +1224,8 @@ VRT_D void atrous_filter(const Uniforms& u, const AtrousLds<STEP>& lds, bool gen
         const float2 gq = sg[l];
         const float4 cq = sc[l];
         const float wKN = __builtin_bit_cast(uint32_t, gq.y) == cc ? K[dx < 0 ? -dx : dx] * K[dy < 0 ? -dy : dy] : 0.0f;
-        const float wDepth = exp_w(-div_rcp(vabs(g.x - gq.x), dscale, rdscale));
-        const float wLum = exp_w(-div_rcp(vabs(lumC - cq.w), sigmaL, rsigmaL));
+        const float wDepth = exp_w(-(vabs(g.x - gq.x) * rdscale));
+        const float wLum = exp_w(-(vabs(lumC - cq.w) * rsigmaL));
         const float wt = (wKN * wDepth) * wLum;
         sum = sum + mk3(cq.x, cq.y, cq.z) * wt;
         wSum = wSum + wt;
@@ -1242,8 +1244,8 @@ VRT_D void atrous_filter(const Uniforms& u, const AtrousLds<STEP>& lds, bool gen
         const float4 nq = gbuf[(uint64_t)(y + dy * STEP) * w + (x + dx * STEP)];
         const float wKernel = K[dx < 0 ? -dx : dx] * K[dy < 0 ? -dy : dy];
         const float wNormal = pow32(vmax(dot(gn, mk3(nq.x, nq.y, nq.z)), 0.0f));
-        const float wDepth = exp_w(-div_rcp(vabs(g.x - gq.x), dscale, rdscale));
-        const float wLum = exp_w(-div_rcp(vabs(lumC - cq.w), sigmaL, rsigmaL));
+        const float wDepth = exp_w(-(vabs(g.x - gq.x) * rdscale));
+        const float wLum = exp_w(-(vabs(lumC - cq.w) * rsigmaL));
         const float wt = ((wKernel * wNormal) * wDepth) * wLum;
         sum = sum + mk3(cq.x, cq.y, cq.z) * wt;
         wSum = wSum + wt;
