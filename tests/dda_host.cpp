@@ -195,7 +195,9 @@ void build(const uint32_t* vox, int nx, int ny, int nz, HostScene& h) {
 // forms of 6 and 3 (brick-entry cell counts from kfine / mfine, KM = true),
 // 10 and 11 the step-level refill kernels' stepping (step_ev, level switches one
 // at a time), 12 and 13 the same with the entry check after begin and resume,
-// 14 and 15 those ending rays at the box of occupied bricks (BOX = true).
+// 14 and 15 those ending rays at the box of occupied bricks (BOX = true),
+// 16 and 17 the path tracer's: also fine walks at the bricks' boxes of solid
+// voxels, with no iteration budget (FBOX, NOITER).
 // hits: n records of 4 u32
 // (t bits, voxel, brick_steps | voxel_steps << 16, face), as vrt_trace_rays
 // writes them.  *iters: loop iterations (step() calls) of variants 3 and 4.
@@ -267,10 +269,10 @@ extern "C" int dda_host_trace(const uint32_t* vox, int nx, int ny, int nz, const
       case 16:  // 14 and 15 also ending fine walks at the entered brick's box of solid voxels
       case 17: {
         if (variant == 16) {
-          vrt::DdaFastT<false, true, true, true> st;
+          vrt::DdaFastT<false, true, true, true, true> st;
           step_ev_walk<true>(h.dev, ls, st, ro, rd, tmax, o, nit);
         } else {
-          vrt::DdaFastT<true, true, true, true> st;
+          vrt::DdaFastT<true, true, true, true, true> st;
           step_ev_walk<true>(h.dev, ls, st, ro, rd, tmax, o, nit);
         }
         break;

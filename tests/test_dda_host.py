@@ -239,3 +239,43 @@ def test_fbox_on_sparse_bricks_skips_whole_bricks(dda):
         assert np.array_equal(got["voxel"], want["voxel"]) and np.array_equal(got["face"], want["face"])
         assert np.array_equal(got["t"].view(np.uint32), want["t"].view(np.uint32))
         assert got["voxel_steps"].sum() * 4 < want["voxel_steps"].sum()
+
+
+def test_noiter_degenerate_rays_end_like_the_budgeted_walk(dda):
+    """The path tracer's DDA has no iteration budget (NOITER): degenerate
+    walks end by the per-axis counts instead.  On zero directions (from solid
+    and from empty voxels), directions with tiny or infinite components and
+    infinite origins, every record's voxel, face and t bits equal the
+    oracle's budgeted walk (step counts may be fewer).  Rays with a NaN
+    component (which the oracle cannot walk: its cell indices would come from
+    NaN) end at setup as a miss in every variant."""
+    vox = _box_scene("world")
+    ext = 64 * v.VOXEL_SIZE
+    rays = []
+    for z, y, x in np.concatenate([np.argwhere(vox != 0)[:64], np.argwhere(vox == 0)[::5000][:64]]):
+        o = (np.array([x, y, z], np.float64) + 0.5) * v.VOXEL_SIZE
+        rays.append([*o, 0.0, 0.0, 0.0])
+        rays.append([*o, -0.0, 0.0, -0.0])
+    inf = float("inf")
+    for o in ([ext / 2, ext / 2, ext / 2], [ext / 3, 0.1, ext / 4], [inf, ext / 2, ext / 2], [-inf, 1.0, 1.0]):
+        for d in ([0.6, 0.0, -0.8], [0.0, -1.0, 0.0], [1e-30, 1e-30, 1.0], [1e-30, -1.0, 1e-38]):
+            rays.append([*o, *d])
+    for o in ([ext / 2, ext / 2, ext / 2], [ext / 3, 0.1, ext / 4]):
+        rays.append([*o, inf, 0.0, 0.0])
+    rays = np.asarray(rays, np.float32)
+    want = oracle.trace_rays(oracle.OracleGrid(vox), rays)
+    for variant in FBOX_VARIANTS.values():
+        got = dda(vox, rays, variant)
+        for f in ("voxel", "face"):
+            bad = np.flatnonzero(got[f] != want[f])
+            assert len(bad) == 0, (variant, f, bad[:5], rays[bad[:5]])
+        assert np.array_equal(got["t"].view(np.uint32), want["t"].view(np.uint32))
+    assert (want["t"][:128:2] >= 0).all()  # zero directions from solid voxels hit at once
+    nan = float("nan")
+    bad_rays = np.asarray([[nan, 1.0, 1.0, 0.0, 1.0, 0.0], [1.0, 1.0, 1.0, nan, 0.0, 0.0],
+                           [1.0, 1.0, 1.0, 0.0, nan, 1.0], [1.0, nan, nan, nan, nan, nan],
+                           [inf, 1.0, 1.0, inf, 0.0, 0.0], [1.0, 1.0, 1.0, nan, nan, nan]], np.float32)
+    for variant in [*VARIANTS.values(), *BOX_VARIANTS.values(), *FBOX_VARIANTS.values()]:
+        got = dda(vox, bad_rays, variant)
+        assert (got["t"] == -1.0).all() and (got["voxel"] == v.VOXEL_MISS).all(), variant
+        assert (got["brick_steps"] == 0).all() and (got["voxel_steps"] == 0).all(), variant

@@ -144,8 +144,20 @@ VRT_D bool bit_at(uint32_t w, uint32_t i) {
 // saved state either way, so every hit (voxel, face, t) and the brick steps
 // are unchanged; the voxel steps spent in bricks passed without a hit are
 // fewer.
-template <bool GM, bool KM = false, bool BOX = false, bool FBOX = false>
+// NOITER (with BOX and FBOX: the path tracer's form): no iteration budget.
+// Every step spends one cell of the count of the axis it selects, and an
+// axis the ray does not move along starts with 2 instead of 255.  A ray with
+// a finite origin and a finite, non-zero direction takes at most one step
+// along such an axis per level (only where its tMax starts at 0, or
+// stepping reaches it: its tDelta is ~1e8), so the counts and every result
+// are those of the budgeted walk, whose budget never decides for it.  A
+// degenerate walk (zero or NaN direction, NaN origin) that only the budget
+// would end -- it tests the same cell again and again -- now ends by the
+// counts instead: same cells tested, same hit or miss, fewer steps.
+template <bool GM, bool KM = false, bool BOX = false, bool FBOX = false, bool NOITER = false>
 struct DdaFastT {
+  static_assert(!NOITER || (BOX && FBOX), "NOITER sets the counts of BOX and FBOX");
+  static constexpr uint32_t STILL = NOITER ? 2u : 255u;  // count of an axis the ray does not move along
   float ro[3], rdv[3], inv[3];
   int sd[3];
   float tExit, maxT;
@@ -182,6 +194,7 @@ struct DdaFastT {
     rdv[0] = rd.x;
     rdv[1] = rd.y;
     rdv[2] = rd.z;
+    if (ray_undefined(ro, rdv)) return true;
     float tmin[3], tmax[3];
     const int G[3] = {s.nx, s.ny, s.nz};
     const int BG[3] = {s.bgx, s.bgy, s.bgz};
@@ -222,7 +235,7 @@ struct DdaFastT {
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         const int lo = s.box_lo[c], hi = s.box_hi[c];
-        const int left = sd[c] > 0 ? hi - bp[c] : (sd[c] < 0 ? bp[c] - lo : (bp[c] < lo || bp[c] > hi ? -1 : 255));
+        const int left = sd[c] > 0 ? hi - bp[c] : (sd[c] < 0 ? bp[c] - lo : (bp[c] < lo || bp[c] > hi ? -1 : (int)STILL));
         out |= left < 0;
         cnt |= (uint32_t)(left & 255) << (8 * c);
       }
@@ -271,9 +284,14 @@ struct DdaFastT {
     idx += (uint32_t)dsel;
     const uint32_t sh = c0 ? 0u : (c1 ? 8u : 16u);
     axs = sh;
-    const bool out = ((cnt >> sh) & 255u) == 0u || iter == 1u;
-    cnt -= (dsel != 0 ? 1u : 0u) << sh;
-    iter -= 1u;
+    bool out = ((cnt >> sh) & 255u) == 0u;
+    if constexpr (NOITER) {
+      cnt -= 1u << sh;
+    } else {
+      out |= iter == 1u;
+      cnt -= (dsel != 0 ? 1u : 0u) << sh;
+      iter -= 1u;
+    }
     return out;
   }
 
@@ -323,7 +341,7 @@ struct DdaFastT {
     di[2] = sd[2] * 64;
     idxB = idx;
     cntB = cnt;
-    iterB = iter;
+    if constexpr (!NOITER) iterB = iter;
     idx = (uint32_t)lp[0] + 8u * (uint32_t)lp[1] + 64u * (uint32_t)lp[2];
     bool reach = true;
     if constexpr (FBOX) {
@@ -342,14 +360,14 @@ struct DdaFastT {
       const uint32_t S = (A & mu) | (B & md);
       const uint32_t mov = (mu | md) & 0x080808u, still = ~(mu | md) & 0x080808u;
       reach = (S & mov) == mov && (A & B & still) == still;
-      cnt = (S - mov) | ((still >> 3) * 0xFFu);
+      cnt = (S - mov) | ((still >> 3) * STILL);
     } else if constexpr (KM) {
       cnt = kfine ^ (((uint32_t)lp[0] | ((uint32_t)lp[1] << 8) | ((uint32_t)lp[2] << 16)) & mfine);
     } else {
       const int n8[3] = {8, 8, 8};
       cnt = pack_left(lp, n8, sd);
     }
-    iter = 24u;
+    if constexpr (!NOITER) iter = 24u;
     if constexpr (!GM) {
       base4 = 4u * L.lane;
       wsh = 10u;
@@ -371,7 +389,7 @@ struct DdaFastT {
     di[2] = sd[2] * (int)(1u << s.psh2);
     idx = idxB;
     cnt = cntB;
-    iter = iterB;
+    if constexpr (!NOITER) iter = iterB;
     if constexpr (GM) {
       gbase = (const char*)s.pbits;
     } else {

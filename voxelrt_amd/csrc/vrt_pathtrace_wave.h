@@ -193,8 +193,8 @@ VRT_D void wave_paths(V& v, const LdsScene& L0, uint32_t min_wait, uint32_t shad
   const uint32_t lane = threadIdx.x & 63u;
   const uint64_t lt_mask = lane == 0u ? 0ull : (~0ull >> (64u - lane));
   // rays end at the box of occupied bricks, fine walks at the entered
-  // brick's box of solid voxels
-  DdaFastT<GM, true, true, true> st;
+  // brick's box of solid voxels; no iteration budget
+  DdaFastT<GM, true, true, true, true> st;
   WavePath P;
   P.pix = 0u;
   P.smp = 0u;

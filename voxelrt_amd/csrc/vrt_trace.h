@@ -140,6 +140,16 @@ struct LdsMem {
   VRT_D uint32_t mask_word(const uint32_t*, uint32_t w) const { return mask[w * stride]; }
 };
 
+// A ray with a NaN component or an origin that is not finite has no defined
+// walk (its cell indices would come from NaN, and an index from a NaN can
+// leave the grid); every DDA ends it at setup as a miss.
+VRT_D bool ray_undefined(const float ro[3], const float rd[3]) {
+  bool bad = false;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) bad |= !__builtin_isfinite(ro[c]) || rd[c] != rd[c];
+  return bad;
+}
+
 // voxel.wgsl:73-153.  Non-cubic grids: clip against [0,n]^3 per axis; the
 // brick-loop bound 3*BGRID+2 becomes bgx+bgy+bgz+2 (identical for cubes).
 template <class Mem>
@@ -155,6 +165,7 @@ VRT_D void trace_m(const SceneDev& s, const Mem& mem, f3 roW, f3 rd, float maxTW
   const float ro[3] = {roW.x * s.inv_voxel, roW.y * s.inv_voxel, roW.z * s.inv_voxel};
   const float maxT = maxTWorld * s.inv_voxel;
   const float rdv[3] = {rd.x, rd.y, rd.z};
+  if (ray_undefined(ro, rdv)) return;
   float inv[3], ge[3], adl[3];
   int sd[3];
   float tmin[3], tmax[3];
