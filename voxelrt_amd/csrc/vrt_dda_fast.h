@@ -150,10 +150,11 @@ VRT_D bool bit_at(uint32_t w, uint32_t i) {
 // a finite origin and a finite, non-zero direction takes at most one step
 // along such an axis per level (only where its tMax starts at 0, or
 // stepping reaches it: its tDelta is ~1e8), so the counts and every result
-// are those of the budgeted walk, whose budget never decides for it.  A
-// degenerate walk (zero or NaN direction, NaN origin) that only the budget
-// would end -- it tests the same cell again and again -- now ends by the
-// counts instead: same cells tested, same hit or miss, fewer steps.
+// are those of the budgeted walk, whose budget never decides for it.  The
+// walk of a zero direction, which only the budget would end -- it tests the
+// same cell again and again -- now ends by the counts instead: same cells
+// tested, same hit or miss, fewer steps.  (Rays with a NaN component end at
+// setup in every form: ray_undefined.)
 template <bool GM, bool KM = false, bool BOX = false, bool FBOX = false, bool NOITER = false>
 struct DdaFastT {
   static_assert(!NOITER || (BOX && FBOX), "NOITER sets the counts of BOX and FBOX");
