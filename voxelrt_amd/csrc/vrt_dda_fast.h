@@ -48,8 +48,10 @@
 // reference's integer bounds tests become counts of cells left along each
 // axis (an axis with zero direction never leaves, as its position never
 // changes) and its iteration caps are kept (they bind for degenerate
-// directions, e.g. a zero vector, whose steps never leave the cell).  Step
-// counts are the reference's loop iterations.
+// directions, e.g. a zero vector, whose steps never leave the cell) -- except
+// in the path tracer's form (NOITER below), which ends those walks by the
+// counts with the same result.  Step counts are the reference's loop
+// iterations (the path tracer's forms count fewer for walks they end early).
 #pragma once
 #include "vrt_trace.h"
 
