@@ -148,15 +148,17 @@ VRT_D bool bit_at(uint32_t w, uint32_t i) {
 // fewer.
 // NOITER (with BOX and FBOX: the path tracer's form): no iteration budget.
 // Every step spends one cell of the count of the axis it selects, and an
-// axis the ray does not move along starts with 2 instead of 255.  A ray with
-// a finite origin and a finite, non-zero direction takes at most one step
-// along such an axis per level (only where its tMax starts at 0, or
-// stepping reaches it: its tDelta is ~1e8), so the counts and every result
-// are those of the budgeted walk, whose budget never decides for it.  The
-// walk of a zero direction, which only the budget would end -- it tests the
-// same cell again and again -- now ends by the counts instead: same cells
-// tested, same hit or miss, fewer steps.  (Rays with a NaN component end at
-// setup in every form: ray_undefined.)
+// axis the ray does not move along starts with 2 instead of 255.  Such an
+// axis has tDelta 1e8 (8e8 at the coarse level: the clamped reciprocal of a
+// zero component), so after its first selection it can only be selected
+// again at t >= 1e8 voxel units; the path tracer's walks end at maxT = 1e4 m
+// = 1.6e5 voxels (the coarse level's t test; a hit beyond maxT is reported
+// as a miss), so every cell that NOITER leaves untested lies where no hit
+// counts.  Hits, misses and their t are the budgeted walk's for every ray --
+// a zero direction, whose walk only the budget would end (it tests the same
+// cell again and again), included -- and only the step counts of walks cut
+// short are fewer.  (Rays with a NaN component end at setup in every form:
+// ray_undefined.)  Requires maxT * inv_voxel < 1e8.
 template <bool GM, bool KM = false, bool BOX = false, bool FBOX = false, bool NOITER = false>
 struct DdaFastT {
   static_assert(!NOITER || (BOX && FBOX), "NOITER sets the counts of BOX and FBOX");
