@@ -1332,7 +1332,7 @@ int vrt_autotune(vrt_ctx* ctx, const vrt_scene* scene, uint64_t nrays, const vrt
     c.dda_batch = 1;
     c.trace_waves = 0;
     for (uint32_t b : {1024u, 1280u, 1536u})
-      for (uint32_t mw : {12u, 16u})
+      for (uint32_t mw : {12u, 16u, 20u})
         for (uint32_t fm : {24u, 32u}) {
           c.trace_blocks = b;
           c.trace_min_refill = mw;
