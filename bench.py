@@ -157,7 +157,7 @@ def traverse(args):
             print(json.dumps(_traverse_cpu(vox, rays, p)), flush=True)
         for name in list(variants) + ["autotuned"]:
             if name == "autotuned":
-                tuned = ctx.autotune(sc, 1 << 22).as_dict()
+                tuned = ctx.autotune(sc, n_rays).as_dict()  # tuned at the measured size
             else:
                 ctx.set_tuning(**dict(base, **variants[name]))
             _traverse_one(args, ctx, sc, p, name, d_rays, d_hits, n_rays,
