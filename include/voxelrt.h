@@ -381,7 +381,9 @@ int vrt_tuning_get(vrt_ctx* ctx, vrt_tuning* out);
 int vrt_tuning_set(vrt_ctx* ctx, const vrt_tuning* in);
 /* Time the candidate variants on `scene` (traversal on `nrays` random rays,
  * path tracing on a width x height view at the scene's spawn-like pose
- * given by u) and keep the fastest.  Either part may be skipped with 0. */
+ * given by u) and keep the fastest.  Either part may be skipped with 0.
+ * The best traversal launch shape depends on the launch size: pass the ray
+ * count of the launches the tuning will serve. */
 int vrt_autotune(vrt_ctx* ctx, const vrt_scene* scene, uint64_t nrays,
                  const vrt_uniforms* u, vrt_tuning* chosen);
 /* Time the batched form's launch shapes on `views` (2 or more: their scenes,
